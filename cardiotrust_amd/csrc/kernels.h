@@ -1,0 +1,74 @@
+// Internal launch interface of the gfx950 kernels (kernels.hip).
+// Device layouts (N voxels, n = 3N states, T steps, Sp = padded sensors):
+//   nbr, coefs, delays, initial_delays   [v][d]          (N, 26)
+//   gT, dGT                              [3d+out][3v+in] (78, n)  gains transposed
+//   y, y_last, fir, iir                  [v][3d+out]     (N, 78)  one filter per (v,d,out)
+//   x                                    [t][i]          (T, n)
+//   xT, eT                               [i][t]          (n, T)
+//   H                                    [b][s][i]       (B, Sp, n), rows s >= S are zero
+//   R (residuals)                        [t][s]          (T, Sp), columns s >= S are zero
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ct {
+
+constexpr int kGemmTileK = 32;
+
+struct DeriveParams {
+    float mse_scaling;
+    float reg_scaling;
+    float threshold;
+    float diff_strength;
+    int32_t freeze_gains;
+    int32_t freeze_delays;
+    int32_t simple;
+};
+
+hipError_t launch_propagate_step(hipStream_t s, const int32_t *nbr, const float *coefs,
+                                 const int32_t *delays, const float *gT, float *y, float *y_last,
+                                 float *x, const float *b, float u_t, int32_t N, int32_t t,
+                                 int32_t T);
+
+hipError_t launch_project(hipStream_t s, const float *x, const float *H, float *partial,
+                          int32_t T, int64_t n, int32_t Sp, int32_t splits, int64_t k_per_split);
+
+hipError_t launch_residuals(hipStream_t s, const float *partial, int32_t splits, const float *mhat,
+                            float *m, float *R, float *loss_mse, int32_t T, int32_t S, int32_t Sp);
+
+hipError_t launch_maxreg_partials(hipStream_t s, const float *x, float *partial, float *mr_last,
+                                  int32_t N, int32_t T, float threshold, int32_t blocks);
+
+hipError_t launch_maxreg_finalize(hipStream_t s, const float *partial, int32_t blocks,
+                                  float *mr_sum, const float *loss_mse, float *loss_mr, float *loss,
+                                  float reg_strength, int32_t T);
+
+hipError_t launch_transpose(hipStream_t s, const float *in, float *out, int64_t rows, int64_t cols);
+
+hipError_t launch_map_residuals(hipStream_t s, const float *H, const float *R, float *eT,
+                                int64_t n, int32_t T, int32_t Sp);
+
+hipError_t launch_derive(hipStream_t s, const int32_t *nbr, const float *coefs,
+                         const int32_t *delays, const float *initial_delays, const float *gT,
+                         const float *xT, const float *eT, float *fir, float *iir, float *dGT,
+                         float *dA, int32_t N, int32_t T, const DeriveParams &p);
+
+hipError_t launch_sgd(hipStream_t s, float *v, const float *d, int64_t len, float scale);
+hipError_t launch_adam(hipStream_t s, float *v, const float *d, float *m1, float *m2, int64_t len,
+                       float c1, float c2, float scale);
+hipError_t launch_roll_delays(hipStream_t s, float *coefs, int32_t *delays, int64_t len);
+
+hipError_t launch_average_delays(hipStream_t s, const float *gT, const float *coefs,
+                                 const int32_t *delays, float *avg, uint8_t *valid, int32_t N);
+hipError_t launch_smoothness(hipStream_t s, const int32_t *nbr, const float *avg,
+                             const uint8_t *valid, float *dA, float strength, int32_t N);
+
+hipError_t launch_batch_metrics(hipStream_t s, const float *loss, const float *loss_mse,
+                                const float *loss_mr, float *loss_b, float *loss_mse_b,
+                                float *loss_mr_b, int32_t idx, int32_t T);
+
+hipError_t launch_expand_rows(hipStream_t s, const float *in, float *out, int64_t N);
+
+hipError_t launch_fill(hipStream_t s, float *p, float value, int64_t len);
+
+}  // namespace ct

@@ -1,0 +1,707 @@
+// gfx950 kernels of the CardioTrust epoch hot path.
+//
+// Built with -ffp-contract=off: every fused multiply-add below is an explicit
+// fmaf() standing where the reference writes `mul_add`; all other products and
+// sums stay separate so propagation results match the reference's CPU epoch
+// bit for bit. Reference citations are relative to the reference's src/core/.
+#include "kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace ct {
+namespace {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float signum(float v) {
+    // Rust f32::signum: NaN stays NaN, -0 -> -1, +0 -> +1.
+    return isnan(v) ? v : copysignf(1.0f, v);
+}
+
+__device__ __forceinline__ float powi5(float v) {
+    // powi(x, 5) as expanded by LLVM: x * (x^2)^2.
+    const float sq = v * v;
+    const float q = sq * sq;
+    return v * q;
+}
+
+// ---------------------------------------------------------------------------
+// Propagation: one launch per time step (prediction.rs:40-105).
+//
+// A workgroup owns kPropVoxels consecutive voxels. Phase 1 runs one lane per
+// connection (v, d): it advances the three all-pass filters of that
+// connection (out = 0..2) from the neighbour's delayed states and stages the
+// outputs in LDS. Phase 2 runs one lane per state 3v+in and accumulates
+// gains * outputs in ascending (d, out) order, then adds the control term.
+// The filter state y is stored once per (v, d, out); the reference keeps one
+// identical copy per input component `in`.
+// ---------------------------------------------------------------------------
+constexpr int kPropVoxels = 64;
+constexpr int kPropThreads = 3 * kPropVoxels;
+
+__global__ __launch_bounds__(kPropThreads) void propagate_step_kernel(
+    const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
+    const int32_t *__restrict__ delays, const float *__restrict__ gT, float *__restrict__ y,
+    float *__restrict__ y_last, float *__restrict__ x, const float *__restrict__ b, float u_t,
+    int32_t N, int32_t t, int32_t T) {
+    __shared__ float ys[kPropVoxels * 78];
+    __shared__ int32_t has[kPropVoxels * 26];
+    const int64_t n = 3LL * N;
+    const int32_t v0 = blockIdx.x * kPropVoxels;
+    const int32_t nv = min(kPropVoxels, N - v0);
+    const bool last = (t == T - 1);
+
+    for (int item = threadIdx.x; item < nv * 26; item += kPropThreads) {
+        const int64_t c = int64_t(v0) * 26 + item;
+        const int32_t w = nbr[c];
+        has[item] = w >= 0;
+        if (w < 0) continue;
+        const float a = coefs[c];
+        const int32_t D = delays[c];
+        float in0[3] = {0.0f, 0.0f, 0.0f};
+        float in1[3] = {0.0f, 0.0f, 0.0f};
+        const int64_t col = 3LL * w;
+        if (D <= t) {
+            const float *r = x + int64_t(t - D) * n + col;
+            in0[0] = r[0];
+            in0[1] = r[1];
+            in0[2] = r[2];
+        }
+        if (D < t) {
+            const float *r = x + int64_t(t - D - 1) * n + col;
+            in1[0] = r[0];
+            in1[1] = r[1];
+            in1[2] = r[2];
+        }
+        float *yp = y + c * 3;
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            const float prev = yp[o];
+            if (last) y_last[c * 3 + o] = prev;
+            const float yn = fmaf(a, in0[o] - prev, in1[o]);
+            yp[o] = yn;
+            ys[item * 3 + o] = yn;
+        }
+    }
+    __syncthreads();
+
+    const int vl = threadIdx.x / 3;
+    if (vl < nv) {
+        const int64_t i = 3LL * v0 + threadIdx.x;
+        float acc = 0.0f;
+        for (int d = 0; d < 26; ++d) {
+            if (!has[vl * 26 + d]) continue;
+#pragma unroll
+            for (int o = 0; o < 3; ++o) {
+                const float g = gT[int64_t(3 * d + o) * n + i];
+                acc = acc + g * ys[(vl * 26 + d) * 3 + o];
+            }
+        }
+        x[int64_t(t) * n + i] = acc + u_t * b[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Projection H·Xᵀ over all steps (prediction.rs:112-130 for every t), FP32
+// MFMA 32x32x2. Split-K: workgroup (t-tile, split, s-tile) writes the partial
+// product of its K range; launch_residuals sums the splits in fixed order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ x,
+                                                      const float *__restrict__ H,
+                                                      float *__restrict__ partial, int32_t T,
+                                                      int64_t n, int32_t Sp, int64_t kps) {
+    __shared__ float As[64][kGemmTileK + 1];
+    __shared__ float Bs[128][kGemmTileK + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int32_t t0 = blockIdx.x * 64;
+    const int32_t split = blockIdx.y;
+    const int32_t s0 = blockIdx.z * 128;
+    const int64_t k0 = int64_t(split) * kps;
+    const int64_t k1 = min(n, k0 + kps);
+    floatx16 acc0 = {0}, acc1 = {0};
+    for (int64_t kb = k0; kb < k1; kb += kGemmTileK) {
+        {
+            const int row = tid >> 2, kq = (tid & 3) * 8;
+            const int32_t t = t0 + row;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int64_t k = kb + kq + j;
+                As[row][kq + j] = (t < T && k < k1) ? x[int64_t(t) * n + k] : 0.0f;
+            }
+        }
+        {
+            const int row = tid >> 1, kq = (tid & 1) * 16;
+            const float *hr = H + int64_t(s0 + row) * n;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int64_t k = kb + kq + j;
+                Bs[row][kq + j] = (k < k1) ? hr[k] : 0.0f;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kGemmTileK; kk += 2) {
+            const int kr = kk + (lane >> 5);
+            const float a0 = As[lane & 31][kr];
+            const float a1 = As[32 + (lane & 31)][kr];
+            const float bv = Bs[wave * 32 + (lane & 31)][kr];
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, bv, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bv, acc1, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const int32_t s = s0 + wave * 32 + (lane & 31);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+        const int32_t ta = t0 + row, tb = t0 + 32 + row;
+        if (ta < T) partial[(int64_t(split) * T + ta) * Sp + s] = acc0[r];
+        if (tb < T) partial[(int64_t(split) * T + tb) * Sp + s] = acc1[r];
+    }
+}
+
+// Residuals and per-step MSE (estimation.rs:146-152, metrics.rs:189-190).
+// One workgroup per step; the squared residuals are summed in the 8-way
+// interleaved order of ndarray's Array::sum.
+__global__ void residuals_kernel(const float *__restrict__ partial, int32_t splits,
+                                 const float *__restrict__ mhat, float *__restrict__ m,
+                                 float *__restrict__ R, float *__restrict__ loss_mse, int32_t T,
+                                 int32_t S, int32_t Sp) {
+    extern __shared__ float sq[];
+    const int32_t t = blockIdx.x;
+    for (int32_t s = threadIdx.x; s < Sp; s += blockDim.x) {
+        float acc = 0.0f;
+        for (int32_t k = 0; k < splits; ++k) acc += partial[(int64_t(k) * T + t) * Sp + s];
+        if (s < S) m[int64_t(t) * S + s] = acc;
+        if (!mhat) continue;  // forward-only simulation: predictions only
+        if (s < S) {
+            const float r = acc - mhat[int64_t(t) * S + s];
+            R[int64_t(t) * Sp + s] = r;
+            sq[s] = r * r;
+        } else {
+            R[int64_t(t) * Sp + s] = 0.0f;
+        }
+    }
+    __syncthreads();
+    if (mhat && threadIdx.x == 0) {
+        float p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int32_t s = 0;
+        for (; S - s >= 8; s += 8)
+            for (int j = 0; j < 8; ++j) p[j] = p[j] + sq[s + j];
+        float acc = 0.0f;
+        acc = acc + (p[0] + p[4]);
+        acc = acc + (p[1] + p[5]);
+        acc = acc + (p[2] + p[6]);
+        acc = acc + (p[3] + p[7]);
+        for (; s < S; ++s) acc = acc + sq[s];
+        loss_mse[t] = acc / float(S);
+    }
+}
+
+// Maximum regularisation (derivation.rs:509-535): per step, the sum over
+// voxels of (|x|_1 - threshold)^2 where positive; partial sums per workgroup.
+__global__ __launch_bounds__(256) void maxreg_partials_kernel(const float *__restrict__ x,
+                                                              float *__restrict__ partial,
+                                                              float *__restrict__ mr_last,
+                                                              int32_t N, int32_t T,
+                                                              float threshold) {
+    __shared__ float red[256];
+    const int32_t t = blockIdx.y;
+    const int32_t v = blockIdx.x * 256 + threadIdx.x;
+    float f2 = 0.0f;
+    if (v < N) {
+        const float *xv = x + int64_t(t) * 3 * N + 3LL * v;
+        const float x0 = xv[0], x1 = xv[1], x2 = xv[2];
+        const float sum = fabsf(x0) + fabsf(x1) + fabsf(x2);
+        float f = 0.0f;
+        const bool on = sum > threshold;
+        if (on) {
+            f = sum - threshold;
+            f2 = f * f;
+        }
+        if (t == T - 1) {
+            float *mv = mr_last + 3LL * v;
+            mv[0] = on ? f * signum(x0) : 0.0f;
+            mv[1] = on ? f * signum(x1) : 0.0f;
+            mv[2] = on ? f * signum(x2) : 0.0f;
+        }
+    }
+    red[threadIdx.x] = f2;
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1) {
+        if (threadIdx.x < stride) red[threadIdx.x] += red[threadIdx.x + stride];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[int64_t(t) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// Running maximum-regularisation sum (cumulative over steps and beats,
+// derivation.rs:523) and the per-step losses (metrics.rs:191-195).
+__global__ __launch_bounds__(256) void maxreg_finalize_kernel(
+    const float *__restrict__ partial, int32_t blocks, float *__restrict__ mr_sum,
+    const float *__restrict__ loss_mse, float *__restrict__ loss_mr, float *__restrict__ loss,
+    float reg_strength, int32_t T) {
+    for (int32_t t = threadIdx.x; t < T; t += blockDim.x) {
+        float s = 0.0f;
+        for (int32_t k = 0; k < blocks; ++k) s += partial[int64_t(t) * blocks + k];
+        loss_mr[t] = s;
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float cum = *mr_sum;
+        for (int32_t t = 0; t < T; ++t) {
+            cum = cum + loss_mr[t];
+            loss_mr[t] = cum;
+            loss[t] = fmaf(reg_strength, cum, loss_mse[t]);
+        }
+        *mr_sum = cum;
+    }
+}
+
+__global__ __launch_bounds__(256) void transpose_kernel(const float *__restrict__ in,
+                                                        float *__restrict__ out, int64_t rows,
+                                                        int64_t cols) {
+    __shared__ float tile[32][33];
+    const int64_t c0 = int64_t(blockIdx.x) * 32, r0 = int64_t(blockIdx.y) * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int j = ty; j < 32; j += 8) {
+        const int64_t r = r0 + j, c = c0 + tx;
+        if (r < rows && c < cols) tile[j][tx] = in[r * cols + c];
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {
+        const int64_t c = c0 + j, r = r0 + tx;
+        if (r < rows && c < cols) out[c * rows + r] = tile[tx][j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mapped residuals Hᵀ·Rᵀ for all steps (derivation.rs:538-551 for every t),
+// FP32 MFMA 32x32x2, K = padded sensor count. Output eT is (n, T).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void map_residuals_kernel(const float *__restrict__ H,
+                                                            const float *__restrict__ R,
+                                                            float *__restrict__ eT, int64_t n,
+                                                            int32_t T, int32_t Sp) {
+    __shared__ float Hs[kGemmTileK][128 + 4];
+    __shared__ float Rs[64][kGemmTileK + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t i0 = int64_t(blockIdx.x) * 128;
+    const int32_t t0 = blockIdx.y * 64;
+    floatx16 acc0 = {0}, acc1 = {0};
+    for (int32_t sb = 0; sb < Sp; sb += kGemmTileK) {
+        {
+            const int row = tid >> 3, cq = (tid & 7) * 16;
+            const float *hr = H + int64_t(sb + row) * n;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int64_t i = i0 + cq + j;
+                Hs[row][cq + j] = (i < n) ? hr[i] : 0.0f;
+            }
+        }
+        {
+            const int row = tid >> 2, kq = (tid & 3) * 8;
+            const int32_t t = t0 + row;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                Rs[row][kq + j] = (t < T) ? R[int64_t(t) * Sp + sb + kq + j] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kGemmTileK; kk += 2) {
+            const int kr = kk + (lane >> 5);
+            const float a = Hs[kr][wave * 32 + (lane & 31)];
+            const float b0 = Rs[lane & 31][kr];
+            const float b1 = Rs[32 + (lane & 31)][kr];
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1, acc1, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+        const int64_t i = i0 + wave * 32 + row;
+        const int32_t ta = t0 + (lane & 31), tb = t0 + 32 + (lane & 31);
+        if (i < n) {
+            if (ta < T) eT[i * T + ta] = acc0[r];
+            if (tb < T) eT[i * T + tb] = acc1[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Derivatives over all steps of one beat (derivation.rs:173-229, 307-329,
+// 338-398, 408-501). One lane per connection (v, d) runs the whole time loop:
+// it recomputes the connection's three all-pass outputs exactly as the
+// forward pass did, tracks the FIR/IIR sensitivities, and keeps the nine gain
+// derivatives and the coefficient derivative in registers.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void derive_kernel(
+    const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
+    const int32_t *__restrict__ delays, const float *__restrict__ initial_delays,
+    const float *__restrict__ gT, const float *__restrict__ xT, const float *__restrict__ eT,
+    float *__restrict__ fir, float *__restrict__ iir, float *__restrict__ dGT,
+    float *__restrict__ dA, int32_t N, int32_t T, DeriveParams p) {
+    const int64_t item = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (item >= int64_t(N) * 26) return;
+    const int64_t n = 3LL * N;
+    const int32_t v = int32_t(item / 26), d = int32_t(item % 26);
+    const int32_t w = nbr[item];
+    const bool has = w >= 0;
+    const float a = coefs[item];
+    const int32_t D = delays[item];
+    const float delay = float(D) + (1.0f - a) / (a + 1.0f);
+    const float reg = p.diff_strength * powi5(initial_delays[item] - delay);
+
+    float G[3][3], g[3][3];
+#pragma unroll
+    for (int in = 0; in < 3; ++in)
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            const int64_t k = int64_t(3 * d + o) * n + 3LL * v + in;
+            G[in][o] = gT[k];
+            g[in][o] = dGT[k];
+        }
+    float f[3], h[3], yv[3] = {0.0f, 0.0f, 0.0f}, prev_in[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+        f[o] = fir[item * 3 + o];
+        h[o] = iir[item * 3 + o];
+    }
+    float A = dA[item];
+    const float *xw = xT + (has ? 3LL * w : 0) * T;
+    const float *xv = xT + 3LL * v * T;
+    const float *ev = eT + 3LL * v * T;
+    const bool do_gains = !p.freeze_gains, do_coefs = !p.freeze_delays;
+
+    for (int32_t t = 0; t < T; ++t) {
+        float in0[3] = {0.0f, 0.0f, 0.0f}, yl[3];
+        if (has && D <= t) {
+#pragma unroll
+            for (int o = 0; o < 3; ++o) in0[o] = xw[int64_t(o) * T + (t - D)];
+        }
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            yl[o] = yv[o];
+            if (has) yv[o] = fmaf(a, in0[o] - yv[o], prev_in[o]);
+        }
+        float e[3], xo[3];
+#pragma unroll
+        for (int in = 0; in < 3; ++in) {
+            e[in] = ev[int64_t(in) * T + t];
+            xo[in] = xv[int64_t(in) * T + t];
+        }
+        if (do_gains) {
+            const float sum = fabsf(xo[0]) + fabsf(xo[1]) + fabsf(xo[2]);
+            const bool on = sum > p.threshold;
+            const float fac = sum - p.threshold;
+#pragma unroll
+            for (int in = 0; in < 3; ++in) {
+                const float mr = on ? fac * signum(xo[in]) : 0.0f;
+                const float c = fmaf(e[in], p.mse_scaling, mr * p.reg_scaling);
+#pragma unroll
+                for (int o = 0; o < 3; ++o) g[in][o] += yv[o] * c;
+            }
+        }
+        if (do_coefs) {
+            if (!p.simple) {
+                if (has && t >= D) {
+#pragma unroll
+                    for (int o = 0; o < 3; ++o) f[o] = fmaf(-a, f[o], in0[o]);
+                }
+                if (t >= D) {
+#pragma unroll
+                    for (int o = 0; o < 3; ++o) h[o] = fmaf(-a, h[o], yl[o]);
+                }
+#pragma unroll
+                for (int in = 0; in < 3; ++in)
+#pragma unroll
+                    for (int o = 0; o < 3; ++o)
+                        A += fmaf((f[o] - h[o]) * G[in][o] * e[in], p.mse_scaling, reg);
+            } else if (has && t >= D) {
+#pragma unroll
+                for (int in = 0; in < 3; ++in)
+#pragma unroll
+                    for (int o = 0; o < 3; ++o)
+                        A += fmaf((in0[o] - yl[o]) * G[in][o] * e[in], p.mse_scaling, reg);
+            }
+        }
+#pragma unroll
+        for (int o = 0; o < 3; ++o) prev_in[o] = in0[o];
+    }
+
+#pragma unroll
+    for (int in = 0; in < 3; ++in)
+#pragma unroll
+        for (int o = 0; o < 3; ++o) dGT[int64_t(3 * d + o) * n + 3LL * v + in] = g[in][o];
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+        fir[item * 3 + o] = f[o];
+        iir[item * 3 + o] = h[o];
+    }
+    dA[item] = A;
+}
+
+// update.rs:106-114, 158-168
+__global__ void sgd_kernel(float *__restrict__ v, const float *__restrict__ d, int64_t len,
+                           float scale) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < len) v[k] = v[k] - scale * d[k];
+}
+
+// update.rs:119-146, 173-200
+__global__ void adam_kernel(float *__restrict__ v, const float *__restrict__ d,
+                            float *__restrict__ m1, float *__restrict__ m2, int64_t len, float c1,
+                            float c2, float scale) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= len) return;
+    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+    const float omb1 = 1.0f - beta1, omb2 = 1.0f - beta2;
+    const float g = d[k];
+    const float a = m1[k] * beta1 + omb1 * g;
+    const float b = m2[k] * beta2 + omb2 * g * g;
+    m1[k] = a;
+    m2[k] = b;
+    const float ac = a / c1;
+    const float bc = b / c2;
+    const float factor = ac / (sqrtf(bc) + eps);
+    v[k] = v[k] - scale * factor;
+}
+
+// update.rs:206-228
+__global__ void roll_delays_kernel(float *__restrict__ coefs, int32_t *__restrict__ delays,
+                                   int64_t len) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= len) return;
+    const float margin = 1e-4f;
+    float a = coefs[k];
+    int32_t D = delays[k];
+    if (a > 1.0f - margin) {
+        if (D > 1) {
+            a = 2.0f * margin;
+            D -= 1;
+        } else {
+            a = 1.0f - margin;
+        }
+    } else if (a < margin) {
+        if (D < 1000) {
+            a = fmaf(2.0f, -margin, 1.0f);
+            D += 1;
+        } else {
+            a = margin;
+        }
+    }
+    coefs[k] = a;
+    delays[k] = D;
+}
+
+// derivation.rs:554-594
+__global__ void average_delays_kernel(const float *__restrict__ gT, const float *__restrict__ coefs,
+                                      const int32_t *__restrict__ delays, float *__restrict__ avg,
+                                      uint8_t *__restrict__ valid, int32_t N) {
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const int64_t n = 3LL * N;
+    float delay_sum = 0.0f, gain_sum = 0.0f;
+    for (int d = 0; d < 26; ++d) {
+        const int c = d > 12 ? d + 1 : d;
+        const int xyz = abs(c % 3 - 1) + abs((c / 3) % 3 - 1) + abs((c / 9) % 3 - 1);
+        const float delay = float(delays[int64_t(v) * 26 + d]) +
+                            (1.0f - coefs[int64_t(v) * 26 + d]) / (coefs[int64_t(v) * 26 + d] + 1.0f);
+        const float corrected = delay / sqrtf(float(xyz));
+        for (int in = 0; in < 3; ++in)
+            for (int o = 0; o < 3; ++o) {
+                const float gain = gT[int64_t(3 * d + o) * n + 3LL * v + in];
+                delay_sum += fabsf(gain) * corrected;
+                gain_sum += fabsf(gain);
+            }
+    }
+    valid[v] = gain_sum != 0.0f;
+    avg[v] = gain_sum != 0.0f ? delay_sum / gain_sum : 0.0f;
+}
+
+// derivation.rs:260-302
+__global__ void smoothness_kernel(const int32_t *__restrict__ nbr, const float *__restrict__ avg,
+                                  const uint8_t *__restrict__ valid, float *__restrict__ dA,
+                                  float strength, int32_t N) {
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N || !valid[v]) return;
+    const float own = avg[v];
+    float hood = own, divisor = 1.0f;
+    for (int d = 0; d < 26; ++d) {
+        const int32_t w = nbr[int64_t(v) * 26 + d];
+        if (w < 0) continue;
+        if (valid[w]) {
+            hood += avg[w];
+            divisor += 1.0f;
+        }
+    }
+    hood /= divisor;
+    const float diff = hood - own;
+    for (int d = 0; d < 26; ++d) dA[int64_t(v) * 26 + d] += strength * diff;
+}
+
+__device__ float unrolled_mean(const float *xs, int32_t len) {
+    float p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int32_t i = 0;
+    for (; len - i >= 8; i += 8)
+        for (int j = 0; j < 8; ++j) p[j] = p[j] + xs[i + j];
+    float acc = 0.0f;
+    acc = acc + (p[0] + p[4]);
+    acc = acc + (p[1] + p[5]);
+    acc = acc + (p[2] + p[6]);
+    acc = acc + (p[3] + p[7]);
+    for (; i < len; ++i) acc = acc + xs[i];
+    return acc / float(len);
+}
+
+// metrics.rs:204-219
+__global__ void batch_metrics_kernel(const float *loss, const float *loss_mse,
+                                     const float *loss_mr, float *loss_b, float *loss_mse_b,
+                                     float *loss_mr_b, int32_t idx, int32_t T) {
+    if (threadIdx.x == 0) loss_mse_b[idx] = unrolled_mean(loss_mse, T);
+    if (threadIdx.x == 1) loss_mr_b[idx] = unrolled_mean(loss_mr, T);
+    if (threadIdx.x == 2) loss_b[idx] = unrolled_mean(loss, T);
+}
+
+// (N, 78) per-connection filter rows -> (3N, 78) per-state rows.
+__global__ void expand_rows_kernel(const float *__restrict__ in, float *__restrict__ out,
+                                   int64_t N) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= N * 234) return;
+    const int64_t row = k / 78, g = k % 78;
+    out[k] = in[(row / 3) * 78 + g];
+}
+
+__global__ void fill_kernel(float *p, float value, int64_t len) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < len) p[k] = value;
+}
+
+inline unsigned blocks_for(int64_t len, int threads) {
+    return unsigned((len + threads - 1) / threads);
+}
+
+}  // namespace
+
+hipError_t launch_propagate_step(hipStream_t s, const int32_t *nbr, const float *coefs,
+                                 const int32_t *delays, const float *gT, float *y, float *y_last,
+                                 float *x, const float *b, float u_t, int32_t N, int32_t t,
+                                 int32_t T) {
+    if (N <= 0) return hipSuccess;
+    propagate_step_kernel<<<blocks_for(N, kPropVoxels), kPropThreads, 0, s>>>(
+        nbr, coefs, delays, gT, y, y_last, x, b, u_t, N, t, T);
+    return hipGetLastError();
+}
+
+hipError_t launch_project(hipStream_t s, const float *x, const float *H, float *partial,
+                          int32_t T, int64_t n, int32_t Sp, int32_t splits, int64_t kps) {
+    dim3 grid(blocks_for(T, 64), splits, Sp / 128);
+    project_kernel<<<grid, 256, 0, s>>>(x, H, partial, T, n, Sp, kps);
+    return hipGetLastError();
+}
+
+hipError_t launch_residuals(hipStream_t s, const float *partial, int32_t splits, const float *mhat,
+                            float *m, float *R, float *loss_mse, int32_t T, int32_t S, int32_t Sp) {
+    const int threads = Sp < 1024 ? Sp : 1024;
+    residuals_kernel<<<T, threads, sizeof(float) * Sp, s>>>(partial, splits, mhat, m, R, loss_mse,
+                                                            T, S, Sp);
+    return hipGetLastError();
+}
+
+hipError_t launch_maxreg_partials(hipStream_t s, const float *x, float *partial, float *mr_last,
+                                  int32_t N, int32_t T, float threshold, int32_t blocks) {
+    dim3 grid(blocks, T);
+    maxreg_partials_kernel<<<grid, 256, 0, s>>>(x, partial, mr_last, N, T, threshold);
+    return hipGetLastError();
+}
+
+hipError_t launch_maxreg_finalize(hipStream_t s, const float *partial, int32_t blocks,
+                                  float *mr_sum, const float *loss_mse, float *loss_mr, float *loss,
+                                  float reg_strength, int32_t T) {
+    maxreg_finalize_kernel<<<1, 256, 0, s>>>(partial, blocks, mr_sum, loss_mse, loss_mr, loss,
+                                             reg_strength, T);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose(hipStream_t s, const float *in, float *out, int64_t rows, int64_t cols) {
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    dim3 grid(blocks_for(cols, 32), blocks_for(rows, 32));
+    transpose_kernel<<<grid, 256, 0, s>>>(in, out, rows, cols);
+    return hipGetLastError();
+}
+
+hipError_t launch_map_residuals(hipStream_t s, const float *H, const float *R, float *eT,
+                                int64_t n, int32_t T, int32_t Sp) {
+    if (n <= 0) return hipSuccess;
+    dim3 grid(blocks_for(n, 128), blocks_for(T, 64));
+    map_residuals_kernel<<<grid, 256, 0, s>>>(H, R, eT, n, T, Sp);
+    return hipGetLastError();
+}
+
+hipError_t launch_derive(hipStream_t s, const int32_t *nbr, const float *coefs,
+                         const int32_t *delays, const float *initial_delays, const float *gT,
+                         const float *xT, const float *eT, float *fir, float *iir, float *dGT,
+                         float *dA, int32_t N, int32_t T, const DeriveParams &p) {
+    if (N <= 0) return hipSuccess;
+    derive_kernel<<<blocks_for(int64_t(N) * 26, 256), 256, 0, s>>>(
+        nbr, coefs, delays, initial_delays, gT, xT, eT, fir, iir, dGT, dA, N, T, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_sgd(hipStream_t s, float *v, const float *d, int64_t len, float scale) {
+    if (len <= 0) return hipSuccess;
+    sgd_kernel<<<blocks_for(len, 256), 256, 0, s>>>(v, d, len, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(hipStream_t s, float *v, const float *d, float *m1, float *m2, int64_t len,
+                       float c1, float c2, float scale) {
+    if (len <= 0) return hipSuccess;
+    adam_kernel<<<blocks_for(len, 256), 256, 0, s>>>(v, d, m1, m2, len, c1, c2, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_roll_delays(hipStream_t s, float *coefs, int32_t *delays, int64_t len) {
+    if (len <= 0) return hipSuccess;
+    roll_delays_kernel<<<blocks_for(len, 256), 256, 0, s>>>(coefs, delays, len);
+    return hipGetLastError();
+}
+
+hipError_t launch_average_delays(hipStream_t s, const float *gT, const float *coefs,
+                                 const int32_t *delays, float *avg, uint8_t *valid, int32_t N) {
+    if (N <= 0) return hipSuccess;
+    average_delays_kernel<<<blocks_for(N, 256), 256, 0, s>>>(gT, coefs, delays, avg, valid, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_smoothness(hipStream_t s, const int32_t *nbr, const float *avg,
+                             const uint8_t *valid, float *dA, float strength, int32_t N) {
+    if (N <= 0) return hipSuccess;
+    smoothness_kernel<<<blocks_for(N, 256), 256, 0, s>>>(nbr, avg, valid, dA, strength, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_metrics(hipStream_t s, const float *loss, const float *loss_mse,
+                                const float *loss_mr, float *loss_b, float *loss_mse_b,
+                                float *loss_mr_b, int32_t idx, int32_t T) {
+    batch_metrics_kernel<<<1, 64, 0, s>>>(loss, loss_mse, loss_mr, loss_b, loss_mse_b, loss_mr_b,
+                                          idx, T);
+    return hipGetLastError();
+}
+
+hipError_t launch_expand_rows(hipStream_t s, const float *in, float *out, int64_t N) {
+    if (N <= 0) return hipSuccess;
+    expand_rows_kernel<<<blocks_for(N * 234, 256), 256, 0, s>>>(in, out, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(hipStream_t s, float *p, float value, int64_t len) {
+    if (len <= 0) return hipSuccess;
+    fill_kernel<<<blocks_for(len, 256), 256, 0, s>>>(p, value, len);
+    return hipGetLastError();
+}
+
+}  // namespace ct

@@ -1,0 +1,185 @@
+/*
+ * cardiotrust_hip.h -- C ABI of the MI355X (gfx950) CardioTrust epoch library.
+ *
+ * Drop-in replacement for the reference's OpenCL backend
+ * (src/core/algorithm/gpu/*.rs). Each entry point names the reference
+ * interface it replaces (paths relative to the reference root). All pointers
+ * are plain host pointers; every array is in the reference's row-major layout:
+ *
+ *   gains, output_state_indices, ap_outputs_*, coefs_fir/iir   (3N, 78)
+ *   coefs, delays, initial_delays                              (N, 26)
+ *   control_matrix (3N)   control_function (T)
+ *   measurement_matrix (B, S, 3N)   measurements (B, T, S)   system_states (T, 3N)
+ *
+ * The library copies what it is given and never frees caller memory.
+ * Functions return CT_OK (0) or a negative status; ct_last_error() returns the
+ * message of the calling thread's last failure. Handles are not thread-safe:
+ * use one ct_epoch_kernel per thread, as the reference uses one OpenCL queue
+ * per scenario thread (src/scheduler.rs:93, src/core/scenario.rs:807).
+ */
+#ifndef CARDIOTRUST_HIP_H
+#define CARDIOTRUST_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CT_OK 0
+#define CT_ERR_INVALID (-1)   /* bad argument, shape or parameter table */
+#define CT_ERR_DEVICE (-2)    /* HIP runtime / device failure */
+#define CT_ERR_STATE (-3)     /* call out of order, index out of range */
+
+typedef struct ct_gpu ct_gpu;
+typedef struct ct_buffer ct_buffer;
+typedef struct ct_results_gpu ct_results_gpu;
+typedef struct ct_epoch_kernel ct_epoch_kernel;
+
+/* Hot-path fields of config::algorithm::Algorithm
+ * (src/core/config/algorithm.rs:25-58; defaults 59-85). */
+typedef struct {
+    float learning_rate;
+    int32_t batch_size;                   /* 0 = all beats per update */
+    int32_t optimizer;                    /* 0 = Sgd, 1 = Adam (refinement.rs:7-12) */
+    float mse_strength;
+    float maximum_regularization_strength;
+    float maximum_regularization_threshold;
+    float difference_regularization_strength;
+    float smoothness_regularization_strength;
+    int32_t freeze_gains;
+    int32_t freeze_delays;
+    int32_t ap_derivative;                /* 0 = Textbook, 1 = Simple */
+} ct_algorithm;
+
+/* Host arrays uploaded by ct_results_to_gpu: the model part of Results
+ * (APParameters allpass.rs:29-36, ControlMatrix/ControlFunction control.rs,
+ * MeasurementMatrix measurement.rs:22). */
+typedef struct {
+    int32_t num_voxels;                   /* N (states 3N); 0 is allowed */
+    int32_t num_sensors;                  /* S */
+    int32_t num_steps;                    /* T */
+    int32_t num_beats;                    /* B */
+    int32_t num_batch_entries;            /* length of the *_batch metrics (epochs x batches) */
+    int32_t optimizer;                    /* 1 allocates the Adam moments */
+    const float *gains;                   /* (3N,78) */
+    const int32_t *output_state_indices;  /* (3N,78), -1 = None */
+    const float *coefs;                   /* (N,26) */
+    const int32_t *delays;                /* (N,26) */
+    const float *initial_delays;          /* (N,26) */
+    const float *control_matrix;          /* (3N) */
+    const float *control_function;        /* (T) */
+    const float *measurement_matrix;      /* (B,S,3N) */
+} ct_results_desc;
+
+/* Host destinations for ct_results_update_from_gpu; any pointer may be NULL.
+ * Mirrors Results::update_from_gpu (scenario/results.rs:136-145) and the
+ * per-struct update_from_gpu it calls. */
+typedef struct {
+    float *gains;                         /* (3N,78) */
+    float *coefs;                         /* (N,26) */
+    int32_t *delays;                      /* (N,26) */
+    float *ap_outputs_now;                /* (3N,78) */
+    float *ap_outputs_last;               /* (3N,78) */
+    float *system_states;                 /* (T,3N), last processed beat */
+    float *measurements;                  /* (B,T,S) */
+    float *residuals;                     /* (S), last step */
+    float *d_gains;                       /* (3N,78) */
+    float *d_coefs;                       /* (N,26) */
+    float *d_coefs_iir;                   /* (3N,78) */
+    float *d_coefs_fir;                   /* (3N,78) */
+    float *mapped_residuals;              /* (3N), last step */
+    float *maximum_regularization;        /* (3N), last step */
+    float *maximum_regularization_sum;    /* (1) */
+    float *average_delays;                /* (N); NaN where None */
+    float *loss;                          /* (T) */
+    float *loss_mse;                      /* (T) */
+    float *loss_maximum_regularization;   /* (T) */
+    float *loss_batch;                    /* (num_batch_entries) */
+    float *loss_mse_batch;
+    float *loss_maximum_regularization_batch;
+    int64_t *derivatives_step;            /* (1) Derivatives::step */
+} ct_results_host;
+
+/* Per-phase device time of the last ct_epoch_kernel_execute, from HIP events
+ * on the library's stream (filled only while profiling is enabled). */
+typedef struct {
+    double propagate_ms;                  /* sum over propagation launches */
+    int64_t propagate_launches;
+    double project_ms;                    /* H·Xᵀ contraction + residuals + step losses */
+    double regularize_ms;                 /* maximum-regularisation sums */
+    double transpose_ms;
+    double map_residuals_ms;              /* Hᵀ·R contraction */
+    double derive_ms;                     /* gain / coefficient derivatives */
+    double update_ms;                     /* batch derivatives, update, batch metrics */
+    double total_ms;                      /* whole epoch, first to last event */
+} ct_profile;
+
+const char *ct_last_error(void);
+
+/* GPU::new (src/core/algorithm/gpu.rs:31-63): open `device` and create the
+ * library's stream (default priority). */
+int ct_gpu_new(int32_t device, ct_gpu **out);
+void ct_gpu_free(ct_gpu *gpu);
+int ct_gpu_synchronize(ct_gpu *gpu);
+
+/* Measurements::to_gpu (src/core/data/shapes.rs:451-463): (B,T,S) f32. */
+int ct_measurements_to_gpu(ct_gpu *gpu, const float *measurements, int32_t num_beats,
+                           int32_t num_steps, int32_t num_sensors, ct_buffer **out);
+void ct_buffer_free(ct_buffer *buffer);
+
+/* Results::to_gpu (src/core/scenario/results.rs:121-132). Validates that
+ * output_state_indices has the 26-neighbour form of allpass.rs:212-289
+ * (row 3v+in, column 3d+out holds 3w+out for every in, or -1) and that every
+ * connected delay is >= 1. */
+int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu **out);
+void ct_results_gpu_free(ct_results_gpu *results);
+
+/* Results::update_from_gpu (src/core/scenario/results.rs:136-145). Blocks. */
+int ct_results_update_from_gpu(ct_results_gpu *results, const ct_results_host *out);
+
+/* Simulation::run without noise (src/core/data/simulation.rs:129-150): the
+ * forward propagation and H·x of every beat with the uploaded parameters.
+ * Fills the device measurements (download with ct_results_update_from_gpu)
+ * and, when `measurements_out` is not NULL, copies them there (B,T,S). */
+int ct_results_simulate(ct_results_gpu *results, float *measurements_out);
+
+/* EpochKernel::new (src/core/algorithm/gpu/epoch.rs:29-93). The sizes must
+ * match the results; `actual_measurements` must be (B,T,S). */
+int ct_epoch_kernel_new(ct_gpu *gpu, ct_results_gpu *results, const ct_buffer *actual_measurements,
+                        const ct_algorithm *config, int32_t number_of_states,
+                        int32_t number_of_sensors, int32_t number_of_steps,
+                        ct_epoch_kernel **out);
+void ct_epoch_kernel_free(ct_epoch_kernel *kernel);
+
+/* EpochKernel::execute (gpu/epoch.rs:96-117) with the semantics of the CPU
+ * epoch algorithm::run_epoch (src/core/algorithm.rs:124-281): every beat,
+ * batch_size, SGD or Adam, all regularisers. Enqueues on the library stream
+ * and returns; results are read with ct_results_update_from_gpu. */
+int ct_epoch_kernel_execute(ct_epoch_kernel *kernel);
+
+/* EpochKernel::set_freeze_delays / set_freeze_gains (gpu/epoch.rs:118-125). */
+int ct_epoch_kernel_set_freeze_delays(ct_epoch_kernel *kernel, int32_t value);
+int ct_epoch_kernel_set_freeze_gains(ct_epoch_kernel *kernel, int32_t value);
+
+/* Replaces the unseeded beat shuffle of run_epoch (algorithm.rs:139-141):
+ * the beats of the next epochs are visited in `order` (a permutation of
+ * 0..B-1). NULL restores 0..B-1. */
+int ct_epoch_kernel_set_beat_order(ct_epoch_kernel *kernel, const int32_t *order, int32_t len);
+
+/* The `batch_index` argument of run_epoch (algorithm.rs:126): index of the
+ * next *_batch metric entry. */
+int ct_epoch_kernel_set_batch_index(ct_epoch_kernel *kernel, int32_t batch_index);
+int ct_epoch_kernel_get_batch_index(const ct_epoch_kernel *kernel, int32_t *batch_index);
+
+/* Per-phase HIP-event timing of ct_epoch_kernel_execute (no reference
+ * counterpart; the reference times epochs with Criterion,
+ * benches/gpu_epoch_benches.rs:40-53). Reading the profile synchronises. */
+int ct_epoch_kernel_set_profiling(ct_epoch_kernel *kernel, int32_t enabled);
+int ct_epoch_kernel_get_profile(ct_epoch_kernel *kernel, ct_profile *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,131 @@
+/*
+ * CPU oracle for the CardioTrust epoch hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A from-source C++ restatement of the reference's CPU epoch (`run_epoch` and
+ * everything it calls), in two precisions: an FP32 variant that follows the
+ * reference's operation order (fused multiply-add exactly where the Rust code
+ * writes `mul_add`, plain multiply-then-add elsewhere, built with
+ * -ffp-contract=off), and an FP64 twin of the same algorithm used to report the
+ * absolute error of every FP32 result.
+ *
+ * Only the test suite, the smoke check and the CPU-baseline leg of the
+ * benchmark may load this library, and only as the checker. The product path
+ * (libcardiotrust_hip.so) never links or calls it.
+ *
+ * Array layouts are the reference's row-major ndarray layouts:
+ *   gains, output_state_indices, ap_outputs_*, coefs_fir/iir : (3N, 78)
+ *   coefs, delays, initial_delays                            : (N, 26)
+ *   system_states                                            : (T, 3N)
+ *   measurement_matrix                                       : (B, S, 3N)
+ *   measurements, actual_measurements                        : (B, T, S)
+ * `real` pointers are float* for the *_f32 entry points and double* for *_f64.
+ */
+#ifndef CT_ORACLE_H
+#define CT_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int32_t num_voxels;   /* N; states n = 3N */
+    int32_t num_sensors;  /* S */
+    int32_t num_steps;    /* T */
+    int32_t num_beats;    /* B */
+
+    /* model (APParameters, ControlMatrix, ControlFunction, MeasurementMatrix) */
+    void *gains;                           /* (n,78)  in/out */
+    const int32_t *output_state_indices;   /* (n,78)  -1 = None */
+    void *coefs;                           /* (N,26)  in/out */
+    int32_t *delays;                       /* (N,26)  in/out */
+    const void *initial_delays;            /* (N,26) */
+    const void *control_matrix;            /* (n) */
+    const void *control_function;          /* (T) */
+    const void *measurement_matrix;        /* (B,S,n) */
+    const void *actual_measurements;       /* (B,T,S) */
+
+    /* estimations */
+    void *ap_outputs_now;                  /* (n,78) */
+    void *ap_outputs_last;                 /* (n,78) */
+    void *system_states;                   /* (T,n) */
+    void *measurements;                    /* (B,T,S) */
+    void *residuals;                       /* (S) */
+    void *average_delays;                  /* (N) */
+    uint8_t *average_delays_valid;         /* (N) 0 = None */
+
+    /* derivatives */
+    void *d_gains;                         /* (n,78) */
+    void *d_coefs;                         /* (N,26) */
+    void *d_coefs_iir;                     /* (n,78) */
+    void *d_coefs_fir;                     /* (n,78) */
+    void *mapped_residuals;                /* (n) */
+    void *maximum_regularization;          /* (n) */
+    void *maximum_regularization_sum;      /* (1) */
+    int64_t *d_step;                       /* (1) Adam step, starts at 1 */
+    void *gains_first_moment;              /* (n,78) Adam only, else NULL */
+    void *gains_second_moment;
+    void *coefs_first_moment;              /* (N,26) Adam only, else NULL */
+    void *coefs_second_moment;
+
+    /* metrics */
+    void *loss;                            /* (T) */
+    void *loss_mse;                        /* (T) */
+    void *loss_maximum_regularization;     /* (T) */
+    void *loss_batch;                      /* (num_batch_entries) */
+    void *loss_mse_batch;
+    void *loss_maximum_regularization_batch;
+    int32_t num_batch_entries;
+} ct_oracle_problem;
+
+typedef struct {
+    float learning_rate;
+    int32_t batch_size;                    /* 0 = all beats */
+    int32_t optimizer;                     /* 0 = SGD, 1 = Adam */
+    float mse_strength;
+    float maximum_regularization_strength;
+    float maximum_regularization_threshold;
+    float difference_regularization_strength;
+    float smoothness_regularization_strength;
+    int32_t freeze_gains;
+    int32_t freeze_delays;
+    int32_t ap_derivative;                 /* 0 = Textbook, 1 = Simple */
+} ct_oracle_algorithm;
+
+/* One epoch (algorithm.rs:124-281). `beat_order` replaces the reference's
+ * unseeded shuffle (NULL = 0..B-1). Returns 0, or <0 on error. */
+int ct_oracle_run_epoch_f32(ct_oracle_problem *p, const ct_oracle_algorithm *cfg,
+                            const int32_t *beat_order, int32_t *batch_index);
+int ct_oracle_run_epoch_f64(ct_oracle_problem *p, const ct_oracle_algorithm *cfg,
+                            const int32_t *beat_order, int32_t *batch_index);
+
+/* Forward-only simulation of every beat (simulation.rs:129-150 without noise):
+ * fills system_states (last beat) and measurements for all beats. */
+int ct_oracle_simulate_f32(ct_oracle_problem *p);
+int ct_oracle_simulate_f64(ct_oracle_problem *p);
+
+/* Scalar helpers (allpass.rs:532-668). Index helpers return -1 for None. */
+int32_t ct_oracle_offset_to_gain_index(int32_t x, int32_t y, int32_t z, int32_t out);
+int32_t ct_oracle_gain_index_to_offset(int32_t gain_index, int32_t *xyz_out4);
+int32_t ct_oracle_offset_to_delay_index(int32_t x, int32_t y, int32_t z);
+int32_t ct_oracle_delay_index_to_offset(int32_t delay_index, int32_t *xyz3);
+float ct_oracle_from_samples_to_coef(float samples);
+int64_t ct_oracle_from_samples_to_usize(float samples);
+float ct_oracle_from_coef_to_samples(float coef);
+
+/* Piecewise kernels exposed for golden-vector tests. */
+int ct_oracle_average_delays_f32(int32_t num_voxels, const float *gains, const float *coefs,
+                                 const int32_t *delays, float *average_delays,
+                                 uint8_t *valid);
+void ct_oracle_update_sgd_f32(float *values, const float *derivatives, int64_t len,
+                              float learning_rate, int64_t batch_size);
+void ct_oracle_roll_delays_f32(float *coefs, int32_t *delays, int64_t len);
+void ct_oracle_mapped_residuals_f32(const float *measurement_matrix, const float *residuals,
+                                    int32_t num_sensors, int32_t num_states, float *out);
+float ct_oracle_unrolled_dot_f32(const float *a, const float *b, int64_t len);
+float ct_oracle_unrolled_sum_f32(const float *a, int64_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
