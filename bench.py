@@ -1,0 +1,168 @@
+#!/usr/bin/env python3
+"""Epoch throughput of the gfx950 CardioTrust hot path on one MI355X.
+
+Workload: one plain epoch (forward propagation, projection, residuals,
+gain and coefficient derivatives, SGD update) of a 64^3 grid masked to an
+ellipsoidal shell: N = 120,120 voxels (360,360 states), 26 connections per
+voxel with delays 1-40, 108 sensors, T = 2000 steps at 2 kHz, one beat,
+learning rate 1e-3, delays not frozen. Inputs are synthetic (seeded); the
+measurements are the noise-free GPU simulation of the ground truth.
+
+Prints one JSON line; per-phase timings go to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "epochs/s at 120k voxels x 2000 steps (fwd+grad+update), HBM GB/s vs peak"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes of one propagation step per active voxel: 4 B x (234 gains
+# + 26 coefs + 26 delays + 2 x 78 filter outputs read/write + 3 states written
+# + 3 states read).
+PROP_BYTES_PER_VOXEL_STEP = 4 * (234 + 26 + 26 + 2 * 78 + 3 + 3)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(problem, measurements, sample_steps: int):
+    """Oracle FP32 epoch (single thread) on the first `sample_steps` steps of
+    the same problem, scaled to epochs/s of the full T."""
+    from oracle import oracle
+    from cardiotrust_amd import Algorithm
+    from cardiotrust_amd.synth import Problem
+
+    p = Problem(problem.geometry, problem.num_sensors, sample_steps, 1, problem.gains,
+                problem.output_state_indices, problem.coefs, problem.delays,
+                problem.initial_delays, problem.control_matrix,
+                problem.control_function[:sample_steps], problem.measurement_matrix,
+                np.ascontiguousarray(measurements[:, :sample_steps]))
+    r = oracle.OracleResults(p, np.float32)
+    cfg = Algorithm(learning_rate=1e-3, freeze_delays=False)
+    t0 = time.perf_counter()
+    r.run_epoch(cfg)
+    dt = time.perf_counter() - t0
+    per_epoch = dt * problem.num_steps / sample_steps
+    return {
+        "value": 1.0 / per_epoch,
+        "unit": "epochs/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle FP32 run_epoch on the first {sample_steps} of {problem.num_steps} steps "
+                  f"of the same problem ({dt:.1f} s), scaled by T/{sample_steps}",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed epochs")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed epochs")
+    ap.add_argument("--num-steps", type=int, default=2000, help="time steps per beat")
+    ap.add_argument("--cpu-sample-steps", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs the epoch on a single MI355X; "
+              "scenarios would shard one per GPU with no communication (see DESIGN.md).",
+              file=sys.stderr)
+        return 2
+
+    from cardiotrust_amd import Algorithm, synth
+    from cardiotrust_amd.gpu import GPU, Buffer, EpochKernel, ResultsGPU
+
+    t0 = time.perf_counter()
+    problem = synth.config1(num_steps=args.num_steps)
+    N, T, S = problem.num_voxels, problem.num_steps, problem.num_sensors
+    log(f"synthetic problem: N={N} voxels, T={T}, S={S} ({time.perf_counter() - t0:.1f} s)")
+    gpu = GPU(0)
+    truth = ResultsGPU(gpu, problem.truth())
+    measurements = truth.simulate()
+    truth.close()
+    problem.measurements = measurements
+    log(f"ground truth simulated on GPU ({time.perf_counter() - t0:.1f} s)")
+
+    cfg = Algorithm(learning_rate=1e-3, freeze_delays=False)
+    results = ResultsGPU(gpu, problem, num_batch_entries=args.warmup + args.steps + 1)
+    actual = Buffer(gpu, measurements)
+    kernel = EpochKernel(gpu, results, actual, cfg, 3 * N, S, T)
+    for _ in range(args.warmup):
+        kernel.execute()
+    gpu.synchronize()
+    log(f"warmup done ({time.perf_counter() - t0:.1f} s)")
+
+    # Timed region: K epochs with per-launch HIP events on the library stream.
+    kernel.set_profiling(True)
+    prof_sum = {}
+    gpu.synchronize()
+    start = time.perf_counter()
+    for _ in range(args.steps):
+        kernel.execute()
+        p = kernel.profile()  # synchronises
+        for k, v in p.items():
+            prof_sum[k] = prof_sum.get(k, 0.0) + v
+    gpu.synchronize()
+    elapsed = time.perf_counter() - start
+    kernel.set_profiling(False)
+    ms_per_epoch = 1000.0 * elapsed / args.steps
+    prof = {k: v / args.steps for k, v in prof_sum.items()}
+    log("per-epoch phases (ms): " + json.dumps({k: round(v, 3) for k, v in prof.items()}))
+
+    prop_ms = prof["propagate_ms"] / max(prof["propagate_launches"], 1)
+    achieved = PROP_BYTES_PER_VOXEL_STEP * N / (prop_ms * 1e-3) / 1e9
+    loss = results.update_from_gpu(["loss_batch"])["loss_batch"]
+    log(f"loss_batch per epoch: {loss.tolist()}")
+
+    out = {
+        "metric": METRIC,
+        "value": 1000.0 / ms_per_epoch,
+        "unit": "epochs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_epoch,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain epoch (fwd+grad+update), 64^3 ellipsoidal shell, N={N} voxels "
+                        f"({3 * N} states), 26 neighbours, delays 1-40, S={S} sensors, T={T} "
+                        f"steps, 1 beat, SGD lr 1e-3",
+            "voxels": N, "steps_per_beat": T, "sensors": S, "beats": 1,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "propagate_step_kernel",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_ms": prop_ms,
+            "bytes_per_launch": PROP_BYTES_PER_VOXEL_STEP * N,
+        },
+    }
+    kernel.close()
+    actual.close()
+    results.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(problem, measurements, args.cpu_sample_steps)
+    print(json.dumps(out), flush=True)
+    gpu.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

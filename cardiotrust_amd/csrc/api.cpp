@@ -1,0 +1,742 @@
+// Host runtime behind include/cardiotrust_hip.h.
+//
+// Owns device memory and the stream, validates the reference-layout inputs,
+// converts them to the device layouts described in kernels.h, and drives the
+// epoch in the order of the reference's CPU run_epoch
+// (src/core/algorithm.rs:124-281): per beat, the forward propagation over all
+// steps, then the deferred projection, residuals, losses and derivatives;
+// per batch, the smoothness term, the update and the batch metrics.
+#include "cardiotrust_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string &msg) {
+    g_error = msg;
+    return code;
+}
+
+#define CT_HIP(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(CT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CT_CHECK(expr)          \
+    do {                        \
+        int rc_ = (expr);       \
+        if (rc_ != CT_OK) return rc_; \
+    } while (0)
+
+}  // namespace
+
+struct ct_gpu {
+    int32_t device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct ct_buffer {
+    ct_gpu *gpu = nullptr;
+    float *data = nullptr;
+    int32_t beats = 0, steps = 0, sensors = 0;
+};
+
+struct ct_results_gpu {
+    ct_gpu *gpu = nullptr;
+    int32_t N = 0, S = 0, Sp = 0, T = 0, B = 0, nbatch = 0, optimizer = 0;
+    int64_t n = 0;
+    std::vector<void *> allocations;
+    std::vector<float> control_function;  // host copy: u[t] is a launch argument
+
+    // model
+    int32_t *nbr = nullptr, *delays = nullptr;
+    float *coefs = nullptr, *initial_delays = nullptr, *gT = nullptr, *b = nullptr, *H = nullptr;
+    // estimations
+    float *x = nullptr, *y = nullptr, *y_last = nullptr, *m = nullptr, *R = nullptr;
+    float *xT = nullptr, *eT = nullptr;
+    float *avg = nullptr;
+    uint8_t *avg_valid = nullptr;
+    // derivatives
+    float *dGT = nullptr, *dA = nullptr, *fir = nullptr, *iir = nullptr, *mr_last = nullptr,
+          *mr_sum = nullptr;
+    float *gm1 = nullptr, *gm2 = nullptr, *cm1 = nullptr, *cm2 = nullptr;
+    int64_t step = 1;
+    // metrics
+    float *loss = nullptr, *loss_mse = nullptr, *loss_mr = nullptr;
+    float *loss_b = nullptr, *loss_mse_b = nullptr, *loss_mr_b = nullptr;
+    // scratch
+    float *partial = nullptr, *mr_partial = nullptr, *staging = nullptr;
+    int32_t splits = 1, mr_blocks = 1;
+    int64_t kps = ct::kGemmTileK;
+
+    ~ct_results_gpu() {
+        for (void *p : allocations) (void)hipFree(p);
+    }
+
+    template <typename V>
+    int alloc(V **p, int64_t count) {
+        const size_t bytes = std::max<size_t>(sizeof(V) * size_t(std::max<int64_t>(count, 0)), 256);
+        void *raw = nullptr;
+        hipError_t e = hipMalloc(&raw, bytes);
+        if (e != hipSuccess)
+            return fail(CT_ERR_DEVICE, "hipMalloc(" + std::to_string(bytes) +
+                                           " bytes): " + hipGetErrorString(e));
+        allocations.push_back(raw);
+        *p = static_cast<V *>(raw);
+        return CT_OK;
+    }
+};
+
+struct ct_epoch_kernel {
+    ct_gpu *gpu = nullptr;
+    ct_results_gpu *r = nullptr;
+    const ct_buffer *mhat = nullptr;
+    ct_algorithm cfg{};
+    std::vector<int32_t> order;
+    int32_t batch_index = 0;
+    bool profiling = false;
+    std::vector<hipEvent_t> events;
+    size_t used_events = 0;
+    // (phase, start event, end event)
+    std::vector<std::pair<int, std::pair<size_t, size_t>>> spans;
+
+    ~ct_epoch_kernel() {
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    }
+};
+
+namespace {
+
+enum Phase { kPropagate, kProject, kRegularize, kTranspose, kMapResiduals, kDerive, kUpdate, kTotal };
+
+struct Span {
+    ct_epoch_kernel *k;
+    int phase;
+    size_t start = 0;
+    bool on = false;
+    Span(ct_epoch_kernel *kk, int ph) : k(kk), phase(ph) {
+        if (!k || !k->profiling) return;
+        if (k->used_events + 2 > k->events.size()) return;
+        start = k->used_events;
+        k->used_events += 2;
+        on = hipEventRecord(k->events[start], k->r->gpu->stream) == hipSuccess;
+    }
+    ~Span() {
+        if (!on) return;
+        if (hipEventRecord(k->events[start + 1], k->r->gpu->stream) == hipSuccess)
+            k->spans.push_back({phase, {start, start + 1}});
+    }
+};
+
+int check_gpu(ct_gpu *gpu) {
+    if (!gpu) return fail(CT_ERR_INVALID, "null ct_gpu");
+    CT_HIP(hipSetDevice(gpu->device));
+    return CT_OK;
+}
+
+int upload(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return CT_OK;
+    if (!src) return fail(CT_ERR_INVALID, "null input array");
+    CT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return CT_OK;
+}
+
+int download(void *dst, const void *src, size_t bytes) {
+    if (!dst || !bytes) return CT_OK;
+    CT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return CT_OK;
+}
+
+// Checks the 26-neighbour structure of output_state_indices (allpass.rs:212-289)
+// and extracts the neighbour table; checks delays >= 1 on connections.
+int neighbours_from_indices(int32_t N, const int32_t *osi, const int32_t *delays,
+                            std::vector<int32_t> &nbr) {
+    nbr.assign(size_t(N) * 26, -1);
+    for (int64_t v = 0; v < N; ++v) {
+        for (int d = 0; d < 26; ++d) {
+            const int32_t first = osi[(3 * v) * 78 + 3 * d];
+            int32_t w = -1;
+            if (first >= 0) {
+                if (first % 3 != 0 || first / 3 >= N)
+                    return fail(CT_ERR_INVALID, "output_state_indices[" + std::to_string(3 * v) +
+                                                    "," + std::to_string(3 * d) +
+                                                    "] is not 3*voxel of a valid voxel");
+                w = first / 3;
+            }
+            for (int in = 0; in < 3; ++in)
+                for (int o = 0; o < 3; ++o) {
+                    const int32_t got = osi[(3 * v + in) * 78 + 3 * d + o];
+                    const int32_t want = w < 0 ? -1 : 3 * w + o;
+                    if (got != want && !(w < 0 && got < 0))
+                        return fail(CT_ERR_INVALID,
+                                    "output_state_indices[" + std::to_string(3 * v + in) + "," +
+                                        std::to_string(3 * d + o) +
+                                        "] breaks the 26-neighbour form (expected " +
+                                        std::to_string(want) + ", got " + std::to_string(got) + ")");
+                }
+            const int32_t D = delays[v * 26 + d];
+            if (D < 0)
+                return fail(CT_ERR_INVALID, "negative delay at (" + std::to_string(v) + "," +
+                                                std::to_string(d) + ")");
+            if (w >= 0 && D < 1)
+                return fail(CT_ERR_INVALID, "delay < 1 on connection (" + std::to_string(v) + "," +
+                                                std::to_string(d) + ")");
+            nbr[v * 26 + d] = w;
+        }
+    }
+    return CT_OK;
+}
+
+int reset_derivatives(ct_results_gpu &r) {
+    hipStream_t s = r.gpu->stream;
+    CT_HIP(hipMemsetAsync(r.dGT, 0, sizeof(float) * r.n * 78, s));
+    CT_HIP(hipMemsetAsync(r.dA, 0, sizeof(float) * r.N * 26, s));
+    CT_HIP(hipMemsetAsync(r.fir, 0, sizeof(float) * r.N * 78, s));
+    CT_HIP(hipMemsetAsync(r.iir, 0, sizeof(float) * r.N * 78, s));
+    CT_HIP(hipMemsetAsync(r.mr_last, 0, sizeof(float) * r.n, s));
+    CT_HIP(hipMemsetAsync(r.mr_sum, 0, sizeof(float), s));
+    return CT_OK;
+}
+
+// Forward propagation of one beat and the projected measurements
+// (prediction.rs:15-130 for every step). With `mhat` also residuals and
+// per-step MSE.
+int forward_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_kernel *k) {
+    hipStream_t s = r.gpu->stream;
+    CT_HIP(hipMemsetAsync(r.y, 0, sizeof(float) * r.N * 78, s));
+    CT_HIP(hipMemsetAsync(r.y_last, 0, sizeof(float) * r.N * 78, s));
+    for (int32_t t = 0; t < r.T; ++t) {
+        Span sp(k, kPropagate);
+        CT_HIP(ct::launch_propagate_step(s, r.nbr, r.coefs, r.delays, r.gT, r.y, r.y_last, r.x,
+                                         r.b, r.control_function[t], r.N, t, r.T));
+    }
+    {
+        Span sp(k, kProject);
+        const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
+        CT_HIP(ct::launch_project(s, r.x, Hb, r.partial, r.T, r.n, r.Sp, r.splits, r.kps));
+        CT_HIP(ct::launch_residuals(s, r.partial, r.splits,
+                                    mhat ? mhat + int64_t(beat) * r.T * r.S : nullptr,
+                                    r.m + int64_t(beat) * r.T * r.S, r.R, r.loss_mse, r.T, r.S,
+                                    r.Sp));
+    }
+    return CT_OK;
+}
+
+int run_beat(ct_epoch_kernel &k, int32_t beat) {
+    ct_results_gpu &r = *k.r;
+    hipStream_t s = r.gpu->stream;
+    const ct_algorithm &c = k.cfg;
+    CT_CHECK(forward_beat(r, beat, k.mhat->data, &k));
+    {
+        Span sp(&k, kRegularize);
+        CT_HIP(ct::launch_maxreg_partials(s, r.x, r.mr_partial, r.mr_last, r.N, r.T,
+                                          c.maximum_regularization_threshold, r.mr_blocks));
+        CT_HIP(ct::launch_maxreg_finalize(s, r.mr_partial, r.mr_blocks, r.mr_sum, r.loss_mse,
+                                          r.loss_mr, r.loss, c.maximum_regularization_strength,
+                                          r.T));
+    }
+    const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
+    {
+        Span sp(&k, kMapResiduals);
+        CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.eT, r.n, r.T, r.Sp));
+    }
+    if (c.freeze_gains && c.freeze_delays) return CT_OK;
+    {
+        Span sp(&k, kTranspose);
+        CT_HIP(ct::launch_transpose(s, r.x, r.xT, r.T, r.n));
+    }
+    {
+        Span sp(&k, kDerive);
+        ct::DeriveParams p;
+        p.mse_scaling = 1.0f / float(r.S) * c.mse_strength;
+        p.reg_scaling = c.maximum_regularization_strength;
+        p.threshold = c.maximum_regularization_threshold;
+        p.diff_strength = c.difference_regularization_strength;
+        p.freeze_gains = c.freeze_gains;
+        p.freeze_delays = c.freeze_delays;
+        p.simple = c.ap_derivative == 1;
+        CT_HIP(ct::launch_derive(s, r.nbr, r.coefs, r.delays, r.initial_delays, r.gT, r.xT, r.eT,
+                                 r.fir, r.iir, r.dGT, r.dA, r.N, r.T, p));
+    }
+    return CT_OK;
+}
+
+// algorithm.rs:188-212 / 224-247 / 255-278: average delays, batch derivatives,
+// parameter update (update.rs:27-97) and batch metrics.
+int finish_batch(ct_epoch_kernel &k, int64_t num_beats, bool reset_after) {
+    ct_results_gpu &r = *k.r;
+    hipStream_t s = r.gpu->stream;
+    const ct_algorithm &c = k.cfg;
+    Span sp(&k, kUpdate);
+    if (k.batch_index < 0 || k.batch_index >= r.nbatch)
+        return fail(CT_ERR_STATE, "batch index " + std::to_string(k.batch_index) +
+                                      " outside the batch metrics (" + std::to_string(r.nbatch) +
+                                      " entries)");
+    CT_HIP(ct::launch_average_delays(s, r.gT, r.coefs, r.delays, r.avg, r.avg_valid, r.N));
+    if (!c.freeze_delays && std::fabs(c.smoothness_regularization_strength - 0.0f) > 1.1920929e-7f)
+        CT_HIP(ct::launch_smoothness(s, r.nbr, r.avg, r.avg_valid, r.dA,
+                                     c.smoothness_regularization_strength, r.N));
+    const int64_t bs = c.batch_size == 0 ? int64_t(r.T) * num_beats : int64_t(r.T) * c.batch_size;
+    const float scale = c.learning_rate / float(bs);
+    const float c1 = 1.0f - std::pow(0.9f, float(r.step));
+    const float c2 = 1.0f - std::pow(0.999f, float(r.step));
+    if (!c.freeze_gains) {
+        if (c.optimizer == 1) {
+            if (!r.gm1) return fail(CT_ERR_STATE, "Adam update without Adam moments (results were "
+                                                  "uploaded with optimizer = Sgd)");
+            CT_HIP(ct::launch_adam(s, r.gT, r.dGT, r.gm1, r.gm2, r.n * 78, c1, c2, scale));
+        } else {
+            CT_HIP(ct::launch_sgd(s, r.gT, r.dGT, r.n * 78, scale));
+        }
+    }
+    if (!c.freeze_delays) {
+        if (c.optimizer == 1) {
+            if (!r.cm1) return fail(CT_ERR_STATE, "Adam update without Adam moments (results were "
+                                                  "uploaded with optimizer = Sgd)");
+            CT_HIP(ct::launch_adam(s, r.coefs, r.dA, r.cm1, r.cm2, int64_t(r.N) * 26, c1, c2, scale));
+        } else {
+            CT_HIP(ct::launch_sgd(s, r.coefs, r.dA, int64_t(r.N) * 26, scale));
+        }
+        CT_HIP(ct::launch_roll_delays(s, r.coefs, r.delays, int64_t(r.N) * 26));
+    }
+    r.step += 1;
+    if (reset_after) CT_CHECK(reset_derivatives(r));
+    CT_HIP(ct::launch_batch_metrics(s, r.loss, r.loss_mse, r.loss_mr, r.loss_b, r.loss_mse_b,
+                                    r.loss_mr_b, k.batch_index, r.T));
+    k.batch_index += 1;
+    return CT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ct_last_error(void) { return g_error.c_str(); }
+
+int ct_gpu_new(int32_t device, ct_gpu **out) {
+    if (!out) return fail(CT_ERR_INVALID, "null output handle");
+    int count = 0;
+    CT_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return fail(CT_ERR_DEVICE, "no HIP device " + std::to_string(device) + " (" +
+                                       std::to_string(count) + " present)");
+    CT_HIP(hipSetDevice(device));
+    ct_gpu *g = new ct_gpu;
+    g->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(CT_ERR_DEVICE, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
+    }
+    *out = g;
+    return CT_OK;
+}
+
+void ct_gpu_free(ct_gpu *gpu) {
+    if (!gpu) return;
+    (void)hipSetDevice(gpu->device);
+    if (gpu->stream) (void)hipStreamDestroy(gpu->stream);
+    delete gpu;
+}
+
+int ct_gpu_synchronize(ct_gpu *gpu) {
+    CT_CHECK(check_gpu(gpu));
+    CT_HIP(hipStreamSynchronize(gpu->stream));
+    return CT_OK;
+}
+
+int ct_measurements_to_gpu(ct_gpu *gpu, const float *measurements, int32_t num_beats,
+                           int32_t num_steps, int32_t num_sensors, ct_buffer **out) {
+    CT_CHECK(check_gpu(gpu));
+    if (!out || !measurements) return fail(CT_ERR_INVALID, "null argument");
+    if (num_beats <= 0 || num_steps <= 0 || num_sensors <= 0)
+        return fail(CT_ERR_INVALID, "measurements need positive beats, steps and sensors");
+    const size_t bytes = sizeof(float) * size_t(num_beats) * num_steps * num_sensors;
+    ct_buffer *b = new ct_buffer;
+    b->gpu = gpu;
+    b->beats = num_beats;
+    b->steps = num_steps;
+    b->sensors = num_sensors;
+    hipError_t e = hipMalloc(&b->data, bytes);
+    if (e == hipSuccess) e = hipMemcpy(b->data, measurements, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (b->data) (void)hipFree(b->data);
+        delete b;
+        return fail(CT_ERR_DEVICE, std::string("measurements upload: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return CT_OK;
+}
+
+void ct_buffer_free(ct_buffer *buffer) {
+    if (!buffer) return;
+    if (buffer->data) (void)hipFree(buffer->data);
+    delete buffer;
+}
+
+int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu **out) {
+    CT_CHECK(check_gpu(gpu));
+    if (!desc || !out) return fail(CT_ERR_INVALID, "null argument");
+    const ct_results_desc &d = *desc;
+    if (d.num_voxels < 0 || d.num_sensors <= 0 || d.num_steps <= 0 || d.num_beats <= 0 ||
+        d.num_batch_entries <= 0)
+        return fail(CT_ERR_INVALID, "need N >= 0 and positive sensors, steps, beats and batch "
+                                    "entries");
+    if (d.num_steps > 65535)
+        return fail(CT_ERR_INVALID, "at most 65535 steps per beat are supported");
+    if (int64_t(d.num_voxels) * 3 > 0x7fffffffLL)
+        return fail(CT_ERR_INVALID, "too many voxels for 32-bit state indices");
+    const int32_t N = d.num_voxels;
+    std::vector<int32_t> nbr;
+    if (N > 0) {
+        if (!d.output_state_indices || !d.delays || !d.gains || !d.coefs || !d.initial_delays ||
+            !d.control_matrix)
+            return fail(CT_ERR_INVALID, "null parameter array");
+        CT_CHECK(neighbours_from_indices(N, d.output_state_indices, d.delays, nbr));
+    }
+    if (!d.control_function || !d.measurement_matrix)
+        return fail(CT_ERR_INVALID, "null control function or measurement matrix");
+
+    ct_results_gpu *r = new ct_results_gpu;
+    auto cleanup = [&](int rc) {
+        delete r;
+        return rc;
+    };
+    r->gpu = gpu;
+    r->N = N;
+    r->n = 3LL * N;
+    r->S = d.num_sensors;
+    r->Sp = (d.num_sensors + 127) / 128 * 128;
+    r->T = d.num_steps;
+    r->B = d.num_beats;
+    r->nbatch = d.num_batch_entries;
+    r->optimizer = d.optimizer;
+    r->control_function.assign(d.control_function, d.control_function + d.num_steps);
+    const int64_t n = r->n, T = r->T, S = r->S, Sp = r->Sp, B = r->B;
+
+    // GEMM split-K plan: about 2048 workgroups for the projection.
+    const int64_t tiles = (T + 63) / 64 * (Sp / 128);
+    int64_t splits = std::max<int64_t>(1, std::min<int64_t>((n + 31) / 32, 2048 / std::max<int64_t>(tiles, 1)));
+    int64_t kps = ((n + splits - 1) / splits + ct::kGemmTileK - 1) / ct::kGemmTileK * ct::kGemmTileK;
+    kps = std::max<int64_t>(kps, ct::kGemmTileK);
+    splits = std::max<int64_t>(1, (n + kps - 1) / kps);
+    r->splits = int32_t(splits);
+    r->kps = kps;
+    r->mr_blocks = std::max<int32_t>(1, (N + 255) / 256);
+
+    int rc = CT_OK;
+#define ALLOC(ptr, count)                                  \
+    do {                                                   \
+        if ((rc = r->alloc(&(ptr), (count))) != CT_OK) return cleanup(rc); \
+    } while (0)
+    ALLOC(r->nbr, int64_t(N) * 26);
+    ALLOC(r->delays, int64_t(N) * 26);
+    ALLOC(r->coefs, int64_t(N) * 26);
+    ALLOC(r->initial_delays, int64_t(N) * 26);
+    ALLOC(r->gT, n * 78);
+    ALLOC(r->b, n);
+    ALLOC(r->H, B * Sp * n);
+    ALLOC(r->x, T * n);
+    ALLOC(r->y, int64_t(N) * 78);
+    ALLOC(r->y_last, int64_t(N) * 78);
+    ALLOC(r->m, B * T * S);
+    ALLOC(r->R, T * Sp);
+    ALLOC(r->xT, n * T);
+    ALLOC(r->eT, n * T);
+    ALLOC(r->avg, N);
+    ALLOC(r->avg_valid, N);
+    ALLOC(r->dGT, n * 78);
+    ALLOC(r->dA, int64_t(N) * 26);
+    ALLOC(r->fir, int64_t(N) * 78);
+    ALLOC(r->iir, int64_t(N) * 78);
+    ALLOC(r->mr_last, n);
+    ALLOC(r->mr_sum, 1);
+    if (d.optimizer == 1) {
+        ALLOC(r->gm1, n * 78);
+        ALLOC(r->gm2, n * 78);
+        ALLOC(r->cm1, int64_t(N) * 26);
+        ALLOC(r->cm2, int64_t(N) * 26);
+    }
+    ALLOC(r->loss, T);
+    ALLOC(r->loss_mse, T);
+    ALLOC(r->loss_mr, T);
+    ALLOC(r->loss_b, r->nbatch);
+    ALLOC(r->loss_mse_b, r->nbatch);
+    ALLOC(r->loss_mr_b, r->nbatch);
+    ALLOC(r->partial, splits * T * Sp);
+    ALLOC(r->mr_partial, T * r->mr_blocks);
+    ALLOC(r->staging, n * 78);
+#undef ALLOC
+
+    hipStream_t s = gpu->stream;
+    auto zero = [&](void *p, size_t bytes) -> int {
+        CT_HIP(hipMemsetAsync(p, 0, bytes, s));
+        return CT_OK;
+    };
+    if ((rc = zero(r->H, sizeof(float) * B * Sp * n)) != CT_OK) return cleanup(rc);
+    if ((rc = zero(r->m, sizeof(float) * B * T * S)) != CT_OK) return cleanup(rc);
+    if ((rc = zero(r->x, sizeof(float) * T * n)) != CT_OK) return cleanup(rc);
+    if ((rc = zero(r->eT, sizeof(float) * T * n)) != CT_OK) return cleanup(rc);
+    if ((rc = zero(r->R, sizeof(float) * T * Sp)) != CT_OK) return cleanup(rc);
+    for (float *p : {r->loss, r->loss_mse, r->loss_mr})
+        if ((rc = zero(p, sizeof(float) * T)) != CT_OK) return cleanup(rc);
+    for (float *p : {r->loss_b, r->loss_mse_b, r->loss_mr_b})
+        if ((rc = zero(p, sizeof(float) * r->nbatch)) != CT_OK) return cleanup(rc);
+    if ((rc = zero(r->y, sizeof(float) * N * 78)) != CT_OK) return cleanup(rc);
+    if ((rc = zero(r->y_last, sizeof(float) * N * 78)) != CT_OK) return cleanup(rc);
+    if ((rc = zero(r->avg, sizeof(float) * N)) != CT_OK) return cleanup(rc);
+    if ((rc = zero(r->avg_valid, N)) != CT_OK) return cleanup(rc);
+    if (r->gm1) {
+        for (float *p : {r->gm1, r->gm2})
+            if ((rc = zero(p, sizeof(float) * n * 78)) != CT_OK) return cleanup(rc);
+        for (float *p : {r->cm1, r->cm2})
+            if ((rc = zero(p, sizeof(float) * N * 26)) != CT_OK) return cleanup(rc);
+    }
+    if ((rc = reset_derivatives(*r)) != CT_OK) return cleanup(rc);
+    if (hipStreamSynchronize(s) != hipSuccess) return cleanup(fail(CT_ERR_DEVICE, "memset failed"));
+
+    if (N > 0) {
+        if ((rc = upload(r->nbr, nbr.data(), sizeof(int32_t) * N * 26)) != CT_OK) return cleanup(rc);
+        if ((rc = upload(r->delays, d.delays, sizeof(int32_t) * N * 26)) != CT_OK) return cleanup(rc);
+        if ((rc = upload(r->coefs, d.coefs, sizeof(float) * N * 26)) != CT_OK) return cleanup(rc);
+        if ((rc = upload(r->initial_delays, d.initial_delays, sizeof(float) * N * 26)) != CT_OK)
+            return cleanup(rc);
+        if ((rc = upload(r->b, d.control_matrix, sizeof(float) * n)) != CT_OK) return cleanup(rc);
+        // gains (n, 78) -> gT (78, n) through the staging buffer
+        if ((rc = upload(r->staging, d.gains, sizeof(float) * n * 78)) != CT_OK) return cleanup(rc);
+        if (ct::launch_transpose(s, r->staging, r->gT, n, 78) != hipSuccess)
+            return cleanup(fail(CT_ERR_DEVICE, "gain transpose launch failed"));
+        for (int64_t bb = 0; bb < B; ++bb) {
+            hipError_t e = hipMemcpy2DAsync(r->H + bb * Sp * n, sizeof(float) * n,
+                                            d.measurement_matrix + bb * S * n, sizeof(float) * n,
+                                            sizeof(float) * n, S, hipMemcpyHostToDevice, s);
+            if (e != hipSuccess)
+                return cleanup(fail(CT_ERR_DEVICE, std::string("H upload: ") + hipGetErrorString(e)));
+        }
+        if (hipStreamSynchronize(s) != hipSuccess)
+            return cleanup(fail(CT_ERR_DEVICE, "upload failed"));
+    }
+    *out = r;
+    return CT_OK;
+}
+
+void ct_results_gpu_free(ct_results_gpu *results) {
+    if (!results) return;
+    (void)hipSetDevice(results->gpu->device);
+    (void)hipStreamSynchronize(results->gpu->stream);
+    delete results;
+}
+
+int ct_results_update_from_gpu(ct_results_gpu *results, const ct_results_host *o) {
+    if (!results || !o) return fail(CT_ERR_INVALID, "null argument");
+    ct_results_gpu &r = *results;
+    CT_CHECK(check_gpu(r.gpu));
+    hipStream_t s = r.gpu->stream;
+    CT_HIP(hipStreamSynchronize(s));
+    const int64_t n = r.n, N = r.N, T = r.T, S = r.S;
+    auto via_staging = [&](float *dst, auto &&produce) -> int {
+        if (!dst || n == 0) return CT_OK;
+        CT_HIP(produce());
+        CT_HIP(hipStreamSynchronize(s));
+        return download(dst, r.staging, sizeof(float) * n * 78);
+    };
+    CT_CHECK(via_staging(o->gains, [&] { return ct::launch_transpose(s, r.gT, r.staging, 78, n); }));
+    CT_CHECK(via_staging(o->d_gains, [&] { return ct::launch_transpose(s, r.dGT, r.staging, 78, n); }));
+    CT_CHECK(via_staging(o->ap_outputs_now, [&] { return ct::launch_expand_rows(s, r.y, r.staging, N); }));
+    CT_CHECK(via_staging(o->ap_outputs_last, [&] { return ct::launch_expand_rows(s, r.y_last, r.staging, N); }));
+    CT_CHECK(via_staging(o->d_coefs_fir, [&] { return ct::launch_expand_rows(s, r.fir, r.staging, N); }));
+    CT_CHECK(via_staging(o->d_coefs_iir, [&] { return ct::launch_expand_rows(s, r.iir, r.staging, N); }));
+    CT_CHECK(download(o->coefs, r.coefs, sizeof(float) * N * 26));
+    CT_CHECK(download(o->delays, r.delays, sizeof(int32_t) * N * 26));
+    CT_CHECK(download(o->d_coefs, r.dA, sizeof(float) * N * 26));
+    CT_CHECK(download(o->system_states, r.x, sizeof(float) * T * n));
+    CT_CHECK(download(o->measurements, r.m, sizeof(float) * r.B * T * S));
+    CT_CHECK(download(o->residuals, r.R + (T - 1) * r.Sp, sizeof(float) * S));
+    if (o->mapped_residuals && n > 0)
+        CT_HIP(hipMemcpy2D(o->mapped_residuals, sizeof(float), r.eT + (T - 1), sizeof(float) * T,
+                           sizeof(float), n, hipMemcpyDeviceToHost));
+    CT_CHECK(download(o->maximum_regularization, r.mr_last, sizeof(float) * n));
+    CT_CHECK(download(o->maximum_regularization_sum, r.mr_sum, sizeof(float)));
+    if (o->average_delays && N > 0) {
+        std::vector<uint8_t> valid(N);
+        CT_CHECK(download(o->average_delays, r.avg, sizeof(float) * N));
+        CT_CHECK(download(valid.data(), r.avg_valid, N));
+        for (int64_t v = 0; v < N; ++v)
+            if (!valid[v]) o->average_delays[v] = std::nanf("");
+    }
+    CT_CHECK(download(o->loss, r.loss, sizeof(float) * T));
+    CT_CHECK(download(o->loss_mse, r.loss_mse, sizeof(float) * T));
+    CT_CHECK(download(o->loss_maximum_regularization, r.loss_mr, sizeof(float) * T));
+    CT_CHECK(download(o->loss_batch, r.loss_b, sizeof(float) * r.nbatch));
+    CT_CHECK(download(o->loss_mse_batch, r.loss_mse_b, sizeof(float) * r.nbatch));
+    CT_CHECK(download(o->loss_maximum_regularization_batch, r.loss_mr_b, sizeof(float) * r.nbatch));
+    if (o->derivatives_step) *o->derivatives_step = r.step;
+    return CT_OK;
+}
+
+int ct_results_simulate(ct_results_gpu *results, float *measurements_out) {
+    if (!results) return fail(CT_ERR_INVALID, "null argument");
+    ct_results_gpu &r = *results;
+    CT_CHECK(check_gpu(r.gpu));
+    for (int32_t beat = 0; beat < r.B; ++beat) CT_CHECK(forward_beat(r, beat, nullptr, nullptr));
+    CT_HIP(hipStreamSynchronize(r.gpu->stream));
+    if (measurements_out)
+        CT_CHECK(download(measurements_out, r.m, sizeof(float) * r.B * r.T * r.S));
+    return CT_OK;
+}
+
+int ct_epoch_kernel_new(ct_gpu *gpu, ct_results_gpu *results, const ct_buffer *actual,
+                        const ct_algorithm *config, int32_t number_of_states,
+                        int32_t number_of_sensors, int32_t number_of_steps, ct_epoch_kernel **out) {
+    CT_CHECK(check_gpu(gpu));
+    if (!results || !actual || !config || !out) return fail(CT_ERR_INVALID, "null argument");
+    if (results->gpu != gpu) return fail(CT_ERR_INVALID, "results belong to another ct_gpu");
+    if (int64_t(number_of_states) != results->n || number_of_sensors != results->S ||
+        number_of_steps != results->T)
+        return fail(CT_ERR_INVALID, "states/sensors/steps do not match the results");
+    if (actual->beats != results->B || actual->steps != results->T || actual->sensors != results->S)
+        return fail(CT_ERR_INVALID, "actual measurements are not (B, T, S) of the results");
+    if (config->optimizer == 1 && !results->gm1)
+        return fail(CT_ERR_INVALID, "Adam needs results uploaded with optimizer = 1");
+    if (config->batch_size < 0) return fail(CT_ERR_INVALID, "negative batch size");
+    ct_epoch_kernel *k = new ct_epoch_kernel;
+    k->gpu = gpu;
+    k->r = results;
+    k->mhat = actual;
+    k->cfg = *config;
+    for (int32_t b = 0; b < results->B; ++b) k->order.push_back(b);
+    *out = k;
+    return CT_OK;
+}
+
+void ct_epoch_kernel_free(ct_epoch_kernel *kernel) {
+    if (!kernel) return;
+    (void)hipSetDevice(kernel->gpu->device);
+    (void)hipStreamSynchronize(kernel->gpu->stream);
+    delete kernel;
+}
+
+int ct_epoch_kernel_execute(ct_epoch_kernel *k) {
+    if (!k) return fail(CT_ERR_INVALID, "null argument");
+    ct_results_gpu &r = *k->r;
+    CT_CHECK(check_gpu(r.gpu));
+    const ct_algorithm &c = k->cfg;
+    k->used_events = 0;
+    k->spans.clear();
+    if (k->profiling) {
+        const size_t need = 2 * size_t(r.T) * r.B + 64 * size_t(r.B) + 64;
+        while (k->events.size() < need) {
+            hipEvent_t e;
+            CT_HIP(hipEventCreate(&e));
+            k->events.push_back(e);
+        }
+    }
+    Span total(k, kTotal);
+    CT_CHECK(reset_derivatives(r));
+    const bool batched = c.batch_size != 0;
+    int64_t in_batch = 0;
+    for (int32_t beat : k->order) {
+        CT_CHECK(run_beat(*k, beat));
+        if (batched) {
+            in_batch += 1;
+            if (in_batch == c.batch_size) {
+                CT_CHECK(finish_batch(*k, in_batch, true));
+                in_batch = 0;
+            }
+        }
+    }
+    if (batched) {
+        if (in_batch > 0) CT_CHECK(finish_batch(*k, in_batch, false));
+    } else {
+        CT_CHECK(finish_batch(*k, r.B, false));
+    }
+    return CT_OK;
+}
+
+int ct_epoch_kernel_set_freeze_delays(ct_epoch_kernel *k, int32_t value) {
+    if (!k) return fail(CT_ERR_INVALID, "null argument");
+    k->cfg.freeze_delays = value != 0;
+    return CT_OK;
+}
+
+int ct_epoch_kernel_set_freeze_gains(ct_epoch_kernel *k, int32_t value) {
+    if (!k) return fail(CT_ERR_INVALID, "null argument");
+    k->cfg.freeze_gains = value != 0;
+    return CT_OK;
+}
+
+int ct_epoch_kernel_set_beat_order(ct_epoch_kernel *k, const int32_t *order, int32_t len) {
+    if (!k) return fail(CT_ERR_INVALID, "null argument");
+    const int32_t B = k->r->B;
+    std::vector<int32_t> o;
+    if (!order) {
+        for (int32_t b = 0; b < B; ++b) o.push_back(b);
+    } else {
+        if (len != B) return fail(CT_ERR_INVALID, "beat order must list every beat once");
+        std::vector<char> seen(B, 0);
+        for (int32_t i = 0; i < len; ++i) {
+            if (order[i] < 0 || order[i] >= B || seen[order[i]])
+                return fail(CT_ERR_INVALID, "beat order is not a permutation of 0..B-1");
+            seen[order[i]] = 1;
+            o.push_back(order[i]);
+        }
+    }
+    k->order = o;
+    return CT_OK;
+}
+
+int ct_epoch_kernel_set_batch_index(ct_epoch_kernel *k, int32_t batch_index) {
+    if (!k) return fail(CT_ERR_INVALID, "null argument");
+    if (batch_index < 0) return fail(CT_ERR_INVALID, "negative batch index");
+    k->batch_index = batch_index;
+    return CT_OK;
+}
+
+int ct_epoch_kernel_get_batch_index(const ct_epoch_kernel *k, int32_t *batch_index) {
+    if (!k || !batch_index) return fail(CT_ERR_INVALID, "null argument");
+    *batch_index = k->batch_index;
+    return CT_OK;
+}
+
+int ct_epoch_kernel_set_profiling(ct_epoch_kernel *k, int32_t enabled) {
+    if (!k) return fail(CT_ERR_INVALID, "null argument");
+    k->profiling = enabled != 0;
+    return CT_OK;
+}
+
+int ct_epoch_kernel_get_profile(ct_epoch_kernel *k, ct_profile *out) {
+    if (!k || !out) return fail(CT_ERR_INVALID, "null argument");
+    CT_CHECK(check_gpu(k->gpu));
+    CT_HIP(hipStreamSynchronize(k->gpu->stream));
+    std::memset(out, 0, sizeof(*out));
+    for (auto &sp : k->spans) {
+        float ms = 0.0f;
+        CT_HIP(hipEventElapsedTime(&ms, k->events[sp.second.first], k->events[sp.second.second]));
+        switch (sp.first) {
+            case kPropagate: out->propagate_ms += ms; out->propagate_launches += 1; break;
+            case kProject: out->project_ms += ms; break;
+            case kRegularize: out->regularize_ms += ms; break;
+            case kTranspose: out->transpose_ms += ms; break;
+            case kMapResiduals: out->map_residuals_ms += ms; break;
+            case kDerive: out->derive_ms += ms; break;
+            case kUpdate: out->update_ms += ms; break;
+            case kTotal: out->total_ms += ms; break;
+        }
+    }
+    return CT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,316 @@
+"""Python mirror of the reference's GPU epoch API over the C ABI.
+
+Names follow `src/core/algorithm/gpu/*.rs`: `GPU` (gpu.rs:12-63),
+`EpochKernel` (gpu/epoch.rs:10-125), `Results::to_gpu` / `update_from_gpu`
+(scenario/results.rs:121-145) and `Measurements::to_gpu`
+(data/shapes.rs:451-463). Every call goes to libcardiotrust_hip.so; there is
+no CPU fallback: a missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from .config import Algorithm, CAlgorithm
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcardiotrust_hip.so")
+
+# Every symbol declared in include/cardiotrust_hip.h.
+EXPORTS = (
+    "ct_last_error", "ct_gpu_new", "ct_gpu_free", "ct_gpu_synchronize",
+    "ct_measurements_to_gpu", "ct_buffer_free", "ct_results_to_gpu", "ct_results_gpu_free",
+    "ct_results_update_from_gpu", "ct_results_simulate", "ct_epoch_kernel_new",
+    "ct_epoch_kernel_free", "ct_epoch_kernel_execute", "ct_epoch_kernel_set_freeze_delays",
+    "ct_epoch_kernel_set_freeze_gains", "ct_epoch_kernel_set_beat_order",
+    "ct_epoch_kernel_set_batch_index", "ct_epoch_kernel_get_batch_index",
+    "ct_epoch_kernel_set_profiling", "ct_epoch_kernel_get_profile",
+)
+
+
+class CTError(RuntimeError):
+    pass
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [
+        ("num_voxels", ctypes.c_int32), ("num_sensors", ctypes.c_int32),
+        ("num_steps", ctypes.c_int32), ("num_beats", ctypes.c_int32),
+        ("num_batch_entries", ctypes.c_int32), ("optimizer", ctypes.c_int32),
+    ] + [(name, ctypes.c_void_p) for name in (
+        "gains", "output_state_indices", "coefs", "delays", "initial_delays",
+        "control_matrix", "control_function", "measurement_matrix")]
+
+
+_HOST_FIELDS = (
+    "gains", "coefs", "delays", "ap_outputs_now", "ap_outputs_last", "system_states",
+    "measurements", "residuals", "d_gains", "d_coefs", "d_coefs_iir", "d_coefs_fir",
+    "mapped_residuals", "maximum_regularization", "maximum_regularization_sum",
+    "average_delays", "loss", "loss_mse", "loss_maximum_regularization", "loss_batch",
+    "loss_mse_batch", "loss_maximum_regularization_batch", "derivatives_step",
+)
+
+
+class _Host(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in _HOST_FIELDS]
+
+
+class Profile(ctypes.Structure):
+    _fields_ = [
+        ("propagate_ms", ctypes.c_double), ("propagate_launches", ctypes.c_int64),
+        ("project_ms", ctypes.c_double), ("regularize_ms", ctypes.c_double),
+        ("transpose_ms", ctypes.c_double), ("map_residuals_ms", ctypes.c_double),
+        ("derive_ms", ctypes.c_double), ("update_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> Dict[str, float]:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libcardiotrust_hip.so (built by cardiotrust_amd/build.py)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CTError(f"{LIB_PATH} is not built; run cardiotrust_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, P = ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER
+    sig = {
+        "ct_last_error": ([], ctypes.c_char_p),
+        "ct_gpu_new": ([i32, P(vp)], ctypes.c_int),
+        "ct_gpu_free": ([vp], None),
+        "ct_gpu_synchronize": ([vp], ctypes.c_int),
+        "ct_measurements_to_gpu": ([vp, vp, i32, i32, i32, P(vp)], ctypes.c_int),
+        "ct_buffer_free": ([vp], None),
+        "ct_results_to_gpu": ([vp, P(_Desc), P(vp)], ctypes.c_int),
+        "ct_results_gpu_free": ([vp], None),
+        "ct_results_update_from_gpu": ([vp, P(_Host)], ctypes.c_int),
+        "ct_results_simulate": ([vp, vp], ctypes.c_int),
+        "ct_epoch_kernel_new": ([vp, vp, vp, P(CAlgorithm), i32, i32, i32, P(vp)], ctypes.c_int),
+        "ct_epoch_kernel_free": ([vp], None),
+        "ct_epoch_kernel_execute": ([vp], ctypes.c_int),
+        "ct_epoch_kernel_set_freeze_delays": ([vp, i32], ctypes.c_int),
+        "ct_epoch_kernel_set_freeze_gains": ([vp, i32], ctypes.c_int),
+        "ct_epoch_kernel_set_beat_order": ([vp, vp, i32], ctypes.c_int),
+        "ct_epoch_kernel_set_batch_index": ([vp, i32], ctypes.c_int),
+        "ct_epoch_kernel_get_batch_index": ([vp, P(i32)], ctypes.c_int),
+        "ct_epoch_kernel_set_profiling": ([vp, i32], ctypes.c_int),
+        "ct_epoch_kernel_get_profile": ([vp, P(Profile)], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().ct_last_error().decode(errors="replace")
+        raise CTError(f"{what} failed ({rc}): {msg}")
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class GPU:
+    """`GPU::new` (gpu.rs:31-63): one device and the library's stream."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().ct_gpu_new(int(device), ctypes.byref(h)), "GPU::new")
+        self.handle = h
+
+    @classmethod
+    def new(cls, device: int = 0) -> "GPU":
+        return cls(device)
+
+    def synchronize(self) -> None:
+        _check(lib().ct_gpu_synchronize(self.handle), "synchronize")
+
+    def close(self) -> None:
+        if self.handle:
+            lib().ct_gpu_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Buffer:
+    """Device copy of measurements (`Measurements::to_gpu`, shapes.rs:451-463)."""
+
+    def __init__(self, gpu: GPU, measurements: np.ndarray):
+        m = np.ascontiguousarray(measurements, dtype=np.float32)
+        if m.ndim != 3:
+            raise ValueError("measurements must be (beats, steps, sensors)")
+        h = ctypes.c_void_p()
+        _check(lib().ct_measurements_to_gpu(gpu.handle, _ptr(m), *map(int, m.shape),
+                                            ctypes.byref(h)), "Measurements::to_gpu")
+        self.gpu, self.handle, self.shape = gpu, h, m.shape
+
+    def close(self) -> None:
+        if self.handle:
+            lib().ct_buffer_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ResultsGPU:
+    """Device state of one scenario (`ResultsGPU`, scenario/results.rs:36-41)."""
+
+    def __init__(self, gpu: GPU, problem, num_batch_entries: int = 1, optimizer: int = 0):
+        self.gpu = gpu
+        self.N, self.S = problem.num_voxels, problem.num_sensors
+        self.T, self.B = problem.num_steps, problem.num_beats
+        self.nbatch = int(num_batch_entries)
+        keep = lambda a, dt: np.ascontiguousarray(a, dtype=dt)
+        self._arrays = dict(
+            gains=keep(problem.gains, np.float32),
+            output_state_indices=keep(problem.output_state_indices, np.int32),
+            coefs=keep(problem.coefs, np.float32),
+            delays=keep(problem.delays, np.int32),
+            initial_delays=keep(problem.initial_delays, np.float32),
+            control_matrix=keep(problem.control_matrix, np.float32),
+            control_function=keep(problem.control_function, np.float32),
+            measurement_matrix=keep(problem.measurement_matrix, np.float32),
+        )
+        d = _Desc(self.N, self.S, self.T, self.B, self.nbatch, int(optimizer),
+                  *[_ptr(self._arrays[k]) for k in (
+                      "gains", "output_state_indices", "coefs", "delays", "initial_delays",
+                      "control_matrix", "control_function", "measurement_matrix")])
+        h = ctypes.c_void_p()
+        _check(lib().ct_results_to_gpu(gpu.handle, ctypes.byref(d), ctypes.byref(h)),
+               "Results::to_gpu")
+        self.handle = h
+        self._arrays = None
+
+    def update_from_gpu(self, fields: Optional[Sequence[str]] = None) -> Dict[str, np.ndarray]:
+        """`Results::update_from_gpu`: download the requested arrays."""
+        N, n, T, S, B = self.N, 3 * self.N, self.T, self.S, self.B
+        shapes = dict(
+            gains=((n, 78), np.float32), coefs=((N, 26), np.float32), delays=((N, 26), np.int32),
+            ap_outputs_now=((n, 78), np.float32), ap_outputs_last=((n, 78), np.float32),
+            system_states=((T, n), np.float32), measurements=((B, T, S), np.float32),
+            residuals=((S,), np.float32), d_gains=((n, 78), np.float32),
+            d_coefs=((N, 26), np.float32), d_coefs_iir=((n, 78), np.float32),
+            d_coefs_fir=((n, 78), np.float32), mapped_residuals=((n,), np.float32),
+            maximum_regularization=((n,), np.float32),
+            maximum_regularization_sum=((1,), np.float32), average_delays=((N,), np.float32),
+            loss=((T,), np.float32), loss_mse=((T,), np.float32),
+            loss_maximum_regularization=((T,), np.float32),
+            loss_batch=((self.nbatch,), np.float32), loss_mse_batch=((self.nbatch,), np.float32),
+            loss_maximum_regularization_batch=((self.nbatch,), np.float32),
+            derivatives_step=((1,), np.int64),
+        )
+        want = list(shapes) if fields is None else list(fields)
+        out = {k: np.zeros(*shapes[k]) for k in want}
+        h = _Host(*[_ptr(out.get(k)) for k in _HOST_FIELDS])
+        _check(lib().ct_results_update_from_gpu(self.handle, ctypes.byref(h)),
+               "Results::update_from_gpu")
+        return out
+
+    def simulate(self) -> np.ndarray:
+        """Noise-free forward simulation of every beat; returns (B, T, S)."""
+        out = np.zeros((self.B, self.T, self.S), dtype=np.float32)
+        _check(lib().ct_results_simulate(self.handle, _ptr(out)), "simulate")
+        return out
+
+    def close(self) -> None:
+        if self.handle:
+            lib().ct_results_gpu_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class EpochKernel:
+    """`EpochKernel` (gpu/epoch.rs:10-125) with `run_epoch` semantics."""
+
+    def __init__(self, gpu: GPU, results: ResultsGPU, actual_measurements: Buffer,
+                 config: Algorithm, number_of_states: int, number_of_sensors: int,
+                 number_of_steps: int):
+        h = ctypes.c_void_p()
+        c = config.to_c()
+        _check(lib().ct_epoch_kernel_new(gpu.handle, results.handle, actual_measurements.handle,
+                                         ctypes.byref(c), int(number_of_states),
+                                         int(number_of_sensors), int(number_of_steps),
+                                         ctypes.byref(h)), "EpochKernel::new")
+        self.gpu, self.results, self.actual, self.handle = gpu, results, actual_measurements, h
+
+    def execute(self) -> None:
+        _check(lib().ct_epoch_kernel_execute(self.handle), "EpochKernel::execute")
+
+    def set_freeze_delays(self, value: bool) -> None:
+        _check(lib().ct_epoch_kernel_set_freeze_delays(self.handle, int(bool(value))),
+               "set_freeze_delays")
+
+    def set_freeze_gains(self, value: bool) -> None:
+        _check(lib().ct_epoch_kernel_set_freeze_gains(self.handle, int(bool(value))),
+               "set_freeze_gains")
+
+    def set_beat_order(self, order: Optional[Sequence[int]]) -> None:
+        o = None if order is None else np.ascontiguousarray(order, dtype=np.int32)
+        _check(lib().ct_epoch_kernel_set_beat_order(self.handle, _ptr(o),
+                                                    0 if o is None else len(o)),
+               "set_beat_order")
+
+    def set_batch_index(self, index: int) -> None:
+        _check(lib().ct_epoch_kernel_set_batch_index(self.handle, int(index)), "set_batch_index")
+
+    def batch_index(self) -> int:
+        v = ctypes.c_int32()
+        _check(lib().ct_epoch_kernel_get_batch_index(self.handle, ctypes.byref(v)),
+               "get_batch_index")
+        return int(v.value)
+
+    def set_profiling(self, enabled: bool) -> None:
+        _check(lib().ct_epoch_kernel_set_profiling(self.handle, int(bool(enabled))),
+               "set_profiling")
+
+    def profile(self) -> Dict[str, float]:
+        p = Profile()
+        _check(lib().ct_epoch_kernel_get_profile(self.handle, ctypes.byref(p)), "get_profile")
+        return p.as_dict()
+
+    def close(self) -> None:
+        if self.handle:
+            lib().ct_epoch_kernel_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def simulate(gpu: GPU, problem) -> np.ndarray:
+    """Noise-free measurements of `problem`'s parameters, computed on the GPU."""
+    r = ResultsGPU(gpu, problem)
+    try:
+        return r.simulate()
+    finally:
+        r.close()
