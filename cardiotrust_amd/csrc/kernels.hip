@@ -18,6 +18,17 @@ __device__ __forceinline__ float signum(float v) {
     return isnan(v) ? v : copysignf(1.0f, v);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2). Remap the launch order so that XCD k runs one contiguous range of
+// blocks: neighbouring voxels, which read the same state rows, then share an
+// L2.
+constexpr int kNumXcds = 8;
+__device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t nb) {
+    const int32_t xcd = b % kNumXcds, local = b / kNumXcds;
+    const int32_t q = nb / kNumXcds, r = nb % kNumXcds;
+    return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
 __device__ __forceinline__ float powi5(float v) {
     // powi(x, 5) as expanded by LLVM: x * (x^2)^2.
     const float sq = v * v;
@@ -38,48 +49,67 @@ __device__ __forceinline__ float powi5(float v) {
 // ---------------------------------------------------------------------------
 constexpr int kPropVoxels = 64;
 constexpr int kPropThreads = 3 * kPropVoxels;
+constexpr int kPropItems = (kPropVoxels * 26 + kPropThreads - 1) / kPropThreads;
 
+// Each lane first issues every load it needs for all of its connections (or
+// all 78 gains in phase 2) and only then runs the dependent arithmetic, so a
+// wave keeps dozens of memory requests in flight instead of one per step of
+// the accumulation chain.
 __global__ __launch_bounds__(kPropThreads) void propagate_step_kernel(
     const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
     const int32_t *__restrict__ delays, const float *__restrict__ gT, float *__restrict__ y,
     float *__restrict__ y_last, float *__restrict__ x, const float *__restrict__ b, float u_t,
     int32_t N, int32_t t, int32_t T) {
     __shared__ float ys[kPropVoxels * 78];
-    __shared__ int32_t has[kPropVoxels * 26];
+    __shared__ uint32_t mask[kPropVoxels];
     const int64_t n = 3LL * N;
-    const int32_t v0 = blockIdx.x * kPropVoxels;
+    const int32_t v0 = xcd_block(blockIdx.x, gridDim.x) * kPropVoxels;
     const int32_t nv = min(kPropVoxels, N - v0);
+    const int32_t items = nv * 26;
     const bool last = (t == T - 1);
+    if (threadIdx.x < kPropVoxels) mask[threadIdx.x] = 0u;
 
-    for (int item = threadIdx.x; item < nv * 26; item += kPropThreads) {
+    int32_t w[kPropItems], D[kPropItems];
+    float a[kPropItems];
+#pragma unroll
+    for (int k = 0; k < kPropItems; ++k) {
+        const int item = threadIdx.x + k * kPropThreads;
         const int64_t c = int64_t(v0) * 26 + item;
-        const int32_t w = nbr[c];
-        has[item] = w >= 0;
-        if (w < 0) continue;
-        const float a = coefs[c];
-        const int32_t D = delays[c];
-        float in0[3] = {0.0f, 0.0f, 0.0f};
-        float in1[3] = {0.0f, 0.0f, 0.0f};
-        const int64_t col = 3LL * w;
-        if (D <= t) {
-            const float *r = x + int64_t(t - D) * n + col;
-            in0[0] = r[0];
-            in0[1] = r[1];
-            in0[2] = r[2];
-        }
-        if (D < t) {
-            const float *r = x + int64_t(t - D - 1) * n + col;
-            in1[0] = r[0];
-            in1[1] = r[1];
-            in1[2] = r[2];
-        }
-        float *yp = y + c * 3;
+        const bool ok = item < items;
+        w[k] = ok ? nbr[c] : -1;
+        D[k] = ok ? delays[c] : 1;
+        a[k] = ok ? coefs[c] : 0.0f;
+    }
+    float in0[kPropItems][3], in1[kPropItems][3], prev[kPropItems][3];
+#pragma unroll
+    for (int k = 0; k < kPropItems; ++k) {
+        const int item = threadIdx.x + k * kPropThreads;
+        const int64_t c = int64_t(v0) * 26 + item;
+        const bool on = w[k] >= 0;
+        const int64_t col = 3LL * (on ? w[k] : 0);
+        const bool l0 = on && D[k] <= t, l1 = on && D[k] < t;
+        const float *r0 = x + int64_t(l0 ? t - D[k] : 0) * n + col;
+        const float *r1 = x + int64_t(l1 ? t - D[k] - 1 : 0) * n + col;
 #pragma unroll
         for (int o = 0; o < 3; ++o) {
-            const float prev = yp[o];
-            if (last) y_last[c * 3 + o] = prev;
-            const float yn = fmaf(a, in0[o] - prev, in1[o]);
-            yp[o] = yn;
+            in0[k][o] = l0 ? r0[o] : 0.0f;
+            in1[k][o] = l1 ? r1[o] : 0.0f;
+            prev[k][o] = on ? y[c * 3 + o] : 0.0f;
+        }
+    }
+    __syncthreads();  // mask cleared
+#pragma unroll
+    for (int k = 0; k < kPropItems; ++k) {
+        const int item = threadIdx.x + k * kPropThreads;
+        if (w[k] < 0) continue;
+        const int64_t c = int64_t(v0) * 26 + item;
+        const int vl = item / 26;
+        atomicOr(&mask[vl], 1u << (item - vl * 26));
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            if (last) y_last[c * 3 + o] = prev[k][o];
+            const float yn = fmaf(a[k], in0[k][o] - prev[k][o], in1[k][o]);
+            y[c * 3 + o] = yn;
             ys[item * 3 + o] = yn;
         }
     }
@@ -88,14 +118,16 @@ __global__ __launch_bounds__(kPropThreads) void propagate_step_kernel(
     const int vl = threadIdx.x / 3;
     if (vl < nv) {
         const int64_t i = 3LL * v0 + threadIdx.x;
-        float acc = 0.0f;
-        for (int d = 0; d < 26; ++d) {
-            if (!has[vl * 26 + d]) continue;
+        float g[78];
 #pragma unroll
-            for (int o = 0; o < 3; ++o) {
-                const float g = gT[int64_t(3 * d + o) * n + i];
-                acc = acc + g * ys[(vl * 26 + d) * 3 + o];
-            }
+        for (int k = 0; k < 78; ++k) g[k] = gT[int64_t(k) * n + i];
+        const uint32_t m = mask[vl];
+        float acc = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 26; ++d) {
+            if (!((m >> d) & 1u)) continue;
+#pragma unroll
+            for (int o = 0; o < 3; ++o) acc = acc + g[3 * d + o] * ys[(vl * 26 + d) * 3 + o];
         }
         x[int64_t(t) * n + i] = acc + u_t * b[i];
     }
@@ -337,111 +369,161 @@ __global__ __launch_bounds__(256) void map_residuals_kernel(const float *__restr
 // it recomputes the connection's three all-pass outputs exactly as the
 // forward pass did, tracks the FIR/IIR sensitivities, and keeps the nine gain
 // derivatives and the coefficient derivative in registers.
+//
+// Time advances in chunks of kDeriveChunk steps. Per chunk, the workgroup
+// stages the mapped residuals and states of its own voxels (shared by the 26
+// lanes of a voxel) in LDS, and every lane loads its neighbour's three state
+// rows for the chunk into registers, so each cache line of xT / eT is
+// fetched once per chunk rather than once per step.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void derive_kernel(
+constexpr int kDeriveThreads = 256;
+constexpr int kDeriveChunk = 16;
+constexpr int kDeriveMaxVoxels = kDeriveThreads / 26 + 2;
+
+struct ConnState {
+    float G[3][3], g[3][3];
+    float f[3], h[3], y[3], prev_in[3];
+    float A;
+};
+
+__device__ __forceinline__ void derive_step(ConnState &c, const float in0[3], const float e[3],
+                                            const float xo[3], bool has, float a, int32_t D,
+                                            int32_t t, float reg, const DeriveParams &p) {
+    float yl[3];
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+        yl[o] = c.y[o];
+        if (has) c.y[o] = fmaf(a, in0[o] - c.y[o], c.prev_in[o]);
+    }
+    if (!p.freeze_gains) {
+        const float sum = fabsf(xo[0]) + fabsf(xo[1]) + fabsf(xo[2]);
+        const bool on = sum > p.threshold;
+        const float fac = sum - p.threshold;
+#pragma unroll
+        for (int in = 0; in < 3; ++in) {
+            const float mr = on ? fac * signum(xo[in]) : 0.0f;
+            const float cc = fmaf(e[in], p.mse_scaling, mr * p.reg_scaling);
+#pragma unroll
+            for (int o = 0; o < 3; ++o) c.g[in][o] += c.y[o] * cc;
+        }
+    }
+    if (!p.freeze_delays) {
+        if (!p.simple) {
+            if (has && t >= D) {
+#pragma unroll
+                for (int o = 0; o < 3; ++o) c.f[o] = fmaf(-a, c.f[o], in0[o]);
+            }
+            if (t >= D) {
+#pragma unroll
+                for (int o = 0; o < 3; ++o) c.h[o] = fmaf(-a, c.h[o], yl[o]);
+            }
+#pragma unroll
+            for (int in = 0; in < 3; ++in)
+#pragma unroll
+                for (int o = 0; o < 3; ++o)
+                    c.A += fmaf((c.f[o] - c.h[o]) * c.G[in][o] * e[in], p.mse_scaling, reg);
+        } else if (has && t >= D) {
+#pragma unroll
+            for (int in = 0; in < 3; ++in)
+#pragma unroll
+                for (int o = 0; o < 3; ++o)
+                    c.A += fmaf((in0[o] - yl[o]) * c.G[in][o] * e[in], p.mse_scaling, reg);
+        }
+    }
+#pragma unroll
+    for (int o = 0; o < 3; ++o) c.prev_in[o] = in0[o];
+}
+
+__global__ __launch_bounds__(kDeriveThreads) void derive_kernel(
     const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
     const int32_t *__restrict__ delays, const float *__restrict__ initial_delays,
     const float *__restrict__ gT, const float *__restrict__ xT, const float *__restrict__ eT,
     float *__restrict__ fir, float *__restrict__ iir, float *__restrict__ dGT,
     float *__restrict__ dA, int32_t N, int32_t T, DeriveParams p) {
-    const int64_t item = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (item >= int64_t(N) * 26) return;
+    __shared__ float es[kDeriveMaxVoxels][3][kDeriveChunk];
+    __shared__ float xs[kDeriveMaxVoxels][3][kDeriveChunk];
     const int64_t n = 3LL * N;
-    const int32_t v = int32_t(item / 26), d = int32_t(item % 26);
-    const int32_t w = nbr[item];
-    const bool has = w >= 0;
-    const float a = coefs[item];
-    const int32_t D = delays[item];
-    const float delay = float(D) + (1.0f - a) / (a + 1.0f);
-    const float reg = p.diff_strength * powi5(initial_delays[item] - delay);
+    const int64_t total = int64_t(N) * 26;
+    const int64_t item0 = int64_t(xcd_block(blockIdx.x, gridDim.x)) * kDeriveThreads;
+    const int64_t item = item0 + threadIdx.x;
+    const bool active = item < total;
+    const int32_t vbeg = int32_t(item0 / 26);
+    const int32_t vend = int32_t(min(total - 1, item0 + kDeriveThreads - 1) / 26) + 1;
+    const int32_t nvox = vend - vbeg;
 
-    float G[3][3], g[3][3];
+    const int64_t it = active ? item : item0;
+    const int32_t v = int32_t(it / 26), d = int32_t(it % 26);
+    const int32_t vl = v - vbeg;
+    const int32_t w = active ? nbr[it] : -1;
+    const bool has = w >= 0;
+    const float a = coefs[it];
+    const int32_t D = delays[it];
+    const float delay = float(D) + (1.0f - a) / (a + 1.0f);
+    const float reg = p.diff_strength * powi5(initial_delays[it] - delay);
+
+    ConnState c;
 #pragma unroll
     for (int in = 0; in < 3; ++in)
 #pragma unroll
         for (int o = 0; o < 3; ++o) {
             const int64_t k = int64_t(3 * d + o) * n + 3LL * v + in;
-            G[in][o] = gT[k];
-            g[in][o] = dGT[k];
+            c.G[in][o] = gT[k];
+            c.g[in][o] = dGT[k];
         }
-    float f[3], h[3], yv[3] = {0.0f, 0.0f, 0.0f}, prev_in[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int o = 0; o < 3; ++o) {
-        f[o] = fir[item * 3 + o];
-        h[o] = iir[item * 3 + o];
+        c.f[o] = fir[it * 3 + o];
+        c.h[o] = iir[it * 3 + o];
+        c.y[o] = 0.0f;
+        c.prev_in[o] = 0.0f;
     }
-    float A = dA[item];
+    c.A = dA[it];
     const float *xw = xT + (has ? 3LL * w : 0) * T;
-    const float *xv = xT + 3LL * v * T;
-    const float *ev = eT + 3LL * v * T;
-    const bool do_gains = !p.freeze_gains, do_coefs = !p.freeze_delays;
 
-    for (int32_t t = 0; t < T; ++t) {
-        float in0[3] = {0.0f, 0.0f, 0.0f}, yl[3];
-        if (has && D <= t) {
-#pragma unroll
-            for (int o = 0; o < 3; ++o) in0[o] = xw[int64_t(o) * T + (t - D)];
+    for (int32_t t0 = 0; t0 < T; t0 += kDeriveChunk) {
+        for (int idx = threadIdx.x; idx < nvox * 3 * kDeriveChunk; idx += kDeriveThreads) {
+            const int vv = idx / (3 * kDeriveChunk);
+            const int rem = idx - vv * 3 * kDeriveChunk;
+            const int comp = rem / kDeriveChunk, j = rem - comp * kDeriveChunk;
+            const int32_t t = t0 + j;
+            const int64_t row = 3LL * (vbeg + vv) + comp;
+            es[vv][comp][j] = t < T ? eT[row * T + t] : 0.0f;
+            xs[vv][comp][j] = t < T ? xT[row * T + t] : 0.0f;
         }
+        float xin[3][kDeriveChunk];
 #pragma unroll
-        for (int o = 0; o < 3; ++o) {
-            yl[o] = yv[o];
-            if (has) yv[o] = fmaf(a, in0[o] - yv[o], prev_in[o]);
-        }
-        float e[3], xo[3];
+        for (int o = 0; o < 3; ++o)
 #pragma unroll
-        for (int in = 0; in < 3; ++in) {
-            e[in] = ev[int64_t(in) * T + t];
-            xo[in] = xv[int64_t(in) * T + t];
-        }
-        if (do_gains) {
-            const float sum = fabsf(xo[0]) + fabsf(xo[1]) + fabsf(xo[2]);
-            const bool on = sum > p.threshold;
-            const float fac = sum - p.threshold;
+            for (int j = 0; j < kDeriveChunk; ++j) {
+                const int32_t s = t0 + j - D;
+                xin[o][j] = (has && s >= 0 && t0 + j < T) ? xw[int64_t(o) * T + s] : 0.0f;
+            }
+        __syncthreads();
+        if (active) {
 #pragma unroll
-            for (int in = 0; in < 3; ++in) {
-                const float mr = on ? fac * signum(xo[in]) : 0.0f;
-                const float c = fmaf(e[in], p.mse_scaling, mr * p.reg_scaling);
-#pragma unroll
-                for (int o = 0; o < 3; ++o) g[in][o] += yv[o] * c;
+            for (int j = 0; j < kDeriveChunk; ++j) {
+                if (t0 + j < T) {
+                    const float in0[3] = {xin[0][j], xin[1][j], xin[2][j]};
+                    const float e[3] = {es[vl][0][j], es[vl][1][j], es[vl][2][j]};
+                    const float xo[3] = {xs[vl][0][j], xs[vl][1][j], xs[vl][2][j]};
+                    derive_step(c, in0, e, xo, has, a, D, t0 + j, reg, p);
+                }
             }
         }
-        if (do_coefs) {
-            if (!p.simple) {
-                if (has && t >= D) {
-#pragma unroll
-                    for (int o = 0; o < 3; ++o) f[o] = fmaf(-a, f[o], in0[o]);
-                }
-                if (t >= D) {
-#pragma unroll
-                    for (int o = 0; o < 3; ++o) h[o] = fmaf(-a, h[o], yl[o]);
-                }
-#pragma unroll
-                for (int in = 0; in < 3; ++in)
-#pragma unroll
-                    for (int o = 0; o < 3; ++o)
-                        A += fmaf((f[o] - h[o]) * G[in][o] * e[in], p.mse_scaling, reg);
-            } else if (has && t >= D) {
-#pragma unroll
-                for (int in = 0; in < 3; ++in)
-#pragma unroll
-                    for (int o = 0; o < 3; ++o)
-                        A += fmaf((in0[o] - yl[o]) * G[in][o] * e[in], p.mse_scaling, reg);
-            }
-        }
-#pragma unroll
-        for (int o = 0; o < 3; ++o) prev_in[o] = in0[o];
+        __syncthreads();
     }
+    if (!active) return;
 
 #pragma unroll
     for (int in = 0; in < 3; ++in)
 #pragma unroll
-        for (int o = 0; o < 3; ++o) dGT[int64_t(3 * d + o) * n + 3LL * v + in] = g[in][o];
+        for (int o = 0; o < 3; ++o) dGT[int64_t(3 * d + o) * n + 3LL * v + in] = c.g[in][o];
 #pragma unroll
     for (int o = 0; o < 3; ++o) {
-        fir[item * 3 + o] = f[o];
-        iir[item * 3 + o] = h[o];
+        fir[item * 3 + o] = c.f[o];
+        iir[item * 3 + o] = c.h[o];
     }
-    dA[item] = A;
+    dA[item] = c.A;
 }
 
 // update.rs:106-114, 158-168
@@ -646,7 +728,7 @@ hipError_t launch_derive(hipStream_t s, const int32_t *nbr, const float *coefs,
                          const float *xT, const float *eT, float *fir, float *iir, float *dGT,
                          float *dA, int32_t N, int32_t T, const DeriveParams &p) {
     if (N <= 0) return hipSuccess;
-    derive_kernel<<<blocks_for(int64_t(N) * 26, 256), 256, 0, s>>>(
+    derive_kernel<<<blocks_for(int64_t(N) * 26, kDeriveThreads), kDeriveThreads, 0, s>>>(
         nbr, coefs, delays, initial_delays, gT, xT, eT, fir, iir, dGT, dA, N, T, p);
     return hipGetLastError();
 }
