@@ -71,6 +71,7 @@ def main() -> int:
     ap.add_argument("--num-steps", type=int, default=2000, help="time steps per beat")
     ap.add_argument("--cpu-sample-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--propagation", choices=["auto", "per-step", "resident"], default="auto")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the epoch on a single MI355X; "
@@ -94,6 +95,9 @@ def main() -> int:
 
     cfg = Algorithm(learning_rate=1e-3, freeze_delays=False)
     results = ResultsGPU(gpu, problem, num_batch_entries=args.warmup + args.steps + 1)
+    mode = {"auto": 0, "per-step": 1, "resident": 2}[args.propagation]
+    resident = results.set_propagation(mode)
+    log(f"propagation path: {'resident' if resident else 'per-step'}")
     actual = Buffer(gpu, measurements)
     kernel = EpochKernel(gpu, results, actual, cfg, 3 * N, S, T)
     for _ in range(args.warmup):
@@ -118,8 +122,11 @@ def main() -> int:
     prof = {k: v / args.steps for k, v in prof_sum.items()}
     log("per-epoch phases (ms): " + json.dumps({k: round(v, 3) for k, v in prof.items()}))
 
-    prop_ms = prof["propagate_ms"] / max(prof["propagate_launches"], 1)
-    achieved = PROP_BYTES_PER_VOXEL_STEP * N / (prop_ms * 1e-3) / 1e9
+    launches = max(prof["propagate_launches"], 1)
+    launch_ms = prof["propagate_ms"] / launches
+    steps_per_launch = prof["propagate_steps"] / launches
+    bytes_per_launch = PROP_BYTES_PER_VOXEL_STEP * N * steps_per_launch
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
     loss = results.update_from_gpu(["loss_batch"])["loss_batch"]
     log(f"loss_batch per epoch: {loss.tolist()}")
 
@@ -141,17 +148,20 @@ def main() -> int:
                         f"({3 * N} states), 26 neighbours, delays 1-40, S={S} sensors, T={T} "
                         f"steps, 1 beat, SGD lr 1e-3",
             "voxels": N, "steps_per_beat": T, "sensors": S, "beats": 1,
+            "propagation": "resident" if resident else "per-step",
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "propagate_step_kernel",
+            "kernel": "propagate_resident_kernel" if resident else "propagate_step_kernel",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "avg_launch_ms": prop_ms,
-            "bytes_per_launch": PROP_BYTES_PER_VOXEL_STEP * N,
+            "avg_launch_ms": launch_ms,
+            "steps_per_launch": steps_per_launch,
+            "us_per_step": 1e3 * launch_ms / steps_per_launch,
+            "bytes_per_launch": bytes_per_launch,
         },
     }
     kernel.close()

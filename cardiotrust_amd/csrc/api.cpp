@@ -77,6 +77,12 @@ struct ct_results_gpu {
     // metrics
     float *loss = nullptr, *loss_mse = nullptr, *loss_mr = nullptr;
     float *loss_b = nullptr, *loss_mse_b = nullptr, *loss_mr_b = nullptr;
+    float *u = nullptr;  // control function on the device (resident propagation)
+    // propagation path: resident (one cooperative launch per beat) when the
+    // problem fits on chip, else one launch per step
+    int32_t propagation_mode = 0;  // 0 auto, 1 per-step launches, 2 resident
+    bool resident_ok = false;
+    int32_t resident_blocks = 0;
     // scratch
     float *partial = nullptr, *mr_partial = nullptr, *staging = nullptr;
     int32_t splits = 1, mr_blocks = 1;
@@ -110,8 +116,12 @@ struct ct_epoch_kernel {
     bool profiling = false;
     std::vector<hipEvent_t> events;
     size_t used_events = 0;
-    // (phase, start event, end event)
-    std::vector<std::pair<int, std::pair<size_t, size_t>>> spans;
+    struct SpanRec {
+        int phase;
+        int64_t steps;
+        size_t start, end;
+    };
+    std::vector<SpanRec> spans;
 
     ~ct_epoch_kernel() {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
@@ -125,9 +135,10 @@ enum Phase { kPropagate, kProject, kRegularize, kTranspose, kMapResiduals, kDeri
 struct Span {
     ct_epoch_kernel *k;
     int phase;
+    int64_t steps;
     size_t start = 0;
     bool on = false;
-    Span(ct_epoch_kernel *kk, int ph) : k(kk), phase(ph) {
+    Span(ct_epoch_kernel *kk, int ph, int64_t st = 0) : k(kk), phase(ph), steps(st) {
         if (!k || !k->profiling) return;
         if (k->used_events + 2 > k->events.size()) return;
         start = k->used_events;
@@ -137,7 +148,7 @@ struct Span {
     ~Span() {
         if (!on) return;
         if (hipEventRecord(k->events[start + 1], k->r->gpu->stream) == hipSuccess)
-            k->spans.push_back({phase, {start, start + 1}});
+            k->spans.push_back({phase, steps, start, start + 1});
     }
 };
 
@@ -218,10 +229,17 @@ int forward_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_ke
     hipStream_t s = r.gpu->stream;
     CT_HIP(hipMemsetAsync(r.y, 0, sizeof(float) * r.N * 78, s));
     CT_HIP(hipMemsetAsync(r.y_last, 0, sizeof(float) * r.N * 78, s));
-    for (int32_t t = 0; t < r.T; ++t) {
-        Span sp(k, kPropagate);
-        CT_HIP(ct::launch_propagate_step(s, r.nbr, r.coefs, r.delays, r.gT, r.y, r.y_last, r.x,
-                                         r.b, r.control_function[t], r.N, t, r.T));
+    const bool resident = r.resident_ok && r.propagation_mode != 1;
+    if (resident && r.N > 0) {
+        Span sp(k, kPropagate, r.T);
+        CT_HIP(ct::launch_propagate_resident(s, r.resident_blocks, r.nbr, r.coefs, r.delays, r.gT,
+                                             r.y, r.y_last, r.x, r.b, r.u, r.N, 0, r.T, r.T));
+    } else {
+        for (int32_t t = 0; t < r.T; ++t) {
+            Span sp(k, kPropagate, 1);
+            CT_HIP(ct::launch_propagate_step(s, r.nbr, r.coefs, r.delays, r.gT, r.y, r.y_last,
+                                             r.x, r.b, r.control_function[t], r.N, t, r.T));
+        }
     }
     {
         Span sp(k, kProject);
@@ -317,6 +335,33 @@ int finish_batch(ct_epoch_kernel &k, int64_t num_beats, bool reset_after) {
     CT_HIP(ct::launch_batch_metrics(s, r.loss, r.loss_mse, r.loss_mr, r.loss_b, r.loss_mse_b,
                                     r.loss_mr_b, k.batch_index, r.T));
     k.batch_index += 1;
+    return CT_OK;
+}
+
+// Decides whether the resident (cooperative, on-chip filter state) kernel can
+// run this problem: cooperative launch support, at most kResMaxVoxels voxels
+// per workgroup with every workgroup co-resident, neighbour indices below
+// 2^17 - 1, delays and step offsets that fit its 15-bit / 32-bit fields.
+int plan_resident(ct_results_gpu &r) {
+    r.resident_ok = false;
+    r.resident_blocks = 0;
+    if (r.N <= 0) return CT_OK;
+    int coop = 0, cus = 0;
+    CT_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, r.gpu->device));
+    CT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, r.gpu->device));
+    if (!coop || cus <= 0) return CT_OK;
+    int32_t per_cu = 0;
+    if (ct::resident_capacity(&per_cu) != hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();
+        return CT_OK;
+    }
+    const int64_t capacity = int64_t(cus) * per_cu;
+    const int64_t blocks = std::min<int64_t>(capacity, std::max<int64_t>(1, (r.N + 127) / 128));
+    if ((r.N + blocks - 1) / blocks > ct::kResMaxVoxels) return CT_OK;
+    if (r.N >= (1 << 17) - 1 || r.T > 32767 || int64_t(r.T) * r.n >= (int64_t(1) << 32))
+        return CT_OK;
+    r.resident_blocks = int32_t(blocks);
+    r.resident_ok = true;
     return CT_OK;
 }
 
@@ -479,6 +524,7 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
     ALLOC(r->partial, splits * T * Sp);
     ALLOC(r->mr_partial, T * r->mr_blocks);
     ALLOC(r->staging, n * 78);
+    ALLOC(r->u, T);
 #undef ALLOC
 
     hipStream_t s = gpu->stream;
@@ -515,6 +561,7 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
         if ((rc = upload(r->initial_delays, d.initial_delays, sizeof(float) * N * 26)) != CT_OK)
             return cleanup(rc);
         if ((rc = upload(r->b, d.control_matrix, sizeof(float) * n)) != CT_OK) return cleanup(rc);
+        if ((rc = upload(r->u, d.control_function, sizeof(float) * T)) != CT_OK) return cleanup(rc);
         // gains (n, 78) -> gT (78, n) through the staging buffer
         if ((rc = upload(r->staging, d.gains, sizeof(float) * n * 78)) != CT_OK) return cleanup(rc);
         if (ct::launch_transpose(s, r->staging, r->gT, n, 78) != hipSuccess)
@@ -529,6 +576,7 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
         if (hipStreamSynchronize(s) != hipSuccess)
             return cleanup(fail(CT_ERR_DEVICE, "upload failed"));
     }
+    if ((rc = plan_resident(*r)) != CT_OK) return cleanup(rc);
     *out = r;
     return CT_OK;
 }
@@ -666,6 +714,16 @@ int ct_epoch_kernel_execute(ct_epoch_kernel *k) {
     return CT_OK;
 }
 
+int ct_results_set_propagation(ct_results_gpu *results, int32_t mode, int32_t *resident) {
+    if (!results) return fail(CT_ERR_INVALID, "null argument");
+    if (mode < 0 || mode > 2) return fail(CT_ERR_INVALID, "propagation mode must be 0, 1 or 2");
+    if (mode == 2 && !results->resident_ok && results->N > 0)
+        return fail(CT_ERR_INVALID, "the resident propagation kernel cannot hold this problem");
+    results->propagation_mode = mode;
+    if (resident) *resident = (results->resident_ok && mode != 1) ? 1 : 0;
+    return CT_OK;
+}
+
 int ct_epoch_kernel_set_freeze_delays(ct_epoch_kernel *k, int32_t value) {
     if (!k) return fail(CT_ERR_INVALID, "null argument");
     k->cfg.freeze_delays = value != 0;
@@ -724,9 +782,13 @@ int ct_epoch_kernel_get_profile(ct_epoch_kernel *k, ct_profile *out) {
     std::memset(out, 0, sizeof(*out));
     for (auto &sp : k->spans) {
         float ms = 0.0f;
-        CT_HIP(hipEventElapsedTime(&ms, k->events[sp.second.first], k->events[sp.second.second]));
-        switch (sp.first) {
-            case kPropagate: out->propagate_ms += ms; out->propagate_launches += 1; break;
+        CT_HIP(hipEventElapsedTime(&ms, k->events[sp.start], k->events[sp.end]));
+        switch (sp.phase) {
+            case kPropagate:
+                out->propagate_ms += ms;
+                out->propagate_launches += 1;
+                out->propagate_steps += sp.steps;
+                break;
             case kProject: out->project_ms += ms; break;
             case kRegularize: out->regularize_ms += ms; break;
             case kTranspose: out->transpose_ms += ms; break;

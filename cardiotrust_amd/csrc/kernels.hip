@@ -6,10 +6,13 @@
 // bit for bit. Reference citations are relative to the reference's src/core/.
 #include "kernels.h"
 
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 namespace ct {
 namespace {
+
+namespace cg = cooperative_groups;
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
@@ -131,6 +134,133 @@ __global__ __launch_bounds__(kPropThreads) void propagate_step_kernel(
         }
         x[int64_t(t) * n + i] = acc + u_t * b[i];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Resident propagation: one cooperative launch runs a range of time steps.
+//
+// Workgroup g owns a contiguous slab of voxels for the whole launch. Its
+// filter outputs y (78 per voxel) live in LDS and its connection tables
+// (neighbour, delay, coefficient) in registers, so per step only the gains
+// and the delayed neighbour states are read from memory. Phase 1 (one lane
+// per connection) and phase 2 (one lane per state) are the same arithmetic as
+// propagate_step_kernel; a grid-wide barrier separates the steps.
+// ---------------------------------------------------------------------------
+constexpr int kResThreads = 1024;
+constexpr int kResItems = (kResMaxVoxels * 26 + kResThreads - 1) / kResThreads;
+constexpr int kResStates = (kResMaxVoxels * 3 + kResThreads - 1) / kResThreads;
+constexpr int kResBatch = 7;
+constexpr uint32_t kNoConnection = 0xFFFFFFFFu;
+constexpr int kResNeighbourBits = 17;  // voxel index field; delay in the upper 15 bits
+
+__global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
+    const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
+    const int32_t *__restrict__ delays, const float *__restrict__ gT, float *__restrict__ y,
+    float *__restrict__ y_last, float *__restrict__ x, const float *__restrict__ b,
+    const float *__restrict__ u, int32_t N, int32_t t_begin, int32_t t_end, int32_t T) {
+    extern __shared__ float lds[];
+    float *ys = lds;                                                       // [voxel][78]
+    uint32_t *mask = reinterpret_cast<uint32_t *>(lds + kResMaxVoxels * 78);  // [voxel]
+    cg::grid_group grid = cg::this_grid();
+    const int64_t n = 3LL * N;
+    const int64_t slab = xcd_block(blockIdx.x, gridDim.x);
+    const int32_t vb = int32_t(slab * N / gridDim.x);
+    const int32_t ve = int32_t((slab + 1) * N / gridDim.x);
+    const int32_t nv = ve - vb;
+    const int32_t items = nv * 26;
+    const int tid = threadIdx.x;
+
+    for (int k = tid; k < kResMaxVoxels; k += kResThreads) mask[k] = 0u;
+    __syncthreads();
+    uint32_t pk[kResItems];
+    float ak[kResItems];
+#pragma unroll
+    for (int k = 0; k < kResItems; ++k) {
+        const int it = tid + k * kResThreads;
+        pk[k] = kNoConnection;
+        ak[k] = 0.0f;
+        if (it < items) {
+            const int64_t c = int64_t(vb) * 26 + it;
+            const int32_t w = nbr[c];
+            if (w >= 0) {
+                const uint32_t D = uint32_t(min(delays[c], 32767));
+                pk[k] = uint32_t(w) | (D << kResNeighbourBits);
+                ak[k] = coefs[c];
+                atomicOr(&mask[it / 26], 1u << (it % 26));
+            }
+        }
+    }
+    for (int idx = tid; idx < nv * 78; idx += kResThreads) ys[idx] = y[int64_t(vb) * 78 + idx];
+    __syncthreads();
+
+    for (int32_t t = t_begin; t < t_end; ++t) {
+        const bool last = (t == T - 1);
+#pragma unroll
+        for (int k0 = 0; k0 < kResItems; k0 += kResBatch) {
+            float in0[kResBatch][3], in1[kResBatch][3];
+#pragma unroll
+            for (int j = 0; j < kResBatch; ++j) {
+                const int k = k0 + j;
+#pragma unroll
+                for (int o = 0; o < 3; ++o) in0[j][o] = in1[j][o] = 0.0f;
+                if (k >= kResItems || pk[k] == kNoConnection) continue;
+                // 32-bit element offsets: the host only picks this kernel when T * n < 2^32
+                const uint32_t w = pk[k] & ((1u << kResNeighbourBits) - 1);
+                const int32_t D = int32_t(pk[k] >> kResNeighbourBits);
+                const uint32_t col = 3u * w;
+                if (D <= t) {
+                    const uint32_t r0 = uint32_t(t - D) * uint32_t(n) + col;
+#pragma unroll
+                    for (int o = 0; o < 3; ++o) in0[j][o] = x[r0 + o];
+                }
+                if (D < t) {
+                    const uint32_t r1 = uint32_t(t - D - 1) * uint32_t(n) + col;
+#pragma unroll
+                    for (int o = 0; o < 3; ++o) in1[j][o] = x[r1 + o];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kResBatch; ++j) {
+                const int k = k0 + j;
+                if (k >= kResItems || pk[k] == kNoConnection) continue;
+                const int it = tid + k * kResThreads;
+#pragma unroll
+                for (int o = 0; o < 3; ++o) {
+                    const float prev = ys[it * 3 + o];
+                    if (last) y_last[(int64_t(vb) * 26 + it) * 3 + o] = prev;
+                    ys[it * 3 + o] = fmaf(ak[k], in0[j][o] - prev, in1[j][o]);
+                }
+            }
+        }
+        __syncthreads();
+        const float ut = u[t];
+#pragma unroll 1
+        for (int k = 0; k < kResStates; ++k) {
+            const int sidx = tid + k * kResThreads;
+            if (sidx >= nv * 3) break;
+            const int vl = sidx / 3;
+            const int64_t i = 3LL * vb + sidx;
+            const uint32_t m = mask[vl];
+            float acc = 0.0f;
+#pragma unroll 1
+            for (int h = 0; h < 2; ++h) {  // gains in two halves of 13 connections
+                float g[39];
+#pragma unroll
+                for (int q = 0; q < 39; ++q) g[q] = gT[uint32_t(39 * h + q) * uint32_t(n) + uint32_t(i)];
+#pragma unroll
+                for (int dd = 0; dd < 13; ++dd) {
+                    const int d = 13 * h + dd;
+                    if (!((m >> d) & 1u)) continue;
+#pragma unroll
+                    for (int o = 0; o < 3; ++o)
+                        acc = acc + g[3 * dd + o] * ys[(vl * 26 + d) * 3 + o];
+                }
+            }
+            x[int64_t(t) * n + i] = acc + ut * b[i];
+        }
+        grid.sync();
+    }
+    for (int idx = tid; idx < nv * 78; idx += kResThreads) y[int64_t(vb) * 78 + idx] = ys[idx];
 }
 
 // ---------------------------------------------------------------------------
@@ -676,6 +806,32 @@ hipError_t launch_propagate_step(hipStream_t s, const int32_t *nbr, const float 
     propagate_step_kernel<<<blocks_for(N, kPropVoxels), kPropThreads, 0, s>>>(
         nbr, coefs, delays, gT, y, y_last, x, b, u_t, N, t, T);
     return hipGetLastError();
+}
+
+size_t resident_lds_bytes() { return sizeof(float) * kResMaxVoxels * 78 + 4 * kResMaxVoxels; }
+
+hipError_t resident_capacity(int32_t *blocks_per_cu) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&propagate_resident_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       int(resident_lds_bytes()));
+    if (e != hipSuccess) return e;
+    int nb = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &nb, reinterpret_cast<const void *>(&propagate_resident_kernel), kResThreads,
+        resident_lds_bytes());
+    *blocks_per_cu = nb;
+    return e;
+}
+
+hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const int32_t *nbr,
+                                     const float *coefs, const int32_t *delays, const float *gT,
+                                     float *y, float *y_last, float *x, const float *b,
+                                     const float *u, int32_t N, int32_t t_begin, int32_t t_end,
+                                     int32_t T) {
+    void *args[] = {&nbr, &coefs, &delays, &gT, &y, &y_last, &x, &b, &u, &N, &t_begin, &t_end, &T};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&propagate_resident_kernel),
+                                      dim3(blocks), dim3(kResThreads), args, resident_lds_bytes(),
+                                      s);
 }
 
 hipError_t launch_project(hipStream_t s, const float *x, const float *H, float *partial,

@@ -23,7 +23,8 @@ LIB_PATH = os.path.join(_HERE, "libcardiotrust_hip.so")
 EXPORTS = (
     "ct_last_error", "ct_gpu_new", "ct_gpu_free", "ct_gpu_synchronize",
     "ct_measurements_to_gpu", "ct_buffer_free", "ct_results_to_gpu", "ct_results_gpu_free",
-    "ct_results_update_from_gpu", "ct_results_simulate", "ct_epoch_kernel_new",
+    "ct_results_update_from_gpu", "ct_results_simulate", "ct_results_set_propagation",
+    "ct_epoch_kernel_new",
     "ct_epoch_kernel_free", "ct_epoch_kernel_execute", "ct_epoch_kernel_set_freeze_delays",
     "ct_epoch_kernel_set_freeze_gains", "ct_epoch_kernel_set_beat_order",
     "ct_epoch_kernel_set_batch_index", "ct_epoch_kernel_get_batch_index",
@@ -61,6 +62,7 @@ class _Host(ctypes.Structure):
 class Profile(ctypes.Structure):
     _fields_ = [
         ("propagate_ms", ctypes.c_double), ("propagate_launches", ctypes.c_int64),
+        ("propagate_steps", ctypes.c_int64),
         ("project_ms", ctypes.c_double), ("regularize_ms", ctypes.c_double),
         ("transpose_ms", ctypes.c_double), ("map_residuals_ms", ctypes.c_double),
         ("derive_ms", ctypes.c_double), ("update_ms", ctypes.c_double),
@@ -94,6 +96,7 @@ def lib() -> ctypes.CDLL:
         "ct_results_gpu_free": ([vp], None),
         "ct_results_update_from_gpu": ([vp, P(_Host)], ctypes.c_int),
         "ct_results_simulate": ([vp, vp], ctypes.c_int),
+        "ct_results_set_propagation": ([vp, i32, P(i32)], ctypes.c_int),
         "ct_epoch_kernel_new": ([vp, vp, vp, P(CAlgorithm), i32, i32, i32, P(vp)], ctypes.c_int),
         "ct_epoch_kernel_free": ([vp], None),
         "ct_epoch_kernel_execute": ([vp], ctypes.c_int),
@@ -227,6 +230,14 @@ class ResultsGPU:
         _check(lib().ct_results_update_from_gpu(self.handle, ctypes.byref(h)),
                "Results::update_from_gpu")
         return out
+
+    def set_propagation(self, mode: int) -> bool:
+        """0 automatic, 1 one launch per step, 2 resident; returns whether the
+        resident kernel will run."""
+        res = ctypes.c_int32()
+        _check(lib().ct_results_set_propagation(self.handle, int(mode), ctypes.byref(res)),
+               "set_propagation")
+        return bool(res.value)
 
     def simulate(self) -> np.ndarray:
         """Noise-free forward simulation of every beat; returns (B, T, S)."""

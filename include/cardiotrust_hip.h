@@ -107,6 +107,7 @@ typedef struct {
 typedef struct {
     double propagate_ms;                  /* sum over propagation launches */
     int64_t propagate_launches;
+    int64_t propagate_steps;              /* time steps those launches advanced */
     double project_ms;                    /* H·Xᵀ contraction + residuals + step losses */
     double regularize_ms;                 /* maximum-regularisation sums */
     double transpose_ms;
@@ -144,6 +145,13 @@ int ct_results_update_from_gpu(ct_results_gpu *results, const ct_results_host *o
  * Fills the device measurements (download with ct_results_update_from_gpu)
  * and, when `measurements_out` is not NULL, copies them there (B,T,S). */
 int ct_results_simulate(ct_results_gpu *results, float *measurements_out);
+
+/* Propagation path (no reference counterpart; a tuning switch of this
+ * library): 0 = automatic (resident when the problem fits on chip), 1 = one
+ * launch per time step, 2 = resident (one cooperative launch per beat that
+ * keeps the filter outputs in LDS; CT_ERR_INVALID if the problem does not
+ * fit). `resident` (may be NULL) receives 1 if the resident path will run. */
+int ct_results_set_propagation(ct_results_gpu *results, int32_t mode, int32_t *resident);
 
 /* EpochKernel::new (src/core/algorithm/gpu/epoch.rs:29-93). The sizes must
  * match the results; `actual_measurements` must be (B,T,S). */

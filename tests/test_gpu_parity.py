@@ -41,9 +41,13 @@ def gpu():
     g.close()
 
 
-def run_gpu_epochs(gpu, p, cfg, epochs=1, beat_order=None, nbatch=1, optimizer=0, profile=False):
+def run_gpu_epochs(gpu, p, cfg, epochs=1, beat_order=None, nbatch=1, optimizer=0, profile=False,
+                   mode=0):
     from cardiotrust_amd.gpu import Buffer, EpochKernel, ResultsGPU
     r = ResultsGPU(gpu, p, num_batch_entries=nbatch, optimizer=optimizer)
+    resident = r.set_propagation(mode)
+    if mode == 2:
+        assert resident
     buf = Buffer(gpu, p.measurements)
     k = EpochKernel(gpu, r, buf, cfg, 3 * p.num_voxels, p.num_sensors, p.num_steps)
     if beat_order is not None:
@@ -92,11 +96,17 @@ def compare(g, o32, o64, exact=EXACT, close=CLOSE, rel=REL):
     return report
 
 
-def test_epoch_config0_parity(gpu):
-    note("case: config0 cube 10^3, T=1000, delays 1-20, lr 1e-3, freeze_delays=False")
-    p = synth.config0(simulate=oracle.simulate_f32)
+@pytest.fixture(scope="module")
+def config0():
+    return synth.config0(simulate=oracle.simulate_f32)
+
+
+@pytest.mark.parametrize("mode", [1, 2], ids=["per-step", "resident"])
+def test_epoch_config0_parity(gpu, config0, mode):
+    note(f"case: config0 cube 10^3, T=1000, delays 1-20, lr 1e-3, freeze_delays=False, mode {mode}")
+    p = config0
     cfg = Algorithm(learning_rate=1e-3, freeze_delays=False)
-    g, prof = run_gpu_epochs(gpu, p, cfg, profile=True)
+    g, prof = run_gpu_epochs(gpu, p, cfg, profile=True, mode=mode)
     o32 = run_oracle_epochs(p, cfg, np.float32)
     o64 = run_oracle_epochs(p, cfg, np.float64)
     compare(g, o32, o64)
@@ -141,11 +151,12 @@ def test_epoch_small_cases(gpu, case):
     compare(g, o32, o64, exact=exact, close=CLOSE + ("system_states", "ap_outputs_now"))
 
 
-def test_first_epoch_states_bit_exact_on_ragged_mask(gpu):
-    note("case: ragged blob 9x4x6 seed 8, T=80, one epoch, textbook")
+@pytest.mark.parametrize("mode", [1, 2], ids=["per-step", "resident"])
+def test_first_epoch_states_bit_exact_on_ragged_mask(gpu, mode):
+    note(f"case: ragged blob 9x4x6 seed 8, T=80, one epoch, textbook, mode {mode}")
     p = blob_problem(8, shape=(9, 4, 6), T=80, max_delay=20, fill=0.45)
     cfg = Algorithm(learning_rate=0.2, freeze_delays=False)
-    g, _ = run_gpu_epochs(gpu, p, cfg)
+    g, _ = run_gpu_epochs(gpu, p, cfg, mode=mode)
     o32 = run_oracle_epochs(p, cfg, np.float32)
     o64 = run_oracle_epochs(p, cfg, np.float64)
     compare(g, o32, o64)
@@ -162,6 +173,25 @@ def test_multibeat_batches_adam(gpu):
     assert g["batch_index"] == o32.batch_index.value == 4
     assert g["derivatives_step"][0] == o32.d_step[0] == 5
     compare(g, o32, o64, exact=("delays",), close=CLOSE + ("system_states",), rel=1e-3)
+
+
+def test_resident_and_per_step_paths_agree_bitwise(gpu):
+    from cardiotrust_amd.gpu import ResultsGPU
+    note("case: shell 24^3 (several resident workgroups), simulate, modes 1 vs 2")
+    g0 = synth.ellipsoid_shell(shape=(24, 24, 24), outer=(12, 12, 12), inner=(6, 6, 6))
+    p = synth.make_problem(g0, num_steps=150, max_delay=40, seed=9)
+    outs = []
+    for mode in (1, 2):
+        r = ResultsGPU(gpu, p)
+        assert r.set_propagation(mode) == (mode == 2)
+        m = r.simulate()
+        x = r.update_from_gpu(["system_states", "ap_outputs_now", "ap_outputs_last"])
+        r.close()
+        outs.append((m, x))
+    assert g0.num_voxels > 5000
+    for k in ("system_states", "ap_outputs_now", "ap_outputs_last"):
+        assert np.array_equal(outs[0][1][k].view(np.uint32), outs[1][1][k].view(np.uint32)), k
+    assert np.array_equal(outs[0][0], outs[1][0])
 
 
 def test_simulate_matches_oracle(gpu):
