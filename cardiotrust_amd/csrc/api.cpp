@@ -78,11 +78,13 @@ struct ct_results_gpu {
     float *loss = nullptr, *loss_mse = nullptr, *loss_mr = nullptr;
     float *loss_b = nullptr, *loss_mse_b = nullptr, *loss_mr_b = nullptr;
     float *u = nullptr;  // control function on the device (resident propagation)
-    // propagation path: resident (one cooperative launch per beat) when the
+    // propagation path: resident (one launch per beat) when the
     // problem fits on chip, else one launch per step
     int32_t propagation_mode = 0;  // 0 auto, 1 per-step launches, 2 resident
     bool resident_ok = false;
+    bool resident_used = false;  // a resident launch is in flight since the last check
     int32_t resident_blocks = 0;
+    ct::GridBarrier *barrier = nullptr;
     // scratch
     float *partial = nullptr, *mr_partial = nullptr, *staging = nullptr;
     int32_t splits = 1, mr_blocks = 1;
@@ -233,7 +235,9 @@ int forward_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_ke
     if (resident && r.N > 0) {
         Span sp(k, kPropagate, r.T);
         CT_HIP(ct::launch_propagate_resident(s, r.resident_blocks, r.nbr, r.coefs, r.delays, r.gT,
-                                             r.y, r.y_last, r.x, r.b, r.u, r.N, 0, r.T, r.T));
+                                             r.y, r.y_last, r.x, r.b, r.u, r.barrier, r.N, 0, r.T,
+                                             r.T));
+        r.resident_used = true;
     } else {
         for (int32_t t = 0; t < r.T; ++t) {
             Span sp(k, kPropagate, 1);
@@ -338,18 +342,17 @@ int finish_batch(ct_epoch_kernel &k, int64_t num_beats, bool reset_after) {
     return CT_OK;
 }
 
-// Decides whether the resident (cooperative, on-chip filter state) kernel can
-// run this problem: cooperative launch support, at most kResMaxVoxels voxels
-// per workgroup with every workgroup co-resident, neighbour indices below
-// 2^17 - 1, delays and step offsets that fit its 15-bit / 32-bit fields.
+// Decides whether the resident (on-chip filter state) kernel can run this
+// problem: at most kResMaxVoxels voxels per workgroup with every workgroup
+// resident on the device at once, neighbour indices below 2^17 - 1, delays
+// and step offsets that fit its 15-bit / 32-bit fields.
 int plan_resident(ct_results_gpu &r) {
     r.resident_ok = false;
     r.resident_blocks = 0;
     if (r.N <= 0) return CT_OK;
-    int coop = 0, cus = 0;
-    CT_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, r.gpu->device));
+    int cus = 0;
     CT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, r.gpu->device));
-    if (!coop || cus <= 0) return CT_OK;
+    if (cus <= 0) return CT_OK;
     int32_t per_cu = 0;
     if (ct::resident_capacity(&per_cu) != hipSuccess || per_cu < 1) {
         (void)hipGetLastError();
@@ -362,6 +365,23 @@ int plan_resident(ct_results_gpu &r) {
         return CT_OK;
     r.resident_blocks = int32_t(blocks);
     r.resident_ok = true;
+    return CT_OK;
+}
+
+// After resident launches: waits for the stream and reports a grid-barrier
+// timeout (workgroups that were not all resident), which leaves the results of
+// that launch invalid. The path is then disabled for this problem.
+int check_resident(ct_results_gpu &r) {
+    if (!r.resident_used) return CT_OK;
+    r.resident_used = false;
+    ct::GridBarrier gb{};
+    CT_HIP(hipStreamSynchronize(r.gpu->stream));
+    CT_HIP(hipMemcpy(&gb, r.barrier, sizeof(gb), hipMemcpyDeviceToHost));
+    if (gb.abort) {
+        r.resident_ok = false;
+        return fail(CT_ERR_DEVICE, "resident propagation: grid barrier timed out (workgroups not "
+                                   "all resident); the per-step path is used from now on");
+    }
     return CT_OK;
 }
 
@@ -525,6 +545,7 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
     ALLOC(r->mr_partial, T * r->mr_blocks);
     ALLOC(r->staging, n * 78);
     ALLOC(r->u, T);
+    ALLOC(r->barrier, 1);
 #undef ALLOC
 
     hipStream_t s = gpu->stream;
@@ -640,6 +661,7 @@ int ct_results_simulate(ct_results_gpu *results, float *measurements_out) {
     ct_results_gpu &r = *results;
     CT_CHECK(check_gpu(r.gpu));
     for (int32_t beat = 0; beat < r.B; ++beat) CT_CHECK(forward_beat(r, beat, nullptr, nullptr));
+    CT_CHECK(check_resident(r));
     CT_HIP(hipStreamSynchronize(r.gpu->stream));
     if (measurements_out)
         CT_CHECK(download(measurements_out, r.m, sizeof(float) * r.B * r.T * r.S));
@@ -711,7 +733,7 @@ int ct_epoch_kernel_execute(ct_epoch_kernel *k) {
     } else {
         CT_CHECK(finish_batch(*k, r.B, false));
     }
-    return CT_OK;
+    return check_resident(r);
 }
 
 int ct_results_set_propagation(ct_results_gpu *results, int32_t mode, int32_t *resident) {

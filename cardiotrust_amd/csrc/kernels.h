@@ -30,17 +30,24 @@ hipError_t launch_propagate_step(hipStream_t s, const int32_t *nbr, const float 
                                  float *x, const float *b, float u_t, int32_t N, int32_t t,
                                  int32_t T);
 
-// Resident propagation (one cooperative launch for steps [t_begin, t_end)).
-// A workgroup holds at most kResMaxVoxels voxels; requires N < 2^17 - 1 and
-// blocks * kResMaxVoxels >= N, blocks <= co-resident capacity.
+// Resident propagation (one launch for steps [t_begin, t_end)). A workgroup
+// holds at most kResMaxVoxels voxels; requires N < 2^17 - 1,
+// blocks * kResMaxVoxels >= N and blocks <= what fits on the device at once.
+// `abort` != 0 after the launch means the grid barrier timed out.
 constexpr int kResMaxVoxels = 500;
+struct GridBarrier {
+    uint32_t count;
+    uint32_t generation;
+    uint32_t abort;
+    uint32_t pad;
+};
 size_t resident_lds_bytes();
 hipError_t resident_capacity(int32_t *blocks_per_cu);
 hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const int32_t *nbr,
                                      const float *coefs, const int32_t *delays, const float *gT,
                                      float *y, float *y_last, float *x, const float *b,
-                                     const float *u, int32_t N, int32_t t_begin, int32_t t_end,
-                                     int32_t T);
+                                     const float *u, GridBarrier *gb, int32_t N, int32_t t_begin,
+                                     int32_t t_end, int32_t T);
 
 hipError_t launch_project(hipStream_t s, const float *x, const float *H, float *partial,
                           int32_t T, int64_t n, int32_t Sp, int32_t splits, int64_t k_per_split);

@@ -6,13 +6,10 @@
 // bit for bit. Reference citations are relative to the reference's src/core/.
 #include "kernels.h"
 
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 namespace ct {
 namespace {
-
-namespace cg = cooperative_groups;
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
@@ -137,7 +134,7 @@ __global__ __launch_bounds__(kPropThreads) void propagate_step_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Resident propagation: one cooperative launch runs a range of time steps.
+// Resident propagation: one launch runs a range of time steps.
 //
 // Workgroup g owns a contiguous slab of voxels for the whole launch. Its
 // filter outputs y (78 per voxel) live in LDS and its connection tables
@@ -145,7 +142,50 @@ __global__ __launch_bounds__(kPropThreads) void propagate_step_kernel(
 // and the delayed neighbour states are read from memory. Phase 1 (one lane
 // per connection) and phase 2 (one lane per state) are the same arithmetic as
 // propagate_step_kernel; a grid-wide barrier separates the steps.
+//
+// The host launches no more workgroups than fit on the device at once. The
+// barrier's wait is bounded: if a workgroup waits longer than
+// kBarrierSpinLimit polls (about a second), it raises `abort`, every
+// workgroup leaves the time loop at its next barrier, and the host reports
+// the failure instead of the kernel waiting forever.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kBarrierSpinLimit = 1u << 21;
+
+__device__ __forceinline__ bool grid_barrier(GridBarrier *gb, uint32_t nblocks, uint32_t &gen,
+                                             int *ok_lds) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        __threadfence();  // publish this workgroup's state rows
+        const uint32_t arrived =
+            __hip_atomic_fetch_add(&gb->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == nblocks - 1) {
+            __hip_atomic_store(&gb->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&gb->generation, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&gb->generation, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) ==
+                   gen) {
+                if (__hip_atomic_load(&gb->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    ok = 0;
+                    break;
+                }
+                if (++spins > kBarrierSpinLimit) {
+                    __hip_atomic_store(&gb->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __threadfence();  // observe the other workgroups' rows
+        *ok_lds = ok;
+    }
+    gen += 1;
+    __syncthreads();
+    return *ok_lds != 0;
+}
+
 constexpr int kResThreads = 1024;
 constexpr int kResItems = (kResMaxVoxels * 26 + kResThreads - 1) / kResThreads;
 constexpr int kResStates = (kResMaxVoxels * 3 + kResThreads - 1) / kResThreads;
@@ -157,11 +197,13 @@ __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
     const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
     const int32_t *__restrict__ delays, const float *__restrict__ gT, float *__restrict__ y,
     float *__restrict__ y_last, float *__restrict__ x, const float *__restrict__ b,
-    const float *__restrict__ u, int32_t N, int32_t t_begin, int32_t t_end, int32_t T) {
+    const float *__restrict__ u, GridBarrier *__restrict__ gb, int32_t N, int32_t t_begin,
+    int32_t t_end, int32_t T) {
     extern __shared__ float lds[];
     float *ys = lds;                                                       // [voxel][78]
     uint32_t *mask = reinterpret_cast<uint32_t *>(lds + kResMaxVoxels * 78);  // [voxel]
-    cg::grid_group grid = cg::this_grid();
+    int *ok_lds = reinterpret_cast<int *>(mask + kResMaxVoxels);
+    uint32_t gen = 0;
     const int64_t n = 3LL * N;
     const int64_t slab = xcd_block(blockIdx.x, gridDim.x);
     const int32_t vb = int32_t(slab * N / gridDim.x);
@@ -258,7 +300,7 @@ __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
             }
             x[int64_t(t) * n + i] = acc + ut * b[i];
         }
-        grid.sync();
+        if (!grid_barrier(gb, gridDim.x, gen, ok_lds)) return;  // host reports the abort
     }
     for (int idx = tid; idx < nv * 78; idx += kResThreads) y[int64_t(vb) * 78 + idx] = ys[idx];
 }
@@ -808,7 +850,7 @@ hipError_t launch_propagate_step(hipStream_t s, const int32_t *nbr, const float 
     return hipGetLastError();
 }
 
-size_t resident_lds_bytes() { return sizeof(float) * kResMaxVoxels * 78 + 4 * kResMaxVoxels; }
+size_t resident_lds_bytes() { return sizeof(float) * kResMaxVoxels * 78 + 4 * kResMaxVoxels + 16; }
 
 hipError_t resident_capacity(int32_t *blocks_per_cu) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&propagate_resident_kernel),
@@ -826,12 +868,14 @@ hipError_t resident_capacity(int32_t *blocks_per_cu) {
 hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const int32_t *nbr,
                                      const float *coefs, const int32_t *delays, const float *gT,
                                      float *y, float *y_last, float *x, const float *b,
-                                     const float *u, int32_t N, int32_t t_begin, int32_t t_end,
-                                     int32_t T) {
-    void *args[] = {&nbr, &coefs, &delays, &gT, &y, &y_last, &x, &b, &u, &N, &t_begin, &t_end, &T};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&propagate_resident_kernel),
-                                      dim3(blocks), dim3(kResThreads), args, resident_lds_bytes(),
-                                      s);
+                                     const float *u, GridBarrier *gb, int32_t N, int32_t t_begin,
+                                     int32_t t_end, int32_t T) {
+    // the barrier starts from zero: a fill kernel rather than a memset node
+    hipError_t e = launch_fill(s, reinterpret_cast<float *>(gb), 0.0f, sizeof(GridBarrier) / 4);
+    if (e != hipSuccess) return e;
+    propagate_resident_kernel<<<blocks, kResThreads, resident_lds_bytes(), s>>>(
+        nbr, coefs, delays, gT, y, y_last, x, b, u, gb, N, t_begin, t_end, T);
+    return hipGetLastError();
 }
 
 hipError_t launch_project(hipStream_t s, const float *x, const float *H, float *partial,
