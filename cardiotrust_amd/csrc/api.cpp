@@ -377,7 +377,7 @@ int check_resident(ct_results_gpu &r) {
     ct::GridBarrier gb{};
     CT_HIP(hipStreamSynchronize(r.gpu->stream));
     CT_HIP(hipMemcpy(&gb, r.barrier, sizeof(gb), hipMemcpyDeviceToHost));
-    if (gb.abort) {
+    if (gb.abort.count) {
         r.resident_ok = false;
         return fail(CT_ERR_DEVICE, "resident propagation: grid barrier timed out (workgroups not "
                                    "all resident); the per-step path is used from now on");

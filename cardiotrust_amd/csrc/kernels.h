@@ -34,12 +34,17 @@ hipError_t launch_propagate_step(hipStream_t s, const int32_t *nbr, const float 
 // holds at most kResMaxVoxels voxels; requires N < 2^17 - 1,
 // blocks * kResMaxVoxels >= N and blocks <= what fits on the device at once.
 // `abort` != 0 after the launch means the grid barrier timed out.
-constexpr int kResMaxVoxels = 500;
-struct GridBarrier {
+constexpr int kResMaxVoxels = 472;
+constexpr int kBarrierGroups = 8;
+struct alignas(128) BarrierWord {
     uint32_t count;
-    uint32_t generation;
-    uint32_t abort;
-    uint32_t pad;
+    uint32_t pad[31];
+};
+struct GridBarrier {
+    BarrierWord group[kBarrierGroups];
+    BarrierWord top;
+    BarrierWord generation;
+    BarrierWord abort;
 };
 size_t resident_lds_bytes();
 hipError_t resident_capacity(int32_t *blocks_per_cu);

@@ -151,34 +151,55 @@ __global__ __launch_bounds__(kPropThreads) void propagate_step_kernel(
 // ---------------------------------------------------------------------------
 constexpr uint32_t kBarrierSpinLimit = 1u << 21;
 
+// Two-level counting barrier. Workgroup b first counts into group b % 8, the
+// last arrival of a group counts into the top counter, and the last group
+// arrival publishes the next generation. Counters only grow within a launch
+// (they are zeroed before it), so "last" is recognised from the running
+// count and nothing has to be reset between steps.
 __device__ __forceinline__ bool grid_barrier(GridBarrier *gb, uint32_t nblocks, uint32_t &gen,
                                              int *ok_lds) {
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
-        __threadfence();  // publish this workgroup's state rows
-        const uint32_t arrived =
-            __hip_atomic_fetch_add(&gb->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == nblocks - 1) {
-            __hip_atomic_store(&gb->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&gb->generation, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
+        const uint32_t next = gen + 1;
+        const uint32_t grp = blockIdx.x % kBarrierGroups;
+        const uint32_t gsize = (nblocks - grp + kBarrierGroups - 1) / kBarrierGroups;
+        const uint32_t ngroups = nblocks < kBarrierGroups ? nblocks : kBarrierGroups;
+        // release: this workgroup's state rows reach device-coherent memory
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const uint32_t a1 = __hip_atomic_fetch_add(&gb->group[grp].count, 1u, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+        bool released = false;
+        if (a1 == next * gsize - 1) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            const uint32_t a2 = __hip_atomic_fetch_add(&gb->top.count, 1u, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+            if (a2 == next * ngroups - 1) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+                __hip_atomic_store(&gb->generation.count, next, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                released = true;
+            }
+        }
+        if (!released) {
             uint32_t spins = 0;
-            while (__hip_atomic_load(&gb->generation, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) ==
-                   gen) {
-                if (__hip_atomic_load(&gb->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            while (__hip_atomic_load(&gb->generation.count, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) < next) {
+                if (__hip_atomic_load(&gb->abort.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                     ok = 0;
                     break;
                 }
                 if (++spins > kBarrierSpinLimit) {
-                    __hip_atomic_store(&gb->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&gb->abort.count, 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
                     ok = 0;
                     break;
                 }
-                __builtin_amdgcn_s_sleep(2);
+                __builtin_amdgcn_s_sleep(1);
             }
         }
-        __threadfence();  // observe the other workgroups' rows
+        // acquire: drop stale cached copies of the other workgroups' rows
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         *ok_lds = ok;
     }
     gen += 1;
@@ -186,10 +207,10 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier *gb, uint32_t nblocks, 
     return *ok_lds != 0;
 }
 
-constexpr int kResThreads = 1024;
+constexpr int kResThreads = 512;
 constexpr int kResItems = (kResMaxVoxels * 26 + kResThreads - 1) / kResThreads;
 constexpr int kResStates = (kResMaxVoxels * 3 + kResThreads - 1) / kResThreads;
-constexpr int kResBatch = 7;
+constexpr int kResBatch = 9;
 constexpr uint32_t kNoConnection = 0xFFFFFFFFu;
 constexpr int kResNeighbourBits = 17;  // voxel index field; delay in the upper 15 bits
 
@@ -216,19 +237,29 @@ __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
     __syncthreads();
     uint32_t pk[kResItems];
     float ak[kResItems];
+    // pv = the neighbour states this connection read one step earlier, i.e.
+    // x[t-D-1] at step t: only x[t-D] is gathered per step.
+    float pv[kResItems][3];
 #pragma unroll
     for (int k = 0; k < kResItems; ++k) {
         const int it = tid + k * kResThreads;
         pk[k] = kNoConnection;
         ak[k] = 0.0f;
+#pragma unroll
+        for (int o = 0; o < 3; ++o) pv[k][o] = 0.0f;
         if (it < items) {
             const int64_t c = int64_t(vb) * 26 + it;
             const int32_t w = nbr[c];
             if (w >= 0) {
-                const uint32_t D = uint32_t(min(delays[c], 32767));
-                pk[k] = uint32_t(w) | (D << kResNeighbourBits);
+                const int32_t D = min(delays[c], 32767);
+                pk[k] = uint32_t(w) | (uint32_t(D) << kResNeighbourBits);
                 ak[k] = coefs[c];
                 atomicOr(&mask[it / 26], 1u << (it % 26));
+                if (D < t_begin) {
+                    const int64_t r1 = int64_t(t_begin - D - 1) * n + 3LL * w;
+#pragma unroll
+                    for (int o = 0; o < 3; ++o) pv[k][o] = x[r1 + o];
+                }
             }
         }
     }
@@ -239,26 +270,20 @@ __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
         const bool last = (t == T - 1);
 #pragma unroll
         for (int k0 = 0; k0 < kResItems; k0 += kResBatch) {
-            float in0[kResBatch][3], in1[kResBatch][3];
+            float in0[kResBatch][3];
 #pragma unroll
             for (int j = 0; j < kResBatch; ++j) {
                 const int k = k0 + j;
 #pragma unroll
-                for (int o = 0; o < 3; ++o) in0[j][o] = in1[j][o] = 0.0f;
+                for (int o = 0; o < 3; ++o) in0[j][o] = 0.0f;
                 if (k >= kResItems || pk[k] == kNoConnection) continue;
                 // 32-bit element offsets: the host only picks this kernel when T * n < 2^32
                 const uint32_t w = pk[k] & ((1u << kResNeighbourBits) - 1);
                 const int32_t D = int32_t(pk[k] >> kResNeighbourBits);
-                const uint32_t col = 3u * w;
                 if (D <= t) {
-                    const uint32_t r0 = uint32_t(t - D) * uint32_t(n) + col;
+                    const uint32_t r0 = uint32_t(t - D) * uint32_t(n) + 3u * w;
 #pragma unroll
                     for (int o = 0; o < 3; ++o) in0[j][o] = x[r0 + o];
-                }
-                if (D < t) {
-                    const uint32_t r1 = uint32_t(t - D - 1) * uint32_t(n) + col;
-#pragma unroll
-                    for (int o = 0; o < 3; ++o) in1[j][o] = x[r1 + o];
                 }
             }
 #pragma unroll
@@ -270,7 +295,8 @@ __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
                 for (int o = 0; o < 3; ++o) {
                     const float prev = ys[it * 3 + o];
                     if (last) y_last[(int64_t(vb) * 26 + it) * 3 + o] = prev;
-                    ys[it * 3 + o] = fmaf(ak[k], in0[j][o] - prev, in1[j][o]);
+                    ys[it * 3 + o] = fmaf(ak[k], in0[j][o] - prev, pv[k][o]);
+                    pv[k][o] = in0[j][o];
                 }
             }
         }

@@ -2,7 +2,7 @@
  * cardiotrust_hip.h -- C ABI of the MI355X (gfx950) CardioTrust epoch library.
  *
  * Drop-in replacement for the reference's OpenCL backend
- * (src/core/algorithm/gpu/*.rs). Each entry point names the reference
+ * (the src/core/algorithm/gpu module). Each entry point names the reference
  * interface it replaces (paths relative to the reference root). All pointers
  * are plain host pointers; every array is in the reference's row-major layout:
  *
