@@ -127,6 +127,11 @@ def main() -> int:
     steps_per_launch = prof["propagate_steps"] / launches
     bytes_per_launch = PROP_BYTES_PER_VOXEL_STEP * N * steps_per_launch
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "propagate_resident_cfg1_traffic.json")
+    if resident and N == 120120 and os.path.exists(tf):
+        # HBM bytes per step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+        traffic = json.load(open(tf))["bytes_per_step"] * steps_per_launch
     loss = results.update_from_gpu(["loss_batch"])["loss_batch"]
     log(f"loss_batch per epoch: {loss.tolist()}")
 
@@ -157,7 +162,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "avg_launch_ms": launch_ms,
             "steps_per_launch": steps_per_launch,
             "us_per_step": 1e3 * launch_ms / steps_per_launch,

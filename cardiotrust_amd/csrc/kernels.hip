@@ -584,8 +584,23 @@ struct ConnState {
     float A;
 };
 
+// Per-state factor of the gain derivative at one step (derivation.rs:315-326
+// with the maximum regularisation of derivation.rs:509-535): shared by the
+// 26 connections of a voxel, so it is computed once while staging.
+__device__ __forceinline__ void gain_factors(const float xo[3], const float e[3],
+                                             const DeriveParams &p, float cg[3]) {
+    const float sum = fabsf(xo[0]) + fabsf(xo[1]) + fabsf(xo[2]);
+    const bool on = sum > p.threshold;
+    const float fac = sum - p.threshold;
+#pragma unroll
+    for (int in = 0; in < 3; ++in) {
+        const float mr = on ? fac * signum(xo[in]) : 0.0f;
+        cg[in] = fmaf(e[in], p.mse_scaling, mr * p.reg_scaling);
+    }
+}
+
 __device__ __forceinline__ void derive_step(ConnState &c, const float in0[3], const float e[3],
-                                            const float xo[3], bool has, float a, int32_t D,
+                                            const float cg[3], bool has, float a, int32_t D,
                                             int32_t t, float reg, const DeriveParams &p) {
     float yl[3];
 #pragma unroll
@@ -594,16 +609,10 @@ __device__ __forceinline__ void derive_step(ConnState &c, const float in0[3], co
         if (has) c.y[o] = fmaf(a, in0[o] - c.y[o], c.prev_in[o]);
     }
     if (!p.freeze_gains) {
-        const float sum = fabsf(xo[0]) + fabsf(xo[1]) + fabsf(xo[2]);
-        const bool on = sum > p.threshold;
-        const float fac = sum - p.threshold;
 #pragma unroll
-        for (int in = 0; in < 3; ++in) {
-            const float mr = on ? fac * signum(xo[in]) : 0.0f;
-            const float cc = fmaf(e[in], p.mse_scaling, mr * p.reg_scaling);
+        for (int in = 0; in < 3; ++in)
 #pragma unroll
-            for (int o = 0; o < 3; ++o) c.g[in][o] += c.y[o] * cc;
-        }
+            for (int o = 0; o < 3; ++o) c.g[in][o] += c.y[o] * cg[in];
     }
     if (!p.freeze_delays) {
         if (!p.simple) {
@@ -632,14 +641,14 @@ __device__ __forceinline__ void derive_step(ConnState &c, const float in0[3], co
     for (int o = 0; o < 3; ++o) c.prev_in[o] = in0[o];
 }
 
-__global__ __launch_bounds__(kDeriveThreads) void derive_kernel(
+__global__ __launch_bounds__(kDeriveThreads, 4) void derive_kernel(
     const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
     const int32_t *__restrict__ delays, const float *__restrict__ initial_delays,
     const float *__restrict__ gT, const float *__restrict__ xT, const float *__restrict__ eT,
     float *__restrict__ fir, float *__restrict__ iir, float *__restrict__ dGT,
     float *__restrict__ dA, int32_t N, int32_t T, DeriveParams p) {
-    __shared__ float es[kDeriveMaxVoxels][3][kDeriveChunk];
-    __shared__ float xs[kDeriveMaxVoxels][3][kDeriveChunk];
+    __shared__ float es[kDeriveMaxVoxels][3][kDeriveChunk];  // mapped residuals
+    __shared__ float cs[kDeriveMaxVoxels][3][kDeriveChunk];  // gain-derivative factors
     const int64_t n = 3LL * N;
     const int64_t total = int64_t(N) * 26;
     const int64_t item0 = int64_t(xcd_block(blockIdx.x, gridDim.x)) * kDeriveThreads;
@@ -679,14 +688,24 @@ __global__ __launch_bounds__(kDeriveThreads) void derive_kernel(
     const float *xw = xT + (has ? 3LL * w : 0) * T;
 
     for (int32_t t0 = 0; t0 < T; t0 += kDeriveChunk) {
-        for (int idx = threadIdx.x; idx < nvox * 3 * kDeriveChunk; idx += kDeriveThreads) {
-            const int vv = idx / (3 * kDeriveChunk);
-            const int rem = idx - vv * 3 * kDeriveChunk;
-            const int comp = rem / kDeriveChunk, j = rem - comp * kDeriveChunk;
+        for (int idx = threadIdx.x; idx < nvox * kDeriveChunk; idx += kDeriveThreads) {
+            const int vv = idx / kDeriveChunk, j = idx - vv * kDeriveChunk;
             const int32_t t = t0 + j;
-            const int64_t row = 3LL * (vbeg + vv) + comp;
-            es[vv][comp][j] = t < T ? eT[row * T + t] : 0.0f;
-            xs[vv][comp][j] = t < T ? xT[row * T + t] : 0.0f;
+            const int64_t row = 3LL * (vbeg + vv) * T + t;
+            float e[3] = {0.0f, 0.0f, 0.0f}, xo[3] = {0.0f, 0.0f, 0.0f}, cg[3];
+            if (t < T) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    e[q] = eT[row + int64_t(q) * T];
+                    xo[q] = xT[row + int64_t(q) * T];
+                }
+            }
+            gain_factors(xo, e, p, cg);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                es[vv][q][j] = e[q];
+                cs[vv][q][j] = cg[q];
+            }
         }
         float xin[3][kDeriveChunk];
 #pragma unroll
@@ -703,8 +722,8 @@ __global__ __launch_bounds__(kDeriveThreads) void derive_kernel(
                 if (t0 + j < T) {
                     const float in0[3] = {xin[0][j], xin[1][j], xin[2][j]};
                     const float e[3] = {es[vl][0][j], es[vl][1][j], es[vl][2][j]};
-                    const float xo[3] = {xs[vl][0][j], xs[vl][1][j], xs[vl][2][j]};
-                    derive_step(c, in0, e, xo, has, a, D, t0 + j, reg, p);
+                    const float cg[3] = {cs[vl][0][j], cs[vl][1][j], cs[vl][2][j]};
+                    derive_step(c, in0, e, cg, has, a, D, t0 + j, reg, p);
                 }
             }
         }
