@@ -465,28 +465,37 @@ __global__ __launch_bounds__(256) void maxreg_partials_kernel(const float *__res
     if (threadIdx.x == 0) partial[int64_t(t) * gridDim.x + blockIdx.x] = red[0];
 }
 
+// Per-step sum of the workgroup partials (fixed tree order, one workgroup per
+// step), stored in loss_mr as a temporary.
+__global__ __launch_bounds__(256) void maxreg_step_sums_kernel(const float *__restrict__ partial,
+                                                               int32_t blocks,
+                                                               float *__restrict__ loss_mr) {
+    __shared__ float red[256];
+    const int32_t t = blockIdx.x;
+    float s = 0.0f;
+    for (int32_t k = threadIdx.x; k < blocks; k += 256) s += partial[int64_t(t) * blocks + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1) {
+        if (threadIdx.x < stride) red[threadIdx.x] += red[threadIdx.x + stride];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss_mr[t] = red[0];
+}
+
 // Running maximum-regularisation sum (cumulative over steps and beats,
 // derivation.rs:523) and the per-step losses (metrics.rs:191-195).
-__global__ __launch_bounds__(256) void maxreg_finalize_kernel(
-    const float *__restrict__ partial, int32_t blocks, float *__restrict__ mr_sum,
-    const float *__restrict__ loss_mse, float *__restrict__ loss_mr, float *__restrict__ loss,
-    float reg_strength, int32_t T) {
-    for (int32_t t = threadIdx.x; t < T; t += blockDim.x) {
-        float s = 0.0f;
-        for (int32_t k = 0; k < blocks; ++k) s += partial[int64_t(t) * blocks + k];
-        loss_mr[t] = s;
+__global__ void maxreg_prefix_kernel(float *__restrict__ mr_sum, const float *__restrict__ loss_mse,
+                                     float *__restrict__ loss_mr, float *__restrict__ loss,
+                                     float reg_strength, int32_t T) {
+    if (threadIdx.x != 0) return;
+    float cum = *mr_sum;
+    for (int32_t t = 0; t < T; ++t) {
+        cum = cum + loss_mr[t];
+        loss_mr[t] = cum;
+        loss[t] = fmaf(reg_strength, cum, loss_mse[t]);
     }
-    __threadfence_block();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float cum = *mr_sum;
-        for (int32_t t = 0; t < T; ++t) {
-            cum = cum + loss_mr[t];
-            loss_mr[t] = cum;
-            loss[t] = fmaf(reg_strength, cum, loss_mse[t]);
-        }
-        *mr_sum = cum;
-    }
+    *mr_sum = cum;
 }
 
 __global__ __launch_bounds__(256) void transpose_kernel(const float *__restrict__ in,
@@ -948,8 +957,10 @@ hipError_t launch_maxreg_partials(hipStream_t s, const float *x, float *partial,
 hipError_t launch_maxreg_finalize(hipStream_t s, const float *partial, int32_t blocks,
                                   float *mr_sum, const float *loss_mse, float *loss_mr, float *loss,
                                   float reg_strength, int32_t T) {
-    maxreg_finalize_kernel<<<1, 256, 0, s>>>(partial, blocks, mr_sum, loss_mse, loss_mr, loss,
-                                             reg_strength, T);
+    maxreg_step_sums_kernel<<<T, 256, 0, s>>>(partial, blocks, loss_mr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    maxreg_prefix_kernel<<<1, 64, 0, s>>>(mr_sum, loss_mse, loss_mr, loss, reg_strength, T);
     return hipGetLastError();
 }
 

@@ -594,9 +594,137 @@ int simulate(const ct_oracle_problem *pp) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Full reverse-mode gradient of one beat (the build's own derivation; no
+// reference counterpart). Objective, with s_mse = mse_strength / S and
+// s_reg = maximum_regularization_strength:
+//   J = sum_t [ s_mse/2 |H x_t - mhat_t|^2 + s_reg/2 sum_v relu(|x_{t,v}|_1 - theta)^2 ]
+// whose partial derivative in x_t is the reference's per-step factor
+// c_t = s_mse H^T r_t + s_reg mr_t (derivation.rs:315-326). Adjoint recursion,
+// for connection (v,d) reading neighbour w with delay D and coefficient a:
+//   xbar_t[3w+o] = c_t[3w+o] + sum_{(v,d): nbr(v,d)=w} (a ybar_{t+D}[v,d,o] + ybar_{t+D+1}[v,d,o])
+//   ybar_t[v,d,o] = sum_i G[3v+i,3d+o] xbar_t[3v+i] - a ybar_{t+1}[v,d,o]
+//   dJ/dG[3v+i,3d+o] = sum_t xbar_t[3v+i] y_t[v,d,o]
+//   dJ/da[v,d] = sum_t sum_o ybar_t[v,d,o] (xi_{t-D}[o] - y_{t-1}[v,d,o])
+// ---------------------------------------------------------------------------
+template <typename R>
+int adjoint(const ct_oracle_problem *pp, const ct_oracle_algorithm *c, int32_t beat, double *J_out,
+            R *dG, R *dA) {
+    Problem<R> q = view<R>(pp);
+    if (check(q) || beat < 0 || beat >= q.B) return -1;
+    const int64_t N = q.N, n = q.n, T = q.T, S = q.S;
+    const R s_mse = R(1) / R(S) * R(c->mse_strength);
+    const R s_reg = R(c->maximum_regularization_strength);
+    const R theta = R(c->maximum_regularization_threshold);
+    // forward, keeping every filter output (one per (v, d, out))
+    std::vector<R> x(T * n, R(0)), y((T + 1) * N * 78, R(0));  // y[t+1] = y_t, y[0] = y_{-1} = 0
+    for (int64_t t = 0; t < T; ++t) {
+        R *yt = y.data() + (t + 1) * N * 78;
+        const R *yp = y.data() + t * N * 78;
+        for (int64_t v = 0; v < N; ++v)
+            for (int64_t d = 0; d < 26; ++d) {
+                const int32_t first = q.osi[(3 * v) * 78 + 3 * d];
+                if (first < 0) continue;
+                const R a = q.coefs[v * 26 + d];
+                const int64_t D = q.delays[v * 26 + d];
+                for (int o = 0; o < 3; ++o) {
+                    const R i0 = D <= t ? x[(t - D) * n + first + o] : R(0);
+                    const R i1 = D < t ? x[(t - D - 1) * n + first + o] : R(0);
+                    yt[(v * 26 + d) * 3 + o] = std::fma(a, i0 - yp[(v * 26 + d) * 3 + o], i1);
+                }
+            }
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t v = i / 3;
+            R acc = 0;
+            for (int64_t g = 0; g < 78; ++g) {
+                if (q.osi[i * 78 + g] < 0) continue;
+                acc += q.gains[i * 78 + g] * yt[v * 78 + g];
+            }
+            x[t * n + i] = acc + q.control_function[t] * q.control_matrix[i];
+        }
+    }
+    // objective and per-step factors c_t
+    const R *Hb = q.H + beat * S * n;
+    const R *mh = q.mhat + beat * T * S;
+    std::vector<R> cst(T * n), r(S);
+    double J = 0;
+    for (int64_t t = 0; t < T; ++t) {
+        for (int64_t s = 0; s < S; ++s) {
+            R m = 0;
+            for (int64_t i = 0; i < n; ++i) m += Hb[s * n + i] * x[t * n + i];
+            r[s] = m - mh[t * S + s];
+            J += 0.5 * double(s_mse) * double(r[s]) * double(r[s]);
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            R e = 0;
+            for (int64_t s = 0; s < S; ++s) e += Hb[s * n + i] * r[s];
+            cst[t * n + i] = e * s_mse;
+        }
+        for (int64_t v = 0; v < N; ++v) {
+            const R *xv = x.data() + t * n + 3 * v;
+            const R sum = std::fabs(xv[0]) + std::fabs(xv[1]) + std::fabs(xv[2]);
+            if (sum > theta) {
+                const R f = sum - theta;
+                J += 0.5 * double(s_reg) * double(f) * double(f);
+                for (int k = 0; k < 3; ++k) cst[t * n + 3 * v + k] += s_reg * f * signum(xv[k]);
+            }
+        }
+    }
+    // backward
+    std::vector<R> xbar(n), ybar((T + 1) * N * 78, R(0));  // ybar[t] = ybar_t, ybar[T] = 0
+    for (int64_t k = 0; k < n * 78; ++k) dG[k] = 0;
+    for (int64_t k = 0; k < N * 26; ++k) dA[k] = 0;
+    for (int64_t t = T - 1; t >= 0; --t) {
+        for (int64_t i = 0; i < n; ++i) xbar[i] = cst[t * n + i];
+        for (int64_t v = 0; v < N; ++v)
+            for (int64_t d = 0; d < 26; ++d) {
+                const int32_t first = q.osi[(3 * v) * 78 + 3 * d];
+                if (first < 0) continue;
+                const R a = q.coefs[v * 26 + d];
+                const int64_t D = q.delays[v * 26 + d];
+                for (int o = 0; o < 3; ++o) {
+                    R contrib = 0;
+                    if (t + D < T) contrib += a * ybar[(t + D) * N * 78 + (v * 26 + d) * 3 + o];
+                    if (t + D + 1 < T) contrib += ybar[(t + D + 1) * N * 78 + (v * 26 + d) * 3 + o];
+                    xbar[first + o] += contrib;
+                }
+            }
+        for (int64_t v = 0; v < N; ++v)
+            for (int64_t d = 0; d < 26; ++d) {
+                const int32_t first = q.osi[(3 * v) * 78 + 3 * d];
+                if (first < 0) continue;
+                const R a = q.coefs[v * 26 + d];
+                const int64_t D = q.delays[v * 26 + d];
+                for (int o = 0; o < 3; ++o) {
+                    R yb = -a * ybar[(t + 1) * N * 78 + (v * 26 + d) * 3 + o];
+                    for (int in = 0; in < 3; ++in)
+                        yb += q.gains[(3 * v + in) * 78 + 3 * d + o] * xbar[3 * v + in];
+                    ybar[t * N * 78 + (v * 26 + d) * 3 + o] = yb;
+                    const R yt = y[(t + 1) * N * 78 + (v * 26 + d) * 3 + o];
+                    const R ytm1 = y[t * N * 78 + (v * 26 + d) * 3 + o];
+                    const R xi = D <= t ? x[(t - D) * n + first + o] : R(0);
+                    for (int in = 0; in < 3; ++in)
+                        dG[(3 * v + in) * 78 + 3 * d + o] += xbar[3 * v + in] * yt;
+                    dA[v * 26 + d] += yb * (xi - ytm1);
+                }
+            }
+    }
+    *J_out = J;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int ct_oracle_adjoint_f64(ct_oracle_problem *p, const ct_oracle_algorithm *cfg, int32_t beat,
+                          double *J, double *d_gains, double *d_coefs) {
+    return adjoint<double>(p, cfg, beat, J, d_gains, d_coefs);
+}
+int ct_oracle_adjoint_f32(ct_oracle_problem *p, const ct_oracle_algorithm *cfg, int32_t beat,
+                          double *J, float *d_gains, float *d_coefs) {
+    return adjoint<float>(p, cfg, beat, J, d_gains, d_coefs);
+}
 
 int ct_oracle_run_epoch_f32(ct_oracle_problem *p, const ct_oracle_algorithm *cfg,
                             const int32_t *beat_order, int32_t *batch_index) {

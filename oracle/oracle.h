@@ -104,6 +104,17 @@ int ct_oracle_run_epoch_f64(ct_oracle_problem *p, const ct_oracle_algorithm *cfg
 int ct_oracle_simulate_f32(ct_oracle_problem *p);
 int ct_oracle_simulate_f64(ct_oracle_problem *p);
 
+/* Full reverse-mode gradient of one beat (no reference counterpart): the
+ * objective J (double) and dJ/dgains (3N,78), dJ/dcoefs (N,26) in the
+ * problem's precision. J = sum_t [ s_mse/2 |H x_t - mhat_t|^2
+ * + s_reg/2 sum_v relu(|x_{t,v}|_1 - threshold)^2 ], s_mse = mse_strength/S,
+ * s_reg = maximum_regularization_strength. Uses the problem's parameters,
+ * control and measurements; state arrays are not touched. */
+int ct_oracle_adjoint_f64(ct_oracle_problem *p, const ct_oracle_algorithm *cfg, int32_t beat,
+                          double *J, double *d_gains, double *d_coefs);
+int ct_oracle_adjoint_f32(ct_oracle_problem *p, const ct_oracle_algorithm *cfg, int32_t beat,
+                          double *J, float *d_gains, float *d_coefs);
+
 /* Scalar helpers (allpass.rs:532-668). Index helpers return -1 for None. */
 int32_t ct_oracle_offset_to_gain_index(int32_t x, int32_t y, int32_t z, int32_t out);
 int32_t ct_oracle_gain_index_to_offset(int32_t gain_index, int32_t *xyz_out4);

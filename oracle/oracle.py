@@ -76,6 +76,11 @@ def lib():
             f = getattr(L, name)
             f.argtypes = [P(_Problem)]
             f.restype = ctypes.c_int
+        for name in ("ct_oracle_adjoint_f64", "ct_oracle_adjoint_f32"):
+            f = getattr(L, name)
+            f.argtypes = [P(_Problem), P(_Algorithm), ctypes.c_int32, P(ctypes.c_double),
+                          ctypes.c_void_p, ctypes.c_void_p]
+            f.restype = ctypes.c_int
         L.ct_oracle_offset_to_gain_index.argtypes = [ctypes.c_int32] * 4
         L.ct_oracle_offset_to_gain_index.restype = ctypes.c_int32
         L.ct_oracle_gain_index_to_offset.argtypes = [ctypes.c_int32, P(ctypes.c_int32)]
@@ -111,6 +116,16 @@ def lib():
 
 def _ptr(a: Optional[np.ndarray]):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _algorithm(config) -> _Algorithm:
+    return _Algorithm(float(config.learning_rate), int(config.batch_size), int(config.optimizer),
+                      float(config.mse_strength), float(config.maximum_regularization_strength),
+                      float(config.maximum_regularization_threshold),
+                      float(config.difference_regularization_strength),
+                      float(config.smoothness_regularization_strength),
+                      int(bool(config.freeze_gains)), int(bool(config.freeze_delays)),
+                      int(config.ap_derivative))
 
 
 class OracleResults:
@@ -176,13 +191,7 @@ class OracleResults:
     def run_epoch(self, config, beat_order=None) -> None:
         """One epoch (`algorithm.rs:124-281`); `beat_order` replaces the
         reference's unseeded shuffle."""
-        a = _Algorithm(float(config.learning_rate), int(config.batch_size), int(config.optimizer),
-                       float(config.mse_strength), float(config.maximum_regularization_strength),
-                       float(config.maximum_regularization_threshold),
-                       float(config.difference_regularization_strength),
-                       float(config.smoothness_regularization_strength),
-                       int(bool(config.freeze_gains)), int(bool(config.freeze_delays)),
-                       int(config.ap_derivative))
+        a = _algorithm(config)
         order = None
         if beat_order is not None:
             order = np.ascontiguousarray(beat_order, dtype=np.int32)
@@ -191,6 +200,20 @@ class OracleResults:
         rc = f(ctypes.byref(s), ctypes.byref(a), _ptr(order), ctypes.byref(self.batch_index))
         if rc != 0:
             raise RuntimeError(f"oracle run_epoch failed ({rc})")
+
+    def adjoint(self, config, beat: int = 0):
+        """Objective J and its exact gradient (dJ/dgains (3N,78), dJ/dcoefs
+        (N,26)) for one beat, by reverse mode (see oracle.h)."""
+        a = _algorithm(config)
+        J = ctypes.c_double()
+        dG = np.zeros((3 * self.N, 78), dtype=self.dtype)
+        dA = np.zeros((self.N, 26), dtype=self.dtype)
+        s = self._struct()
+        f = lib().ct_oracle_adjoint_f64 if self.dtype == np.float64 else lib().ct_oracle_adjoint_f32
+        rc = f(ctypes.byref(s), ctypes.byref(a), int(beat), ctypes.byref(J), _ptr(dG), _ptr(dA))
+        if rc != 0:
+            raise RuntimeError(f"oracle adjoint failed ({rc})")
+        return J.value, dG, dA
 
     def simulate(self) -> np.ndarray:
         """Forward-only pass over every beat; returns measurements (B,T,S)."""
