@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libcardiotrust_hip.so")
 PROBE = os.path.join(HERE, "copy_probe")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("kernels.hip", "api.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("kernels.hip", "adjoint.hip", "api.cpp")]
 ARCH = os.environ.get("CT_OFFLOAD_ARCH", "gfx950")
 
 
@@ -35,7 +35,7 @@ def build(verbose: bool = False) -> str:
         "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-Wno-unused-command-line-argument",
         "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
-        "-x", "hip", SOURCES[0], "-x", "hip", SOURCES[1], "-o", tmp,
+        *[a for src in SOURCES for a in ("-x", "hip", src)], "-o", tmp,
     ]
     if verbose:
         print(" ".join(cmd), flush=True)

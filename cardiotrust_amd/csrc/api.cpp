@@ -86,7 +86,16 @@ struct ct_results_gpu {
     int32_t resident_blocks = 0;
     ct::GridBarrier *barrier = nullptr;
     // scratch
-    float *partial = nullptr, *mr_partial = nullptr, *staging = nullptr;
+    float *partial = nullptr, *mr_partial = nullptr, *staging = nullptr, *step_mr = nullptr;
+    double *objective = nullptr;  // J of the last forward (adjoint objective)
+    // reverse-mode gradient: host neighbour table (for the consumer lists) and
+    // buffers allocated on first use
+    std::vector<int32_t> nbr_host;
+    int32_t *cons_ptr = nullptr, *cons_idx = nullptr, *dmax = nullptr;
+    float *E = nullptr, *xbar = nullptr, *xbarT = nullptr, *ybar = nullptr;
+    float *qring = nullptr, *ckpt = nullptr, *yseg = nullptr;
+    int64_t qring_slots = 0, ckpt_segments = 0, yseg_steps = 0;
+    int32_t forward_beat = -1;  // beat whose forward pass the device state holds
     int32_t splits = 1, mr_blocks = 1;
     int64_t kps = ct::kGemmTileK;
 
@@ -116,6 +125,8 @@ struct ct_epoch_kernel {
     std::vector<int32_t> order;
     int32_t batch_index = 0;
     bool profiling = false;
+    int32_t gradient = 0;            // 0 = reference derivation, 1 = reverse-mode adjoint
+    int32_t checkpoint_interval = 64;
     std::vector<hipEvent_t> events;
     size_t used_events = 0;
     struct SpanRec {
@@ -229,6 +240,7 @@ int reset_derivatives(ct_results_gpu &r) {
 // per-step MSE.
 int forward_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_kernel *k) {
     hipStream_t s = r.gpu->stream;
+    r.forward_beat = beat;
     CT_HIP(hipMemsetAsync(r.y, 0, sizeof(float) * r.N * 78, s));
     CT_HIP(hipMemsetAsync(r.y_last, 0, sizeof(float) * r.N * 78, s));
     const bool resident = r.resident_ok && r.propagation_mode != 1;
@@ -257,43 +269,57 @@ int forward_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_ke
     return CT_OK;
 }
 
-int run_beat(ct_epoch_kernel &k, int32_t beat) {
-    ct_results_gpu &r = *k.r;
+// Per-step maximum regularisation sums, the running sum and the per-step
+// losses (derivation.rs:509-535, metrics.rs:180-196), and the adjoint
+// objective J of the beat.
+int losses_beat(ct_results_gpu &r, const ct_algorithm &c, ct_epoch_kernel *k) {
     hipStream_t s = r.gpu->stream;
-    const ct_algorithm &c = k.cfg;
-    CT_CHECK(forward_beat(r, beat, k.mhat->data, &k));
-    {
-        Span sp(&k, kRegularize);
-        CT_HIP(ct::launch_maxreg_partials(s, r.x, r.mr_partial, r.mr_last, r.N, r.T,
-                                          c.maximum_regularization_threshold, r.mr_blocks));
-        CT_HIP(ct::launch_maxreg_finalize(s, r.mr_partial, r.mr_blocks, r.mr_sum, r.loss_mse,
-                                          r.loss_mr, r.loss, c.maximum_regularization_strength,
-                                          r.T));
-    }
+    Span sp(k, kRegularize);
+    CT_HIP(ct::launch_maxreg_partials(s, r.x, r.mr_partial, r.mr_last, r.N, r.T,
+                                      c.maximum_regularization_threshold, r.mr_blocks));
+    CT_HIP(ct::launch_maxreg_finalize(s, r.mr_partial, r.mr_blocks, r.step_mr, r.mr_sum,
+                                      r.loss_mse, r.loss_mr, r.loss, r.objective,
+                                      c.mse_strength, c.maximum_regularization_strength, r.T));
+    return CT_OK;
+}
+
+// The reference's per-step derivation over the whole beat (derivation.rs:173-229).
+int derive_reference(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, ct_epoch_kernel *k) {
+    hipStream_t s = r.gpu->stream;
     const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
     {
-        Span sp(&k, kMapResiduals);
+        Span sp(k, kMapResiduals);
         CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.eT, r.n, r.T, r.Sp));
     }
     if (c.freeze_gains && c.freeze_delays) return CT_OK;
     {
-        Span sp(&k, kTranspose);
+        Span sp(k, kTranspose);
         CT_HIP(ct::launch_transpose(s, r.x, r.xT, r.T, r.n));
     }
-    {
-        Span sp(&k, kDerive);
-        ct::DeriveParams p;
-        p.mse_scaling = 1.0f / float(r.S) * c.mse_strength;
-        p.reg_scaling = c.maximum_regularization_strength;
-        p.threshold = c.maximum_regularization_threshold;
-        p.diff_strength = c.difference_regularization_strength;
-        p.freeze_gains = c.freeze_gains;
-        p.freeze_delays = c.freeze_delays;
-        p.simple = c.ap_derivative == 1;
-        CT_HIP(ct::launch_derive(s, r.nbr, r.coefs, r.delays, r.initial_delays, r.gT, r.xT, r.eT,
-                                 r.fir, r.iir, r.dGT, r.dA, r.N, r.T, p));
-    }
+    Span sp(k, kDerive);
+    ct::DeriveParams p;
+    p.mse_scaling = 1.0f / float(r.S) * c.mse_strength;
+    p.reg_scaling = c.maximum_regularization_strength;
+    p.threshold = c.maximum_regularization_threshold;
+    p.diff_strength = c.difference_regularization_strength;
+    p.freeze_gains = c.freeze_gains;
+    p.freeze_delays = c.freeze_delays;
+    p.simple = c.ap_derivative == 1;
+    CT_HIP(ct::launch_derive(s, r.nbr, r.coefs, r.delays, r.initial_delays, r.gT, r.xT, r.eT,
+                             r.fir, r.iir, r.dGT, r.dA, r.N, r.T, p));
     return CT_OK;
+}
+
+int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32_t K,
+                   ct_epoch_kernel *k);
+
+int run_beat(ct_epoch_kernel &k, int32_t beat) {
+    ct_results_gpu &r = *k.r;
+    CT_CHECK(forward_beat(r, beat, k.mhat->data, &k));
+    CT_CHECK(losses_beat(r, k.cfg, &k));
+    if (k.gradient == 1 && !(k.cfg.freeze_gains && k.cfg.freeze_delays))
+        return derive_adjoint(r, beat, k.cfg, k.checkpoint_interval, &k);
+    return derive_reference(r, beat, k.cfg, &k);
 }
 
 // algorithm.rs:188-212 / 224-247 / 255-278: average delays, batch derivatives,
@@ -382,6 +408,101 @@ int check_resident(ct_results_gpu &r) {
         return fail(CT_ERR_DEVICE, "resident propagation: grid barrier timed out (workgroups not "
                                    "all resident); the per-step path is used from now on");
     }
+    return CT_OK;
+}
+
+// Grows a lazily allocated adjoint buffer to `count` elements.
+template <typename V>
+int ensure(ct_results_gpu &r, V **p, int64_t &have, int64_t count, int64_t unit) {
+    if (*p && have >= count) return CT_OK;
+    if (*p) {
+        auto it = std::find(r.allocations.begin(), r.allocations.end(), static_cast<void *>(*p));
+        if (it != r.allocations.end()) r.allocations.erase(it);
+        CT_HIP(hipStreamSynchronize(r.gpu->stream));
+        CT_HIP(hipFree(*p));
+        *p = nullptr;
+    }
+    CT_CHECK(r.alloc(p, std::max<int64_t>(count, 1) * unit));
+    have = count;
+    return CT_OK;
+}
+
+// Consumer lists (CSR): for each voxel w, the connections (v, d) that read
+// it, in ascending connection order.
+int build_consumers(ct_results_gpu &r) {
+    if (r.cons_ptr) return CT_OK;
+    const int64_t N = r.N;
+    std::vector<int32_t> ptr(N + 1, 0), idx;
+    for (int64_t c = 0; c < N * 26; ++c)
+        if (r.nbr_host[c] >= 0) ptr[r.nbr_host[c] + 1] += 1;
+    for (int64_t w = 0; w < N; ++w) ptr[w + 1] += ptr[w];
+    idx.resize(ptr[N]);
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t c = 0; c < N * 26; ++c)
+        if (r.nbr_host[c] >= 0) idx[fill[r.nbr_host[c]]++] = int32_t(c);
+    CT_CHECK(r.alloc(&r.cons_ptr, N + 1));
+    CT_CHECK(r.alloc(&r.cons_idx, std::max<int64_t>(int64_t(idx.size()), 1)));
+    CT_CHECK(upload(r.cons_ptr, ptr.data(), sizeof(int32_t) * (N + 1)));
+    CT_CHECK(upload(r.cons_idx, idx.data(), sizeof(int32_t) * idx.size()));
+    return CT_OK;
+}
+
+// Reverse-mode gradient of the forward pass held on the device (adjoint.hip);
+// adds dJ/dG and dJ/da of this beat to the derivative accumulators.
+int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32_t K,
+                   ct_epoch_kernel *k) {
+    if (r.N <= 0) return CT_OK;
+    if (K < 1) return fail(CT_ERR_INVALID, "checkpoint interval must be >= 1");
+    hipStream_t s = r.gpu->stream;
+    const int64_t n = r.n, T = r.T, M = int64_t(r.N) * 78;
+    CT_CHECK(build_consumers(r));
+    int64_t have_e = r.E ? n * T : 0, have_xb = r.xbar ? n * T : 0, have_xbt = r.xbarT ? n * T : 0,
+            have_yb = r.ybar ? M : 0;
+    CT_CHECK(ensure(r, &r.E, have_e, n * T, 1));
+    CT_CHECK(ensure(r, &r.xbar, have_xb, n * T, 1));
+    CT_CHECK(ensure(r, &r.xbarT, have_xbt, n * T, 1));
+    CT_CHECK(ensure(r, &r.ybar, have_yb, M, 1));
+    // ring of q over R >= Dmax + 1 slots (R a power of two)
+    CT_HIP(ct::launch_max_delay(s, r.nbr, r.delays, int64_t(r.N) * 26, r.dmax));
+    int32_t dmax = 0;
+    CT_HIP(hipMemcpyAsync(&dmax, r.dmax, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    CT_HIP(hipStreamSynchronize(s));
+    int64_t slots = 1;
+    while (slots < int64_t(dmax) + 1) slots <<= 1;
+    int64_t have_q = r.qring_slots * M;
+    CT_CHECK(ensure(r, &r.qring, have_q, slots * M, 1));
+    r.qring_slots = std::max(r.qring_slots, slots);
+    const int64_t nseg = (T + K - 1) / K;
+    int64_t have_c = r.ckpt_segments * M, have_y = r.yseg_steps * M;
+    CT_CHECK(ensure(r, &r.ckpt, have_c, std::max<int64_t>(nseg - 1, 1) * M, 1));
+    r.ckpt_segments = std::max(r.ckpt_segments, std::max<int64_t>(nseg - 1, 1));
+    CT_CHECK(ensure(r, &r.yseg, have_y, int64_t(K) * M, 1));
+    r.yseg_steps = std::max(r.yseg_steps, int64_t(K));
+
+    const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
+    {
+        Span sp(k, kMapResiduals);
+        CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.eT, n, r.T, r.Sp));
+    }
+    {
+        Span sp(k, kTranspose);
+        CT_HIP(ct::launch_transpose(s, r.eT, r.E, n, T));      // mapped residuals by step
+        CT_HIP(ct::launch_transpose(s, r.gT, r.staging, 78, n));  // gains by state row
+    }
+    Span sp(k, kDerive);
+    ct::AdjointParams p;
+    p.mse_scaling = 1.0f / float(r.S) * c.mse_strength;
+    p.reg_scaling = c.maximum_regularization_strength;
+    p.threshold = c.maximum_regularization_threshold;
+    CT_HIP(hipMemsetAsync(r.ybar, 0, sizeof(float) * M, s));
+    for (int32_t step = r.T - 1; step >= 0; --step)
+        CT_HIP(ct::launch_adjoint_step(s, r.nbr, r.coefs, r.delays, r.staging, r.cons_ptr,
+                                       r.cons_idx, r.x, r.E, r.ybar, r.qring, r.xbar, r.N, r.T,
+                                       step, uint32_t(slots - 1), p));
+    CT_HIP(ct::launch_transpose(s, r.xbar, r.xbarT, T, n));
+    CT_HIP(ct::launch_transpose(s, r.x, r.xT, T, n));
+    CT_HIP(ct::launch_adjoint_gradient(s, r.nbr, r.coefs, r.delays, r.gT, r.xT, r.xbarT, r.ckpt,
+                                       r.yseg, r.dGT, r.dA, r.N, r.T, K));
     return CT_OK;
 }
 
@@ -490,6 +611,7 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
     r->nbatch = d.num_batch_entries;
     r->optimizer = d.optimizer;
     r->control_function.assign(d.control_function, d.control_function + d.num_steps);
+    r->nbr_host = nbr;
     const int64_t n = r->n, T = r->T, S = r->S, Sp = r->Sp, B = r->B;
 
     // GEMM split-K plan: about 2048 workgroups for the projection.
@@ -544,6 +666,9 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
     ALLOC(r->partial, splits * T * Sp);
     ALLOC(r->mr_partial, T * r->mr_blocks);
     ALLOC(r->staging, n * 78);
+    ALLOC(r->step_mr, T);
+    ALLOC(r->objective, 1);
+    ALLOC(r->dmax, 1);
     ALLOC(r->u, T);
     ALLOC(r->barrier, 1);
 #undef ALLOC
@@ -743,6 +868,51 @@ int ct_results_set_propagation(ct_results_gpu *results, int32_t mode, int32_t *r
         return fail(CT_ERR_INVALID, "the resident propagation kernel cannot hold this problem");
     results->propagation_mode = mode;
     if (resident) *resident = (results->resident_ok && mode != 1) ? 1 : 0;
+    return CT_OK;
+}
+
+int ct_epoch_kernel_set_gradient(ct_epoch_kernel *k, int32_t mode, int32_t checkpoint_interval) {
+    if (!k) return fail(CT_ERR_INVALID, "null argument");
+    if (mode < 0 || mode > 1) return fail(CT_ERR_INVALID, "gradient mode must be 0 or 1");
+    if (checkpoint_interval < 1) return fail(CT_ERR_INVALID, "checkpoint interval must be >= 1");
+    k->gradient = mode;
+    k->checkpoint_interval = checkpoint_interval;
+    return CT_OK;
+}
+
+int ct_results_forward(ct_results_gpu *results, const ct_buffer *actual, int32_t beat,
+                       const ct_algorithm *config, double *objective) {
+    if (!results || !actual || !config) return fail(CT_ERR_INVALID, "null argument");
+    ct_results_gpu &r = *results;
+    CT_CHECK(check_gpu(r.gpu));
+    if (beat < 0 || beat >= r.B) return fail(CT_ERR_INVALID, "beat out of range");
+    if (actual->beats != r.B || actual->steps != r.T || actual->sensors != r.S)
+        return fail(CT_ERR_INVALID, "actual measurements are not (B, T, S) of the results");
+    CT_HIP(hipMemsetAsync(r.mr_sum, 0, sizeof(float), r.gpu->stream));
+    CT_CHECK(forward_beat(r, beat, actual->data, nullptr));
+    CT_CHECK(losses_beat(r, *config, nullptr));
+    CT_CHECK(check_resident(r));
+    CT_HIP(hipStreamSynchronize(r.gpu->stream));
+    if (objective) CT_CHECK(download(objective, r.objective, sizeof(double)));
+    return CT_OK;
+}
+
+int ct_results_backward(ct_results_gpu *results, const ct_buffer *actual, int32_t beat,
+                        const ct_algorithm *config, int32_t truncate,
+                        int32_t checkpoint_interval) {
+    if (!results || !actual || !config) return fail(CT_ERR_INVALID, "null argument");
+    ct_results_gpu &r = *results;
+    CT_CHECK(check_gpu(r.gpu));
+    if (r.forward_beat != beat)
+        return fail(CT_ERR_STATE, "backward needs the forward pass of the same beat first");
+    CT_CHECK(reset_derivatives(r));
+    if (truncate) {
+        ct_algorithm c = *config;
+        CT_CHECK(derive_reference(r, beat, c, nullptr));
+    } else {
+        CT_CHECK(derive_adjoint(r, beat, *config, checkpoint_interval, nullptr));
+    }
+    CT_HIP(hipStreamSynchronize(r.gpu->stream));
     return CT_OK;
 }
 

@@ -64,8 +64,9 @@ hipError_t launch_maxreg_partials(hipStream_t s, const float *x, float *partial,
                                   int32_t N, int32_t T, float threshold, int32_t blocks);
 
 hipError_t launch_maxreg_finalize(hipStream_t s, const float *partial, int32_t blocks,
-                                  float *mr_sum, const float *loss_mse, float *loss_mr, float *loss,
-                                  float reg_strength, int32_t T);
+                                  float *step_mr, float *mr_sum, const float *loss_mse,
+                                  float *loss_mr, float *loss, double *objective,
+                                  float mse_strength, float reg_strength, int32_t T);
 
 hipError_t launch_transpose(hipStream_t s, const float *in, float *out, int64_t rows, int64_t cols);
 
@@ -76,6 +77,24 @@ hipError_t launch_derive(hipStream_t s, const int32_t *nbr, const float *coefs,
                          const int32_t *delays, const float *initial_delays, const float *gT,
                          const float *xT, const float *eT, float *fir, float *iir, float *dGT,
                          float *dA, int32_t N, int32_t T, const DeriveParams &p);
+
+// Reverse-mode gradient (adjoint.hip).
+struct AdjointParams {
+    float mse_scaling;
+    float reg_scaling;
+    float threshold;
+};
+hipError_t launch_max_delay(hipStream_t s, const int32_t *nbr, const int32_t *delays, int64_t len,
+                            int32_t *out);
+hipError_t launch_adjoint_step(hipStream_t s, const int32_t *nbr, const float *coefs,
+                               const int32_t *delays, const float *gains_rows,
+                               const int32_t *cons_ptr, const int32_t *cons_idx, const float *x,
+                               const float *E, float *ybar, float *qring, float *xbar, int32_t N,
+                               int32_t T, int32_t step, uint32_t rmask, const AdjointParams &p);
+hipError_t launch_adjoint_gradient(hipStream_t s, const int32_t *nbr, const float *coefs,
+                                   const int32_t *delays, const float *gT, const float *xT,
+                                   const float *xbarT, float *ckpt, float *yseg, float *dGT,
+                                   float *dA, int32_t N, int32_t T, int32_t K);
 
 hipError_t launch_sgd(hipStream_t s, float *v, const float *d, int64_t len, float scale);
 hipError_t launch_adam(hipStream_t s, float *v, const float *d, float *m1, float *m2, int64_t len,

@@ -8,33 +8,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include "device.h"
+
 namespace ct {
 namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
-
-__device__ __forceinline__ float signum(float v) {
-    // Rust f32::signum: NaN stays NaN, -0 -> -1, +0 -> +1.
-    return isnan(v) ? v : copysignf(1.0f, v);
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
-// L2). Remap the launch order so that XCD k runs one contiguous range of
-// blocks: neighbouring voxels, which read the same state rows, then share an
-// L2.
-constexpr int kNumXcds = 8;
-__device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t nb) {
-    const int32_t xcd = b % kNumXcds, local = b / kNumXcds;
-    const int32_t q = nb / kNumXcds, r = nb % kNumXcds;
-    return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
-}
-
-__device__ __forceinline__ float powi5(float v) {
-    // powi(x, 5) as expanded by LLVM: x * (x^2)^2.
-    const float sq = v * v;
-    const float q = sq * sq;
-    return v * q;
-}
 
 // ---------------------------------------------------------------------------
 // Propagation: one launch per time step (prediction.rs:40-105).
@@ -466,10 +445,10 @@ __global__ __launch_bounds__(256) void maxreg_partials_kernel(const float *__res
 }
 
 // Per-step sum of the workgroup partials (fixed tree order, one workgroup per
-// step), stored in loss_mr as a temporary.
+// step).
 __global__ __launch_bounds__(256) void maxreg_step_sums_kernel(const float *__restrict__ partial,
                                                                int32_t blocks,
-                                                               float *__restrict__ loss_mr) {
+                                                               float *__restrict__ step_mr) {
     __shared__ float red[256];
     const int32_t t = blockIdx.x;
     float s = 0.0f;
@@ -480,22 +459,30 @@ __global__ __launch_bounds__(256) void maxreg_step_sums_kernel(const float *__re
         if (threadIdx.x < stride) red[threadIdx.x] += red[threadIdx.x + stride];
         __syncthreads();
     }
-    if (threadIdx.x == 0) loss_mr[t] = red[0];
+    if (threadIdx.x == 0) step_mr[t] = red[0];
 }
 
 // Running maximum-regularisation sum (cumulative over steps and beats,
 // derivation.rs:523) and the per-step losses (metrics.rs:191-195).
-__global__ void maxreg_prefix_kernel(float *__restrict__ mr_sum, const float *__restrict__ loss_mse,
+// Also the objective of the reverse-mode gradient (in double):
+// J = sum_t mse_strength/2 * loss_mse[t] + reg_strength/2 * step_mr[t].
+__global__ void maxreg_prefix_kernel(const float *__restrict__ step_mr, float *__restrict__ mr_sum,
+                                     const float *__restrict__ loss_mse,
                                      float *__restrict__ loss_mr, float *__restrict__ loss,
+                                     double *__restrict__ objective, float mse_strength,
                                      float reg_strength, int32_t T) {
     if (threadIdx.x != 0) return;
     float cum = *mr_sum;
+    double J = 0.0;
     for (int32_t t = 0; t < T; ++t) {
-        cum = cum + loss_mr[t];
+        cum = cum + step_mr[t];
         loss_mr[t] = cum;
         loss[t] = fmaf(reg_strength, cum, loss_mse[t]);
+        J += 0.5 * double(mse_strength) * double(loss_mse[t]) +
+             0.5 * double(reg_strength) * double(step_mr[t]);
     }
     *mr_sum = cum;
+    *objective = J;
 }
 
 __global__ __launch_bounds__(256) void transpose_kernel(const float *__restrict__ in,
@@ -955,12 +942,14 @@ hipError_t launch_maxreg_partials(hipStream_t s, const float *x, float *partial,
 }
 
 hipError_t launch_maxreg_finalize(hipStream_t s, const float *partial, int32_t blocks,
-                                  float *mr_sum, const float *loss_mse, float *loss_mr, float *loss,
-                                  float reg_strength, int32_t T) {
-    maxreg_step_sums_kernel<<<T, 256, 0, s>>>(partial, blocks, loss_mr);
+                                  float *step_mr, float *mr_sum, const float *loss_mse,
+                                  float *loss_mr, float *loss, double *objective,
+                                  float mse_strength, float reg_strength, int32_t T) {
+    maxreg_step_sums_kernel<<<T, 256, 0, s>>>(partial, blocks, step_mr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    maxreg_prefix_kernel<<<1, 64, 0, s>>>(mr_sum, loss_mse, loss_mr, loss, reg_strength, T);
+    maxreg_prefix_kernel<<<1, 64, 0, s>>>(step_mr, mr_sum, loss_mse, loss_mr, loss, objective,
+                                          mse_strength, reg_strength, T);
     return hipGetLastError();
 }
 

@@ -24,6 +24,7 @@ EXPORTS = (
     "ct_last_error", "ct_gpu_new", "ct_gpu_free", "ct_gpu_synchronize",
     "ct_measurements_to_gpu", "ct_buffer_free", "ct_results_to_gpu", "ct_results_gpu_free",
     "ct_results_update_from_gpu", "ct_results_simulate", "ct_results_set_propagation",
+    "ct_results_forward", "ct_results_backward", "ct_epoch_kernel_set_gradient",
     "ct_epoch_kernel_new",
     "ct_epoch_kernel_free", "ct_epoch_kernel_execute", "ct_epoch_kernel_set_freeze_delays",
     "ct_epoch_kernel_set_freeze_gains", "ct_epoch_kernel_set_beat_order",
@@ -97,6 +98,9 @@ def lib() -> ctypes.CDLL:
         "ct_results_update_from_gpu": ([vp, P(_Host)], ctypes.c_int),
         "ct_results_simulate": ([vp, vp], ctypes.c_int),
         "ct_results_set_propagation": ([vp, i32, P(i32)], ctypes.c_int),
+        "ct_results_forward": ([vp, vp, i32, P(CAlgorithm), P(ctypes.c_double)], ctypes.c_int),
+        "ct_results_backward": ([vp, vp, i32, P(CAlgorithm), i32, i32], ctypes.c_int),
+        "ct_epoch_kernel_set_gradient": ([vp, i32, i32], ctypes.c_int),
         "ct_epoch_kernel_new": ([vp, vp, vp, P(CAlgorithm), i32, i32, i32, P(vp)], ctypes.c_int),
         "ct_epoch_kernel_free": ([vp], None),
         "ct_epoch_kernel_execute": ([vp], ctypes.c_int),
@@ -239,6 +243,24 @@ class ResultsGPU:
                "set_propagation")
         return bool(res.value)
 
+    def forward(self, actual: "Buffer", beat: int, config: Algorithm) -> float:
+        """Forward pass of one beat; returns the objective J."""
+        J = ctypes.c_double()
+        c = config.to_c()
+        _check(lib().ct_results_forward(self.handle, actual.handle, int(beat), ctypes.byref(c),
+                                        ctypes.byref(J)), "forward")
+        return J.value
+
+    def backward(self, actual: "Buffer", beat: int, config: Algorithm, truncate: bool = False,
+                 checkpoint_interval: int = 64) -> Dict[str, np.ndarray]:
+        """Gradient of the last forward's J (or the reference's truncated
+        derivation); returns d_gains (3N, 78) and d_coefs (N, 26)."""
+        c = config.to_c()
+        _check(lib().ct_results_backward(self.handle, actual.handle, int(beat), ctypes.byref(c),
+                                         int(bool(truncate)), int(checkpoint_interval)),
+               "backward")
+        return self.update_from_gpu(["d_gains", "d_coefs"])
+
     def simulate(self) -> np.ndarray:
         """Noise-free forward simulation of every beat; returns (B, T, S)."""
         out = np.zeros((self.B, self.T, self.S), dtype=np.float32)
@@ -273,6 +295,11 @@ class EpochKernel:
 
     def execute(self) -> None:
         _check(lib().ct_epoch_kernel_execute(self.handle), "EpochKernel::execute")
+
+    def set_gradient(self, mode: int, checkpoint_interval: int = 64) -> None:
+        """0 = the reference's derivation, 1 = exact reverse-mode gradient."""
+        _check(lib().ct_epoch_kernel_set_gradient(self.handle, int(mode), int(checkpoint_interval)),
+               "set_gradient")
 
     def set_freeze_delays(self, value: bool) -> None:
         _check(lib().ct_epoch_kernel_set_freeze_delays(self.handle, int(bool(value))),

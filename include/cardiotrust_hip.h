@@ -167,6 +167,31 @@ void ct_epoch_kernel_free(ct_epoch_kernel *kernel);
  * and returns; results are read with ct_results_update_from_gpu. */
 int ct_epoch_kernel_execute(ct_epoch_kernel *kernel);
 
+/* Gradient used by ct_epoch_kernel_execute (no reference counterpart):
+ * mode 0 = the reference's per-step derivation (derivation.rs:173-229, the
+ * default; truncated in time), mode 1 = the exact reverse-mode gradient of
+ *   J = sum_t [ mse_strength/(2S) |H x_t - mhat_t|^2
+ *             + maximum_regularization_strength/2 sum_v relu(|x_{t,v}|_1 - threshold)^2 ]
+ * through the all-pass recursion and the gains, recomputing filter outputs
+ * from checkpoints taken every `checkpoint_interval` steps. The difference and
+ * smoothness regularisers are not part of J. */
+int ct_epoch_kernel_set_gradient(ct_epoch_kernel *kernel, int32_t mode,
+                                 int32_t checkpoint_interval);
+
+/* Forward / backward pair of one beat (the counterpart of an autograd
+ * Function over the epoch's parameters; no reference counterpart).
+ * ct_results_forward runs the propagation, projection, residuals and per-step
+ * losses of `beat` with the current parameters and returns J (above).
+ * ct_results_backward then writes dJ/dgains and dJ/dcoefs of that beat into
+ * the derivative arrays (read with ct_results_update_from_gpu: d_gains,
+ * d_coefs); truncate = 1 gives the reference's truncated derivation instead.
+ * Neither updates the parameters. */
+int ct_results_forward(ct_results_gpu *results, const ct_buffer *actual_measurements, int32_t beat,
+                       const ct_algorithm *config, double *objective);
+int ct_results_backward(ct_results_gpu *results, const ct_buffer *actual_measurements,
+                        int32_t beat, const ct_algorithm *config, int32_t truncate,
+                        int32_t checkpoint_interval);
+
 /* EpochKernel::set_freeze_delays / set_freeze_gains (gpu/epoch.rs:118-125). */
 int ct_epoch_kernel_set_freeze_delays(ct_epoch_kernel *kernel, int32_t value);
 int ct_epoch_kernel_set_freeze_gains(ct_epoch_kernel *kernel, int32_t value);
