@@ -72,6 +72,10 @@ def main() -> int:
     ap.add_argument("--cpu-sample-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--propagation", choices=["auto", "per-step", "resident"], default="auto")
+    ap.add_argument("--gradient", choices=["reference", "adjoint"], default="reference",
+                    help="reference: the reference's per-step derivation; adjoint: exact "
+                         "reverse-mode gradient with checkpoints")
+    ap.add_argument("--checkpoint", type=int, default=64, help="adjoint checkpoint interval")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the epoch on a single MI355X; "
@@ -100,6 +104,8 @@ def main() -> int:
     log(f"propagation path: {'resident' if resident else 'per-step'}")
     actual = Buffer(gpu, measurements)
     kernel = EpochKernel(gpu, results, actual, cfg, 3 * N, S, T)
+    if args.gradient == "adjoint":
+        kernel.set_gradient(1, args.checkpoint)
     for _ in range(args.warmup):
         kernel.execute()
     gpu.synchronize()
@@ -154,6 +160,8 @@ def main() -> int:
                         f"steps, 1 beat, SGD lr 1e-3",
             "voxels": N, "steps_per_beat": T, "sensors": S, "beats": 1,
             "propagation": "resident" if resident else "per-step",
+            "gradient": args.gradient if args.gradient == "reference"
+            else f"adjoint, checkpoint every {args.checkpoint} steps",
         },
         "roofline": {
             "bound": "hbm",

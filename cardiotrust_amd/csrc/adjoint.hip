@@ -13,9 +13,9 @@
 //
 // adjoint_step_kernel runs one backward step for all voxels (xbar, then ybar
 // and q); q lives in a ring of R >= Dmax + 1 time slots. adjoint_gradient_kernel
-// then runs one lane per connection: a forward sweep that stores the filter
-// outputs every K steps, and a backward sweep that recomputes each K-step
-// segment from its checkpoint and accumulates dG and da.
+// then runs one lane per connection over the filter-output checkpoints the
+// forward pass stored every K steps: it recomputes each K-step segment and
+// walks it backwards, accumulating dG and da.
 #include <hip/hip_runtime.h>
 
 #include "device.h"
@@ -25,7 +25,7 @@ namespace ct {
 namespace {
 
 constexpr int kAdjVoxels = 64;
-constexpr int kAdjThreads = 3 * kAdjVoxels;
+constexpr int kAdjThreads = 256;
 
 __global__ void max_delay_kernel(const int32_t *__restrict__ nbr, const int32_t *__restrict__ delays,
                                  int64_t len, int32_t *__restrict__ out) {
@@ -42,6 +42,13 @@ __global__ void max_delay_kernel(const int32_t *__restrict__ nbr, const int32_t 
     if (threadIdx.x == 0) atomicMax(out, red[0]);
 }
 
+__global__ void consumer_delays_kernel(const int32_t *__restrict__ cons_idx,
+                                       const int32_t *__restrict__ delays,
+                                       int32_t *__restrict__ cons_delay, int64_t len) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < len) cons_delay[k] = delays[cons_idx[k]];
+}
+
 // c_t for one state from the state triple of its voxel and its mapped residual.
 __device__ __forceinline__ float step_factor(const float xv[3], int o, float e,
                                              const AdjointParams &p) {
@@ -50,10 +57,13 @@ __device__ __forceinline__ float step_factor(const float xv[3], int o, float e,
     return fmaf(e, p.mse_scaling, mr * p.reg_scaling);
 }
 
+// One backward step s. Phase A: one lane per voxel w computes xbar_s of its
+// three states (c_s, then the q of every connection reading w, in ascending
+// connection order). Phase B: one lane per connection computes ybar_s and q_s.
 __global__ __launch_bounds__(kAdjThreads) void adjoint_step_kernel(
     const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
-    const int32_t *__restrict__ delays, const float *__restrict__ gains_rows,
-    const int32_t *__restrict__ cons_ptr, const int32_t *__restrict__ cons_idx,
+    const float *__restrict__ gains_rows, const int32_t *__restrict__ cons_ptr,
+    const int32_t *__restrict__ cons_idx, const int32_t *__restrict__ cons_delay,
     const float *__restrict__ x, const float *__restrict__ E, float *__restrict__ ybar,
     float *__restrict__ qring, float *__restrict__ xbar, int32_t N, int32_t T, int32_t s,
     uint32_t rmask, AdjointParams p) {
@@ -63,32 +73,39 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_step_kernel(
     const int32_t v0 = xcd_block(blockIdx.x, gridDim.x) * kAdjVoxels;
     const int32_t nv = min(kAdjVoxels, N - v0);
 
-    const int vl = threadIdx.x / 3, o = threadIdx.x - 3 * vl;
-    if (vl < nv) {
-        const int32_t w = v0 + vl;
-        const int64_t i = 3LL * w + o;
-        const float *xs = x + int64_t(s) * n + 3LL * w;
-        const float xv[3] = {xs[0], xs[1], xs[2]};
-        float acc = step_factor(xv, o, E[int64_t(s) * n + i], p);
-        for (int32_t j = cons_ptr[w]; j < cons_ptr[w + 1]; ++j) {
-            const int32_t c = cons_idx[j];
-            const int32_t t = s + delays[c];
-            if (t < T) acc += qring[int64_t(uint32_t(t) & rmask) * M + int64_t(c) * 3 + o];
+    if (threadIdx.x < nv) {
+        const int32_t w = v0 + threadIdx.x;
+        const int64_t row = int64_t(s) * n + 3LL * w;
+        const float xv[3] = {x[row], x[row + 1], x[row + 2]};
+        float acc[3];
+#pragma unroll
+        for (int o = 0; o < 3; ++o) acc[o] = step_factor(xv, o, E[row + o], p);
+        const int32_t j1 = cons_ptr[w + 1];
+        for (int32_t j = cons_ptr[w]; j < j1; ++j) {
+            const int32_t t = s + cons_delay[j];
+            if (t < T) {
+                const float *q = qring + int64_t(uint32_t(t) & rmask) * M + int64_t(cons_idx[j]) * 3;
+#pragma unroll
+                for (int o = 0; o < 3; ++o) acc[o] += q[o];
+            }
         }
-        xb[threadIdx.x] = acc;
-        xbar[int64_t(s) * n + i] = acc;
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            xb[threadIdx.x * 3 + o] = acc[o];
+            xbar[row + o] = acc[o];
+        }
     }
     __syncthreads();
     for (int item = threadIdx.x; item < nv * 26; item += kAdjThreads) {
         const int64_t c = int64_t(v0) * 26 + item;
         if (nbr[c] < 0) continue;
         const int vv = item / 26, d = item - 26 * vv;
-        const int64_t row = 3LL * (v0 + vv);
+        const int64_t grow = 3LL * (v0 + vv);
         const float a = coefs[c];
 #pragma unroll
         for (int oo = 0; oo < 3; ++oo) {
             const float next = ybar[c * 3 + oo];
-            const float *g = gains_rows + row * 78 + 3 * d + oo;
+            const float *g = gains_rows + grow * 78 + 3 * d + oo;
             const float yb =
                 g[0] * xb[3 * vv] + g[78] * xb[3 * vv + 1] + g[156] * xb[3 * vv + 2] - a * next;
             qring[int64_t(uint32_t(s) & rmask) * M + c * 3 + oo] = a * yb + next;
@@ -97,43 +114,39 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_step_kernel(
     }
 }
 
+// Gradient pass: one lane per connection. For each K-step segment, from the
+// last to the first: recompute the filter outputs from the checkpoint the
+// forward pass stored (y_{t0-1}), then walk the segment backwards in chunks of
+// kGradChunk steps (own-voxel xbar staged in LDS, the neighbour's states in
+// registers), recomputing ybar and accumulating dJ/dG and dJ/da.
 constexpr int kGradThreads = 256;
+constexpr int kGradChunk = 16;
+constexpr int kGradMaxVoxels = kGradThreads / 26 + 2;
 
 __global__ __launch_bounds__(kGradThreads) void adjoint_gradient_kernel(
     const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
     const int32_t *__restrict__ delays, const float *__restrict__ gT,
-    const float *__restrict__ xT, const float *__restrict__ xbarT, float *__restrict__ ckpt,
+    const float *__restrict__ xT, const float *__restrict__ xbarT, const float *__restrict__ ckpt,
     float *__restrict__ yseg, float *__restrict__ dGT, float *__restrict__ dA, int32_t N,
     int32_t T, int32_t K) {
-    const int64_t item = int64_t(xcd_block(blockIdx.x, gridDim.x)) * kGradThreads + threadIdx.x;
+    __shared__ float xs[kGradMaxVoxels][3][kGradChunk];
     const int64_t total = int64_t(N) * 26;
-    if (item >= total) return;
-    const int32_t w = nbr[item];
-    if (w < 0) return;  // no connection: no dependence on these parameters
+    const int64_t item0 = int64_t(xcd_block(blockIdx.x, gridDim.x)) * kGradThreads;
+    const int64_t item = item0 + threadIdx.x;
+    const bool active = item < total;
+    const int32_t vbeg = int32_t(item0 / 26);
+    const int32_t vend = int32_t(min(total - 1, item0 + kGradThreads - 1) / 26) + 1;
+    const int32_t nvox = vend - vbeg;
+    const int64_t it = active ? item : item0;
+    const int32_t w = active ? nbr[it] : -1;
+    const bool has = w >= 0;
     const int64_t n = 3LL * N;
     const int64_t M = total * 3;
-    const int32_t v = int32_t(item / 26), d = int32_t(item % 26);
-    const float a = coefs[item];
-    const int32_t D = delays[item];
-    const float *xw = xT + 3LL * w * T;
-    const float *xbv = xbarT + 3LL * v * T;
+    const int32_t v = int32_t(it / 26), d = int32_t(it % 26), vl = v - vbeg;
+    const float a = coefs[it];
+    const int32_t D = delays[it];
+    const float *xw = xT + (has ? 3LL * w : 0) * T;
     const int32_t nseg = (T + K - 1) / K;
-
-    // forward sweep: filter outputs, stored at the end of every segment but the last
-    float y[3] = {0.0f, 0.0f, 0.0f}, prev_in[3] = {0.0f, 0.0f, 0.0f};
-    for (int32_t t = 0; t < T; ++t) {
-#pragma unroll
-        for (int o = 0; o < 3; ++o) {
-            const float in0 = D <= t ? xw[int64_t(o) * T + t - D] : 0.0f;
-            y[o] = fmaf(a, in0 - y[o], prev_in[o]);
-            prev_in[o] = in0;
-        }
-        if ((t + 1) % K == 0 && t + 1 < T) {
-            const int64_t k = (t + 1) / K;
-#pragma unroll
-            for (int o = 0; o < 3; ++o) ckpt[(k - 1) * M + item * 3 + o] = y[o];
-        }
-    }
 
     float G[3][3], g[3][3];
 #pragma unroll
@@ -143,44 +156,70 @@ __global__ __launch_bounds__(kGradThreads) void adjoint_gradient_kernel(
             G[in][o] = gT[int64_t(3 * d + o) * n + 3LL * v + in];
             g[in][o] = 0.0f;
         }
-    float A = 0.0f;
-    float ybn[3] = {0.0f, 0.0f, 0.0f};
+    float A = 0.0f, ybn[3] = {0.0f, 0.0f, 0.0f};
     for (int32_t k = nseg - 1; k >= 0; --k) {
         const int32_t t0 = k * K, t1 = min(T, t0 + K);
-        // recompute the segment from its checkpoint (y_{t0-1})
-        float yc[3], ys[3], pin[3];
-#pragma unroll
-        for (int o = 0; o < 3; ++o) {
-            yc[o] = k == 0 ? 0.0f : ckpt[int64_t(k - 1) * M + item * 3 + o];
-            ys[o] = yc[o];
-            pin[o] = D <= t0 - 1 ? xw[int64_t(o) * T + t0 - 1 - D] : 0.0f;
-        }
-        for (int32_t t = t0; t < t1; ++t) {
+        float yc[3] = {0.0f, 0.0f, 0.0f}, ycur[3] = {0.0f, 0.0f, 0.0f};
+        if (has) {
+            float pin[3];
 #pragma unroll
             for (int o = 0; o < 3; ++o) {
-                const float in0 = D <= t ? xw[int64_t(o) * T + t - D] : 0.0f;
-                ys[o] = fmaf(a, in0 - ys[o], pin[o]);
-                pin[o] = in0;
-                yseg[int64_t(t - t0) * M + item * 3 + o] = ys[o];
+                yc[o] = k == 0 ? 0.0f : ckpt[int64_t(k - 1) * M + it * 3 + o];
+                ycur[o] = yc[o];
+                pin[o] = D <= t0 - 1 ? xw[int64_t(o) * T + t0 - 1 - D] : 0.0f;
+            }
+            for (int32_t t = t0; t < t1; ++t) {
+#pragma unroll
+                for (int o = 0; o < 3; ++o) {
+                    const float in0 = D <= t ? xw[int64_t(o) * T + t - D] : 0.0f;
+                    ycur[o] = fmaf(a, in0 - ycur[o], pin[o]);
+                    pin[o] = in0;
+                    yseg[int64_t(t - t0) * M + it * 3 + o] = ycur[o];
+                }
             }
         }
-        for (int32_t s = t1 - 1; s >= t0; --s) {
-            float xb[3], yb[3];
+        // ycur = y_{t1-1}; walk back
+        for (int32_t cend = t1; cend > t0; cend -= kGradChunk) {
+            const int32_t cbeg = max(t0, cend - kGradChunk), len = cend - cbeg;
+            for (int idx = threadIdx.x; idx < nvox * len; idx += kGradThreads) {
+                const int vv = idx / len, j = idx - vv * len;
+                const int64_t row = 3LL * (vbeg + vv) * T + cbeg + j;
 #pragma unroll
-            for (int in = 0; in < 3; ++in) xb[in] = xbv[int64_t(in) * T + s];
-#pragma unroll
-            for (int o = 0; o < 3; ++o) {
-                yb[o] = G[0][o] * xb[0] + G[1][o] * xb[1] + G[2][o] * xb[2] - a * ybn[o];
-                const float yt = yseg[int64_t(s - t0) * M + item * 3 + o];
-                const float ym = s > t0 ? yseg[int64_t(s - 1 - t0) * M + item * 3 + o] : yc[o];
-                const float xi = D <= s ? xw[int64_t(o) * T + s - D] : 0.0f;
-#pragma unroll
-                for (int in = 0; in < 3; ++in) g[in][o] += xb[in] * yt;
-                A += yb[o] * (xi - ym);
-                ybn[o] = yb[o];
+                for (int q = 0; q < 3; ++q) xs[vv][q][j] = xbarT[row + int64_t(q) * T];
             }
+            float xi[3][kGradChunk];
+#pragma unroll
+            for (int o = 0; o < 3; ++o)
+#pragma unroll
+                for (int j = 0; j < kGradChunk; ++j) {
+                    const int32_t src = cbeg + j - D;
+                    xi[o][j] = (has && j < len && src >= 0) ? xw[int64_t(o) * T + src] : 0.0f;
+                }
+            __syncthreads();
+            if (has) {
+#pragma unroll
+                for (int j = kGradChunk - 1; j >= 0; --j) {
+                    if (j >= len) continue;
+                    const int32_t s = cbeg + j;
+                    const float xb[3] = {xs[vl][0][j], xs[vl][1][j], xs[vl][2][j]};
+#pragma unroll
+                    for (int o = 0; o < 3; ++o) {
+                        const float yb =
+                            G[0][o] * xb[0] + G[1][o] * xb[1] + G[2][o] * xb[2] - a * ybn[o];
+                        const float ym =
+                            s > t0 ? yseg[int64_t(s - 1 - t0) * M + it * 3 + o] : yc[o];
+#pragma unroll
+                        for (int in = 0; in < 3; ++in) g[in][o] += xb[in] * ycur[o];
+                        A += yb * (xi[o][j] - ym);
+                        ybn[o] = yb;
+                        ycur[o] = ym;
+                    }
+                }
+            }
+            __syncthreads();
         }
     }
+    if (!active || !has) return;  // no connection: no dependence on these parameters
 #pragma unroll
     for (int in = 0; in < 3; ++in)
 #pragma unroll
@@ -198,22 +237,30 @@ hipError_t launch_max_delay(hipStream_t s, const int32_t *nbr, const int32_t *de
     return hipGetLastError();
 }
 
+hipError_t launch_consumer_delays(hipStream_t s, const int32_t *cons_idx, const int32_t *delays,
+                                  int32_t *cons_delay, int64_t len) {
+    if (len <= 0) return hipSuccess;
+    consumer_delays_kernel<<<unsigned((len + 255) / 256), 256, 0, s>>>(cons_idx, delays, cons_delay,
+                                                                     len);
+    return hipGetLastError();
+}
+
 hipError_t launch_adjoint_step(hipStream_t s, const int32_t *nbr, const float *coefs,
-                               const int32_t *delays, const float *gains_rows,
-                               const int32_t *cons_ptr, const int32_t *cons_idx, const float *x,
+                               const float *gains_rows, const int32_t *cons_ptr,
+                               const int32_t *cons_idx, const int32_t *cons_delay, const float *x,
                                const float *E, float *ybar, float *qring, float *xbar, int32_t N,
                                int32_t T, int32_t step, uint32_t rmask, const AdjointParams &p) {
     if (N <= 0) return hipSuccess;
     const unsigned blocks = unsigned((N + kAdjVoxels - 1) / kAdjVoxels);
-    adjoint_step_kernel<<<blocks, kAdjThreads, 0, s>>>(nbr, coefs, delays, gains_rows, cons_ptr,
-                                                       cons_idx, x, E, ybar, qring, xbar, N, T,
+    adjoint_step_kernel<<<blocks, kAdjThreads, 0, s>>>(nbr, coefs, gains_rows, cons_ptr, cons_idx,
+                                                       cons_delay, x, E, ybar, qring, xbar, N, T,
                                                        step, rmask, p);
     return hipGetLastError();
 }
 
 hipError_t launch_adjoint_gradient(hipStream_t s, const int32_t *nbr, const float *coefs,
                                    const int32_t *delays, const float *gT, const float *xT,
-                                   const float *xbarT, float *ckpt, float *yseg, float *dGT,
+                                   const float *xbarT, const float *ckpt, float *yseg, float *dGT,
                                    float *dA, int32_t N, int32_t T, int32_t K) {
     if (N <= 0) return hipSuccess;
     const unsigned blocks = unsigned((int64_t(N) * 26 + kGradThreads - 1) / kGradThreads);

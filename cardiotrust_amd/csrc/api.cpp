@@ -91,7 +91,9 @@ struct ct_results_gpu {
     // reverse-mode gradient: host neighbour table (for the consumer lists) and
     // buffers allocated on first use
     std::vector<int32_t> nbr_host;
-    int32_t *cons_ptr = nullptr, *cons_idx = nullptr, *dmax = nullptr;
+    int32_t *cons_ptr = nullptr, *cons_idx = nullptr, *cons_delay = nullptr, *dmax = nullptr;
+    int64_t cons_len = 0;
+    int32_t ckpt_interval = 0;  // checkpoint interval of the last forward pass (0 = none)
     float *E = nullptr, *xbar = nullptr, *xbarT = nullptr, *ybar = nullptr;
     float *qring = nullptr, *ckpt = nullptr, *yseg = nullptr;
     int64_t qring_slots = 0, ckpt_segments = 0, yseg_steps = 0;
@@ -143,7 +145,10 @@ struct ct_epoch_kernel {
 
 namespace {
 
-enum Phase { kPropagate, kProject, kRegularize, kTranspose, kMapResiduals, kDerive, kUpdate, kTotal };
+enum Phase {
+    kPropagate, kProject, kRegularize, kTranspose, kMapResiduals, kDerive, kUpdate, kTotal,
+    kAdjointSweep
+};
 
 struct Span {
     ct_epoch_kernel *k;
@@ -238,23 +243,34 @@ int reset_derivatives(ct_results_gpu &r) {
 // Forward propagation of one beat and the projected measurements
 // (prediction.rs:15-130 for every step). With `mhat` also residuals and
 // per-step MSE.
-int forward_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_kernel *k) {
+int ensure_checkpoints(ct_results_gpu &r, int32_t K);
+
+// With ckpt_interval = K > 0 the pass also stores the filter outputs y_t at
+// t = kK - 1 (k = 1 .. ceil(T/K) - 1) for the reverse-mode gradient.
+int forward_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_kernel *k,
+                 int32_t ckpt_interval = 0) {
     hipStream_t s = r.gpu->stream;
     r.forward_beat = beat;
+    if (ckpt_interval > 0) CT_CHECK(ensure_checkpoints(r, ckpt_interval));
+    r.ckpt_interval = ckpt_interval;
+    const int64_t M = int64_t(r.N) * 78;
     CT_HIP(hipMemsetAsync(r.y, 0, sizeof(float) * r.N * 78, s));
     CT_HIP(hipMemsetAsync(r.y_last, 0, sizeof(float) * r.N * 78, s));
     const bool resident = r.resident_ok && r.propagation_mode != 1;
     if (resident && r.N > 0) {
         Span sp(k, kPropagate, r.T);
         CT_HIP(ct::launch_propagate_resident(s, r.resident_blocks, r.nbr, r.coefs, r.delays, r.gT,
-                                             r.y, r.y_last, r.x, r.b, r.u, r.barrier, r.N, 0, r.T,
-                                             r.T));
+                                             r.y, r.y_last, r.x, r.b, r.u, r.barrier, r.ckpt,
+                                             ckpt_interval, r.N, 0, r.T, r.T));
         r.resident_used = true;
     } else {
         for (int32_t t = 0; t < r.T; ++t) {
             Span sp(k, kPropagate, 1);
             CT_HIP(ct::launch_propagate_step(s, r.nbr, r.coefs, r.delays, r.gT, r.y, r.y_last,
                                              r.x, r.b, r.control_function[t], r.N, t, r.T));
+            if (ckpt_interval > 0 && (t + 1) % ckpt_interval == 0 && t + 1 < r.T && M > 0)
+                CT_HIP(hipMemcpyAsync(r.ckpt + int64_t((t + 1) / ckpt_interval - 1) * M, r.y,
+                                      sizeof(float) * M, hipMemcpyDeviceToDevice, s));
         }
     }
     {
@@ -315,10 +331,10 @@ int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32
 
 int run_beat(ct_epoch_kernel &k, int32_t beat) {
     ct_results_gpu &r = *k.r;
-    CT_CHECK(forward_beat(r, beat, k.mhat->data, &k));
+    const bool adjoint = k.gradient == 1 && !(k.cfg.freeze_gains && k.cfg.freeze_delays);
+    CT_CHECK(forward_beat(r, beat, k.mhat->data, &k, adjoint ? k.checkpoint_interval : 0));
     CT_CHECK(losses_beat(r, k.cfg, &k));
-    if (k.gradient == 1 && !(k.cfg.freeze_gains && k.cfg.freeze_delays))
-        return derive_adjoint(r, beat, k.cfg, k.checkpoint_interval, &k);
+    if (adjoint) return derive_adjoint(r, beat, k.cfg, k.checkpoint_interval, &k);
     return derive_reference(r, beat, k.cfg, &k);
 }
 
@@ -427,6 +443,16 @@ int ensure(ct_results_gpu &r, V **p, int64_t &have, int64_t count, int64_t unit)
     return CT_OK;
 }
 
+int ensure_checkpoints(ct_results_gpu &r, int32_t K) {
+    const int64_t M = int64_t(r.N) * 78;
+    const int64_t nseg = (int64_t(r.T) + K - 1) / K;
+    const int64_t need = std::max<int64_t>(nseg - 1, 1);
+    int64_t have = r.ckpt_segments * M;
+    CT_CHECK(ensure(r, &r.ckpt, have, need * M, 1));
+    r.ckpt_segments = std::max(r.ckpt_segments, need);
+    return CT_OK;
+}
+
 // Consumer lists (CSR): for each voxel w, the connections (v, d) that read
 // it, in ascending connection order.
 int build_consumers(ct_results_gpu &r) {
@@ -442,6 +468,8 @@ int build_consumers(ct_results_gpu &r) {
         if (r.nbr_host[c] >= 0) idx[fill[r.nbr_host[c]]++] = int32_t(c);
     CT_CHECK(r.alloc(&r.cons_ptr, N + 1));
     CT_CHECK(r.alloc(&r.cons_idx, std::max<int64_t>(int64_t(idx.size()), 1)));
+    CT_CHECK(r.alloc(&r.cons_delay, std::max<int64_t>(int64_t(idx.size()), 1)));
+    r.cons_len = int64_t(idx.size());
     CT_CHECK(upload(r.cons_ptr, ptr.data(), sizeof(int32_t) * (N + 1)));
     CT_CHECK(upload(r.cons_idx, idx.data(), sizeof(int32_t) * idx.size()));
     return CT_OK;
@@ -472,10 +500,10 @@ int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32
     int64_t have_q = r.qring_slots * M;
     CT_CHECK(ensure(r, &r.qring, have_q, slots * M, 1));
     r.qring_slots = std::max(r.qring_slots, slots);
-    const int64_t nseg = (T + K - 1) / K;
-    int64_t have_c = r.ckpt_segments * M, have_y = r.yseg_steps * M;
-    CT_CHECK(ensure(r, &r.ckpt, have_c, std::max<int64_t>(nseg - 1, 1) * M, 1));
-    r.ckpt_segments = std::max(r.ckpt_segments, std::max<int64_t>(nseg - 1, 1));
+    if (r.ckpt_interval != K || r.forward_beat != beat)
+        return fail(CT_ERR_STATE, "the reverse-mode gradient needs the forward pass of this beat "
+                                  "with checkpoints every " + std::to_string(K) + " steps");
+    int64_t have_y = r.yseg_steps * M;
     CT_CHECK(ensure(r, &r.yseg, have_y, int64_t(K) * M, 1));
     r.yseg_steps = std::max(r.yseg_steps, int64_t(K));
 
@@ -489,16 +517,20 @@ int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32
         CT_HIP(ct::launch_transpose(s, r.eT, r.E, n, T));      // mapped residuals by step
         CT_HIP(ct::launch_transpose(s, r.gT, r.staging, 78, n));  // gains by state row
     }
-    Span sp(k, kDerive);
     ct::AdjointParams p;
     p.mse_scaling = 1.0f / float(r.S) * c.mse_strength;
     p.reg_scaling = c.maximum_regularization_strength;
     p.threshold = c.maximum_regularization_threshold;
-    CT_HIP(hipMemsetAsync(r.ybar, 0, sizeof(float) * M, s));
-    for (int32_t step = r.T - 1; step >= 0; --step)
-        CT_HIP(ct::launch_adjoint_step(s, r.nbr, r.coefs, r.delays, r.staging, r.cons_ptr,
-                                       r.cons_idx, r.x, r.E, r.ybar, r.qring, r.xbar, r.N, r.T,
-                                       step, uint32_t(slots - 1), p));
+    {
+        Span sp(k, kAdjointSweep);
+        CT_HIP(hipMemsetAsync(r.ybar, 0, sizeof(float) * M, s));
+        CT_HIP(ct::launch_consumer_delays(s, r.cons_idx, r.delays, r.cons_delay, r.cons_len));
+        for (int32_t step = r.T - 1; step >= 0; --step)
+            CT_HIP(ct::launch_adjoint_step(s, r.nbr, r.coefs, r.staging, r.cons_ptr, r.cons_idx,
+                                           r.cons_delay, r.x, r.E, r.ybar, r.qring, r.xbar, r.N,
+                                           r.T, step, uint32_t(slots - 1), p));
+    }
+    Span sp(k, kDerive);
     CT_HIP(ct::launch_transpose(s, r.xbar, r.xbarT, T, n));
     CT_HIP(ct::launch_transpose(s, r.x, r.xT, T, n));
     CT_HIP(ct::launch_adjoint_gradient(s, r.nbr, r.coefs, r.delays, r.gT, r.xT, r.xbarT, r.ckpt,
@@ -881,7 +913,8 @@ int ct_epoch_kernel_set_gradient(ct_epoch_kernel *k, int32_t mode, int32_t check
 }
 
 int ct_results_forward(ct_results_gpu *results, const ct_buffer *actual, int32_t beat,
-                       const ct_algorithm *config, double *objective) {
+                       const ct_algorithm *config, int32_t checkpoint_interval,
+                       double *objective) {
     if (!results || !actual || !config) return fail(CT_ERR_INVALID, "null argument");
     ct_results_gpu &r = *results;
     CT_CHECK(check_gpu(r.gpu));
@@ -889,7 +922,8 @@ int ct_results_forward(ct_results_gpu *results, const ct_buffer *actual, int32_t
     if (actual->beats != r.B || actual->steps != r.T || actual->sensors != r.S)
         return fail(CT_ERR_INVALID, "actual measurements are not (B, T, S) of the results");
     CT_HIP(hipMemsetAsync(r.mr_sum, 0, sizeof(float), r.gpu->stream));
-    CT_CHECK(forward_beat(r, beat, actual->data, nullptr));
+    if (checkpoint_interval < 0) return fail(CT_ERR_INVALID, "negative checkpoint interval");
+    CT_CHECK(forward_beat(r, beat, actual->data, nullptr, checkpoint_interval));
     CT_CHECK(losses_beat(r, *config, nullptr));
     CT_CHECK(check_resident(r));
     CT_HIP(hipStreamSynchronize(r.gpu->stream));
@@ -988,6 +1022,7 @@ int ct_epoch_kernel_get_profile(ct_epoch_kernel *k, ct_profile *out) {
             case kDerive: out->derive_ms += ms; break;
             case kUpdate: out->update_ms += ms; break;
             case kTotal: out->total_ms += ms; break;
+            case kAdjointSweep: out->adjoint_sweep_ms += ms; break;
         }
     }
     return CT_OK;

@@ -51,7 +51,8 @@ hipError_t resident_capacity(int32_t *blocks_per_cu);
 hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const int32_t *nbr,
                                      const float *coefs, const int32_t *delays, const float *gT,
                                      float *y, float *y_last, float *x, const float *b,
-                                     const float *u, GridBarrier *gb, int32_t N, int32_t t_begin,
+                                     const float *u, GridBarrier *gb, float *ckpt,
+                                     int32_t ckpt_interval, int32_t N, int32_t t_begin,
                                      int32_t t_end, int32_t T);
 
 hipError_t launch_project(hipStream_t s, const float *x, const float *H, float *partial,
@@ -86,14 +87,18 @@ struct AdjointParams {
 };
 hipError_t launch_max_delay(hipStream_t s, const int32_t *nbr, const int32_t *delays, int64_t len,
                             int32_t *out);
+hipError_t launch_consumer_delays(hipStream_t s, const int32_t *cons_idx, const int32_t *delays,
+                                  int32_t *cons_delay, int64_t len);
 hipError_t launch_adjoint_step(hipStream_t s, const int32_t *nbr, const float *coefs,
-                               const int32_t *delays, const float *gains_rows,
-                               const int32_t *cons_ptr, const int32_t *cons_idx, const float *x,
+                               const float *gains_rows, const int32_t *cons_ptr,
+                               const int32_t *cons_idx, const int32_t *cons_delay, const float *x,
                                const float *E, float *ybar, float *qring, float *xbar, int32_t N,
                                int32_t T, int32_t step, uint32_t rmask, const AdjointParams &p);
+// `ckpt` holds y at steps kK - 1 (k = 1 .. ceil(T/K) - 1), written by the
+// forward propagation.
 hipError_t launch_adjoint_gradient(hipStream_t s, const int32_t *nbr, const float *coefs,
                                    const int32_t *delays, const float *gT, const float *xT,
-                                   const float *xbarT, float *ckpt, float *yseg, float *dGT,
+                                   const float *xbarT, const float *ckpt, float *yseg, float *dGT,
                                    float *dA, int32_t N, int32_t T, int32_t K);
 
 hipError_t launch_sgd(hipStream_t s, float *v, const float *d, int64_t len, float scale);

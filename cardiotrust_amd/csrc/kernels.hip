@@ -197,8 +197,8 @@ __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
     const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
     const int32_t *__restrict__ delays, const float *__restrict__ gT, float *__restrict__ y,
     float *__restrict__ y_last, float *__restrict__ x, const float *__restrict__ b,
-    const float *__restrict__ u, GridBarrier *__restrict__ gb, int32_t N, int32_t t_begin,
-    int32_t t_end, int32_t T) {
+    const float *__restrict__ u, GridBarrier *__restrict__ gb, float *__restrict__ ckpt,
+    int32_t ckpt_interval, int32_t N, int32_t t_begin, int32_t t_end, int32_t T) {
     extern __shared__ float lds[];
     float *ys = lds;                                                       // [voxel][78]
     uint32_t *mask = reinterpret_cast<uint32_t *>(lds + kResMaxVoxels * 78);  // [voxel]
@@ -304,6 +304,11 @@ __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
                 }
             }
             x[int64_t(t) * n + i] = acc + ut * b[i];
+        }
+        // filter outputs y_t for the reverse-mode gradient, every ckpt_interval steps
+        if (ckpt_interval > 0 && (t + 1) % ckpt_interval == 0 && t + 1 < T) {
+            float *dst = ckpt + int64_t((t + 1) / ckpt_interval - 1) * N * 78 + int64_t(vb) * 78;
+            for (int idx = tid; idx < nv * 78; idx += kResThreads) dst[idx] = ys[idx];
         }
         if (!grid_barrier(gb, gridDim.x, gen, ok_lds)) return;  // host reports the abort
     }
@@ -909,13 +914,15 @@ hipError_t resident_capacity(int32_t *blocks_per_cu) {
 hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const int32_t *nbr,
                                      const float *coefs, const int32_t *delays, const float *gT,
                                      float *y, float *y_last, float *x, const float *b,
-                                     const float *u, GridBarrier *gb, int32_t N, int32_t t_begin,
+                                     const float *u, GridBarrier *gb, float *ckpt,
+                                     int32_t ckpt_interval, int32_t N, int32_t t_begin,
                                      int32_t t_end, int32_t T) {
     // the barrier starts from zero: a fill kernel rather than a memset node
     hipError_t e = launch_fill(s, reinterpret_cast<float *>(gb), 0.0f, sizeof(GridBarrier) / 4);
     if (e != hipSuccess) return e;
     propagate_resident_kernel<<<blocks, kResThreads, resident_lds_bytes(), s>>>(
-        nbr, coefs, delays, gT, y, y_last, x, b, u, gb, N, t_begin, t_end, T);
+        nbr, coefs, delays, gT, y, y_last, x, b, u, gb, ckpt, ckpt_interval, N, t_begin, t_end,
+        T);
     return hipGetLastError();
 }
 

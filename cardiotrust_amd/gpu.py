@@ -67,7 +67,7 @@ class Profile(ctypes.Structure):
         ("project_ms", ctypes.c_double), ("regularize_ms", ctypes.c_double),
         ("transpose_ms", ctypes.c_double), ("map_residuals_ms", ctypes.c_double),
         ("derive_ms", ctypes.c_double), ("update_ms", ctypes.c_double),
-        ("total_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double), ("adjoint_sweep_ms", ctypes.c_double),
     ]
 
     def as_dict(self) -> Dict[str, float]:
@@ -98,7 +98,8 @@ def lib() -> ctypes.CDLL:
         "ct_results_update_from_gpu": ([vp, P(_Host)], ctypes.c_int),
         "ct_results_simulate": ([vp, vp], ctypes.c_int),
         "ct_results_set_propagation": ([vp, i32, P(i32)], ctypes.c_int),
-        "ct_results_forward": ([vp, vp, i32, P(CAlgorithm), P(ctypes.c_double)], ctypes.c_int),
+        "ct_results_forward": ([vp, vp, i32, P(CAlgorithm), i32, P(ctypes.c_double)],
+                               ctypes.c_int),
         "ct_results_backward": ([vp, vp, i32, P(CAlgorithm), i32, i32], ctypes.c_int),
         "ct_epoch_kernel_set_gradient": ([vp, i32, i32], ctypes.c_int),
         "ct_epoch_kernel_new": ([vp, vp, vp, P(CAlgorithm), i32, i32, i32, P(vp)], ctypes.c_int),
@@ -243,12 +244,14 @@ class ResultsGPU:
                "set_propagation")
         return bool(res.value)
 
-    def forward(self, actual: "Buffer", beat: int, config: Algorithm) -> float:
-        """Forward pass of one beat; returns the objective J."""
+    def forward(self, actual: "Buffer", beat: int, config: Algorithm,
+                checkpoint_interval: int = 64) -> float:
+        """Forward pass of one beat (filter outputs stored every
+        `checkpoint_interval` steps); returns the objective J."""
         J = ctypes.c_double()
         c = config.to_c()
         _check(lib().ct_results_forward(self.handle, actual.handle, int(beat), ctypes.byref(c),
-                                        ctypes.byref(J)), "forward")
+                                        int(checkpoint_interval), ctypes.byref(J)), "forward")
         return J.value
 
     def backward(self, actual: "Buffer", beat: int, config: Algorithm, truncate: bool = False,

@@ -115,6 +115,7 @@ typedef struct {
     double derive_ms;                     /* gain / coefficient derivatives */
     double update_ms;                     /* batch derivatives, update, batch metrics */
     double total_ms;                      /* whole epoch, first to last event */
+    double adjoint_sweep_ms;              /* reverse-mode sweep (gradient mode 1) */
 } ct_profile;
 
 const char *ct_last_error(void);
@@ -181,13 +182,15 @@ int ct_epoch_kernel_set_gradient(ct_epoch_kernel *kernel, int32_t mode,
 /* Forward / backward pair of one beat (the counterpart of an autograd
  * Function over the epoch's parameters; no reference counterpart).
  * ct_results_forward runs the propagation, projection, residuals and per-step
- * losses of `beat` with the current parameters and returns J (above).
+ * losses of `beat` with the current parameters and returns J (above); with
+ * checkpoint_interval = K > 0 it also stores the filter outputs every K steps.
  * ct_results_backward then writes dJ/dgains and dJ/dcoefs of that beat into
  * the derivative arrays (read with ct_results_update_from_gpu: d_gains,
- * d_coefs); truncate = 1 gives the reference's truncated derivation instead.
- * Neither updates the parameters. */
+ * d_coefs); it needs the same K as the forward pass, or truncate = 1 for the
+ * reference's truncated derivation. Neither updates the parameters. */
 int ct_results_forward(ct_results_gpu *results, const ct_buffer *actual_measurements, int32_t beat,
-                       const ct_algorithm *config, double *objective);
+                       const ct_algorithm *config, int32_t checkpoint_interval,
+                       double *objective);
 int ct_results_backward(ct_results_gpu *results, const ct_buffer *actual_measurements,
                         int32_t beat, const ct_algorithm *config, int32_t truncate,
                         int32_t checkpoint_interval);

@@ -48,7 +48,7 @@ def gpu_grad(gpu, p, cfg, K, truncate=False, mode=0):
     r = ResultsGPU(gpu, p)
     r.set_propagation(mode)
     buf = Buffer(gpu, p.measurements)
-    J = r.forward(buf, 0, cfg)
+    J = r.forward(buf, 0, cfg, checkpoint_interval=K)
     out = r.backward(buf, 0, cfg, truncate=truncate, checkpoint_interval=K)
     buf.close()
     r.close()
