@@ -149,9 +149,10 @@ int ct_results_simulate(ct_results_gpu *results, float *measurements_out);
 
 /* Propagation path (no reference counterpart; a tuning switch of this
  * library): 0 = automatic (resident when the problem fits on chip), 1 = one
- * launch per time step, 2 = resident (one cooperative launch per beat that
- * keeps the filter outputs in LDS; CT_ERR_INVALID if the problem does not
- * fit). `resident` (may be NULL) receives 1 if the resident path will run. */
+ * launch per time step, 2 = resident (one launch per beat whose workgroups
+ * all stay resident, keep the filter outputs in LDS and meet at a grid
+ * barrier after every step; CT_ERR_INVALID if the problem does not fit).
+ * `resident` (may be NULL) receives 1 if the resident path will run. */
 int ct_results_set_propagation(ct_results_gpu *results, int32_t mode, int32_t *resident);
 
 /* EpochKernel::new (src/core/algorithm/gpu/epoch.rs:29-93). The sizes must
