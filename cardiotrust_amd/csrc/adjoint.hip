@@ -231,7 +231,7 @@ __global__ __launch_bounds__(kGradThreads) void adjoint_gradient_kernel(
 
 hipError_t launch_max_delay(hipStream_t s, const int32_t *nbr, const int32_t *delays, int64_t len,
                             int32_t *out) {
-    hipError_t e = hipMemsetAsync(out, 0, sizeof(int32_t), s);
+    hipError_t e = launch_fill(s, reinterpret_cast<float *>(out), 0.0f, 1);  // bits of int 0
     if (e != hipSuccess || len <= 0) return e;
     max_delay_kernel<<<256, 256, 0, s>>>(nbr, delays, len, out);
     return hipGetLastError();

@@ -6,20 +6,9 @@
 // (src/core/algorithm.rs:124-281): per beat, the forward propagation over all
 // steps, then the deferred projection, residuals, losses and derivatives;
 // per batch, the smoothness term, the update and the batch metrics.
-#include "cardiotrust_hip.h"
+#include "runtime.h"
 
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <string>
-#include <vector>
-
-#include "kernels.h"
-
-namespace {
+namespace ctrt {
 
 thread_local std::string g_error;
 
@@ -27,148 +16,6 @@ int fail(int code, const std::string &msg) {
     g_error = msg;
     return code;
 }
-
-#define CT_HIP(expr)                                                                  \
-    do {                                                                              \
-        hipError_t e_ = (expr);                                                       \
-        if (e_ != hipSuccess)                                                         \
-            return fail(CT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-#define CT_CHECK(expr)          \
-    do {                        \
-        int rc_ = (expr);       \
-        if (rc_ != CT_OK) return rc_; \
-    } while (0)
-
-}  // namespace
-
-struct ct_gpu {
-    int32_t device = 0;
-    hipStream_t stream = nullptr;
-};
-
-struct ct_buffer {
-    ct_gpu *gpu = nullptr;
-    float *data = nullptr;
-    int32_t beats = 0, steps = 0, sensors = 0;
-};
-
-struct ct_results_gpu {
-    ct_gpu *gpu = nullptr;
-    int32_t N = 0, S = 0, Sp = 0, T = 0, B = 0, nbatch = 0, optimizer = 0;
-    int64_t n = 0;
-    std::vector<void *> allocations;
-    std::vector<float> control_function;  // host copy: u[t] is a launch argument
-
-    // model
-    int32_t *nbr = nullptr, *delays = nullptr;
-    float *coefs = nullptr, *initial_delays = nullptr, *gT = nullptr, *b = nullptr, *H = nullptr;
-    // estimations
-    float *x = nullptr, *y = nullptr, *y_last = nullptr, *m = nullptr, *R = nullptr;
-    float *xT = nullptr, *eT = nullptr;
-    float *avg = nullptr;
-    uint8_t *avg_valid = nullptr;
-    // derivatives
-    float *dGT = nullptr, *dA = nullptr, *fir = nullptr, *iir = nullptr, *mr_last = nullptr,
-          *mr_sum = nullptr;
-    float *gm1 = nullptr, *gm2 = nullptr, *cm1 = nullptr, *cm2 = nullptr;
-    int64_t step = 1;
-    // metrics
-    float *loss = nullptr, *loss_mse = nullptr, *loss_mr = nullptr;
-    float *loss_b = nullptr, *loss_mse_b = nullptr, *loss_mr_b = nullptr;
-    float *u = nullptr;  // control function on the device (resident propagation)
-    // propagation path: resident (one launch per beat) when the
-    // problem fits on chip, else one launch per step
-    int32_t propagation_mode = 0;  // 0 auto, 1 per-step launches, 2 resident
-    bool resident_ok = false;
-    bool resident_used = false;  // a resident launch is in flight since the last check
-    int32_t resident_blocks = 0;
-    ct::GridBarrier *barrier = nullptr;
-    // scratch
-    float *partial = nullptr, *mr_partial = nullptr, *staging = nullptr, *step_mr = nullptr;
-    double *objective = nullptr;  // J of the last forward (adjoint objective)
-    // reverse-mode gradient: host neighbour table (for the consumer lists) and
-    // buffers allocated on first use
-    std::vector<int32_t> nbr_host;
-    int32_t *cons_ptr = nullptr, *cons_idx = nullptr, *cons_delay = nullptr, *dmax = nullptr;
-    int64_t cons_len = 0;
-    int32_t ckpt_interval = 0;  // checkpoint interval of the last forward pass (0 = none)
-    float *E = nullptr, *xbar = nullptr, *xbarT = nullptr, *ybar = nullptr;
-    float *qring = nullptr, *ckpt = nullptr, *yseg = nullptr;
-    int64_t qring_slots = 0, ckpt_segments = 0, yseg_steps = 0;
-    int32_t forward_beat = -1;  // beat whose forward pass the device state holds
-    int32_t splits = 1, mr_blocks = 1;
-    int64_t kps = ct::kGemmTileK;
-
-    ~ct_results_gpu() {
-        for (void *p : allocations) (void)hipFree(p);
-    }
-
-    template <typename V>
-    int alloc(V **p, int64_t count) {
-        const size_t bytes = std::max<size_t>(sizeof(V) * size_t(std::max<int64_t>(count, 0)), 256);
-        void *raw = nullptr;
-        hipError_t e = hipMalloc(&raw, bytes);
-        if (e != hipSuccess)
-            return fail(CT_ERR_DEVICE, "hipMalloc(" + std::to_string(bytes) +
-                                           " bytes): " + hipGetErrorString(e));
-        allocations.push_back(raw);
-        *p = static_cast<V *>(raw);
-        return CT_OK;
-    }
-};
-
-struct ct_epoch_kernel {
-    ct_gpu *gpu = nullptr;
-    ct_results_gpu *r = nullptr;
-    const ct_buffer *mhat = nullptr;
-    ct_algorithm cfg{};
-    std::vector<int32_t> order;
-    int32_t batch_index = 0;
-    bool profiling = false;
-    int32_t gradient = 0;            // 0 = reference derivation, 1 = reverse-mode adjoint
-    int32_t checkpoint_interval = 64;
-    std::vector<hipEvent_t> events;
-    size_t used_events = 0;
-    struct SpanRec {
-        int phase;
-        int64_t steps;
-        size_t start, end;
-    };
-    std::vector<SpanRec> spans;
-
-    ~ct_epoch_kernel() {
-        for (hipEvent_t e : events) (void)hipEventDestroy(e);
-    }
-};
-
-namespace {
-
-enum Phase {
-    kPropagate, kProject, kRegularize, kTranspose, kMapResiduals, kDerive, kUpdate, kTotal,
-    kAdjointSweep
-};
-
-struct Span {
-    ct_epoch_kernel *k;
-    int phase;
-    int64_t steps;
-    size_t start = 0;
-    bool on = false;
-    Span(ct_epoch_kernel *kk, int ph, int64_t st = 0) : k(kk), phase(ph), steps(st) {
-        if (!k || !k->profiling) return;
-        if (k->used_events + 2 > k->events.size()) return;
-        start = k->used_events;
-        k->used_events += 2;
-        on = hipEventRecord(k->events[start], k->r->gpu->stream) == hipSuccess;
-    }
-    ~Span() {
-        if (!on) return;
-        if (hipEventRecord(k->events[start + 1], k->r->gpu->stream) == hipSuccess)
-            k->spans.push_back({phase, steps, start, start + 1});
-    }
-};
 
 int check_gpu(ct_gpu *gpu) {
     if (!gpu) return fail(CT_ERR_INVALID, "null ct_gpu");
@@ -231,37 +78,64 @@ int neighbours_from_indices(int32_t N, const int32_t *osi, const int32_t *delays
 
 int reset_derivatives(ct_results_gpu &r) {
     hipStream_t s = r.gpu->stream;
-    CT_HIP(hipMemsetAsync(r.dGT, 0, sizeof(float) * r.n * 78, s));
-    CT_HIP(hipMemsetAsync(r.dA, 0, sizeof(float) * r.N * 26, s));
-    CT_HIP(hipMemsetAsync(r.fir, 0, sizeof(float) * r.N * 78, s));
-    CT_HIP(hipMemsetAsync(r.iir, 0, sizeof(float) * r.N * 78, s));
-    CT_HIP(hipMemsetAsync(r.mr_last, 0, sizeof(float) * r.n, s));
-    CT_HIP(hipMemsetAsync(r.mr_sum, 0, sizeof(float), s));
+    // fill kernels, not memset nodes, so that the epoch can be captured in a graph
+    CT_HIP(ct::launch_fill(s, r.dGT, 0.0f, r.n * 78));
+    CT_HIP(ct::launch_fill(s, r.dA, 0.0f, int64_t(r.N) * 26));
+    CT_HIP(ct::launch_fill(s, r.fir, 0.0f, int64_t(r.N) * 78));
+    CT_HIP(ct::launch_fill(s, r.iir, 0.0f, int64_t(r.N) * 78));
+    CT_HIP(ct::launch_fill(s, r.mr_last, 0.0f, r.n));
+    CT_HIP(ct::launch_fill(s, r.mr_sum, 0.0f, 1));
     return CT_OK;
 }
 
-// Forward propagation of one beat and the projected measurements
-// (prediction.rs:15-130 for every step). With `mhat` also residuals and
-// per-step MSE.
-int ensure_checkpoints(ct_results_gpu &r, int32_t K);
+uint64_t wait_ticks(const ct_gpu &g, double seconds) {
+    return uint64_t(seconds * double(g.wall_ticks_per_second));
+}
 
-// With ckpt_interval = K > 0 the pass also stores the filter outputs y_t at
-// t = kK - 1 (k = 1 .. ceil(T/K) - 1) for the reverse-mode gradient.
-int forward_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_kernel *k,
-                 int32_t ckpt_interval = 0) {
+// Clears y / y_last (estimation.rs:82-86: a beat starts from rest) and records
+// the checkpoint interval of the coming propagation.
+int prepare_propagation(ct_results_gpu &r, int32_t ckpt_interval) {
     hipStream_t s = r.gpu->stream;
-    r.forward_beat = beat;
     if (ckpt_interval > 0) CT_CHECK(ensure_checkpoints(r, ckpt_interval));
     r.ckpt_interval = ckpt_interval;
+    CT_HIP(ct::launch_fill(s, r.y, 0.0f, int64_t(r.N) * 78));
+    CT_HIP(ct::launch_fill(s, r.y_last, 0.0f, int64_t(r.N) * 78));
+    return CT_OK;
+}
+
+ct::ResidentJob resident_job(ct_results_gpu &r, int32_t ckpt_interval) {
+    ct::ResidentJob j{};
+    j.nbr = r.nbr;
+    j.coefs = r.coefs;
+    j.delays = r.delays;
+    j.gT = r.gT;
+    j.y = r.y;
+    j.y_last = r.y_last;
+    j.x = r.x;
+    j.b = r.b;
+    j.u = r.u;
+    j.ckpt = r.ckpt;
+    j.gb = r.barrier;
+    j.N = r.N;
+    j.T = r.T;
+    j.ckpt_interval = ckpt_interval;
+    j.slot0 = 0;
+    j.width = r.resident_blocks;
+    return j;
+}
+
+// Forward propagation of one beat (prediction.rs:15-105 for every step). With
+// ckpt_interval = K > 0 the pass also stores the filter outputs y_t at
+// t = kK - 1 (k = 1 .. ceil(T/K) - 1) for the reverse-mode gradient.
+int propagate_beat(ct_results_gpu &r, ct_epoch_kernel *k, int32_t ckpt_interval) {
+    hipStream_t s = r.gpu->stream;
+    CT_CHECK(prepare_propagation(r, ckpt_interval));
     const int64_t M = int64_t(r.N) * 78;
-    CT_HIP(hipMemsetAsync(r.y, 0, sizeof(float) * r.N * 78, s));
-    CT_HIP(hipMemsetAsync(r.y_last, 0, sizeof(float) * r.N * 78, s));
     const bool resident = r.resident_ok && r.propagation_mode != 1;
     if (resident && r.N > 0) {
         Span sp(k, kPropagate, r.T);
-        CT_HIP(ct::launch_propagate_resident(s, r.resident_blocks, r.nbr, r.coefs, r.delays, r.gT,
-                                             r.y, r.y_last, r.x, r.b, r.u, r.barrier, r.ckpt,
-                                             ckpt_interval, r.N, 0, r.T, r.T));
+        CT_HIP(ct::launch_propagate_resident(s, r.resident_blocks, resident_job(r, ckpt_interval),
+                                             0, r.T, wait_ticks(*r.gpu, 2.0)));
         r.resident_used = true;
     } else {
         for (int32_t t = 0; t < r.T; ++t) {
@@ -273,16 +147,27 @@ int forward_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_ke
                                       sizeof(float) * M, hipMemcpyDeviceToDevice, s));
         }
     }
-    {
-        Span sp(k, kProject);
-        const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
-        CT_HIP(ct::launch_project(s, r.x, Hb, r.partial, r.T, r.n, r.Sp, r.splits, r.kps));
-        CT_HIP(ct::launch_residuals(s, r.partial, r.splits,
-                                    mhat ? mhat + int64_t(beat) * r.T * r.S : nullptr,
-                                    r.m + int64_t(beat) * r.T * r.S, r.R, r.loss_mse, r.T, r.S,
-                                    r.Sp));
-    }
     return CT_OK;
+}
+
+// The projected measurements of the beat (prediction.rs:112-130 for every
+// step); with `mhat` also residuals and per-step MSE.
+int project_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_kernel *k) {
+    hipStream_t s = r.gpu->stream;
+    r.forward_beat = beat;
+    Span sp(k, kProject);
+    const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
+    CT_HIP(ct::launch_project(s, r.x, Hb, r.partial, r.T, r.n, r.Sp, r.splits, r.kps));
+    CT_HIP(ct::launch_residuals(s, r.partial, r.splits,
+                                mhat ? mhat + int64_t(beat) * r.T * r.S : nullptr,
+                                r.m + int64_t(beat) * r.T * r.S, r.R, r.loss_mse, r.T, r.S, r.Sp));
+    return CT_OK;
+}
+
+int forward_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_kernel *k,
+                 int32_t ckpt_interval) {
+    CT_CHECK(propagate_beat(r, k, ckpt_interval));
+    return project_beat(r, beat, mhat, k);
 }
 
 // Per-step maximum regularisation sums, the running sum and the per-step
@@ -326,62 +211,97 @@ int derive_reference(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, ct_
     return CT_OK;
 }
 
-int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32_t K,
-                   ct_epoch_kernel *k);
+bool uses_adjoint(const ct_epoch_kernel &k) {
+    return k.gradient == 1 && !(k.cfg.freeze_gains && k.cfg.freeze_delays);
+}
 
-int run_beat(ct_epoch_kernel &k, int32_t beat) {
+int finish_beat(ct_epoch_kernel &k, int32_t beat) {
     ct_results_gpu &r = *k.r;
-    const bool adjoint = k.gradient == 1 && !(k.cfg.freeze_gains && k.cfg.freeze_delays);
-    CT_CHECK(forward_beat(r, beat, k.mhat->data, &k, adjoint ? k.checkpoint_interval : 0));
+    CT_CHECK(project_beat(r, beat, k.mhat->data, &k));
     CT_CHECK(losses_beat(r, k.cfg, &k));
-    if (adjoint) return derive_adjoint(r, beat, k.cfg, k.checkpoint_interval, &k);
+    if (uses_adjoint(k)) return derive_adjoint(r, beat, k.cfg, k.checkpoint_interval, &k);
     return derive_reference(r, beat, k.cfg, &k);
 }
 
+static int run_beat(ct_epoch_kernel &k, int32_t beat) {
+    CT_CHECK(propagate_beat(*k.r, &k, uses_adjoint(k) ? k.checkpoint_interval : 0));
+    return finish_beat(k, beat);
+}
+
 // algorithm.rs:188-212 / 224-247 / 255-278: average delays, batch derivatives,
-// parameter update (update.rs:27-97) and batch metrics.
-int finish_batch(ct_epoch_kernel &k, int64_t num_beats, bool reset_after) {
+// parameter update (update.rs:27-97) and batch metrics. `update` numbers the
+// updates within the epoch: its Adam step and batch-metric slot come from the
+// device scalars written by begin_epoch.
+int finish_batch(ct_epoch_kernel &k, int64_t num_beats, bool reset_after, int32_t update) {
     ct_results_gpu &r = *k.r;
     hipStream_t s = r.gpu->stream;
     const ct_algorithm &c = k.cfg;
     Span sp(&k, kUpdate);
-    if (k.batch_index < 0 || k.batch_index >= r.nbatch)
-        return fail(CT_ERR_STATE, "batch index " + std::to_string(k.batch_index) +
-                                      " outside the batch metrics (" + std::to_string(r.nbatch) +
-                                      " entries)");
+    const ct::UpdateScalars *sc = r.scal + update;
     CT_HIP(ct::launch_average_delays(s, r.gT, r.coefs, r.delays, r.avg, r.avg_valid, r.N));
     if (!c.freeze_delays && std::fabs(c.smoothness_regularization_strength - 0.0f) > 1.1920929e-7f)
         CT_HIP(ct::launch_smoothness(s, r.nbr, r.avg, r.avg_valid, r.dA,
                                      c.smoothness_regularization_strength, r.N));
     const int64_t bs = c.batch_size == 0 ? int64_t(r.T) * num_beats : int64_t(r.T) * c.batch_size;
     const float scale = c.learning_rate / float(bs);
-    const float c1 = 1.0f - std::pow(0.9f, float(r.step));
-    const float c2 = 1.0f - std::pow(0.999f, float(r.step));
     if (!c.freeze_gains) {
         if (c.optimizer == 1) {
-            if (!r.gm1) return fail(CT_ERR_STATE, "Adam update without Adam moments (results were "
-                                                  "uploaded with optimizer = Sgd)");
-            CT_HIP(ct::launch_adam(s, r.gT, r.dGT, r.gm1, r.gm2, r.n * 78, c1, c2, scale));
+            CT_HIP(ct::launch_adam(s, r.gT, r.dGT, r.gm1, r.gm2, r.n * 78, sc, scale));
         } else {
             CT_HIP(ct::launch_sgd(s, r.gT, r.dGT, r.n * 78, scale));
         }
     }
     if (!c.freeze_delays) {
         if (c.optimizer == 1) {
-            if (!r.cm1) return fail(CT_ERR_STATE, "Adam update without Adam moments (results were "
-                                                  "uploaded with optimizer = Sgd)");
-            CT_HIP(ct::launch_adam(s, r.coefs, r.dA, r.cm1, r.cm2, int64_t(r.N) * 26, c1, c2, scale));
+            CT_HIP(ct::launch_adam(s, r.coefs, r.dA, r.cm1, r.cm2, int64_t(r.N) * 26, sc, scale));
         } else {
             CT_HIP(ct::launch_sgd(s, r.coefs, r.dA, int64_t(r.N) * 26, scale));
         }
         CT_HIP(ct::launch_roll_delays(s, r.coefs, r.delays, int64_t(r.N) * 26));
     }
-    r.step += 1;
     if (reset_after) CT_CHECK(reset_derivatives(r));
     CT_HIP(ct::launch_batch_metrics(s, r.loss, r.loss_mse, r.loss_mr, r.loss_b, r.loss_mse_b,
-                                    r.loss_mr_b, k.batch_index, r.T));
-    k.batch_index += 1;
+                                    r.loss_mr_b, sc, r.T));
     return CT_OK;
+}
+
+int32_t updates_per_epoch(const ct_epoch_kernel &k) {
+    const int32_t B = k.r->B, bs = k.cfg.batch_size;
+    return bs == 0 ? 1 : (B + bs - 1) / bs;
+}
+
+// Checks the epoch's preconditions and writes its update scalars (Adam step
+// and batch-metric slot of every update) to the device. Waits for the stream
+// first, so the previous epoch no longer reads them.
+int begin_epoch(ct_epoch_kernel &k) {
+    ct_results_gpu &r = *k.r;
+    const ct_algorithm &c = k.cfg;
+    const int32_t updates = updates_per_epoch(k);
+    if (k.batch_index < 0 || int64_t(k.batch_index) + updates > r.nbatch)
+        return fail(CT_ERR_STATE, "batch index " + std::to_string(k.batch_index + updates - 1) +
+                                      " outside the batch metrics (" + std::to_string(r.nbatch) +
+                                      " entries)");
+    if (c.optimizer == 1 && ((!c.freeze_gains && !r.gm1) || (!c.freeze_delays && !r.cm1)))
+        return fail(CT_ERR_STATE, "Adam update without Adam moments (results were uploaded with "
+                                  "optimizer = Sgd)");
+    CT_HIP(hipStreamSynchronize(r.gpu->stream));
+    for (int32_t u = 0; u < updates; ++u) {
+        const float step = float(r.step + u);
+        r.scal_host[u].c1 = 1.0f - std::pow(0.9f, step);
+        r.scal_host[u].c2 = 1.0f - std::pow(0.999f, step);
+        r.scal_host[u].batch_index = k.batch_index + u;
+        r.scal_host[u].pad = 0;
+    }
+    CT_HIP(hipMemcpyAsync(r.scal, r.scal_host, sizeof(ct::UpdateScalars) * updates,
+                          hipMemcpyHostToDevice, r.gpu->stream));
+    return CT_OK;
+}
+
+// Host mirror of the device counters after an epoch's updates were enqueued.
+void end_epoch(ct_epoch_kernel &k) {
+    const int32_t updates = updates_per_epoch(k);
+    k.r->step += updates;
+    k.batch_index += updates;
 }
 
 // Decides whether the resident (on-chip filter state) kernel can run this
@@ -453,6 +373,22 @@ int ensure_checkpoints(ct_results_gpu &r, int32_t K) {
     return CT_OK;
 }
 
+int fix_adjoint_ring(ct_results_gpu &r, int64_t headroom) {
+    hipStream_t s = r.gpu->stream;
+    const int64_t M = int64_t(r.N) * 78;
+    CT_HIP(ct::launch_max_delay(s, r.nbr, r.delays, int64_t(r.N) * 26, r.dmax));
+    int32_t dmax = 0;
+    CT_HIP(hipMemcpyAsync(&dmax, r.dmax, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    CT_HIP(hipStreamSynchronize(s));
+    int64_t slots = 1;
+    while (slots < int64_t(dmax) + 1 + headroom) slots <<= 1;
+    int64_t have_q = r.qring_slots * M;
+    CT_CHECK(ensure(r, &r.qring, have_q, slots * M, 1));
+    r.qring_slots = std::max(r.qring_slots, slots);
+    r.ring_slots_fixed = slots;
+    return CT_OK;
+}
+
 // Consumer lists (CSR): for each voxel w, the connections (v, d) that read
 // it, in ascending connection order.
 int build_consumers(ct_results_gpu &r) {
@@ -491,12 +427,15 @@ int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32
     CT_CHECK(ensure(r, &r.xbarT, have_xbt, n * T, 1));
     CT_CHECK(ensure(r, &r.ybar, have_yb, M, 1));
     // ring of q over R >= Dmax + 1 slots (R a power of two)
-    CT_HIP(ct::launch_max_delay(s, r.nbr, r.delays, int64_t(r.N) * 26, r.dmax));
-    int32_t dmax = 0;
-    CT_HIP(hipMemcpyAsync(&dmax, r.dmax, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    CT_HIP(hipStreamSynchronize(s));
-    int64_t slots = 1;
-    while (slots < int64_t(dmax) + 1) slots <<= 1;
+    int64_t slots = r.ring_slots_fixed;
+    if (slots <= 0) {
+        CT_HIP(ct::launch_max_delay(s, r.nbr, r.delays, int64_t(r.N) * 26, r.dmax));
+        int32_t dmax = 0;
+        CT_HIP(hipMemcpyAsync(&dmax, r.dmax, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        CT_HIP(hipStreamSynchronize(s));
+        slots = 1;
+        while (slots < int64_t(dmax) + 1) slots <<= 1;
+    }
     int64_t have_q = r.qring_slots * M;
     CT_CHECK(ensure(r, &r.qring, have_q, slots * M, 1));
     r.qring_slots = std::max(r.qring_slots, slots);
@@ -523,7 +462,7 @@ int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32
     p.threshold = c.maximum_regularization_threshold;
     {
         Span sp(k, kAdjointSweep);
-        CT_HIP(hipMemsetAsync(r.ybar, 0, sizeof(float) * M, s));
+        CT_HIP(ct::launch_fill(s, r.ybar, 0.0f, M));
         CT_HIP(ct::launch_consumer_delays(s, r.cons_idx, r.delays, r.cons_delay, r.cons_len));
         for (int32_t step = r.T - 1; step >= 0; --step)
             CT_HIP(ct::launch_adjoint_step(s, r.nbr, r.coefs, r.staging, r.cons_ptr, r.cons_idx,
@@ -538,7 +477,9 @@ int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32
     return CT_OK;
 }
 
-}  // namespace
+}  // namespace ctrt
+
+using namespace ctrt;
 
 extern "C" {
 
@@ -554,6 +495,10 @@ int ct_gpu_new(int32_t device, ct_gpu **out) {
     CT_HIP(hipSetDevice(device));
     ct_gpu *g = new ct_gpu;
     g->device = device;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess &&
+        khz > 0)
+        g->wall_ticks_per_second = uint64_t(khz) * 1000u;
     hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete g;
@@ -703,7 +648,13 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
     ALLOC(r->dmax, 1);
     ALLOC(r->u, T);
     ALLOC(r->barrier, 1);
+    ALLOC(r->scal, std::max<int64_t>(B, 1));
 #undef ALLOC
+    if (hipHostMalloc(reinterpret_cast<void **>(&r->scal_host),
+                      sizeof(ct::UpdateScalars) * std::max<int64_t>(B, 1)) != hipSuccess) {
+        r->scal_host = nullptr;
+        return cleanup(fail(CT_ERR_DEVICE, "hipHostMalloc of the update scalars failed"));
+    }
 
     hipStream_t s = gpu->stream;
     auto zero = [&](void *p, size_t bytes) -> int {
@@ -719,6 +670,7 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
         if ((rc = zero(p, sizeof(float) * T)) != CT_OK) return cleanup(rc);
     for (float *p : {r->loss_b, r->loss_mse_b, r->loss_mr_b})
         if ((rc = zero(p, sizeof(float) * r->nbatch)) != CT_OK) return cleanup(rc);
+    if ((rc = zero(r->barrier, sizeof(ct::GridBarrier))) != CT_OK) return cleanup(rc);
     if ((rc = zero(r->y, sizeof(float) * N * 78)) != CT_OK) return cleanup(rc);
     if ((rc = zero(r->y_last, sizeof(float) * N * 78)) != CT_OK) return cleanup(rc);
     if ((rc = zero(r->avg, sizeof(float) * N)) != CT_OK) return cleanup(rc);
@@ -871,25 +823,28 @@ int ct_epoch_kernel_execute(ct_epoch_kernel *k) {
             k->events.push_back(e);
         }
     }
+    CT_CHECK(begin_epoch(*k));
     Span total(k, kTotal);
     CT_CHECK(reset_derivatives(r));
     const bool batched = c.batch_size != 0;
     int64_t in_batch = 0;
+    int32_t update = 0;
     for (int32_t beat : k->order) {
         CT_CHECK(run_beat(*k, beat));
         if (batched) {
             in_batch += 1;
             if (in_batch == c.batch_size) {
-                CT_CHECK(finish_batch(*k, in_batch, true));
+                CT_CHECK(finish_batch(*k, in_batch, true, update++));
                 in_batch = 0;
             }
         }
     }
     if (batched) {
-        if (in_batch > 0) CT_CHECK(finish_batch(*k, in_batch, false));
+        if (in_batch > 0) CT_CHECK(finish_batch(*k, in_batch, false, update++));
     } else {
-        CT_CHECK(finish_batch(*k, r.B, false));
+        CT_CHECK(finish_batch(*k, r.B, false, update++));
     }
+    end_epoch(*k);
     return check_resident(r);
 }
 

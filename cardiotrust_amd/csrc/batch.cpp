@@ -1,0 +1,580 @@
+// Batched epochs: many scenarios (each a ct_epoch_kernel with its own results,
+// measurements and configuration) advanced by one epoch together.
+//
+// The epoch of every scenario keeps its own order of operations
+// (algorithm.rs:124-281), so each scenario's results equal those of its own
+// ct_epoch_kernel_execute. What changes is how the work is laid out on the
+// device: the beats are taken in rounds (round k = the k-th beat of every
+// scenario that has one), and the propagation of all scenarios of a round runs
+// in ONE persistent launch (propagate_batch_kernel): a job table holds each
+// scenario's arrays, and every workgroup slot walks its own task list. The
+// planner packs the jobs (team width = ceil(N / voxels per workgroup) slots,
+// length ~ T steps) onto the slots longest first, and the autotuner picks the
+// voxels per workgroup by timing the first round. Optionally the whole epoch
+// is captured once in a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+
+#include "device.h"
+#include "runtime.h"
+
+using namespace ctrt;
+
+struct ct_batch {
+    ct_gpu *gpu = nullptr;
+    std::vector<ct_epoch_kernel *> kernels;
+    int32_t slots = 0;              // resident workgroups of the batched launch
+    int32_t voxels_per_group = 0;   // 0 = autotune on the first epoch
+    std::vector<int32_t> tuned;     // candidates tried, for the profile
+    std::vector<double> tuned_ms;
+
+    // plan: per round, the jobs (scenario indices) and their device tables
+    struct Round {
+        std::vector<int32_t> scenarios;   // all scenarios with a beat in this round
+        std::vector<int32_t> resident;    // those in the batched launch (job order)
+        int32_t job0 = 0;                 // first job / barrier of the round
+        int32_t task0 = 0;                // first task of the round
+        int32_t ptr0 = 0;                 // first task_ptr entry (slots + 1 per round)
+        double predicted_ms = 0.0;
+    };
+    std::vector<Round> rounds;
+    std::vector<int32_t> plan_checkpoints;  // checkpoint interval per scenario the plan used
+    int32_t plan_vpg = -1;
+    ct::ResidentJob *jobs = nullptr;
+    int32_t *task_ptr = nullptr, *task_job = nullptr;
+    ct::GridBarrier *barriers = nullptr;
+    int32_t num_jobs = 0;
+
+    // graph of one epoch
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool warmed = false;  // an eager epoch ran since the last plan (all buffers exist)
+    int32_t graph_epochs_left = 0;  // replays before the adjoint rings are re-sized
+    std::vector<std::vector<int32_t>> graph_orders;  // what the captured epoch was built for
+    std::vector<ct_algorithm> graph_cfgs;
+    std::vector<int32_t> graph_gradients;
+
+    // profiling (whole-epoch and propagation-launch times)
+    bool profiling = false;
+    hipEvent_t ev[8] = {};
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prop_events;
+    double last_epoch_ms = 0.0, last_propagate_ms = 0.0;
+    int64_t last_propagate_launches = 0, last_voxel_steps = 0;
+
+    std::vector<void *> allocations;
+
+    ~ct_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto &p : prop_events) {
+            (void)hipEventDestroy(p.first);
+            (void)hipEventDestroy(p.second);
+        }
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (void *p : allocations) (void)hipFree(p);
+    }
+};
+
+namespace {
+
+int batch_checkpoint(const ct_epoch_kernel &k) {
+    return uses_adjoint(k) ? k.checkpoint_interval : 0;
+}
+
+bool scenario_resident(const ct_results_gpu &r) {
+    return r.N > 0 && r.T > 0 && r.propagation_mode != 1 && r.resident_ok;
+}
+
+// Contiguous-window placement of the jobs of one round onto `slots`
+// workgroup slots (longest first). Fills job slot0 / width and the per-slot
+// task lists in one global order (start time, then placement), which keeps
+// the launch deadlock-free. Returns the predicted makespan (ms).
+double place_round(const std::vector<ct_results_gpu *> &rs, int32_t slots, int32_t vpg,
+                   std::vector<ct::ResidentJob> &jobs, std::vector<std::vector<int32_t>> &tasks) {
+    const int32_t nj = int32_t(rs.size());
+    std::vector<double> cost(nj);
+    std::vector<int32_t> order(nj);
+    for (int32_t j = 0; j < nj; ++j) {
+        const int64_t N = rs[j]->N;
+        int64_t width = std::min<int64_t>(slots, std::max<int64_t>(1, (N + vpg - 1) / vpg));
+        // the LDS holds at most kResMaxVoxels voxels per workgroup
+        width = std::max<int64_t>(width, (N + ct::kResMaxVoxels - 1) / ct::kResMaxVoxels);
+        jobs[j].width = int32_t(width);
+        const double per_group = double((N + width - 1) / width);
+        // a step costs a barrier (~4 us) plus work proportional to the slab
+        cost[j] = double(rs[j]->T) * (4.0e-3 + 1.0e-4 * per_group);
+        order[j] = j;
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+    std::vector<double> avail(slots, 0.0), start(nj, 0.0);
+    const int32_t per_xcd = std::max(1, slots / ct::kNumXcds);
+    for (int32_t j : order) {
+        const int32_t w = jobs[j].width;
+        double best = 1e300;
+        int32_t best_s0 = 0;
+        bool best_cross = true;
+        for (int32_t s0 = 0; s0 + w <= slots; ++s0) {
+            double st = 0.0;
+            for (int32_t q = s0; q < s0 + w; ++q) st = std::max(st, avail[q]);
+            // prefer windows inside one XCD (shared L2) at equal start
+            const bool cross = w <= per_xcd && (s0 / per_xcd) != ((s0 + w - 1) / per_xcd);
+            if (st < best - 1e-9 || (st < best + 1e-9 && best_cross && !cross)) {
+                best = st;
+                best_s0 = s0;
+                best_cross = cross;
+            }
+        }
+        jobs[j].slot0 = best_s0;
+        start[j] = best;
+        for (int32_t q = best_s0; q < best_s0 + w; ++q) avail[q] = best + cost[j];
+    }
+    // one global order for every slot: by start time, ties by placement
+    std::vector<int32_t> rank(nj);
+    for (int32_t i = 0; i < nj; ++i) rank[order[i]] = i;
+    std::vector<int32_t> global(nj);
+    for (int32_t j = 0; j < nj; ++j) global[j] = j;
+    std::stable_sort(global.begin(), global.end(), [&](int32_t a, int32_t b) {
+        if (start[a] != start[b]) return start[a] < start[b];
+        return rank[a] < rank[b];
+    });
+    tasks.assign(slots, {});
+    for (int32_t j : global)
+        for (int32_t q = jobs[j].slot0; q < jobs[j].slot0 + jobs[j].width; ++q)
+            tasks[q].push_back(j);
+    double makespan = 0.0;
+    for (double a : avail) makespan = std::max(makespan, a);
+    return makespan;
+}
+
+template <typename V>
+int batch_alloc(ct_batch &b, V **p, int64_t count) {
+    void *raw = nullptr;
+    CT_HIP(hipMalloc(&raw, std::max<size_t>(sizeof(V) * size_t(std::max<int64_t>(count, 1)), 256)));
+    b.allocations.push_back(raw);
+    *p = static_cast<V *>(raw);
+    return CT_OK;
+}
+
+void batch_free(ct_batch &b, void *p) {
+    if (!p) return;
+    auto it = std::find(b.allocations.begin(), b.allocations.end(), p);
+    if (it != b.allocations.end()) b.allocations.erase(it);
+    (void)hipFree(p);
+}
+
+void drop_graph(ct_batch &b) {
+    if (b.exec) (void)hipGraphExecDestroy(b.exec);
+    if (b.graph) (void)hipGraphDestroy(b.graph);
+    b.exec = nullptr;
+    b.graph = nullptr;
+    b.warmed = false;
+}
+
+// Builds the rounds and the device job tables for `vpg` voxels per group.
+int plan(ct_batch &b, int32_t vpg) {
+    CT_HIP(hipStreamSynchronize(b.gpu->stream));
+    drop_graph(b);
+    const int32_t S = int32_t(b.kernels.size());
+    int32_t rounds = 0;
+    for (ct_epoch_kernel *k : b.kernels) rounds = std::max(rounds, k->r->B);
+    b.rounds.assign(rounds, {});
+    b.plan_checkpoints.resize(S);
+    std::vector<ct::ResidentJob> all_jobs;
+    std::vector<int32_t> all_ptr, all_task;
+    for (int32_t s = 0; s < S; ++s) {
+        ct_epoch_kernel &k = *b.kernels[s];
+        b.plan_checkpoints[s] = batch_checkpoint(k);
+        // checkpoint buffers are allocated now: the job table holds their address
+        if (b.plan_checkpoints[s] > 0 && k.r->N > 0)
+            CT_CHECK(ensure_checkpoints(*k.r, b.plan_checkpoints[s]));
+    }
+    for (int32_t rd = 0; rd < rounds; ++rd) {
+        ct_batch::Round &R = b.rounds[rd];
+        std::vector<ct_results_gpu *> rs;
+        for (int32_t s = 0; s < S; ++s) {
+            ct_results_gpu &r = *b.kernels[s]->r;
+            if (r.B <= rd) continue;
+            R.scenarios.push_back(s);
+            const bool fits = scenario_resident(r) &&
+                              (r.N + ct::kResMaxVoxels - 1) / ct::kResMaxVoxels <= b.slots;
+            if (fits && b.slots > 0) {
+                R.resident.push_back(s);
+                rs.push_back(&r);
+            }
+        }
+        R.job0 = int32_t(all_jobs.size());
+        R.task0 = int32_t(all_task.size());
+        R.ptr0 = int32_t(all_ptr.size());
+        std::vector<ct::ResidentJob> jobs;
+        for (int32_t s : R.resident) {
+            ct_results_gpu &r = *b.kernels[s]->r;
+            jobs.push_back(resident_job(r, b.plan_checkpoints[s]));
+        }
+        std::vector<std::vector<int32_t>> tasks;
+        R.predicted_ms = rs.empty() ? 0.0 : place_round(rs, b.slots, vpg, jobs, tasks);
+        all_jobs.insert(all_jobs.end(), jobs.begin(), jobs.end());
+        // task_ptr of this round: slots + 1 entries into task_job, whose
+        // entries index the round's jobs (the launch gets jobs + job0)
+        int32_t acc = R.task0;
+        for (int32_t q = 0; q < b.slots; ++q) {
+            all_ptr.push_back(acc);
+            if (!tasks.empty())
+                for (int32_t j : tasks[q]) {
+                    all_task.push_back(j);
+                    ++acc;
+                }
+        }
+        all_ptr.push_back(acc);
+    }
+    // device tables (barrier addresses patched in now that the array exists)
+    batch_free(b, b.jobs);
+    batch_free(b, b.task_ptr);
+    batch_free(b, b.task_job);
+    batch_free(b, b.barriers);
+    b.jobs = nullptr;
+    b.task_ptr = b.task_job = nullptr;
+    b.barriers = nullptr;
+    b.num_jobs = int32_t(all_jobs.size());
+    CT_CHECK(batch_alloc(b, &b.jobs, b.num_jobs));
+    CT_CHECK(batch_alloc(b, &b.task_ptr, int64_t(all_ptr.size())));
+    CT_CHECK(batch_alloc(b, &b.task_job, int64_t(all_task.size())));
+    CT_CHECK(batch_alloc(b, &b.barriers, b.num_jobs));
+    for (size_t j = 0; j < all_jobs.size(); ++j) all_jobs[j].gb = b.barriers + j;
+    CT_HIP(hipMemset(b.barriers, 0, sizeof(ct::GridBarrier) * std::max(b.num_jobs, 1)));
+    CT_CHECK(upload(b.jobs, all_jobs.data(), sizeof(ct::ResidentJob) * all_jobs.size()));
+    CT_CHECK(upload(b.task_ptr, all_ptr.data(), sizeof(int32_t) * all_ptr.size()));
+    CT_CHECK(upload(b.task_job, all_task.data(), sizeof(int32_t) * all_task.size()));
+    b.plan_vpg = vpg;
+    return CT_OK;
+}
+
+bool plan_current(const ct_batch &b) {
+    if (b.plan_vpg != b.voxels_per_group) return false;
+    for (size_t s = 0; s < b.kernels.size(); ++s)
+        if (b.plan_checkpoints[s] != batch_checkpoint(*b.kernels[s])) return false;
+    return true;
+}
+
+// The propagation of every scenario of round `rd`: the batched launch for the
+// resident jobs, the per-step path for the rest.
+int propagate_round(ct_batch &b, int32_t rd, bool timed) {
+    const ct_batch::Round &R = b.rounds[rd];
+    hipStream_t s = b.gpu->stream;
+    for (int32_t sc : R.scenarios) {
+        ct_epoch_kernel &k = *b.kernels[sc];
+        const bool in_launch =
+            std::find(R.resident.begin(), R.resident.end(), sc) != R.resident.end();
+        if (in_launch)
+            CT_CHECK(prepare_propagation(*k.r, b.plan_checkpoints[sc]));
+        else
+            CT_CHECK(propagate_beat(*k.r, nullptr, b.plan_checkpoints[sc]));
+    }
+    if (R.resident.empty()) return CT_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timed) {
+        CT_HIP(hipEventCreate(&e0));
+        CT_HIP(hipEventCreate(&e1));
+        b.prop_events.push_back({e0, e1});
+        CT_HIP(hipEventRecord(e0, s));
+    }
+    CT_HIP(ct::launch_propagate_batch(s, b.slots, b.jobs + R.job0, b.task_ptr + R.ptr0,
+                                      b.task_job, b.barriers + R.job0,
+                                      int32_t(R.resident.size()), wait_ticks(*b.gpu, 60.0),
+                                      wait_ticks(*b.gpu, 2.0)));
+    if (timed) CT_HIP(hipEventRecord(e1, s));
+    return CT_OK;
+}
+
+// One epoch of every scenario, enqueued on the stream (no synchronisation).
+int enqueue_epoch(ct_batch &b, bool timed) {
+    const int32_t S = int32_t(b.kernels.size());
+    std::vector<int64_t> in_batch(S, 0);
+    std::vector<int32_t> update(S, 0);
+    for (int32_t s = 0; s < S; ++s) CT_CHECK(reset_derivatives(*b.kernels[s]->r));
+    for (int32_t rd = 0; rd < int32_t(b.rounds.size()); ++rd) {
+        CT_CHECK(propagate_round(b, rd, timed));
+        for (int32_t s : b.rounds[rd].scenarios) {
+            ct_epoch_kernel &k = *b.kernels[s];
+            CT_CHECK(finish_beat(k, k.order[rd]));
+            if (k.cfg.batch_size != 0 && ++in_batch[s] == k.cfg.batch_size) {
+                CT_CHECK(finish_batch(k, in_batch[s], true, update[s]++));
+                in_batch[s] = 0;
+            }
+        }
+    }
+    for (int32_t s = 0; s < S; ++s) {
+        ct_epoch_kernel &k = *b.kernels[s];
+        if (k.cfg.batch_size != 0) {
+            if (in_batch[s] > 0) CT_CHECK(finish_batch(k, in_batch[s], false, update[s]++));
+        } else {
+            CT_CHECK(finish_batch(k, k.r->B, false, update[s]++));
+        }
+    }
+    return CT_OK;
+}
+
+// Reads the abort words of every barrier after an epoch.
+int check_barriers(ct_batch &b) {
+    CT_HIP(hipStreamSynchronize(b.gpu->stream));
+    if (b.num_jobs <= 0) return CT_OK;
+    std::vector<ct::GridBarrier> gb(b.num_jobs);
+    CT_CHECK(download(gb.data(), b.barriers, sizeof(ct::GridBarrier) * gb.size()));
+    for (const auto &g : gb)
+        if (g.abort.count) {
+            b.slots = 0;  // stop using the batched launch
+            drop_graph(b);
+            return fail(CT_ERR_DEVICE, "batched propagation: a barrier timed out (workgroups not "
+                                       "all resident); the per-scenario paths are used from now on");
+        }
+    return CT_OK;
+}
+
+// Autotuner: times round 0's batched propagation for each candidate number of
+// voxels per workgroup and keeps the fastest. The timed launches write the
+// same arrays the epoch then recomputes from scratch.
+int autotune(ct_batch &b) {
+    static const int32_t kCandidates[] = {472, 384, 320, 256, 192, 160, 128, 96, 64};
+    b.tuned.clear();
+    b.tuned_ms.clear();
+    int32_t best = ct::kResMaxVoxels;
+    double best_ms = 1e300;
+    std::vector<std::vector<int32_t>> seen;
+    for (int32_t vpg : kCandidates) {
+        CT_CHECK(plan(b, vpg));
+        if (b.rounds.empty() || b.rounds[0].resident.empty()) return CT_OK;
+        // skip candidates that give the same team widths as one already timed
+        std::vector<ct::ResidentJob> j0(b.rounds[0].resident.size());
+        CT_CHECK(download(j0.data(), b.jobs + b.rounds[0].job0, sizeof(ct::ResidentJob) * j0.size()));
+        std::vector<int32_t> widths;
+        for (auto &j : j0) widths.push_back(j.width);
+        if (std::find(seen.begin(), seen.end(), widths) != seen.end()) continue;
+        seen.push_back(widths);
+        hipEvent_t e0, e1;
+        CT_HIP(hipEventCreate(&e0));
+        CT_HIP(hipEventCreate(&e1));
+        int rc = propagate_round(b, 0, false);
+        if (rc == CT_OK && hipEventRecord(e0, b.gpu->stream) != hipSuccess) rc = CT_ERR_DEVICE;
+        if (rc == CT_OK) rc = propagate_round(b, 0, false);
+        if (rc == CT_OK && hipEventRecord(e1, b.gpu->stream) != hipSuccess) rc = CT_ERR_DEVICE;
+        float ms = 0.0f;
+        if (rc == CT_OK && hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        CT_CHECK(rc);
+        CT_CHECK(check_barriers(b));
+        b.tuned.push_back(vpg);
+        b.tuned_ms.push_back(ms);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = vpg;
+        }
+    }
+    b.voxels_per_group = best;
+    return CT_OK;
+}
+
+constexpr int32_t kGraphEpochs = 16;
+
+bool same_cfg(const ct_algorithm &a, const ct_algorithm &b) {
+    return std::memcmp(&a, &b, sizeof(ct_algorithm)) == 0;
+}
+
+// Is the captured epoch still the epoch the scenarios would run now?
+bool graph_current(const ct_batch &b) {
+    if (!b.exec || b.graph_epochs_left <= 0) return false;
+    for (size_t s = 0; s < b.kernels.size(); ++s) {
+        const ct_epoch_kernel &k = *b.kernels[s];
+        if (k.order != b.graph_orders[s] || !same_cfg(k.cfg, b.graph_cfgs[s]) ||
+            k.gradient * 100000 + k.checkpoint_interval != b.graph_gradients[s])
+            return false;
+    }
+    return true;
+}
+
+// Captures one epoch of the batch into a graph. The adjoint's q rings get a
+// fixed size with room for the delays to grow during kGraphEpochs replays.
+int capture(ct_batch &b) {
+    hipStream_t s = b.gpu->stream;
+    drop_graph(b);
+    b.warmed = true;
+    b.graph_orders.clear();
+    b.graph_cfgs.clear();
+    b.graph_gradients.clear();
+    for (ct_epoch_kernel *k : b.kernels) {
+        if (uses_adjoint(*k))
+            CT_CHECK(fix_adjoint_ring(*k->r, int64_t(kGraphEpochs) * updates_per_epoch(*k)));
+        b.graph_orders.push_back(k->order);
+        b.graph_cfgs.push_back(k->cfg);
+        b.graph_gradients.push_back(k->gradient * 100000 + k->checkpoint_interval);
+    }
+    CT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_epoch(b, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    for (ct_epoch_kernel *k : b.kernels) k->r->ring_slots_fixed = 0;
+    if (rc != CT_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    CT_HIP(e);
+    b.graph = g;
+    CT_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+    b.graph_epochs_left = kGraphEpochs;
+    return CT_OK;
+}
+
+int clear_prop_events(ct_batch &b) {
+    for (auto &p : b.prop_events) {
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    b.prop_events.clear();
+    return CT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ct_batch_new(ct_gpu *gpu, ct_epoch_kernel *const *kernels, int32_t count, ct_batch **out) {
+    if (!gpu || !out || (count > 0 && !kernels)) return fail(CT_ERR_INVALID, "null argument");
+    if (count < 1) return fail(CT_ERR_INVALID, "a batch needs at least one scenario");
+    CT_CHECK(check_gpu(gpu));
+    for (int32_t i = 0; i < count; ++i) {
+        if (!kernels[i]) return fail(CT_ERR_INVALID, "null epoch kernel in the batch");
+        if (kernels[i]->gpu != gpu || kernels[i]->r->gpu != gpu)
+            return fail(CT_ERR_INVALID, "every scenario of a batch must live on the batch's GPU");
+        for (int32_t j = 0; j < i; ++j)
+            if (kernels[j]->r == kernels[i]->r)
+                return fail(CT_ERR_INVALID, "two scenarios of a batch share one results object");
+    }
+    ct_batch *b = new ct_batch;
+    b->gpu = gpu;
+    b->kernels.assign(kernels, kernels + count);
+    int cus = 0;
+    int32_t per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, gpu->device) ==
+            hipSuccess &&
+        ct::batch_capacity(&per_cu) == hipSuccess && cus > 0 && per_cu > 0)
+        b->slots = cus * per_cu;
+    (void)hipGetLastError();
+    *out = b;
+    return CT_OK;
+}
+
+void ct_batch_free(ct_batch *batch) {
+    if (!batch) return;
+    (void)hipSetDevice(batch->gpu->device);
+    (void)hipStreamSynchronize(batch->gpu->stream);
+    delete batch;
+}
+
+int ct_batch_set_voxels_per_group(ct_batch *b, int32_t voxels) {
+    if (!b) return fail(CT_ERR_INVALID, "null argument");
+    if (voxels != 0 && (voxels < 1 || voxels > ct::kResMaxVoxels))
+        return fail(CT_ERR_INVALID, "voxels per workgroup must be 0 (autotune) or 1.." +
+                                        std::to_string(ct::kResMaxVoxels));
+    b->voxels_per_group = voxels;
+    return CT_OK;
+}
+
+int ct_batch_set_graph(ct_batch *b, int32_t enabled) {
+    if (!b) return fail(CT_ERR_INVALID, "null argument");
+    CT_CHECK(check_gpu(b->gpu));
+    CT_HIP(hipStreamSynchronize(b->gpu->stream));
+    drop_graph(*b);
+    b->use_graph = enabled != 0;
+    return CT_OK;
+}
+
+int ct_batch_set_profiling(ct_batch *b, int32_t enabled) {
+    if (!b) return fail(CT_ERR_INVALID, "null argument");
+    b->profiling = enabled != 0;
+    return CT_OK;
+}
+
+int ct_batch_execute(ct_batch *b) {
+    if (!b) return fail(CT_ERR_INVALID, "null argument");
+    ct_batch &B = *b;
+    CT_CHECK(check_gpu(B.gpu));
+    hipStream_t s = B.gpu->stream;
+    for (ct_epoch_kernel *k : B.kernels) {
+        if (k->mhat->beats != k->r->B || k->mhat->steps != k->r->T)
+            return fail(CT_ERR_INVALID, "measurements do not match a scenario of the batch");
+        CT_CHECK(begin_epoch(*k));
+    }
+    if (B.voxels_per_group == 0 && B.slots > 0) CT_CHECK(autotune(B));
+    if (!plan_current(B)) CT_CHECK(plan(B, B.voxels_per_group > 0 ? B.voxels_per_group
+                                                                  : ct::kResMaxVoxels));
+    CT_CHECK(clear_prop_events(B));
+    for (int i = 0; i < 2; ++i)
+        if (!B.ev[i]) CT_HIP(hipEventCreate(&B.ev[i]));
+    // per-scenario phase profiling stays off inside a batch (one timeline)
+    std::vector<bool> prof;
+    for (ct_epoch_kernel *k : B.kernels) {
+        prof.push_back(k->profiling);
+        k->profiling = false;
+    }
+    int rc = CT_OK;
+    const bool graph = B.use_graph && B.warmed;
+    if (graph && !graph_current(B)) rc = capture(B);
+    if (rc == CT_OK && hipEventRecord(B.ev[0], s) != hipSuccess) rc = fail(CT_ERR_DEVICE, "event");
+    if (rc == CT_OK) {
+        if (graph) {
+            if (hipGraphLaunch(B.exec, s) != hipSuccess)
+                rc = fail(CT_ERR_DEVICE, "hipGraphLaunch of the batched epoch failed");
+            B.graph_epochs_left -= 1;
+        } else {
+            rc = enqueue_epoch(B, B.profiling);
+            B.warmed = true;
+        }
+    }
+    for (size_t i = 0; i < B.kernels.size(); ++i) B.kernels[i]->profiling = prof[i];
+    CT_CHECK(rc);
+    CT_HIP(hipEventRecord(B.ev[1], s));
+    for (ct_epoch_kernel *k : B.kernels) end_epoch(*k);
+    CT_CHECK(check_barriers(B));
+    float ms = 0.0f;
+    CT_HIP(hipEventElapsedTime(&ms, B.ev[0], B.ev[1]));
+    B.last_epoch_ms = ms;
+    B.last_propagate_ms = 0.0;
+    for (auto &p : B.prop_events) {
+        float pm = 0.0f;
+        CT_HIP(hipEventElapsedTime(&pm, p.first, p.second));
+        B.last_propagate_ms += pm;
+    }
+    B.last_propagate_launches = int64_t(B.prop_events.size());
+    int64_t vs = 0;
+    for (ct_epoch_kernel *k : B.kernels) vs += int64_t(k->r->N) * k->r->T * k->r->B;
+    B.last_voxel_steps = vs;
+    return CT_OK;
+}
+
+int ct_batch_get_info(ct_batch *b, ct_batch_info *out) {
+    if (!b || !out) return fail(CT_ERR_INVALID, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    out->scenarios = int32_t(b->kernels.size());
+    out->rounds = int32_t(b->rounds.size());
+    out->slots = b->slots;
+    out->voxels_per_group = b->plan_vpg > 0 ? b->plan_vpg : b->voxels_per_group;
+    out->jobs = b->num_jobs;
+    for (const auto &R : b->rounds) out->predicted_propagate_ms += R.predicted_ms;
+    out->epoch_ms = b->last_epoch_ms;
+    out->propagate_ms = b->last_propagate_ms;
+    out->propagate_launches = b->last_propagate_launches;
+    out->voxel_steps = b->last_voxel_steps;
+    out->graph = b->exec != nullptr ? 1 : 0;
+    const int32_t nt = int32_t(std::min<size_t>(b->tuned.size(), 16));
+    out->tuned_candidates = nt;
+    for (int32_t i = 0; i < nt; ++i) {
+        out->tuned_voxels[i] = b->tuned[i];
+        out->tuned_ms[i] = b->tuned_ms[i];
+    }
+    return CT_OK;
+}
+
+}  // extern "C"

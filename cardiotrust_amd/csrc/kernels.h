@@ -44,16 +44,41 @@ struct GridBarrier {
     BarrierWord group[kBarrierGroups];
     BarrierWord top;
     BarrierWord generation;
-    BarrierWord abort;
+    BarrierWord abort;  // sticky: set on a timed-out wait, cleared only by the host
+};
+static_assert(sizeof(GridBarrier) == 11 * sizeof(BarrierWord), "barrier layout");
+// One job of the resident propagation: one scenario's beat. `gb` is the
+// job's barrier; slot0 / width place its team in a batched launch.
+struct ResidentJob {
+    const int32_t *nbr;
+    const float *coefs;
+    const int32_t *delays;
+    const float *gT;
+    float *y;
+    float *y_last;
+    float *x;
+    const float *b;
+    const float *u;
+    float *ckpt;          // filter outputs every ckpt_interval steps (or unused)
+    GridBarrier *gb;
+    int32_t N;
+    int32_t T;
+    int32_t ckpt_interval;  // 0 = no checkpoints
+    int32_t slot0;
+    int32_t width;          // workgroups in the team
 };
 size_t resident_lds_bytes();
 hipError_t resident_capacity(int32_t *blocks_per_cu);
-hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const int32_t *nbr,
-                                     const float *coefs, const int32_t *delays, const float *gT,
-                                     float *y, float *y_last, float *x, const float *b,
-                                     const float *u, GridBarrier *gb, float *ckpt,
-                                     int32_t ckpt_interval, int32_t N, int32_t t_begin,
-                                     int32_t t_end, int32_t T);
+hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const ResidentJob &job,
+                                     int32_t t_begin, int32_t t_end, uint64_t wait_ticks);
+// Batched resident propagation: `slots` workgroups (all resident at once) run
+// the task lists task_job[task_ptr[q] .. task_ptr[q+1]) of slot q; `jobs` and
+// the `num_jobs` barriers are device arrays.
+hipError_t batch_capacity(int32_t *blocks_per_cu);
+hipError_t launch_propagate_batch(hipStream_t s, int32_t slots, const ResidentJob *jobs,
+                                  const int32_t *task_ptr, const int32_t *task_job,
+                                  GridBarrier *barriers, int32_t num_jobs, uint64_t first_wait,
+                                  uint64_t step_wait);
 
 hipError_t launch_project(hipStream_t s, const float *x, const float *H, float *partial,
                           int32_t T, int64_t n, int32_t Sp, int32_t splits, int64_t k_per_split);
@@ -102,8 +127,18 @@ hipError_t launch_adjoint_gradient(hipStream_t s, const int32_t *nbr, const floa
                                    float *dA, int32_t N, int32_t T, int32_t K);
 
 hipError_t launch_sgd(hipStream_t s, float *v, const float *d, int64_t len, float scale);
+// Per-update scalars of the parameter update, read by the kernels from
+// device memory so that a captured epoch replays with each epoch's values:
+// Adam bias corrections c1 = 1 - 0.9^step, c2 = 1 - 0.999^step and the
+// batch-metric slot.
+struct UpdateScalars {
+    float c1;
+    float c2;
+    int32_t batch_index;
+    int32_t pad;
+};
 hipError_t launch_adam(hipStream_t s, float *v, const float *d, float *m1, float *m2, int64_t len,
-                       float c1, float c2, float scale);
+                       const UpdateScalars *sc, float scale);
 hipError_t launch_roll_delays(hipStream_t s, float *coefs, int32_t *delays, int64_t len);
 
 hipError_t launch_average_delays(hipStream_t s, const float *gT, const float *coefs,
@@ -113,7 +148,7 @@ hipError_t launch_smoothness(hipStream_t s, const int32_t *nbr, const float *avg
 
 hipError_t launch_batch_metrics(hipStream_t s, const float *loss, const float *loss_mse,
                                 const float *loss_mr, float *loss_b, float *loss_mse_b,
-                                float *loss_mr_b, int32_t idx, int32_t T);
+                                float *loss_mr_b, const UpdateScalars *sc, int32_t T);
 
 hipError_t launch_expand_rows(hipStream_t s, const float *in, float *out, int64_t N);
 

@@ -115,35 +115,34 @@ __global__ __launch_bounds__(kPropThreads) void propagate_step_kernel(
 // ---------------------------------------------------------------------------
 // Resident propagation: one launch runs a range of time steps.
 //
-// Workgroup g owns a contiguous slab of voxels for the whole launch. Its
-// filter outputs y (78 per voxel) live in LDS and its connection tables
-// (neighbour, delay, coefficient) in registers, so per step only the gains
-// and the delayed neighbour states are read from memory. Phase 1 (one lane
-// per connection) and phase 2 (one lane per state) are the same arithmetic as
-// propagate_step_kernel; a grid-wide barrier separates the steps.
+// A workgroup owns a contiguous slab of voxels of one job (one scenario's
+// beat) for all of its steps. Its filter outputs y (78 per voxel) live in LDS
+// and its connection tables (neighbour, delay, coefficient) in registers, so
+// per step only the gains and the delayed neighbour states are read from
+// memory. Phase 1 (one lane per connection) and phase 2 (one lane per state)
+// are the same arithmetic as propagate_step_kernel; a barrier over the
+// workgroups of the job (its team) separates the steps.
 //
-// The host launches no more workgroups than fit on the device at once. The
-// barrier's wait is bounded: if a workgroup waits longer than
-// kBarrierSpinLimit polls (about a second), it raises `abort`, every
-// workgroup leaves the time loop at its next barrier, and the host reports
-// the failure instead of the kernel waiting forever.
+// The host launches no more workgroups than fit on the device at once. Every
+// barrier wait is bounded in wall-clock time: a workgroup that waits longer
+// raises `abort`, every workgroup of the job leaves at its next barrier, and
+// the host reports the failure instead of the kernel waiting forever.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kBarrierSpinLimit = 1u << 21;
 
-// Two-level counting barrier. Workgroup b first counts into group b % 8, the
-// last arrival of a group counts into the top counter, and the last group
-// arrival publishes the next generation. Counters only grow within a launch
-// (they are zeroed before it), so "last" is recognised from the running
-// count and nothing has to be reset between steps.
-__device__ __forceinline__ bool grid_barrier(GridBarrier *gb, uint32_t nblocks, uint32_t &gen,
-                                             int *ok_lds) {
+// Two-level counting barrier over `members` workgroups. Member m first counts
+// into group m % 8, the last arrival of a group counts into the top counter,
+// and the last group arrival publishes the next generation. Counters only grow
+// within a launch (they are zeroed before it), so "last" is recognised from
+// the running count and nothing has to be reset between steps.
+__device__ __forceinline__ bool team_barrier(GridBarrier *gb, uint32_t members, uint32_t member,
+                                             uint32_t &gen, int *ok_lds, uint64_t wait_ticks) {
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
         const uint32_t next = gen + 1;
-        const uint32_t grp = blockIdx.x % kBarrierGroups;
-        const uint32_t gsize = (nblocks - grp + kBarrierGroups - 1) / kBarrierGroups;
-        const uint32_t ngroups = nblocks < kBarrierGroups ? nblocks : kBarrierGroups;
+        const uint32_t grp = member % kBarrierGroups;
+        const uint32_t gsize = (members - grp + kBarrierGroups - 1) / kBarrierGroups;
+        const uint32_t ngroups = members < kBarrierGroups ? members : kBarrierGroups;
         // release: this workgroup's state rows reach device-coherent memory
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const uint32_t a1 = __hip_atomic_fetch_add(&gb->group[grp].count, 1u, __ATOMIC_RELAXED,
@@ -161,14 +160,15 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier *gb, uint32_t nblocks, 
             }
         }
         if (!released) {
-            uint32_t spins = 0;
+            const uint64_t t0 = uint64_t(wall_clock64());
+            uint32_t polls = 0;
             while (__hip_atomic_load(&gb->generation.count, __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_AGENT) < next) {
                 if (__hip_atomic_load(&gb->abort.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                     ok = 0;
                     break;
                 }
-                if (++spins > kBarrierSpinLimit) {
+                if ((++polls & 63u) == 0 && uint64_t(wall_clock64()) - t0 > wait_ticks) {
                     __hip_atomic_store(&gb->abort.count, 1u, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
                     ok = 0;
@@ -193,21 +193,24 @@ constexpr int kResBatch = 9;
 constexpr uint32_t kNoConnection = 0xFFFFFFFFu;
 constexpr int kResNeighbourBits = 17;  // voxel index field; delay in the upper 15 bits
 
-__global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
-    const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
-    const int32_t *__restrict__ delays, const float *__restrict__ gT, float *__restrict__ y,
-    float *__restrict__ y_last, float *__restrict__ x, const float *__restrict__ b,
-    const float *__restrict__ u, GridBarrier *__restrict__ gb, float *__restrict__ ckpt,
-    int32_t ckpt_interval, int32_t N, int32_t t_begin, int32_t t_end, int32_t T) {
-    extern __shared__ float lds[];
+// Steps [t_begin, t_end) of voxels [vb, ve) of job `j`, as member `member` of
+// the job's team. The first barrier waits up to `first_wait` ticks (team
+// members may still be finishing earlier jobs), the others `step_wait`.
+// Returns false if the job's barrier aborted.
+__device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, int32_t ve,
+                                             uint32_t member, int32_t t_begin, int32_t t_end,
+                                             float *lds, uint64_t first_wait, uint64_t step_wait) {
     float *ys = lds;                                                       // [voxel][78]
     uint32_t *mask = reinterpret_cast<uint32_t *>(lds + kResMaxVoxels * 78);  // [voxel]
     int *ok_lds = reinterpret_cast<int *>(mask + kResMaxVoxels);
+    const int32_t N = j.N, T = j.T;
+    const int32_t *__restrict__ nbr = j.nbr;
+    const float *__restrict__ gT = j.gT;
+    const float *__restrict__ b = j.b;
+    const float *__restrict__ u = j.u;
+    float *__restrict__ x = j.x;
     uint32_t gen = 0;
     const int64_t n = 3LL * N;
-    const int64_t slab = xcd_block(blockIdx.x, gridDim.x);
-    const int32_t vb = int32_t(slab * N / gridDim.x);
-    const int32_t ve = int32_t((slab + 1) * N / gridDim.x);
     const int32_t nv = ve - vb;
     const int32_t items = nv * 26;
     const int tid = threadIdx.x;
@@ -230,9 +233,9 @@ __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
             const int64_t c = int64_t(vb) * 26 + it;
             const int32_t w = nbr[c];
             if (w >= 0) {
-                const int32_t D = min(delays[c], 32767);
+                const int32_t D = min(j.delays[c], 32767);
                 pk[k] = uint32_t(w) | (uint32_t(D) << kResNeighbourBits);
-                ak[k] = coefs[c];
+                ak[k] = j.coefs[c];
                 atomicOr(&mask[it / 26], 1u << (it % 26));
                 if (D < t_begin) {
                     const int64_t r1 = int64_t(t_begin - D - 1) * n + 3LL * w;
@@ -242,7 +245,7 @@ __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
             }
         }
     }
-    for (int idx = tid; idx < nv * 78; idx += kResThreads) ys[idx] = y[int64_t(vb) * 78 + idx];
+    for (int idx = tid; idx < nv * 78; idx += kResThreads) ys[idx] = j.y[int64_t(vb) * 78 + idx];
     __syncthreads();
 
     for (int32_t t = t_begin; t < t_end; ++t) {
@@ -251,10 +254,10 @@ __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
         for (int k0 = 0; k0 < kResItems; k0 += kResBatch) {
             float in0[kResBatch][3];
 #pragma unroll
-            for (int j = 0; j < kResBatch; ++j) {
-                const int k = k0 + j;
+            for (int q = 0; q < kResBatch; ++q) {
+                const int k = k0 + q;
 #pragma unroll
-                for (int o = 0; o < 3; ++o) in0[j][o] = 0.0f;
+                for (int o = 0; o < 3; ++o) in0[q][o] = 0.0f;
                 if (k >= kResItems || pk[k] == kNoConnection) continue;
                 // 32-bit element offsets: the host only picks this kernel when T * n < 2^32
                 const uint32_t w = pk[k] & ((1u << kResNeighbourBits) - 1);
@@ -262,20 +265,20 @@ __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
                 if (D <= t) {
                     const uint32_t r0 = uint32_t(t - D) * uint32_t(n) + 3u * w;
 #pragma unroll
-                    for (int o = 0; o < 3; ++o) in0[j][o] = x[r0 + o];
+                    for (int o = 0; o < 3; ++o) in0[q][o] = x[r0 + o];
                 }
             }
 #pragma unroll
-            for (int j = 0; j < kResBatch; ++j) {
-                const int k = k0 + j;
+            for (int q = 0; q < kResBatch; ++q) {
+                const int k = k0 + q;
                 if (k >= kResItems || pk[k] == kNoConnection) continue;
                 const int it = tid + k * kResThreads;
 #pragma unroll
                 for (int o = 0; o < 3; ++o) {
                     const float prev = ys[it * 3 + o];
-                    if (last) y_last[(int64_t(vb) * 26 + it) * 3 + o] = prev;
-                    ys[it * 3 + o] = fmaf(ak[k], in0[j][o] - prev, pv[k][o]);
-                    pv[k][o] = in0[j][o];
+                    if (last) j.y_last[(int64_t(vb) * 26 + it) * 3 + o] = prev;
+                    ys[it * 3 + o] = fmaf(ak[k], in0[q][o] - prev, pv[k][o]);
+                    pv[k][o] = in0[q][o];
                 }
             }
         }
@@ -306,13 +309,51 @@ __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
             x[int64_t(t) * n + i] = acc + ut * b[i];
         }
         // filter outputs y_t for the reverse-mode gradient, every ckpt_interval steps
-        if (ckpt_interval > 0 && (t + 1) % ckpt_interval == 0 && t + 1 < T) {
-            float *dst = ckpt + int64_t((t + 1) / ckpt_interval - 1) * N * 78 + int64_t(vb) * 78;
+        const int32_t K = j.ckpt_interval;
+        if (K > 0 && (t + 1) % K == 0 && t + 1 < T) {
+            float *dst = j.ckpt + int64_t((t + 1) / K - 1) * N * 78 + int64_t(vb) * 78;
             for (int idx = tid; idx < nv * 78; idx += kResThreads) dst[idx] = ys[idx];
         }
-        if (!grid_barrier(gb, gridDim.x, gen, ok_lds)) return;  // host reports the abort
+        if (!team_barrier(j.gb, uint32_t(j.width), member, gen, ok_lds,
+                          t == t_begin ? first_wait : step_wait))
+            return false;  // host reports the abort
     }
-    for (int idx = tid; idx < nv * 78; idx += kResThreads) y[int64_t(vb) * 78 + idx] = ys[idx];
+    for (int idx = tid; idx < nv * 78; idx += kResThreads) j.y[int64_t(vb) * 78 + idx] = ys[idx];
+    return true;
+}
+
+// One job over the whole grid: workgroup slab = xcd_block(blockIdx.x).
+__global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
+    ResidentJob job, int32_t t_begin, int32_t t_end, uint64_t wait_ticks) {
+    extern __shared__ float lds[];
+    const int64_t slab = xcd_block(blockIdx.x, gridDim.x);
+    const int32_t vb = int32_t(slab * job.N / gridDim.x);
+    const int32_t ve = int32_t((slab + 1) * job.N / gridDim.x);
+    resident_run(job, vb, ve, blockIdx.x, t_begin, t_end, lds, wait_ticks, wait_ticks);
+}
+
+// Many jobs in one persistent launch. Logical slot = xcd_block(blockIdx.x), so
+// consecutive slots share an XCD. Slot q runs the tasks
+// task_job[task_ptr[q] .. task_ptr[q+1]) in that order; a job occupies the
+// contiguous slots [slot0, slot0 + width) and member (q - slot0) owns voxels
+// [m N / width, (m+1) N / width). Every slot lists its jobs in one global
+// order, so the earliest unfinished job always has its whole team and the
+// launch cannot deadlock.
+__global__ __launch_bounds__(kResThreads) void propagate_batch_kernel(
+    const ResidentJob *__restrict__ jobs, const int32_t *__restrict__ task_ptr,
+    const int32_t *__restrict__ task_job, uint64_t first_wait, uint64_t step_wait) {
+    extern __shared__ float lds[];
+    const int32_t slot = xcd_block(blockIdx.x, gridDim.x);
+    const int32_t q1 = task_ptr[slot + 1];
+    for (int32_t q = task_ptr[slot]; q < q1; ++q) {
+        const ResidentJob job = jobs[task_job[q]];
+        const int32_t member = slot - job.slot0;
+        const int32_t vb = int32_t(int64_t(member) * job.N / job.width);
+        const int32_t ve = int32_t(int64_t(member + 1) * job.N / job.width);
+        if (!resident_run(job, vb, ve, uint32_t(member), 0, job.T, lds, first_wait, step_wait))
+            return;
+        __syncthreads();  // LDS is reused by the next job
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -753,10 +794,11 @@ __global__ void sgd_kernel(float *__restrict__ v, const float *__restrict__ d, i
 
 // update.rs:119-146, 173-200
 __global__ void adam_kernel(float *__restrict__ v, const float *__restrict__ d,
-                            float *__restrict__ m1, float *__restrict__ m2, int64_t len, float c1,
-                            float c2, float scale) {
+                            float *__restrict__ m1, float *__restrict__ m2, int64_t len,
+                            const UpdateScalars *__restrict__ sc, float scale) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= len) return;
+    const float c1 = sc->c1, c2 = sc->c2;
     const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
     const float omb1 = 1.0f - beta1, omb2 = 1.0f - beta2;
     const float g = d[k];
@@ -860,7 +902,8 @@ __device__ float unrolled_mean(const float *xs, int32_t len) {
 // metrics.rs:204-219
 __global__ void batch_metrics_kernel(const float *loss, const float *loss_mse,
                                      const float *loss_mr, float *loss_b, float *loss_mse_b,
-                                     float *loss_mr_b, int32_t idx, int32_t T) {
+                                     float *loss_mr_b, const UpdateScalars *sc, int32_t T) {
+    const int32_t idx = sc->batch_index;
     if (threadIdx.x == 0) loss_mse_b[idx] = unrolled_mean(loss_mse, T);
     if (threadIdx.x == 1) loss_mr_b[idx] = unrolled_mean(loss_mr, T);
     if (threadIdx.x == 2) loss_b[idx] = unrolled_mean(loss, T);
@@ -878,6 +921,16 @@ __global__ void expand_rows_kernel(const float *__restrict__ in, float *__restri
 __global__ void fill_kernel(float *p, float value, int64_t len) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k < len) p[k] = value;
+}
+
+// Zeroes the counters of `count` barriers and keeps their abort words, so an
+// abort stays visible to the host until it reads it after the epoch.
+__global__ void barrier_reset_kernel(GridBarrier *gb, int32_t count) {
+    constexpr int kWords = int(sizeof(GridBarrier) / sizeof(BarrierWord));
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count * (kWords - 1)) return;
+    const int32_t j = k / (kWords - 1), w = k % (kWords - 1);  // abort is the last word
+    reinterpret_cast<BarrierWord *>(gb + j)[w].count = 0u;
 }
 
 inline unsigned blocks_for(int64_t len, int threads) {
@@ -911,18 +964,40 @@ hipError_t resident_capacity(int32_t *blocks_per_cu) {
     return e;
 }
 
-hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const int32_t *nbr,
-                                     const float *coefs, const int32_t *delays, const float *gT,
-                                     float *y, float *y_last, float *x, const float *b,
-                                     const float *u, GridBarrier *gb, float *ckpt,
-                                     int32_t ckpt_interval, int32_t N, int32_t t_begin,
-                                     int32_t t_end, int32_t T) {
-    // the barrier starts from zero: a fill kernel rather than a memset node
-    hipError_t e = launch_fill(s, reinterpret_cast<float *>(gb), 0.0f, sizeof(GridBarrier) / 4);
+hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const ResidentJob &job,
+                                     int32_t t_begin, int32_t t_end, uint64_t wait_ticks) {
+    // the barrier starts from zero: a kernel rather than a memset node
+    barrier_reset_kernel<<<1, 64, 0, s>>>(job.gb, 1);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    propagate_resident_kernel<<<blocks, kResThreads, resident_lds_bytes(), s>>>(
-        nbr, coefs, delays, gT, y, y_last, x, b, u, gb, ckpt, ckpt_interval, N, t_begin, t_end,
-        T);
+    propagate_resident_kernel<<<blocks, kResThreads, resident_lds_bytes(), s>>>(job, t_begin, t_end,
+                                                                                wait_ticks);
+    return hipGetLastError();
+}
+
+hipError_t batch_capacity(int32_t *blocks_per_cu) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&propagate_batch_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       int(resident_lds_bytes()));
+    if (e != hipSuccess) return e;
+    int nb = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &nb, reinterpret_cast<const void *>(&propagate_batch_kernel), kResThreads,
+        resident_lds_bytes());
+    *blocks_per_cu = nb;
+    return e;
+}
+
+hipError_t launch_propagate_batch(hipStream_t s, int32_t slots, const ResidentJob *jobs,
+                                  const int32_t *task_ptr, const int32_t *task_job,
+                                  GridBarrier *barriers, int32_t num_jobs, uint64_t first_wait,
+                                  uint64_t step_wait) {
+    if (slots <= 0 || num_jobs <= 0) return hipSuccess;
+    barrier_reset_kernel<<<blocks_for(int64_t(num_jobs) * 10, 256), 256, 0, s>>>(barriers, num_jobs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    propagate_batch_kernel<<<slots, kResThreads, resident_lds_bytes(), s>>>(jobs, task_ptr, task_job,
+                                                                            first_wait, step_wait);
     return hipGetLastError();
 }
 
@@ -992,9 +1067,9 @@ hipError_t launch_sgd(hipStream_t s, float *v, const float *d, int64_t len, floa
 }
 
 hipError_t launch_adam(hipStream_t s, float *v, const float *d, float *m1, float *m2, int64_t len,
-                       float c1, float c2, float scale) {
+                       const UpdateScalars *sc, float scale) {
     if (len <= 0) return hipSuccess;
-    adam_kernel<<<blocks_for(len, 256), 256, 0, s>>>(v, d, m1, m2, len, c1, c2, scale);
+    adam_kernel<<<blocks_for(len, 256), 256, 0, s>>>(v, d, m1, m2, len, sc, scale);
     return hipGetLastError();
 }
 
@@ -1020,9 +1095,9 @@ hipError_t launch_smoothness(hipStream_t s, const int32_t *nbr, const float *avg
 
 hipError_t launch_batch_metrics(hipStream_t s, const float *loss, const float *loss_mse,
                                 const float *loss_mr, float *loss_b, float *loss_mse_b,
-                                float *loss_mr_b, int32_t idx, int32_t T) {
+                                float *loss_mr_b, const UpdateScalars *sc, int32_t T) {
     batch_metrics_kernel<<<1, 64, 0, s>>>(loss, loss_mse, loss_mr, loss_b, loss_mse_b, loss_mr_b,
-                                          idx, T);
+                                          sc, T);
     return hipGetLastError();
 }
 

@@ -1,0 +1,200 @@
+// Internal host-runtime interface shared by api.cpp (single scenario) and
+// batch.cpp (many scenarios per launch): the opaque handle types of
+// cardiotrust_hip.h and the epoch building blocks.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cardiotrust_hip.h"
+#include "kernels.h"
+
+namespace ctrt {
+extern thread_local std::string g_error;
+int fail(int code, const std::string &msg);
+}  // namespace ctrt
+
+#define CT_HIP(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return ctrt::fail(CT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CT_CHECK(expr)          \
+    do {                        \
+        int rc_ = (expr);       \
+        if (rc_ != CT_OK) return rc_; \
+    } while (0)
+
+struct ct_gpu {
+    int32_t device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t wall_ticks_per_second = 100000000;  // wall_clock64() rate
+};
+
+struct ct_buffer {
+    ct_gpu *gpu = nullptr;
+    float *data = nullptr;
+    int32_t beats = 0, steps = 0, sensors = 0;
+};
+
+struct ct_results_gpu {
+    ct_gpu *gpu = nullptr;
+    int32_t N = 0, S = 0, Sp = 0, T = 0, B = 0, nbatch = 0, optimizer = 0;
+    int64_t n = 0;
+    std::vector<void *> allocations;
+    std::vector<float> control_function;  // host copy: u[t] is a launch argument
+
+    // model
+    int32_t *nbr = nullptr, *delays = nullptr;
+    float *coefs = nullptr, *initial_delays = nullptr, *gT = nullptr, *b = nullptr, *H = nullptr;
+    // estimations
+    float *x = nullptr, *y = nullptr, *y_last = nullptr, *m = nullptr, *R = nullptr;
+    float *xT = nullptr, *eT = nullptr;
+    float *avg = nullptr;
+    uint8_t *avg_valid = nullptr;
+    // derivatives
+    float *dGT = nullptr, *dA = nullptr, *fir = nullptr, *iir = nullptr, *mr_last = nullptr,
+          *mr_sum = nullptr;
+    float *gm1 = nullptr, *gm2 = nullptr, *cm1 = nullptr, *cm2 = nullptr;
+    int64_t step = 1;
+    // metrics
+    float *loss = nullptr, *loss_mse = nullptr, *loss_mr = nullptr;
+    float *loss_b = nullptr, *loss_mse_b = nullptr, *loss_mr_b = nullptr;
+    float *u = nullptr;  // control function on the device (resident propagation)
+    // propagation path: resident (one launch per beat) when the
+    // problem fits on chip, else one launch per step
+    int32_t propagation_mode = 0;  // 0 auto, 1 per-step launches, 2 resident
+    bool resident_ok = false;
+    bool resident_used = false;  // a resident launch is in flight since the last check
+    int32_t resident_blocks = 0;
+    ct::GridBarrier *barrier = nullptr;
+    // scratch
+    float *partial = nullptr, *mr_partial = nullptr, *staging = nullptr, *step_mr = nullptr;
+    double *objective = nullptr;  // J of the last forward (adjoint objective)
+    // reverse-mode gradient: host neighbour table (for the consumer lists) and
+    // buffers allocated on first use
+    std::vector<int32_t> nbr_host;
+    int32_t *cons_ptr = nullptr, *cons_idx = nullptr, *cons_delay = nullptr, *dmax = nullptr;
+    int64_t cons_len = 0;
+    int32_t ckpt_interval = 0;  // checkpoint interval of the last forward pass (0 = none)
+    float *E = nullptr, *xbar = nullptr, *xbarT = nullptr, *ybar = nullptr;
+    float *qring = nullptr, *ckpt = nullptr, *yseg = nullptr;
+    int64_t qring_slots = 0, ckpt_segments = 0, yseg_steps = 0;
+    int64_t ring_slots_fixed = 0;  // > 0: q-ring size fixed for a captured epoch
+    int32_t forward_beat = -1;  // beat whose forward pass the device state holds
+    ct::UpdateScalars *scal = nullptr;       // (B) device, per update of an epoch
+    ct::UpdateScalars *scal_host = nullptr;  // (B) pinned host staging
+    int32_t splits = 1, mr_blocks = 1;
+    int64_t kps = ct::kGemmTileK;
+
+    ~ct_results_gpu() {
+        for (void *p : allocations) (void)hipFree(p);
+        if (scal_host) (void)hipHostFree(scal_host);
+    }
+
+    template <typename V>
+    int alloc(V **p, int64_t count) {
+        const size_t bytes = std::max<size_t>(sizeof(V) * size_t(std::max<int64_t>(count, 0)), 256);
+        void *raw = nullptr;
+        hipError_t e = hipMalloc(&raw, bytes);
+        if (e != hipSuccess)
+            return ctrt::fail(CT_ERR_DEVICE, "hipMalloc(" + std::to_string(bytes) +
+                                           " bytes): " + hipGetErrorString(e));
+        allocations.push_back(raw);
+        *p = static_cast<V *>(raw);
+        return CT_OK;
+    }
+};
+
+struct ct_epoch_kernel {
+    ct_gpu *gpu = nullptr;
+    ct_results_gpu *r = nullptr;
+    const ct_buffer *mhat = nullptr;
+    ct_algorithm cfg{};
+    std::vector<int32_t> order;
+    int32_t batch_index = 0;
+    bool profiling = false;
+    int32_t gradient = 0;            // 0 = reference derivation, 1 = reverse-mode adjoint
+    int32_t checkpoint_interval = 64;
+    std::vector<hipEvent_t> events;
+    size_t used_events = 0;
+    struct SpanRec {
+        int phase;
+        int64_t steps;
+        size_t start, end;
+    };
+    std::vector<SpanRec> spans;
+
+    ~ct_epoch_kernel() {
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    }
+};
+
+namespace ctrt {
+
+enum Phase {
+    kPropagate, kProject, kRegularize, kTranspose, kMapResiduals, kDerive, kUpdate, kTotal,
+    kAdjointSweep
+};
+
+struct Span {
+    ct_epoch_kernel *k;
+    int phase;
+    int64_t steps;
+    size_t start = 0;
+    bool on = false;
+    Span(ct_epoch_kernel *kk, int ph, int64_t st = 0) : k(kk), phase(ph), steps(st) {
+        if (!k || !k->profiling) return;
+        if (k->used_events + 2 > k->events.size()) return;
+        start = k->used_events;
+        k->used_events += 2;
+        on = hipEventRecord(k->events[start], k->r->gpu->stream) == hipSuccess;
+    }
+    ~Span() {
+        if (!on) return;
+        if (hipEventRecord(k->events[start + 1], k->r->gpu->stream) == hipSuccess)
+            k->spans.push_back({phase, steps, start, start + 1});
+    }
+};
+
+int check_gpu(ct_gpu *gpu);
+int upload(void *dst, const void *src, size_t bytes);
+int download(void *dst, const void *src, size_t bytes);
+int reset_derivatives(ct_results_gpu &r);
+int ensure_checkpoints(ct_results_gpu &r, int32_t K);
+// Propagation of one beat on its own (resident or per-step launches).
+int propagate_beat(ct_results_gpu &r, ct_epoch_kernel *k, int32_t ckpt_interval);
+// Projection, residuals and per-step MSE of the beat whose states x holds.
+int project_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_kernel *k);
+int forward_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_kernel *k,
+                 int32_t ckpt_interval = 0);
+int losses_beat(ct_results_gpu &r, const ct_algorithm &c, ct_epoch_kernel *k);
+int derive_reference(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, ct_epoch_kernel *k);
+int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32_t K,
+                   ct_epoch_kernel *k);
+// Everything of a beat after its propagation (run_epoch's per-beat body).
+int finish_beat(ct_epoch_kernel &k, int32_t beat);
+bool uses_adjoint(const ct_epoch_kernel &k);
+int finish_batch(ct_epoch_kernel &k, int64_t num_beats, bool reset_after, int32_t update);
+int32_t updates_per_epoch(const ct_epoch_kernel &k);
+int begin_epoch(ct_epoch_kernel &k);
+void end_epoch(ct_epoch_kernel &k);
+ct::ResidentJob resident_job(ct_results_gpu &r, int32_t ckpt_interval);
+int prepare_propagation(ct_results_gpu &r, int32_t ckpt_interval);
+int check_resident(ct_results_gpu &r);
+// Fixes the adjoint's q-ring size for graph capture: at least the current
+// largest delay + 1 + `headroom` (delays grow by at most one per update).
+// Reads the delays (synchronises); allocates the ring.
+int fix_adjoint_ring(ct_results_gpu &r, int64_t headroom);
+int begin_profile(ct_epoch_kernel &k);
+// Seconds -> wall_clock64() ticks of the device.
+uint64_t wait_ticks(const ct_gpu &g, double seconds);
+
+}  // namespace ctrt

@@ -30,6 +30,8 @@ EXPORTS = (
     "ct_epoch_kernel_set_freeze_gains", "ct_epoch_kernel_set_beat_order",
     "ct_epoch_kernel_set_batch_index", "ct_epoch_kernel_get_batch_index",
     "ct_epoch_kernel_set_profiling", "ct_epoch_kernel_get_profile",
+    "ct_batch_new", "ct_batch_free", "ct_batch_execute", "ct_batch_set_voxels_per_group",
+    "ct_batch_set_graph", "ct_batch_set_profiling", "ct_batch_get_info",
 )
 
 
@@ -74,6 +76,23 @@ class Profile(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("scenarios", ctypes.c_int32), ("rounds", ctypes.c_int32), ("slots", ctypes.c_int32),
+        ("voxels_per_group", ctypes.c_int32), ("jobs", ctypes.c_int32), ("graph", ctypes.c_int32),
+        ("predicted_propagate_ms", ctypes.c_double), ("epoch_ms", ctypes.c_double),
+        ("propagate_ms", ctypes.c_double), ("propagate_launches", ctypes.c_int64),
+        ("voxel_steps", ctypes.c_int64), ("tuned_candidates", ctypes.c_int32),
+        ("tuned_voxels", ctypes.c_int32 * 16), ("tuned_ms", ctypes.c_double * 16),
+    ]
+
+    def as_dict(self) -> Dict[str, object]:
+        d = {name: getattr(self, name) for name, _ in self._fields_[:11]}
+        n = self.tuned_candidates
+        d["autotune"] = {int(self.tuned_voxels[i]): float(self.tuned_ms[i]) for i in range(n)}
+        return d
+
+
 _lib = None
 
 
@@ -112,6 +131,13 @@ def lib() -> ctypes.CDLL:
         "ct_epoch_kernel_get_batch_index": ([vp, P(i32)], ctypes.c_int),
         "ct_epoch_kernel_set_profiling": ([vp, i32], ctypes.c_int),
         "ct_epoch_kernel_get_profile": ([vp, P(Profile)], ctypes.c_int),
+        "ct_batch_new": ([vp, P(vp), i32, P(vp)], ctypes.c_int),
+        "ct_batch_free": ([vp], None),
+        "ct_batch_execute": ([vp], ctypes.c_int),
+        "ct_batch_set_voxels_per_group": ([vp, i32], ctypes.c_int),
+        "ct_batch_set_graph": ([vp, i32], ctypes.c_int),
+        "ct_batch_set_profiling": ([vp, i32], ctypes.c_int),
+        "ct_batch_get_info": ([vp, P(BatchInfo)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -355,3 +381,47 @@ def simulate(gpu: GPU, problem) -> np.ndarray:
         return r.simulate()
     finally:
         r.close()
+
+
+class Batch:
+    """Many scenarios advanced one epoch per `execute` (ct_batch_*): each
+    member's results equal those of its own EpochKernel.execute()."""
+
+    def __init__(self, gpu: GPU, kernels: Sequence["EpochKernel"], voxels_per_group: int = 0,
+                 graph: bool = False):
+        arr = (ctypes.c_void_p * len(kernels))(*[k.handle for k in kernels])
+        h = ctypes.c_void_p()
+        _check(lib().ct_batch_new(gpu.handle, arr, len(kernels), ctypes.byref(h)), "ct_batch_new")
+        self.handle = h
+        self.kernels = list(kernels)  # keep the members alive
+        self.set_voxels_per_group(voxels_per_group)
+        self.set_graph(graph)
+
+    def execute(self) -> None:
+        _check(lib().ct_batch_execute(self.handle), "ct_batch_execute")
+
+    def set_voxels_per_group(self, voxels: int) -> None:
+        _check(lib().ct_batch_set_voxels_per_group(self.handle, int(voxels)),
+               "set_voxels_per_group")
+
+    def set_graph(self, enabled: bool) -> None:
+        _check(lib().ct_batch_set_graph(self.handle, int(bool(enabled))), "set_graph")
+
+    def set_profiling(self, enabled: bool) -> None:
+        _check(lib().ct_batch_set_profiling(self.handle, int(bool(enabled))), "set_profiling")
+
+    def info(self) -> Dict[str, object]:
+        out = BatchInfo()
+        _check(lib().ct_batch_get_info(self.handle, ctypes.byref(out)), "get_info")
+        return out.as_dict()
+
+    def close(self) -> None:
+        if self.handle:
+            lib().ct_batch_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

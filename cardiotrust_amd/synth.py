@@ -118,6 +118,46 @@ def ellipsoid_shell(shape=(64, 64, 64), outer=(32.0, 32.0, 32.0), inner=(16.0, 1
     return grid_geometry((ro <= 1.0) & (ri > 1.0), voxel_size_mm)
 
 
+def blob(seed: int, num_voxels: int, voxel_size_mm: float = 2.5) -> Geometry:
+    """Seeded random blob of exactly `num_voxels` voxels: the `num_voxels`
+    highest cells of a sum of six Gaussian bumps (random centres and widths)
+    in a cubic box of about 1.8 x `num_voxels` cells."""
+    side = max(2, int(np.ceil((1.8 * num_voxels) ** (1.0 / 3.0))))
+    u = uniform(seed, 20, 24).reshape(6, 4)
+    idx = np.indices((side, side, side)).reshape(3, -1).T.astype(np.float64) + 0.5
+    field = np.zeros(idx.shape[0])
+    for c in u:
+        centre = (0.2 + 0.6 * c[:3]) * side
+        width = (0.15 + 0.2 * c[3]) * side
+        field += np.exp(-np.sum((idx - centre) ** 2, axis=1) / (2.0 * width * width))
+    # ties broken by cell number, so the count is exact
+    keep = np.lexsort((np.arange(field.size), -field))[:num_voxels]
+    mask = np.zeros(field.size, dtype=bool)
+    mask[keep] = True
+    return grid_geometry(mask.reshape(side, side, side), voxel_size_mm)
+
+
+def sphere_region(geometry: Geometry, seed: int, radius_fraction: float = 0.2) -> np.ndarray:
+    """(N,) bool: voxels within a seeded sphere (radius `radius_fraction` of
+    the grid's smallest side) centred on a random active voxel."""
+    if geometry.num_voxels == 0:
+        return np.zeros(0, dtype=bool)
+    pick = int(uniform(seed, 30, 1)[0] * geometry.num_voxels)
+    centre = geometry.coords[pick].astype(np.float64)
+    r = radius_fraction * min(geometry.shape)
+    return np.sum((geometry.coords - centre) ** 2, axis=1) <= r * r
+
+
+def grid_motion_offsets(steps=(1, 1, 1), range_mm=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """(B, 3) sensor-array offsets of SensorArrayMotion::Grid
+    (`sensors.rs:194-225`): B = steps_x * steps_y * steps_z positions, x
+    outermost, each axis spanning `range_mm` in steps - 1 equal steps."""
+    step = [r / (n - 1) if n > 1 else 0.0 for r, n in zip(range_mm, steps)]
+    out = [[step[0] * x, step[1] * y, step[2] * z]
+           for x in range(steps[0]) for y in range(steps[1]) for z in range(steps[2])]
+    return np.array(out, dtype=np.float32).astype(np.float64)
+
+
 def output_state_indices(neighbours: np.ndarray) -> np.ndarray:
     """(3N, 78) int32: 3w+out at (3v+in, 3d+out), -1 where no neighbour."""
     N = neighbours.shape[0]
@@ -239,6 +279,7 @@ def make_problem(geometry: Geometry, num_steps: int, max_delay: int, seed: int,
                  num_beats: int = 1, gain_sigma: float = 0.1,
                  perturbation: float = 0.2,
                  beat_offsets_mm: Optional[np.ndarray] = None,
+                 slow_region: Optional[np.ndarray] = None,
                  simulate: Optional[Callable[["Problem"], np.ndarray]] = None) -> Problem:
     """Random ground-truth parameters, their perturbed initial estimate and,
     when `simulate` is given, the noise-free measurements of the truth.
@@ -248,7 +289,9 @@ def make_problem(geometry: Geometry, num_steps: int, max_delay: int, seed: int,
     standard deviation) on every existing connection; missing connections keep
     gain 0, delay 1 and coef 0.5. The estimate multiplies gains and coefs by
     (1 + perturbation * N(0, 1)), clamps coefs to [1e-4, 1 - 1e-4] and keeps
-    the true integer delays.
+    the true integer delays. With `slow_region` ((N,) bool) the true delays of
+    the connections of those voxels are doubled (a pathological, slowly
+    conducting region) while the estimate starts from the undoubled ones.
     """
     N = geometry.num_voxels
     nb = geometry.neighbours
@@ -256,6 +299,10 @@ def make_problem(geometry: Geometry, num_steps: int, max_delay: int, seed: int,
     D = (1 + np.floor(uniform(seed, 1, N * 26) * max_delay)).astype(np.int32).reshape(N, 26)
     a = (0.05 + 0.9 * uniform(seed, 2, N * 26)).astype(np.float32).reshape(N, 26)
     D = np.where(exists, D, 1).astype(np.int32)
+    D_estimate = D.copy()
+    if slow_region is not None:
+        D = np.where(exists & np.asarray(slow_region, dtype=bool)[:, None], 2 * D, D)
+        D = D.astype(np.int32)
     a = np.where(exists, a, np.float32(0.5)).astype(np.float32)
     g = (gain_sigma * normal(seed, 3, N * 78 * 3)).astype(np.float32).reshape(3 * N, 78)
     conn = np.repeat(np.repeat(exists, 3, axis=1), 3, axis=0)  # (3N, 78)
@@ -281,8 +328,8 @@ def make_problem(geometry: Geometry, num_steps: int, max_delay: int, seed: int,
         gains=g0,
         output_state_indices=output_state_indices(nb),
         coefs=a0,
-        delays=D.copy(),
-        initial_delays=(D.astype(np.float32) + coef_to_samples(a0)).astype(np.float32),
+        delays=D_estimate.copy(),
+        initial_delays=(D_estimate.astype(np.float32) + coef_to_samples(a0)).astype(np.float32),
         control_matrix=b,
         control_function=u,
         measurement_matrix=H,
@@ -311,3 +358,44 @@ def config1(simulate=None, num_steps: int = 2000) -> Problem:
 def config2(simulate=None, num_steps: int = 2000) -> Problem:
     """32^3 full cube (N = 32,768), delays 1-40, T = 2000, one beat."""
     return make_problem(cube(32), num_steps=num_steps, max_delay=40, seed=2, simulate=simulate)
+
+
+def config3(simulate=None, count: int = 64, seed: int = 3, min_voxels: int = 1000,
+            max_voxels: int = 40000, min_steps: int = 600, max_steps: int = 2000,
+            max_beats: int = 8) -> list:
+    """`count` scenarios of uneven size: voxels log-uniform in
+    [min_voxels, max_voxels] (seeded blobs, 2.5 mm), beats uniform in
+    1..max_beats (sensor array moved along x by 10 mm per beat, grid motion),
+    steps uniform in [min_steps, max_steps] per scenario (the reference's
+    Measurements are (B, T, S): one length per scenario), delays 1-20."""
+    u = uniform(seed, 40, 3 * count).reshape(count, 3)
+    out = []
+    for i in range(count):
+        n = int(round(np.exp(np.log(min_voxels) + u[i, 0] * np.log(max_voxels / min_voxels))))
+        beats = 1 + int(u[i, 1] * max_beats)
+        steps = min_steps + int(u[i, 2] * (max_steps - min_steps + 1))
+        g = blob(seed * 1000 + i, n)
+        offsets = grid_motion_offsets((beats, 1, 1), (10.0 * (beats - 1), 0.0, 0.0))
+        out.append(make_problem(g, num_steps=steps, max_delay=20, seed=seed * 1000 + i,
+                                num_beats=beats, beat_offsets_mm=offsets, simulate=simulate))
+    return out
+
+
+def config4(simulate=None, count: int = 8, seed: int = 4, beats: int = 4,
+            num_steps: int = 2000) -> list:
+    """`count` ellipsoidal shells of about 120k voxels (64^3 grid, semi-axes
+    varied by a few voxels per scenario) with a seeded slow sub-region (true
+    delays doubled), `beats` beats with grid sensor motion, T = num_steps,
+    delays 1-40."""
+    u = uniform(seed, 50, 2 * count).reshape(count, 2)
+    out = []
+    for i in range(count):
+        outer = (32.0, 32.0 - 2.0 * u[i, 0], 32.0)
+        inner = (16.0 + 2.0 * u[i, 1], 16.0, 16.0)
+        g = ellipsoid_shell(outer=outer, inner=inner)
+        region = sphere_region(g, seed * 1000 + i)
+        offsets = grid_motion_offsets((beats, 1, 1), (10.0 * (beats - 1), 0.0, 0.0))
+        out.append(make_problem(g, num_steps=num_steps, max_delay=40, seed=seed * 1000 + i,
+                                num_beats=beats, beat_offsets_mm=offsets, slow_region=region,
+                                simulate=simulate))
+    return out

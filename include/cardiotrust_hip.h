@@ -36,6 +36,7 @@ typedef struct ct_gpu ct_gpu;
 typedef struct ct_buffer ct_buffer;
 typedef struct ct_results_gpu ct_results_gpu;
 typedef struct ct_epoch_kernel ct_epoch_kernel;
+typedef struct ct_batch ct_batch;
 
 /* Hot-path fields of config::algorithm::Algorithm
  * (src/core/config/algorithm.rs:25-58; defaults 59-85). */
@@ -215,6 +216,52 @@ int ct_epoch_kernel_get_batch_index(const ct_epoch_kernel *kernel, int32_t *batc
  * benches/gpu_epoch_benches.rs:40-53). Reading the profile synchronises. */
 int ct_epoch_kernel_set_profiling(ct_epoch_kernel *kernel, int32_t enabled);
 int ct_epoch_kernel_get_profile(ct_epoch_kernel *kernel, ct_profile *out);
+
+/* ---- Batching many scenarios (no reference counterpart: the reference runs
+ * one scenario per thread, each with its own OpenCL queue, src/scheduler.rs:93).
+ *
+ * A batch advances every member scenario by one epoch per ct_batch_execute,
+ * with exactly the results of calling ct_epoch_kernel_execute on each member
+ * (same order of operations per scenario; scenarios are independent). The
+ * beats run in rounds (round k = every scenario's k-th beat in its beat
+ * order); the propagation of a whole round is one persistent launch driven by
+ * a job table (one job per scenario: team of workgroups, voxel slab per
+ * workgroup, its arrays). Scenarios whose propagation does not fit the
+ * resident kernel use their per-step path inside the same epoch. */
+typedef struct {
+    int32_t scenarios;
+    int32_t rounds;                       /* = largest number of beats */
+    int32_t slots;                        /* workgroups of the batched launch (0 = unused) */
+    int32_t voxels_per_group;             /* planned voxels per workgroup */
+    int32_t jobs;                         /* jobs over all rounds */
+    int32_t graph;                        /* 1 if the last epoch replayed a captured graph */
+    double predicted_propagate_ms;        /* planner's makespan estimate, all rounds */
+    double epoch_ms;                      /* device time of the last epoch (HIP events) */
+    double propagate_ms;                  /* batched propagation launches of the last epoch
+                                             (profiling on, eager epochs only) */
+    int64_t propagate_launches;
+    int64_t voxel_steps;                  /* sum over scenarios of N * T * B */
+    int32_t tuned_candidates;             /* autotuner: voxels-per-workgroup values timed */
+    int32_t tuned_voxels[16];
+    double tuned_ms[16];                  /* time of round 0's batched propagation for each */
+} ct_batch_info;
+
+/* All kernels must live on `gpu`, with distinct results objects. The batch
+ * keeps pointers to them: free the batch before its kernels. */
+int ct_batch_new(ct_gpu *gpu, ct_epoch_kernel *const *kernels, int32_t count, ct_batch **out);
+void ct_batch_free(ct_batch *batch);
+/* One epoch of every scenario. Blocks until the epoch is done. */
+int ct_batch_execute(ct_batch *batch);
+/* Voxels per workgroup of the batched propagation: 0 (default) = autotune
+ * by timing round 0 on the first execute; else 1..472. */
+int ct_batch_set_voxels_per_group(ct_batch *batch, int32_t voxels);
+/* 1 = capture the epoch in a hipGraph after one eager epoch and replay it
+ * (re-captured when a member's beat order, configuration or gradient mode
+ * changes, and every 16 epochs for reverse-mode members). */
+int ct_batch_set_graph(ct_batch *batch, int32_t enabled);
+/* Times the batched propagation launches of eager epochs. */
+int ct_batch_set_profiling(ct_batch *batch, int32_t enabled);
+int ct_batch_get_info(ct_batch *batch, ct_batch_info *out);
 
 #ifdef __cplusplus
 }
