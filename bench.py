@@ -63,6 +63,113 @@ def cpu_baseline(problem, measurements, sample_steps: int):
     }
 
 
+def bench_batched(args) -> int:
+    """Many scenarios per epoch through ct_batch (one persistent propagation
+    launch per round of beats, autotuned voxels per workgroup, hipGraph
+    replay), timed beside the same scenarios run one after another."""
+    from cardiotrust_amd import Algorithm, synth
+    from cardiotrust_amd.gpu import GPU, Batch, Buffer, EpochKernel, ResultsGPU
+
+    t0 = time.perf_counter()
+    adjoint = args.workload == "batched-adjoint"
+    if adjoint:
+        problems = synth.config4(count=args.scenarios or 8, num_steps=args.num_steps)
+        label = (f"{len(problems)} ellipsoidal shells of ~120k voxels with slow sub-regions, "
+                 f"4 beats, T={args.num_steps}, reverse-mode gradient (checkpoint every "
+                 f"{args.checkpoint} steps), SGD lr 1e-3")
+    else:
+        problems = synth.config3(count=args.scenarios or 64)
+        label = (f"{len(problems)} blob scenarios, voxels log-uniform 1k-40k, 1-8 beats, "
+                 f"T uniform 600-2000, reference derivation, SGD lr 1e-3")
+    log(f"{len(problems)} synthetic scenarios ({time.perf_counter() - t0:.1f} s)")
+    gpu = GPU(0)
+    for p in problems:
+        tr = ResultsGPU(gpu, p.truth())
+        p.measurements = tr.simulate()
+        tr.close()
+    log(f"ground truths simulated on GPU ({time.perf_counter() - t0:.1f} s)")
+    cfg = Algorithm(learning_rate=1e-3, freeze_delays=False)
+    epochs = args.warmup + args.steps + 2
+    members = []
+    for p in problems:
+        r = ResultsGPU(gpu, p, num_batch_entries=epochs + 1)
+        b = Buffer(gpu, p.measurements)
+        k = EpochKernel(gpu, r, b, cfg, 3 * p.num_voxels, p.num_sensors, p.num_steps)
+        if adjoint:
+            k.set_gradient(1, args.checkpoint)
+        members.append((r, b, k))
+    voxel_steps = sum(p.num_voxels * p.num_steps * p.num_beats for p in problems)
+
+    # the same scenarios one after another (one epoch each)
+    gpu.synchronize()
+    ts = time.perf_counter()
+    for r, b, k in members:
+        k.execute()
+    gpu.synchronize()
+    seq_s = time.perf_counter() - ts
+    log(f"sequential epoch of all scenarios: {seq_s * 1e3:.1f} ms")
+
+    batch = Batch(gpu, [k for _, _, k in members], voxels_per_group=args.voxels_per_group,
+                  graph=not args.no_graph)
+    batch.set_profiling(True)
+    batch.execute()  # eager epoch (autotune, plan, buffers)
+    info0 = batch.info()
+    log("autotune (voxels per workgroup -> ms of round 0): " + json.dumps(info0["autotune"]))
+    log(f"eager batched epoch: {info0['epoch_ms']:.1f} ms, batched propagation "
+        f"{info0['propagate_ms']:.1f} ms in {info0['propagate_launches']} launches")
+    for _ in range(max(args.warmup - 1, 0)):
+        batch.execute()
+    gpu.synchronize()
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        batch.execute()
+    gpu.synchronize()
+    elapsed = time.perf_counter() - ts
+    info = batch.info()
+    ms_per_epoch = 1e3 * elapsed / args.steps
+    # roofline of the batched propagation launches (eager epoch, HIP events)
+    prop_ms = info0["propagate_ms"]
+    resident_vs = 0
+    for (r, _, _), p in zip(members, problems):
+        if r.set_propagation(0):
+            resident_vs += p.num_voxels * p.num_steps * p.num_beats
+    achieved = PROP_BYTES_PER_VOXEL_STEP * resident_vs / (prop_ms * 1e-3) / 1e9 if prop_ms else 0.0
+    out = {
+        "metric": "batched epochs/s (all scenarios advance one epoch)",
+        "value": 1000.0 / ms_per_epoch,
+        "unit": "epochs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_epoch,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": label, "scenarios": len(problems),
+                   "voxel_steps_per_epoch": voxel_steps, "graph": bool(info["graph"]),
+                   "voxels_per_group": info["voxels_per_group"], "rounds": info["rounds"],
+                   "jobs": info["jobs"]},
+        "scenario_epochs_per_s": len(problems) * 1000.0 / ms_per_epoch,
+        "voxel_steps_per_s": voxel_steps * 1000.0 / ms_per_epoch,
+        "sequential": {"ms_per_epoch": seq_s * 1e3,
+                       "speedup_of_batch": seq_s * 1e3 / ms_per_epoch},
+        "roofline": {"bound": "hbm", "kernel": "propagate_batch_kernel", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "propagate_ms_per_epoch": prop_ms,
+                     "predicted_ms": info0["predicted_propagate_ms"]},
+    }
+    batch.close()
+    for r, b, k in members:
+        k.close()
+        b.close()
+        r.close()
+    print(json.dumps(out), flush=True)
+    gpu.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,12 +183,23 @@ def main() -> int:
                     help="reference: the reference's per-step derivation; adjoint: exact "
                          "reverse-mode gradient with checkpoints")
     ap.add_argument("--checkpoint", type=int, default=64, help="adjoint checkpoint interval")
+    ap.add_argument("--workload", choices=["plain", "batched-uneven", "batched-adjoint"],
+                    default="plain",
+                    help="plain: the 120k-voxel epoch (the headline line); batched-uneven: 64 "
+                         "uneven scenarios per batched epoch; batched-adjoint: 8 shells x 4 "
+                         "beats with the reverse-mode gradient")
+    ap.add_argument("--scenarios", type=int, default=0, help="batched workloads: scenario count")
+    ap.add_argument("--voxels-per-group", type=int, default=0,
+                    help="batched workloads: 0 = autotune")
+    ap.add_argument("--no-graph", action="store_true", help="batched workloads: eager epochs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the epoch on a single MI355X; "
               "scenarios would shard one per GPU with no communication (see DESIGN.md).",
               file=sys.stderr)
         return 2
+    if args.workload != "plain":
+        return bench_batched(args)
 
     from cardiotrust_amd import Algorithm, synth
     from cardiotrust_amd.gpu import GPU, Buffer, EpochKernel, ResultsGPU

@@ -30,8 +30,6 @@ import numpy as np
 # physical_constants 0.5.0 VACUUM_MAG_PERMEABILITY (CODATA 2018), N/A^2.
 MU0 = 1.25663706212e-6
 
-_MASK64 = np.uint64(0xFFFFFFFFFFFFFFFF)
-
 # (dx, dy, dz) of connection d, allpass.rs:603-618 (delay_index_to_offset).
 OFFSETS = np.array(
     [
@@ -41,15 +39,19 @@ OFFSETS = np.array(
     dtype=np.int64,
 )
 
-
 def splitmix64(x: np.ndarray) -> np.ndarray:
     """splitmix64 finaliser applied to uint64 counters (wrapping arithmetic)."""
     with np.errstate(over="ignore"):
-        z = (x + np.uint64(0x9E3779B97F4A7C15)) & _MASK64
-        z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & _MASK64
-        z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & _MASK64
-        return z ^ (z >> np.uint64(31))
-
+        z = x + np.uint64(0x9E3779B97F4A7C15)
+        t = z >> np.uint64(30)
+        z ^= t
+        z *= np.uint64(0xBF58476D1CE4E5B9)
+        np.right_shift(z, np.uint64(27), out=t)
+        z ^= t
+        z *= np.uint64(0x94D049BB133111EB)
+        np.right_shift(z, np.uint64(31), out=t)
+        z ^= t
+        return z
 
 def uniform(seed: int, stream: int, count: int, offset: int = 0) -> np.ndarray:
     """`count` float64 uniforms in [0, 1) for (seed, stream)."""
@@ -59,14 +61,12 @@ def uniform(seed: int, stream: int, count: int, offset: int = 0) -> np.ndarray:
         bits = splitmix64(ctr ^ key)
     return (bits >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
 
-
 def normal(seed: int, stream: int, count: int) -> np.ndarray:
     """`count` standard normal float64 deviates (Box-Muller)."""
     u = uniform(seed, stream, 2 * count)
     u1 = 1.0 - u[0::2]  # (0, 1]
     u2 = u[1::2]
     return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
-
 
 @dataclass
 class Geometry:
@@ -84,7 +84,6 @@ class Geometry:
     def num_voxels(self) -> int:
         return int(self.coords.shape[0])
 
-
 def grid_geometry(mask: np.ndarray, voxel_size_mm: float) -> Geometry:
     mask = np.asarray(mask, dtype=bool)
     shape = mask.shape
@@ -101,10 +100,8 @@ def grid_geometry(mask: np.ndarray, voxel_size_mm: float) -> Geometry:
     positions = (coords.astype(np.float64) + 0.5) * voxel_size_mm
     return Geometry(shape, voxel_size_mm, mask, numbers, coords, positions, neighbours)
 
-
 def cube(n: int, voxel_size_mm: float = 2.5) -> Geometry:
     return grid_geometry(np.ones((n, n, n), dtype=bool), voxel_size_mm)
-
 
 def ellipsoid_shell(shape=(64, 64, 64), outer=(32.0, 32.0, 32.0), inner=(16.0, 16.0, 16.0),
                     voxel_size_mm: float = 2.5) -> Geometry:
@@ -116,7 +113,6 @@ def ellipsoid_shell(shape=(64, 64, 64), outer=(32.0, 32.0, 32.0), inner=(16.0, 1
     ro = sum((rel[k] / outer[k]) ** 2 for k in range(3))
     ri = sum((rel[k] / inner[k]) ** 2 for k in range(3))
     return grid_geometry((ro <= 1.0) & (ri > 1.0), voxel_size_mm)
-
 
 def blob(seed: int, num_voxels: int, voxel_size_mm: float = 2.5) -> Geometry:
     """Seeded random blob of exactly `num_voxels` voxels: the `num_voxels`
@@ -136,7 +132,6 @@ def blob(seed: int, num_voxels: int, voxel_size_mm: float = 2.5) -> Geometry:
     mask[keep] = True
     return grid_geometry(mask.reshape(side, side, side), voxel_size_mm)
 
-
 def sphere_region(geometry: Geometry, seed: int, radius_fraction: float = 0.2) -> np.ndarray:
     """(N,) bool: voxels within a seeded sphere (radius `radius_fraction` of
     the grid's smallest side) centred on a random active voxel."""
@@ -147,7 +142,6 @@ def sphere_region(geometry: Geometry, seed: int, radius_fraction: float = 0.2) -
     r = radius_fraction * min(geometry.shape)
     return np.sum((geometry.coords - centre) ** 2, axis=1) <= r * r
 
-
 def grid_motion_offsets(steps=(1, 1, 1), range_mm=(0.0, 0.0, 0.0)) -> np.ndarray:
     """(B, 3) sensor-array offsets of SensorArrayMotion::Grid
     (`sensors.rs:194-225`): B = steps_x * steps_y * steps_z positions, x
@@ -157,7 +151,6 @@ def grid_motion_offsets(steps=(1, 1, 1), range_mm=(0.0, 0.0, 0.0)) -> np.ndarray
            for x in range(steps[0]) for y in range(steps[1]) for z in range(steps[2])]
     return np.array(out, dtype=np.float32).astype(np.float64)
 
-
 def output_state_indices(neighbours: np.ndarray) -> np.ndarray:
     """(3N, 78) int32: 3w+out at (3v+in, 3d+out), -1 where no neighbour."""
     N = neighbours.shape[0]
@@ -166,13 +159,11 @@ def output_state_indices(neighbours: np.ndarray) -> np.ndarray:
     base = np.where(neighbours[:, :, None] >= 0, base, -1).reshape(N, 78)
     return np.repeat(base, 3, axis=0).astype(np.int32)
 
-
 @dataclass
 class Sensors:
     positions_mm: np.ndarray  # (S, 3)
     orientations: np.ndarray  # (S, 3)
     beat_offsets_mm: np.ndarray  # (B, 3)
-
 
 def sensor_grid(geometry: Geometry, per_axis: int = 6, pitch_mm: float = 30.0,
                 height_mm: float = 100.0, num_beats: int = 1,
@@ -195,7 +186,6 @@ def sensor_grid(geometry: Geometry, per_axis: int = 6, pitch_mm: float = 30.0,
     if beat_offsets_mm is None:
         beat_offsets_mm = np.zeros((num_beats, 3))
     return Sensors(np.array(pos), np.array(ori), np.asarray(beat_offsets_mm, dtype=np.float64))
-
 
 def measurement_matrix(geometry: Geometry, sensors: Sensors) -> np.ndarray:
     """(B, S, 3N) float32 Biot-Savart matrix, `measurement.rs:85-117`:
@@ -220,7 +210,6 @@ def measurement_matrix(geometry: Geometry, sensors: Sensors) -> np.ndarray:
             H[b, s] = (c * cross / r3[:, None]).reshape(-1).astype(np.float32)
     return H
 
-
 def gaussian_derivative_pulse(num_steps: int, sample_rate_hz: float = 2000.0,
                               t0_ms: float = 20.0, sigma_ms: float = 4.0) -> np.ndarray:
     """(T,) float32 derivative-of-Gaussian drive, peak-normalised to |u| = 1."""
@@ -228,13 +217,11 @@ def gaussian_derivative_pulse(num_steps: int, sample_rate_hz: float = 2000.0,
     g = -(t_ms - t0_ms) / sigma_ms**2 * np.exp(-((t_ms - t0_ms) ** 2) / (2.0 * sigma_ms**2))
     return (g / np.max(np.abs(g))).astype(np.float32)
 
-
 def coef_to_samples(coef: np.ndarray) -> np.ndarray:
     """allpass.rs:665-668 in float32."""
     coef = np.asarray(coef, dtype=np.float32)
     one = np.float32(1.0)
     return (one - coef) / (coef + one)
-
 
 @dataclass
 class Problem:
@@ -273,7 +260,6 @@ class Problem:
                        self.true_delays, self.initial_delays, self.control_matrix,
                        self.control_function, self.measurement_matrix, self.measurements,
                        self.true_gains, self.true_coefs, self.true_delays, dict(self.meta))
-
 
 def make_problem(geometry: Geometry, num_steps: int, max_delay: int, seed: int,
                  num_beats: int = 1, gain_sigma: float = 0.1,
@@ -342,11 +328,9 @@ def make_problem(geometry: Geometry, num_steps: int, max_delay: int, seed: int,
         p.measurements = np.ascontiguousarray(simulate(p.truth()), dtype=np.float32)
     return p
 
-
 def config0(simulate=None) -> Problem:
     """10^3 cube of 2.5 mm voxels, delays 1-20, T = 1000 at 2 kHz, one beat."""
     return make_problem(cube(10), num_steps=1000, max_delay=20, seed=0, simulate=simulate)
-
 
 def config1(simulate=None, num_steps: int = 2000) -> Problem:
     """64^3 grid masked to an ellipsoidal shell (outer semi-axes 32, inner 16
@@ -354,11 +338,9 @@ def config1(simulate=None, num_steps: int = 2000) -> Problem:
     return make_problem(ellipsoid_shell(), num_steps=num_steps, max_delay=40, seed=1,
                         simulate=simulate)
 
-
 def config2(simulate=None, num_steps: int = 2000) -> Problem:
     """32^3 full cube (N = 32,768), delays 1-40, T = 2000, one beat."""
     return make_problem(cube(32), num_steps=num_steps, max_delay=40, seed=2, simulate=simulate)
-
 
 def config3(simulate=None, count: int = 64, seed: int = 3, min_voxels: int = 1000,
             max_voxels: int = 40000, min_steps: int = 600, max_steps: int = 2000,
@@ -379,7 +361,6 @@ def config3(simulate=None, count: int = 64, seed: int = 3, min_voxels: int = 100
         out.append(make_problem(g, num_steps=steps, max_delay=20, seed=seed * 1000 + i,
                                 num_beats=beats, beat_offsets_mm=offsets, simulate=simulate))
     return out
-
 
 def config4(simulate=None, count: int = 8, seed: int = 4, beats: int = 4,
             num_steps: int = 2000) -> list:
