@@ -17,8 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libcardiotrust_hip.so")
 PROBE = os.path.join(HERE, "copy_probe")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("kernels.hip", "adjoint.hip", "api.cpp",
-                                                  "batch.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("kernels.hip", "adjoint.hip", "gradcheck.hip",
+                                                  "api.cpp", "batch.cpp")]
 ARCH = os.environ.get("CT_OFFLOAD_ARCH", "gfx950")
 
 

@@ -886,6 +886,59 @@ int ct_results_forward(ct_results_gpu *results, const ct_buffer *actual, int32_t
     return CT_OK;
 }
 
+int ct_results_objective_f64(ct_results_gpu *results, const ct_buffer *actual, int32_t beat,
+                             const ct_algorithm *config, int32_t param, int64_t index,
+                             double delta, double *objective) {
+    if (!results || !actual || !config || !objective) return fail(CT_ERR_INVALID, "null argument");
+    ct_results_gpu &r = *results;
+    CT_CHECK(check_gpu(r.gpu));
+    if (beat < 0 || beat >= r.B) return fail(CT_ERR_INVALID, "beat out of range");
+    if (actual->beats != r.B || actual->steps != r.T || actual->sensors != r.S)
+        return fail(CT_ERR_INVALID, "actual measurements are not (B, T, S) of the results");
+    if (param < 0 || param > 2) return fail(CT_ERR_INVALID, "param must be 0, 1 (gains) or 2 (coefs)");
+    if ((param == 1 && (index < 0 || index >= r.n * 78)) ||
+        (param == 2 && (index < 0 || index >= int64_t(r.N) * 26)))
+        return fail(CT_ERR_INVALID, "parameter index out of range");
+    hipStream_t s = r.gpu->stream;
+    const int64_t n = r.n, T = r.T;
+    const int64_t tiles = (r.Sp / 64) * ((T + 63) / 64);
+    const int32_t splits = int32_t(std::min<int64_t>(64, std::max<int64_t>(1, (2048 + tiles - 1) /
+                                                                              std::max<int64_t>(tiles, 1))));
+    const int64_t need = int64_t(r.N) * 78 + T * n + int64_t(splits) * r.Sp * T + T + 1;
+    if (r.f64_have < need) {
+        for (double *p : {r.y64}) {
+            if (!p) continue;
+            auto it = std::find(r.allocations.begin(), r.allocations.end(), static_cast<void *>(p));
+            if (it != r.allocations.end()) r.allocations.erase(it);
+            CT_HIP(hipStreamSynchronize(s));
+            CT_HIP(hipFree(p));
+        }
+        r.y64 = nullptr;
+        CT_CHECK(r.alloc(&r.y64, need));
+        r.f64_have = need;
+    }
+    r.x64 = r.y64 + int64_t(r.N) * 78;
+    r.p64 = r.x64 + T * n;
+    r.terms64 = r.p64 + int64_t(splits) * r.Sp * T;
+    r.j64 = r.terms64 + T;
+    ct::Perturbation pert{};
+    pert.kind = param;
+    pert.index = index;
+    pert.delta = delta;
+    CT_HIP(hipMemsetAsync(r.y64, 0, sizeof(double) * r.N * 78, s));
+    for (int32_t t = 0; t < r.T; ++t)
+        CT_HIP(ct::launch_propagate_f64(s, r.nbr, r.coefs, r.delays, r.gT, r.y64, r.x64, r.b,
+                                        r.control_function[t], r.N, t, pert));
+    const double s_mse = double(config->mse_strength) / double(r.S);
+    CT_HIP(ct::launch_objective_f64(s, r.H + int64_t(beat) * r.Sp * r.n, r.x64,
+                                    actual->data + int64_t(beat) * r.T * r.S, r.p64, splits,
+                                    r.terms64, r.j64, r.S, r.Sp, r.T, r.N, s_mse,
+                                    double(config->maximum_regularization_strength),
+                                    double(config->maximum_regularization_threshold)));
+    CT_HIP(hipStreamSynchronize(s));
+    return download(objective, r.j64, sizeof(double));
+}
+
 int ct_results_backward(ct_results_gpu *results, const ct_buffer *actual, int32_t beat,
                         const ct_algorithm *config, int32_t truncate,
                         int32_t checkpoint_interval) {

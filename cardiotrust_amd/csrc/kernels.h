@@ -154,4 +154,21 @@ hipError_t launch_expand_rows(hipStream_t s, const float *in, float *out, int64_
 
 hipError_t launch_fill(hipStream_t s, float *p, float value, int64_t len);
 
+// FP64 objective for gradient checking (gradcheck.hip). kind 0 = none,
+// 1 = gains[index] in the (3N, 78) row-major layout, 2 = coefs[index] (N, 26).
+struct Perturbation {
+    int32_t kind;
+    int32_t pad;
+    int64_t index;
+    double delta;
+};
+hipError_t launch_propagate_f64(hipStream_t s, const int32_t *nbr, const float *coefs,
+                                const int32_t *delays, const float *gT, double *y, double *x,
+                                const float *b, float u_t, int32_t N, int32_t t,
+                                const Perturbation &pert);
+hipError_t launch_objective_f64(hipStream_t s, const float *H, const double *X,
+                                const float *mhat, double *P, int32_t splits, double *terms,
+                                double *J, int32_t S, int32_t Sp, int32_t T, int32_t N,
+                                double s_mse, double s_reg, double theta);
+
 }  // namespace ct

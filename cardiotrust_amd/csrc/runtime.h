@@ -88,6 +88,9 @@ struct ct_results_gpu {
     float *qring = nullptr, *ckpt = nullptr, *yseg = nullptr;
     int64_t qring_slots = 0, ckpt_segments = 0, yseg_steps = 0;
     int64_t ring_slots_fixed = 0;  // > 0: q-ring size fixed for a captured epoch
+    // FP64 objective (gradient checking), allocated on first use
+    double *y64 = nullptr, *x64 = nullptr, *p64 = nullptr, *terms64 = nullptr, *j64 = nullptr;
+    int64_t f64_have = 0;
     int32_t forward_beat = -1;  // beat whose forward pass the device state holds
     ct::UpdateScalars *scal = nullptr;       // (B) device, per update of an epoch
     ct::UpdateScalars *scal_host = nullptr;  // (B) pinned host staging

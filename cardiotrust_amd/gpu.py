@@ -30,7 +30,7 @@ EXPORTS = (
     "ct_epoch_kernel_set_freeze_gains", "ct_epoch_kernel_set_beat_order",
     "ct_epoch_kernel_set_batch_index", "ct_epoch_kernel_get_batch_index",
     "ct_epoch_kernel_set_profiling", "ct_epoch_kernel_get_profile",
-    "ct_batch_new", "ct_batch_free", "ct_batch_execute", "ct_batch_set_voxels_per_group",
+    "ct_results_objective_f64", "ct_batch_new", "ct_batch_free", "ct_batch_execute", "ct_batch_set_voxels_per_group",
     "ct_batch_set_graph", "ct_batch_set_profiling", "ct_batch_get_info",
 )
 
@@ -131,6 +131,8 @@ def lib() -> ctypes.CDLL:
         "ct_epoch_kernel_get_batch_index": ([vp, P(i32)], ctypes.c_int),
         "ct_epoch_kernel_set_profiling": ([vp, i32], ctypes.c_int),
         "ct_epoch_kernel_get_profile": ([vp, P(Profile)], ctypes.c_int),
+        "ct_results_objective_f64": ([vp, vp, i32, P(CAlgorithm), i32, ctypes.c_int64,
+                                      ctypes.c_double, P(ctypes.c_double)], ctypes.c_int),
         "ct_batch_new": ([vp, P(vp), i32, P(vp)], ctypes.c_int),
         "ct_batch_free": ([vp], None),
         "ct_batch_execute": ([vp], ctypes.c_int),
@@ -269,6 +271,17 @@ class ResultsGPU:
         _check(lib().ct_results_set_propagation(self.handle, int(mode), ctypes.byref(res)),
                "set_propagation")
         return bool(res.value)
+
+    def objective_f64(self, actual: "Buffer", beat: int, config: Algorithm, param: int = 0,
+                      index: int = 0, delta: float = 0.0) -> float:
+        """FP64 objective with parameter `index` of `param` (1 gains, 2
+        coefs, 0 none) shifted by `delta` (gradient checking)."""
+        J = ctypes.c_double()
+        c = config.to_c()
+        _check(lib().ct_results_objective_f64(self.handle, actual.handle, int(beat),
+                                              ctypes.byref(c), int(param), int(index),
+                                              float(delta), ctypes.byref(J)), "objective_f64")
+        return J.value
 
     def forward(self, actual: "Buffer", beat: int, config: Algorithm,
                 checkpoint_interval: int = 64) -> float:

@@ -197,6 +197,17 @@ int ct_results_backward(ct_results_gpu *results, const ct_buffer *actual_measure
                         int32_t beat, const ct_algorithm *config, int32_t truncate,
                         int32_t checkpoint_interval);
 
+/* Gradient check (no reference counterpart; the counterpart of
+ * torch.autograd.gradcheck): the objective J of ct_results_forward for
+ * `beat`, evaluated in FP64 (propagation, projection and sums in double) from
+ * the FP32 parameters widened to double, with one parameter shifted by
+ * `delta` in FP64: param 0 = none, 1 = gains[index] ((3N, 78) row-major),
+ * 2 = coefs[index] ((N, 26) row-major). Central differences of it check
+ * ct_results_backward at step sizes below FP32 resolution. Blocks. */
+int ct_results_objective_f64(ct_results_gpu *results, const ct_buffer *actual_measurements,
+                             int32_t beat, const ct_algorithm *config, int32_t param,
+                             int64_t index, double delta, double *objective);
+
 /* EpochKernel::set_freeze_delays / set_freeze_gains (gpu/epoch.rs:118-125). */
 int ct_epoch_kernel_set_freeze_delays(ct_epoch_kernel *kernel, int32_t value);
 int ct_epoch_kernel_set_freeze_gains(ct_epoch_kernel *kernel, int32_t value);
