@@ -18,6 +18,7 @@
 // walks it backwards, accumulating dG and da.
 #include <hip/hip_runtime.h>
 
+#include "barrier.h"
 #include "device.h"
 #include "kernels.h"
 
@@ -111,6 +112,121 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_step_kernel(
             qring[int64_t(uint32_t(s) & rmask) * M + c * 3 + oo] = a * yb + next;
             ybar[c * 3 + oo] = yb;
         }
+    }
+}
+
+// Resident backward sweep: one launch runs steps T-1 .. 0 for all voxels. A
+// workgroup owns a voxel slab [vb, ve) (at most kResMaxVoxels voxels) for the
+// whole launch: the slab's ybar (78 per voxel) stays in LDS, its consumer lists
+// (consumer connection 22 bits | delay 10 bits) and coefficients in registers.
+// Per step the arithmetic is that of adjoint_step_kernel (same expressions and
+// summation orders, so the same bits); q goes through the ring in memory and
+// a barrier over the workgroups separates the steps.
+constexpr int kSweepThreads = 512;
+constexpr int kSweepItems = (kResMaxVoxels * 26 + kSweepThreads - 1) / kSweepThreads;
+constexpr int kSweepConsumerBits = 22;
+constexpr uint32_t kSweepNoEntry = 0xFFFFFFFFu;
+
+__global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
+    const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
+    const float *__restrict__ gains_rows, const int32_t *__restrict__ cons_ptr,
+    const int32_t *__restrict__ cons_idx, const int32_t *__restrict__ cons_delay,
+    const float *__restrict__ x, const float *__restrict__ E, float *__restrict__ qring,
+    float *__restrict__ xbar, GridBarrier *__restrict__ gb, int32_t N, int32_t T, uint32_t rmask,
+    AdjointParams p, uint64_t wait_ticks) {
+    extern __shared__ float lds[];
+    float *yb_lds = lds;                                   // [voxel][78]
+    float *xb_lds = lds + kResMaxVoxels * 78;              // [voxel][3]
+    int *ok_lds = reinterpret_cast<int *>(xb_lds + kResMaxVoxels * 3);
+    const int64_t n = 3LL * N;
+    const int64_t M = int64_t(N) * 78;  // one ring slot
+    const int32_t slab = xcd_block(blockIdx.x, gridDim.x);
+    const int32_t vb = int32_t(int64_t(slab) * N / gridDim.x);
+    const int32_t ve = int32_t(int64_t(slab + 1) * N / gridDim.x);
+    const int32_t nv = ve - vb, items = nv * 26;
+    const int tid = threadIdx.x;
+    uint32_t gen = 0;
+
+    // consumers of the voxel this lane owns in phase A (lane < nv)
+    uint32_t cons[26];
+    int32_t ncons = 0;
+#pragma unroll
+    for (int j = 0; j < 26; ++j) cons[j] = kSweepNoEntry;
+    if (tid < nv) {
+        const int32_t w = vb + tid;
+        const int32_t j0 = cons_ptr[w], j1 = cons_ptr[w + 1];
+        ncons = j1 - j0;
+#pragma unroll
+        for (int j = 0; j < 26; ++j)
+            if (j < ncons)
+                cons[j] = uint32_t(cons_idx[j0 + j]) |
+                          (uint32_t(min(cons_delay[j0 + j], 1023)) << kSweepConsumerBits);
+    }
+    // coefficients of the connections this lane owns in phase B
+    float ak[kSweepItems];
+    uint32_t has = 0;
+#pragma unroll
+    for (int k = 0; k < kSweepItems; ++k) {
+        const int it = tid + k * kSweepThreads;
+        ak[k] = 0.0f;
+        if (it < items) {
+            const int64_t c = int64_t(vb) * 26 + it;
+            if (nbr[c] >= 0) {
+                has |= 1u << k;
+                ak[k] = coefs[c];
+            }
+        }
+    }
+    for (int idx = tid; idx < nv * 78; idx += kSweepThreads) yb_lds[idx] = 0.0f;
+    __syncthreads();
+
+    for (int32_t s = T - 1; s >= 0; --s) {
+        // phase A: xbar_s of the slab's states
+        if (tid < nv) {
+            const int32_t w = vb + tid;
+            const int64_t row = int64_t(s) * n + 3LL * w;
+            const float xv[3] = {x[row], x[row + 1], x[row + 2]};
+            float acc[3];
+#pragma unroll
+            for (int o = 0; o < 3; ++o) acc[o] = step_factor(xv, o, E[row + o], p);
+#pragma unroll
+            for (int j = 0; j < 26; ++j) {
+                if (j >= ncons) break;
+                const uint32_t e = cons[j];
+                const int32_t t = s + int32_t(e >> kSweepConsumerBits);
+                if (t < T) {
+                    const float *q = qring + int64_t(uint32_t(t) & rmask) * M +
+                                     int64_t(e & ((1u << kSweepConsumerBits) - 1)) * 3;
+#pragma unroll
+                    for (int o = 0; o < 3; ++o) acc[o] += q[o];
+                }
+            }
+#pragma unroll
+            for (int o = 0; o < 3; ++o) {
+                xb_lds[tid * 3 + o] = acc[o];
+                xbar[row + o] = acc[o];
+            }
+        }
+        __syncthreads();
+        // phase B: ybar_s and q_s of the slab's connections
+        float *qs = qring + int64_t(uint32_t(s) & rmask) * M;
+#pragma unroll
+        for (int k = 0; k < kSweepItems; ++k) {
+            if (!((has >> k) & 1u)) continue;
+            const int it = tid + k * kSweepThreads;
+            const int vv = it / 26, d = it - 26 * vv;
+            const int64_t c = int64_t(vb) * 26 + it;
+            const float *g = gains_rows + 3LL * (vb + vv) * 78 + 3 * d;
+            const float x0 = xb_lds[3 * vv], x1 = xb_lds[3 * vv + 1], x2 = xb_lds[3 * vv + 2];
+#pragma unroll
+            for (int oo = 0; oo < 3; ++oo) {
+                const float next = yb_lds[it * 3 + oo];
+                const float yb = g[oo] * x0 + g[78 + oo] * x1 + g[156 + oo] * x2 - ak[k] * next;
+                qs[c * 3 + oo] = ak[k] * yb + next;
+                yb_lds[it * 3 + oo] = yb;
+            }
+        }
+        if (!team_barrier(gb, gridDim.x, blockIdx.x, gen, ok_lds, wait_ticks)) return;
     }
 }
 
@@ -255,6 +371,37 @@ hipError_t launch_adjoint_step(hipStream_t s, const int32_t *nbr, const float *c
     adjoint_step_kernel<<<blocks, kAdjThreads, 0, s>>>(nbr, coefs, gains_rows, cons_ptr, cons_idx,
                                                        cons_delay, x, E, ybar, qring, xbar, N, T,
                                                        step, rmask, p);
+    return hipGetLastError();
+}
+
+size_t sweep_lds_bytes() { return sizeof(float) * kResMaxVoxels * 81 + 16; }
+
+hipError_t sweep_capacity(int32_t *blocks_per_cu) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&adjoint_sweep_resident_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       int(sweep_lds_bytes()));
+    if (e != hipSuccess) return e;
+    int nb = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &nb, reinterpret_cast<const void *>(&adjoint_sweep_resident_kernel), kSweepThreads,
+        sweep_lds_bytes());
+    *blocks_per_cu = nb;
+    return e;
+}
+
+hipError_t launch_adjoint_sweep_resident(hipStream_t s, int32_t blocks, const int32_t *nbr,
+                                         const float *coefs, const float *gains_rows,
+                                         const int32_t *cons_ptr, const int32_t *cons_idx,
+                                         const int32_t *cons_delay, const float *x,
+                                         const float *E, float *qring, float *xbar,
+                                         GridBarrier *gb, int32_t N, int32_t T, uint32_t rmask,
+                                         const AdjointParams &p, uint64_t wait_ticks) {
+    if (N <= 0 || T <= 0) return hipSuccess;
+    hipError_t e = launch_barrier_reset(s, gb, 1);
+    if (e != hipSuccess) return e;
+    adjoint_sweep_resident_kernel<<<blocks, kSweepThreads, sweep_lds_bytes(), s>>>(
+        nbr, coefs, gains_rows, cons_ptr, cons_idx, cons_delay, x, E, qring, xbar, gb, N, T, rmask,
+        p, wait_ticks);
     return hipGetLastError();
 }
 

@@ -327,6 +327,10 @@ int plan_resident(ct_results_gpu &r) {
         return CT_OK;
     r.resident_blocks = int32_t(blocks);
     r.resident_ok = true;
+    int32_t sweep_per_cu = 0;
+    r.sweep_ok = ct::sweep_capacity(&sweep_per_cu) == hipSuccess &&
+                 blocks <= int64_t(cus) * sweep_per_cu;
+    (void)hipGetLastError();
     return CT_OK;
 }
 
@@ -406,6 +410,8 @@ int build_consumers(ct_results_gpu &r) {
     CT_CHECK(r.alloc(&r.cons_idx, std::max<int64_t>(int64_t(idx.size()), 1)));
     CT_CHECK(r.alloc(&r.cons_delay, std::max<int64_t>(int64_t(idx.size()), 1)));
     r.cons_len = int64_t(idx.size());
+    r.max_consumers = 0;
+    for (int64_t w = 0; w < N; ++w) r.max_consumers = std::max(r.max_consumers, ptr[w + 1] - ptr[w]);
     CT_CHECK(upload(r.cons_ptr, ptr.data(), sizeof(int32_t) * (N + 1)));
     CT_CHECK(upload(r.cons_idx, idx.data(), sizeof(int32_t) * idx.size()));
     return CT_OK;
@@ -462,12 +468,23 @@ int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32
     p.threshold = c.maximum_regularization_threshold;
     {
         Span sp(k, kAdjointSweep);
-        CT_HIP(ct::launch_fill(s, r.ybar, 0.0f, M));
         CT_HIP(ct::launch_consumer_delays(s, r.cons_idx, r.delays, r.cons_delay, r.cons_len));
-        for (int32_t step = r.T - 1; step >= 0; --step)
-            CT_HIP(ct::launch_adjoint_step(s, r.nbr, r.coefs, r.staging, r.cons_ptr, r.cons_idx,
-                                           r.cons_delay, r.x, r.E, r.ybar, r.qring, r.xbar, r.N,
-                                           r.T, step, uint32_t(slots - 1), p));
+        if (r.sweep_ok && r.propagation_mode != 1 && slots <= 1024 && r.max_consumers <= 26) {
+            // all steps in one launch (delays < 1024 and at most 26 consumers
+            // per voxel fit its register tables)
+            CT_HIP(ct::launch_adjoint_sweep_resident(
+                s, r.resident_blocks, r.nbr, r.coefs, r.staging, r.cons_ptr, r.cons_idx,
+                r.cons_delay, r.x, r.E, r.qring, r.xbar, r.barrier, r.N, r.T,
+                uint32_t(slots - 1), p, wait_ticks(*r.gpu, 2.0)));
+            r.resident_used = true;
+        } else {
+            CT_HIP(ct::launch_fill(s, r.ybar, 0.0f, M));
+            for (int32_t step = r.T - 1; step >= 0; --step)
+                CT_HIP(ct::launch_adjoint_step(s, r.nbr, r.coefs, r.staging, r.cons_ptr,
+                                               r.cons_idx, r.cons_delay, r.x, r.E, r.ybar,
+                                               r.qring, r.xbar, r.N, r.T, step,
+                                               uint32_t(slots - 1), p));
+        }
     }
     Span sp(k, kDerive);
     CT_HIP(ct::launch_transpose(s, r.xbar, r.xbarT, T, n));
@@ -954,6 +971,7 @@ int ct_results_backward(ct_results_gpu *results, const ct_buffer *actual, int32_
     } else {
         CT_CHECK(derive_adjoint(r, beat, *config, checkpoint_interval, nullptr));
     }
+    CT_CHECK(check_resident(r));
     CT_HIP(hipStreamSynchronize(r.gpu->stream));
     return CT_OK;
 }

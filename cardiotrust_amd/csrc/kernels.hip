@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "barrier.h"
 #include "device.h"
 
 namespace ct {
@@ -128,63 +129,6 @@ __global__ __launch_bounds__(kPropThreads) void propagate_step_kernel(
 // raises `abort`, every workgroup of the job leaves at its next barrier, and
 // the host reports the failure instead of the kernel waiting forever.
 // ---------------------------------------------------------------------------
-
-// Two-level counting barrier over `members` workgroups. Member m first counts
-// into group m % 8, the last arrival of a group counts into the top counter,
-// and the last group arrival publishes the next generation. Counters only grow
-// within a launch (they are zeroed before it), so "last" is recognised from
-// the running count and nothing has to be reset between steps.
-__device__ __forceinline__ bool team_barrier(GridBarrier *gb, uint32_t members, uint32_t member,
-                                             uint32_t &gen, int *ok_lds, uint64_t wait_ticks) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int ok = 1;
-        const uint32_t next = gen + 1;
-        const uint32_t grp = member % kBarrierGroups;
-        const uint32_t gsize = (members - grp + kBarrierGroups - 1) / kBarrierGroups;
-        const uint32_t ngroups = members < kBarrierGroups ? members : kBarrierGroups;
-        // release: this workgroup's state rows reach device-coherent memory
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const uint32_t a1 = __hip_atomic_fetch_add(&gb->group[grp].count, 1u, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-        bool released = false;
-        if (a1 == next * gsize - 1) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-            const uint32_t a2 = __hip_atomic_fetch_add(&gb->top.count, 1u, __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT);
-            if (a2 == next * ngroups - 1) {
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-                __hip_atomic_store(&gb->generation.count, next, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                released = true;
-            }
-        }
-        if (!released) {
-            const uint64_t t0 = uint64_t(wall_clock64());
-            uint32_t polls = 0;
-            while (__hip_atomic_load(&gb->generation.count, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT) < next) {
-                if (__hip_atomic_load(&gb->abort.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    ok = 0;
-                    break;
-                }
-                if ((++polls & 63u) == 0 && uint64_t(wall_clock64()) - t0 > wait_ticks) {
-                    __hip_atomic_store(&gb->abort.count, 1u, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    ok = 0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-        }
-        // acquire: drop stale cached copies of the other workgroups' rows
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        *ok_lds = ok;
-    }
-    gen += 1;
-    __syncthreads();
-    return *ok_lds != 0;
-}
 
 constexpr int kResThreads = 512;
 constexpr int kResItems = (kResMaxVoxels * 26 + kResThreads - 1) / kResThreads;
@@ -972,6 +916,12 @@ hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const Reside
     if (e != hipSuccess) return e;
     propagate_resident_kernel<<<blocks, kResThreads, resident_lds_bytes(), s>>>(job, t_begin, t_end,
                                                                                 wait_ticks);
+    return hipGetLastError();
+}
+
+hipError_t launch_barrier_reset(hipStream_t s, GridBarrier *gb, int32_t count) {
+    if (count <= 0) return hipSuccess;
+    barrier_reset_kernel<<<blocks_for(int64_t(count) * 10, 256), 256, 0, s>>>(gb, count);
     return hipGetLastError();
 }
 

@@ -74,6 +74,7 @@ struct ct_results_gpu {
     bool resident_ok = false;
     bool resident_used = false;  // a resident launch is in flight since the last check
     int32_t resident_blocks = 0;
+    bool sweep_ok = false;  // the resident adjoint sweep fits with resident_blocks workgroups
     ct::GridBarrier *barrier = nullptr;
     // scratch
     float *partial = nullptr, *mr_partial = nullptr, *staging = nullptr, *step_mr = nullptr;
@@ -82,6 +83,7 @@ struct ct_results_gpu {
     // buffers allocated on first use
     std::vector<int32_t> nbr_host;
     int32_t *cons_ptr = nullptr, *cons_idx = nullptr, *cons_delay = nullptr, *dmax = nullptr;
+    int32_t max_consumers = 0;    // longest consumer list (26 on a grid)
     int64_t cons_len = 0;
     int32_t ckpt_interval = 0;  // checkpoint interval of the last forward pass (0 = none)
     float *E = nullptr, *xbar = nullptr, *xbarT = nullptr, *ybar = nullptr;
