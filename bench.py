@@ -97,6 +97,7 @@ def bench_batched(args) -> int:
         k = EpochKernel(gpu, r, b, cfg, 3 * p.num_voxels, p.num_sensors, p.num_steps)
         if adjoint:
             k.set_gradient(1, args.checkpoint)
+        k.set_share_propagation(args.share_propagation)
         members.append((r, b, k))
     voxel_steps = sum(p.num_voxels * p.num_steps * p.num_beats for p in problems)
 
@@ -129,10 +130,11 @@ def bench_batched(args) -> int:
     ms_per_epoch = 1e3 * elapsed / args.steps
     # roofline of the batched propagation launches (eager epoch, HIP events)
     prop_ms = info0["propagate_ms"]
-    resident_vs = 0
+    resident_vs = 0  # voxel-steps the batched launches propagated in one epoch
     for (r, _, _), p in zip(members, problems):
         if r.set_propagation(0):
-            resident_vs += p.num_voxels * p.num_steps * p.num_beats
+            launches = 1 if args.share_propagation else p.num_beats  # batch_size 0
+            resident_vs += p.num_voxels * p.num_steps * launches
     achieved = PROP_BYTES_PER_VOXEL_STEP * resident_vs / (prop_ms * 1e-3) / 1e9 if prop_ms else 0.0
     out = {
         "metric": "batched epochs/s (all scenarios advance one epoch)",
@@ -148,6 +150,7 @@ def bench_batched(args) -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": label, "scenarios": len(problems),
+                   "shared_propagation": bool(args.share_propagation),
                    "voxel_steps_per_epoch": voxel_steps, "graph": bool(info["graph"]),
                    "voxels_per_group": info["voxels_per_group"], "rounds": info["rounds"],
                    "jobs": info["jobs"]},
@@ -192,6 +195,9 @@ def main() -> int:
     ap.add_argument("--voxels-per-group", type=int, default=0,
                     help="batched workloads: 0 = autotune")
     ap.add_argument("--no-graph", action="store_true", help="batched workloads: eager epochs")
+    ap.add_argument("--share-propagation", action="store_true",
+                    help="batched workloads: one propagation (and, with the reverse-mode "
+                         "gradient, one backward pass) per update batch of beats")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the epoch on a single MI355X; "

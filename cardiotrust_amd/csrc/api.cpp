@@ -215,17 +215,45 @@ bool uses_adjoint(const ct_epoch_kernel &k) {
     return k.gradient == 1 && !(k.cfg.freeze_gains && k.cfg.freeze_delays);
 }
 
-int finish_beat(ct_epoch_kernel &k, int32_t beat) {
-    ct_results_gpu &r = *k.r;
-    CT_CHECK(project_beat(r, beat, k.mhat->data, &k));
-    CT_CHECK(losses_beat(r, k.cfg, &k));
-    if (uses_adjoint(k)) return derive_adjoint(r, beat, k.cfg, k.checkpoint_interval, &k);
-    return derive_reference(r, beat, k.cfg, &k);
+static int adjoint_prepare(ct_results_gpu &r, int32_t K, int64_t &slots);
+static int adjoint_forcing(ct_results_gpu &r, int32_t beat, bool accumulate, ct_epoch_kernel *k);
+static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K, int64_t slots,
+                            int32_t num_beats, ct_epoch_kernel *k);
+
+int32_t update_group(const ct_epoch_kernel &k) {
+    return k.cfg.batch_size == 0 ? k.r->B : k.cfg.batch_size;
 }
 
-static int run_beat(ct_epoch_kernel &k, int32_t beat) {
-    CT_CHECK(propagate_beat(*k.r, &k, uses_adjoint(k) ? k.checkpoint_interval : 0));
-    return finish_beat(k, beat);
+bool propagates_at(const ct_epoch_kernel &k, int32_t pos) {
+    return !k.share_propagation || pos % update_group(k) == 0;
+}
+
+int finish_beat(ct_epoch_kernel &k, int32_t pos) {
+    ct_results_gpu &r = *k.r;
+    const int32_t beat = k.order[pos];
+    CT_CHECK(project_beat(r, beat, k.mhat->data, &k));
+    CT_CHECK(losses_beat(r, k.cfg, &k));
+    if (!uses_adjoint(k)) return derive_reference(r, beat, k.cfg, &k);
+    if (!k.share_propagation) return derive_adjoint(r, beat, k.cfg, k.checkpoint_interval, &k);
+    // one backward pass per update batch, for the batch's summed objective
+    if (r.N <= 0) return CT_OK;
+    const int32_t group = update_group(k), first = pos - pos % group;
+    const int32_t last = std::min<int32_t>(first + group, int32_t(k.order.size())) - 1;
+    if (pos == first) {
+        CT_CHECK(adjoint_prepare(r, k.checkpoint_interval, r.pending_slots));
+        if (r.ckpt_interval != k.checkpoint_interval)
+            return fail(CT_ERR_STATE, "shared propagation without the adjoint's checkpoints");
+    }
+    CT_CHECK(adjoint_forcing(r, beat, pos != first, &k));
+    if (pos != last) return CT_OK;
+    return adjoint_backward(r, k.cfg, k.checkpoint_interval, r.pending_slots, last - first + 1,
+                            &k);
+}
+
+static int run_beat(ct_epoch_kernel &k, int32_t pos) {
+    if (propagates_at(k, pos))
+        CT_CHECK(propagate_beat(*k.r, &k, uses_adjoint(k) ? k.checkpoint_interval : 0));
+    return finish_beat(k, pos);
 }
 
 // algorithm.rs:188-212 / 224-247 / 255-278: average delays, batch derivatives,
@@ -419,9 +447,8 @@ int build_consumers(ct_results_gpu &r) {
 
 // Reverse-mode gradient of the forward pass held on the device (adjoint.hip);
 // adds dJ/dG and dJ/da of this beat to the derivative accumulators.
-int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32_t K,
-                   ct_epoch_kernel *k) {
-    if (r.N <= 0) return CT_OK;
+// Buffers and q-ring size of the reverse-mode gradient (checkpoints every K).
+static int adjoint_prepare(ct_results_gpu &r, int32_t K, int64_t &slots) {
     if (K < 1) return fail(CT_ERR_INVALID, "checkpoint interval must be >= 1");
     hipStream_t s = r.gpu->stream;
     const int64_t n = r.n, T = r.T, M = int64_t(r.N) * 78;
@@ -433,7 +460,7 @@ int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32
     CT_CHECK(ensure(r, &r.xbarT, have_xbt, n * T, 1));
     CT_CHECK(ensure(r, &r.ybar, have_yb, M, 1));
     // ring of q over R >= Dmax + 1 slots (R a power of two)
-    int64_t slots = r.ring_slots_fixed;
+    slots = r.ring_slots_fixed;
     if (slots <= 0) {
         CT_HIP(ct::launch_max_delay(s, r.nbr, r.delays, int64_t(r.N) * 26, r.dmax));
         int32_t dmax = 0;
@@ -445,18 +472,34 @@ int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32
     int64_t have_q = r.qring_slots * M;
     CT_CHECK(ensure(r, &r.qring, have_q, slots * M, 1));
     r.qring_slots = std::max(r.qring_slots, slots);
-    if (r.ckpt_interval != K || r.forward_beat != beat)
-        return fail(CT_ERR_STATE, "the reverse-mode gradient needs the forward pass of this beat "
-                                  "with checkpoints every " + std::to_string(K) + " steps");
     int64_t have_y = r.yseg_steps * M;
     CT_CHECK(ensure(r, &r.yseg, have_y, int64_t(K) * M, 1));
     r.yseg_steps = std::max(r.yseg_steps, int64_t(K));
+    return CT_OK;
+}
 
+// Hᵀ r of `beat` for every step into eT (n, T); with `accumulate` added to
+// what eT holds (beats of one update batch, in beat order).
+static int adjoint_forcing(ct_results_gpu &r, int32_t beat, bool accumulate, ct_epoch_kernel *k) {
+    hipStream_t s = r.gpu->stream;
     const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
-    {
-        Span sp(k, kMapResiduals);
-        CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.eT, n, r.T, r.Sp));
+    Span sp(k, kMapResiduals);
+    if (!accumulate) {
+        CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.eT, r.n, r.T, r.Sp));
+    } else {
+        CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.xbarT, r.n, r.T, r.Sp));  // scratch
+        CT_HIP(ct::launch_add(s, r.eT, r.xbarT, r.n * r.T));
     }
+    return CT_OK;
+}
+
+// Backward sweep and gradient pass for the forcing in eT, which sums
+// `num_beats` beats that share the forward states (each beat contributes the
+// maximum-regularisation term once).
+static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K, int64_t slots,
+                            int32_t num_beats, ct_epoch_kernel *k) {
+    hipStream_t s = r.gpu->stream;
+    const int64_t n = r.n, T = r.T, M = int64_t(r.N) * 78;
     {
         Span sp(k, kTranspose);
         CT_HIP(ct::launch_transpose(s, r.eT, r.E, n, T));      // mapped residuals by step
@@ -464,7 +507,7 @@ int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32
     }
     ct::AdjointParams p;
     p.mse_scaling = 1.0f / float(r.S) * c.mse_strength;
-    p.reg_scaling = c.maximum_regularization_strength;
+    p.reg_scaling = c.maximum_regularization_strength * float(num_beats);
     p.threshold = c.maximum_regularization_threshold;
     {
         Span sp(k, kAdjointSweep);
@@ -492,6 +535,18 @@ int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32
     CT_HIP(ct::launch_adjoint_gradient(s, r.nbr, r.coefs, r.delays, r.gT, r.xT, r.xbarT, r.ckpt,
                                        r.yseg, r.dGT, r.dA, r.N, r.T, K));
     return CT_OK;
+}
+
+int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32_t K,
+                   ct_epoch_kernel *k) {
+    if (r.N <= 0) return CT_OK;
+    int64_t slots = 0;
+    CT_CHECK(adjoint_prepare(r, K, slots));
+    if (r.ckpt_interval != K || r.forward_beat != beat)
+        return fail(CT_ERR_STATE, "the reverse-mode gradient needs the forward pass of this beat "
+                                  "with checkpoints every " + std::to_string(K) + " steps");
+    CT_CHECK(adjoint_forcing(r, beat, false, k));
+    return adjoint_backward(r, c, K, slots, 1, k);
 }
 
 }  // namespace ctrt
@@ -846,8 +901,8 @@ int ct_epoch_kernel_execute(ct_epoch_kernel *k) {
     const bool batched = c.batch_size != 0;
     int64_t in_batch = 0;
     int32_t update = 0;
-    for (int32_t beat : k->order) {
-        CT_CHECK(run_beat(*k, beat));
+    for (int32_t pos = 0; pos < int32_t(k->order.size()); ++pos) {
+        CT_CHECK(run_beat(*k, pos));
         if (batched) {
             in_batch += 1;
             if (in_batch == c.batch_size) {
@@ -973,6 +1028,12 @@ int ct_results_backward(ct_results_gpu *results, const ct_buffer *actual, int32_
     }
     CT_CHECK(check_resident(r));
     CT_HIP(hipStreamSynchronize(r.gpu->stream));
+    return CT_OK;
+}
+
+int ct_epoch_kernel_set_share_propagation(ct_epoch_kernel *k, int32_t enabled) {
+    if (!k) return fail(CT_ERR_INVALID, "null argument");
+    k->share_propagation = enabled != 0;
     return CT_OK;
 }
 

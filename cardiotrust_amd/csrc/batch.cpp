@@ -32,7 +32,8 @@ struct ct_batch {
     // plan: per round, the jobs (scenario indices) and their device tables
     struct Round {
         std::vector<int32_t> scenarios;   // all scenarios with a beat in this round
-        std::vector<int32_t> resident;    // those in the batched launch (job order)
+        std::vector<int32_t> propagate;   // those that propagate in this round
+        std::vector<int32_t> resident;    // of those, the ones in the batched launch (job order)
         int32_t job0 = 0;                 // first job / barrier of the round
         int32_t task0 = 0;                // first task of the round
         int32_t ptr0 = 0;                 // first task_ptr entry (slots + 1 per round)
@@ -40,6 +41,7 @@ struct ct_batch {
     };
     std::vector<Round> rounds;
     std::vector<int32_t> plan_checkpoints;  // checkpoint interval per scenario the plan used
+    std::vector<int32_t> plan_groups;       // beats per shared propagation (0 = not shared)
     int32_t plan_vpg = -1;
     ct::ResidentJob *jobs = nullptr;
     int32_t *task_ptr = nullptr, *task_job = nullptr;
@@ -183,11 +185,13 @@ int plan(ct_batch &b, int32_t vpg) {
     for (ct_epoch_kernel *k : b.kernels) rounds = std::max(rounds, k->r->B);
     b.rounds.assign(rounds, {});
     b.plan_checkpoints.resize(S);
+    b.plan_groups.resize(S);
     std::vector<ct::ResidentJob> all_jobs;
     std::vector<int32_t> all_ptr, all_task;
     for (int32_t s = 0; s < S; ++s) {
         ct_epoch_kernel &k = *b.kernels[s];
         b.plan_checkpoints[s] = batch_checkpoint(k);
+        b.plan_groups[s] = k.share_propagation ? update_group(k) : 0;
         // checkpoint buffers are allocated now: the job table holds their address
         if (b.plan_checkpoints[s] > 0 && k.r->N > 0)
             CT_CHECK(ensure_checkpoints(*k.r, b.plan_checkpoints[s]));
@@ -199,6 +203,8 @@ int plan(ct_batch &b, int32_t vpg) {
             ct_results_gpu &r = *b.kernels[s]->r;
             if (r.B <= rd) continue;
             R.scenarios.push_back(s);
+            if (!propagates_at(*b.kernels[s], rd)) continue;
+            R.propagate.push_back(s);
             const bool fits = scenario_resident(r) &&
                               (r.N + ct::kResMaxVoxels - 1) / ct::kResMaxVoxels <= b.slots;
             if (fits && b.slots > 0) {
@@ -254,8 +260,12 @@ int plan(ct_batch &b, int32_t vpg) {
 
 bool plan_current(const ct_batch &b) {
     if (b.plan_vpg != b.voxels_per_group) return false;
-    for (size_t s = 0; s < b.kernels.size(); ++s)
-        if (b.plan_checkpoints[s] != batch_checkpoint(*b.kernels[s])) return false;
+    for (size_t s = 0; s < b.kernels.size(); ++s) {
+        const ct_epoch_kernel &k = *b.kernels[s];
+        if (b.plan_checkpoints[s] != batch_checkpoint(k) ||
+            b.plan_groups[s] != (k.share_propagation ? update_group(k) : 0))
+            return false;
+    }
     return true;
 }
 
@@ -264,7 +274,7 @@ bool plan_current(const ct_batch &b) {
 int propagate_round(ct_batch &b, int32_t rd, bool timed) {
     const ct_batch::Round &R = b.rounds[rd];
     hipStream_t s = b.gpu->stream;
-    for (int32_t sc : R.scenarios) {
+    for (int32_t sc : R.propagate) {
         ct_epoch_kernel &k = *b.kernels[sc];
         const bool in_launch =
             std::find(R.resident.begin(), R.resident.end(), sc) != R.resident.end();
@@ -299,7 +309,7 @@ int enqueue_epoch(ct_batch &b, bool timed) {
         CT_CHECK(propagate_round(b, rd, timed));
         for (int32_t s : b.rounds[rd].scenarios) {
             ct_epoch_kernel &k = *b.kernels[s];
-            CT_CHECK(finish_beat(k, k.order[rd]));
+            CT_CHECK(finish_beat(k, rd));
             if (k.cfg.batch_size != 0 && ++in_batch[s] == k.cfg.batch_size) {
                 CT_CHECK(finish_batch(k, in_batch[s], true, update[s]++));
                 in_batch[s] = 0;
@@ -389,7 +399,8 @@ bool graph_current(const ct_batch &b) {
     for (size_t s = 0; s < b.kernels.size(); ++s) {
         const ct_epoch_kernel &k = *b.kernels[s];
         if (k.order != b.graph_orders[s] || !same_cfg(k.cfg, b.graph_cfgs[s]) ||
-            k.gradient * 100000 + k.checkpoint_interval != b.graph_gradients[s])
+            k.gradient * 100000 + k.checkpoint_interval + (k.share_propagation ? 1 << 30 : 0) !=
+                b.graph_gradients[s])
             return false;
     }
     return true;
@@ -409,7 +420,8 @@ int capture(ct_batch &b) {
             CT_CHECK(fix_adjoint_ring(*k->r, int64_t(kGraphEpochs) * updates_per_epoch(*k)));
         b.graph_orders.push_back(k->order);
         b.graph_cfgs.push_back(k->cfg);
-        b.graph_gradients.push_back(k->gradient * 100000 + k->checkpoint_interval);
+        b.graph_gradients.push_back(k->gradient * 100000 + k->checkpoint_interval +
+                                    (k->share_propagation ? 1 << 30 : 0));
     }
     CT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const int rc = enqueue_epoch(b, false);

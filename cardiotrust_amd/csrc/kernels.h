@@ -167,6 +167,7 @@ hipError_t launch_batch_metrics(hipStream_t s, const float *loss, const float *l
 hipError_t launch_expand_rows(hipStream_t s, const float *in, float *out, int64_t N);
 
 hipError_t launch_fill(hipStream_t s, float *p, float value, int64_t len);
+hipError_t launch_add(hipStream_t s, float *a, const float *b, int64_t len);  // a += b
 
 // FP64 objective for gradient checking (gradcheck.hip). kind 0 = none,
 // 1 = gains[index] in the (3N, 78) row-major layout, 2 = coefs[index] (N, 26).

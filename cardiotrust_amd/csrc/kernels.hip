@@ -862,6 +862,11 @@ __global__ void expand_rows_kernel(const float *__restrict__ in, float *__restri
     out[k] = in[(row / 3) * 78 + g];
 }
 
+__global__ void add_kernel(float *__restrict__ a, const float *__restrict__ b, int64_t len) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < len) a[k] = a[k] + b[k];
+}
+
 __global__ void fill_kernel(float *p, float value, int64_t len) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k < len) p[k] = value;
@@ -916,6 +921,12 @@ hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const Reside
     if (e != hipSuccess) return e;
     propagate_resident_kernel<<<blocks, kResThreads, resident_lds_bytes(), s>>>(job, t_begin, t_end,
                                                                                 wait_ticks);
+    return hipGetLastError();
+}
+
+hipError_t launch_add(hipStream_t s, float *a, const float *b, int64_t len) {
+    if (len <= 0) return hipSuccess;
+    add_kernel<<<blocks_for(len, 256), 256, 0, s>>>(a, b, len);
     return hipGetLastError();
 }
 

@@ -90,6 +90,7 @@ struct ct_results_gpu {
     float *qring = nullptr, *ckpt = nullptr, *yseg = nullptr;
     int64_t qring_slots = 0, ckpt_segments = 0, yseg_steps = 0;
     int64_t ring_slots_fixed = 0;  // > 0: q-ring size fixed for a captured epoch
+    int64_t pending_slots = 0;     // q-ring size of the update batch being differentiated
     // FP64 objective (gradient checking), allocated on first use
     double *y64 = nullptr, *x64 = nullptr, *p64 = nullptr, *terms64 = nullptr, *j64 = nullptr;
     int64_t f64_have = 0;
@@ -128,6 +129,10 @@ struct ct_epoch_kernel {
     bool profiling = false;
     int32_t gradient = 0;            // 0 = reference derivation, 1 = reverse-mode adjoint
     int32_t checkpoint_interval = 64;
+    // beats of one update batch share one propagation (their states are
+    // identical: same parameters and drive); with the reverse-mode gradient
+    // also one backward pass for the batch's summed objective
+    bool share_propagation = false;
     std::vector<hipEvent_t> events;
     size_t used_events = 0;
     struct SpanRec {
@@ -184,8 +189,13 @@ int losses_beat(ct_results_gpu &r, const ct_algorithm &c, ct_epoch_kernel *k);
 int derive_reference(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, ct_epoch_kernel *k);
 int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32_t K,
                    ct_epoch_kernel *k);
-// Everything of a beat after its propagation (run_epoch's per-beat body).
-int finish_beat(ct_epoch_kernel &k, int32_t beat);
+// Everything of the beat at position `pos` of the beat order after its
+// propagation (run_epoch's per-beat body).
+int finish_beat(ct_epoch_kernel &k, int32_t pos);
+// Beats per update batch, and whether the beat at `pos` runs the propagation
+// (always, unless the update batch shares one propagation).
+int32_t update_group(const ct_epoch_kernel &k);
+bool propagates_at(const ct_epoch_kernel &k, int32_t pos);
 bool uses_adjoint(const ct_epoch_kernel &k);
 int finish_batch(ct_epoch_kernel &k, int64_t num_beats, bool reset_after, int32_t update);
 int32_t updates_per_epoch(const ct_epoch_kernel &k);

@@ -30,7 +30,7 @@ EXPORTS = (
     "ct_epoch_kernel_set_freeze_gains", "ct_epoch_kernel_set_beat_order",
     "ct_epoch_kernel_set_batch_index", "ct_epoch_kernel_get_batch_index",
     "ct_epoch_kernel_set_profiling", "ct_epoch_kernel_get_profile",
-    "ct_results_objective_f64", "ct_batch_new", "ct_batch_free", "ct_batch_execute", "ct_batch_set_voxels_per_group",
+    "ct_results_objective_f64", "ct_epoch_kernel_set_share_propagation", "ct_batch_new", "ct_batch_free", "ct_batch_execute", "ct_batch_set_voxels_per_group",
     "ct_batch_set_graph", "ct_batch_set_profiling", "ct_batch_get_info",
 )
 
@@ -133,6 +133,7 @@ def lib() -> ctypes.CDLL:
         "ct_epoch_kernel_get_profile": ([vp, P(Profile)], ctypes.c_int),
         "ct_results_objective_f64": ([vp, vp, i32, P(CAlgorithm), i32, ctypes.c_int64,
                                       ctypes.c_double, P(ctypes.c_double)], ctypes.c_int),
+        "ct_epoch_kernel_set_share_propagation": ([vp, i32], ctypes.c_int),
         "ct_batch_new": ([vp, P(vp), i32, P(vp)], ctypes.c_int),
         "ct_batch_free": ([vp], None),
         "ct_batch_execute": ([vp], ctypes.c_int),
@@ -342,6 +343,12 @@ class EpochKernel:
         """0 = the reference's derivation, 1 = exact reverse-mode gradient."""
         _check(lib().ct_epoch_kernel_set_gradient(self.handle, int(mode), int(checkpoint_interval)),
                "set_gradient")
+
+    def set_share_propagation(self, enabled: bool) -> None:
+        """Propagate once per update batch (the beats' states are identical);
+        with the reverse-mode gradient, one backward pass per update batch."""
+        _check(lib().ct_epoch_kernel_set_share_propagation(self.handle, int(bool(enabled))),
+               "set_share_propagation")
 
     def set_freeze_delays(self, value: bool) -> None:
         _check(lib().ct_epoch_kernel_set_freeze_delays(self.handle, int(bool(value))),

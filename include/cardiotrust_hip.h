@@ -181,6 +181,17 @@ int ct_epoch_kernel_execute(ct_epoch_kernel *kernel);
 int ct_epoch_kernel_set_gradient(ct_epoch_kernel *kernel, int32_t mode,
                                  int32_t checkpoint_interval);
 
+/* Shared propagation (no reference counterpart; off by default). The beats
+ * of one update batch (all beats, or batch_size of them) use the same
+ * parameters and drive, and the propagation does not depend on the beat
+ * (prediction.rs:15-105), so their states are identical: with `enabled` the
+ * epoch propagates once per update batch instead of once per beat, with
+ * bit-identical results under gradient mode 0. Under gradient mode 1 the
+ * batch's summed objective sum_b J_b is differentiated with one backward pass
+ * (forcing summed over the beats), instead of one pass per beat: the same
+ * gradient up to FP32 rounding. */
+int ct_epoch_kernel_set_share_propagation(ct_epoch_kernel *kernel, int32_t enabled);
+
 /* Forward / backward pair of one beat (the counterpart of an autograd
  * Function over the epoch's parameters; no reference counterpart).
  * ct_results_forward runs the propagation, projection, residuals and per-step
