@@ -237,6 +237,23 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
 // registers), recomputing ybar and accumulating dJ/dG and dJ/da.
 constexpr int kGradThreads = 256;
 constexpr int kGradChunk = 16;
+
+// Sixteen consecutive floats of three rows `stride` apart, in 16-byte loads
+// that need only 4-byte alignment.
+typedef float float4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+__device__ __forceinline__ void load_rows16(const float *src, int64_t stride,
+                                            float out[3][kGradChunk]) {
+#pragma unroll
+    for (int o = 0; o < 3; ++o)
+#pragma unroll
+        for (int q = 0; q < kGradChunk / 4; ++q) {
+            const float4_a4 v = *reinterpret_cast<const float4_a4 *>(src + o * stride + 4 * q);
+            out[o][4 * q] = v.x;
+            out[o][4 * q + 1] = v.y;
+            out[o][4 * q + 2] = v.z;
+            out[o][4 * q + 3] = v.w;
+        }
+}
 constexpr int kGradMaxVoxels = kGradThreads / 26 + 2;
 
 __global__ __launch_bounds__(kGradThreads) void adjoint_gradient_kernel(
@@ -284,13 +301,29 @@ __global__ __launch_bounds__(kGradThreads) void adjoint_gradient_kernel(
                 ycur[o] = yc[o];
                 pin[o] = D <= t0 - 1 ? xw[int64_t(o) * T + t0 - 1 - D] : 0.0f;
             }
-            for (int32_t t = t0; t < t1; ++t) {
+            for (int32_t tc = t0; tc < t1; tc += kGradChunk) {
+                float xc[3][kGradChunk];
+                if (tc >= D && tc + kGradChunk <= t1) {
+                    load_rows16(xw + (tc - D), T, xc);
+                } else {
 #pragma unroll
-                for (int o = 0; o < 3; ++o) {
-                    const float in0 = D <= t ? xw[int64_t(o) * T + t - D] : 0.0f;
-                    ycur[o] = fmaf(a, in0 - ycur[o], pin[o]);
-                    pin[o] = in0;
-                    yseg[int64_t(t - t0) * M + it * 3 + o] = ycur[o];
+                    for (int o = 0; o < 3; ++o)
+#pragma unroll
+                        for (int j = 0; j < kGradChunk; ++j) {
+                            const int32_t t = tc + j;
+                            xc[o][j] = (t < t1 && D <= t) ? xw[int64_t(o) * T + t - D] : 0.0f;
+                        }
+                }
+#pragma unroll
+                for (int j = 0; j < kGradChunk; ++j) {
+                    const int32_t t = tc + j;
+                    if (t >= t1) break;
+#pragma unroll
+                    for (int o = 0; o < 3; ++o) {
+                        ycur[o] = fmaf(a, xc[o][j] - ycur[o], pin[o]);
+                        pin[o] = xc[o][j];
+                        yseg[int64_t(t - t0) * M + it * 3 + o] = ycur[o];
+                    }
                 }
             }
         }
@@ -304,13 +337,17 @@ __global__ __launch_bounds__(kGradThreads) void adjoint_gradient_kernel(
                 for (int q = 0; q < 3; ++q) xs[vv][q][j] = xbarT[row + int64_t(q) * T];
             }
             float xi[3][kGradChunk];
+            if (has && len == kGradChunk && cbeg >= D) {
+                load_rows16(xw + (cbeg - D), T, xi);
+            } else {
 #pragma unroll
-            for (int o = 0; o < 3; ++o)
+                for (int o = 0; o < 3; ++o)
 #pragma unroll
-                for (int j = 0; j < kGradChunk; ++j) {
-                    const int32_t src = cbeg + j - D;
-                    xi[o][j] = (has && j < len && src >= 0) ? xw[int64_t(o) * T + src] : 0.0f;
-                }
+                    for (int j = 0; j < kGradChunk; ++j) {
+                        const int32_t src = cbeg + j - D;
+                        xi[o][j] = (has && j < len && src >= 0) ? xw[int64_t(o) * T + src] : 0.0f;
+                    }
+            }
             __syncthreads();
             if (has) {
 #pragma unroll
