@@ -130,6 +130,8 @@ __global__ __launch_bounds__(kPropThreads) void propagate_step_kernel(
 // the host reports the failure instead of the kernel waiting forever.
 // ---------------------------------------------------------------------------
 
+typedef float float3_a4 __attribute__((ext_vector_type(3), aligned(4)));
+
 constexpr int kResThreads = 512;
 constexpr int kResItems = (kResMaxVoxels * 26 + kResThreads - 1) / kResThreads;
 constexpr int kResStates = (kResMaxVoxels * 3 + kResThreads - 1) / kResThreads;
@@ -207,9 +209,12 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
                 const uint32_t w = pk[k] & ((1u << kResNeighbourBits) - 1);
                 const int32_t D = int32_t(pk[k] >> kResNeighbourBits);
                 if (D <= t) {
+                    // the neighbour's three states: one 12-byte load
                     const uint32_t r0 = uint32_t(t - D) * uint32_t(n) + 3u * w;
-#pragma unroll
-                    for (int o = 0; o < 3; ++o) in0[q][o] = x[r0 + o];
+                    const float3_a4 v = *reinterpret_cast<const float3_a4 *>(x + r0);
+                    in0[q][0] = v.x;
+                    in0[q][1] = v.y;
+                    in0[q][2] = v.z;
                 }
             }
 #pragma unroll
