@@ -25,6 +25,8 @@
 namespace ct {
 namespace {
 
+typedef float float3_a4 __attribute__((ext_vector_type(3), aligned(4)));
+
 constexpr int kAdjVoxels = 64;
 constexpr int kAdjThreads = 256;
 
@@ -86,8 +88,10 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_step_kernel(
             const int32_t t = s + cons_delay[j];
             if (t < T) {
                 const float *q = qring + int64_t(uint32_t(t) & rmask) * M + int64_t(cons_idx[j]) * 3;
-#pragma unroll
-                for (int o = 0; o < 3; ++o) acc[o] += q[o];
+                const float3_a4 v = *reinterpret_cast<const float3_a4 *>(q);  // 12-byte load
+                acc[0] += v.x;
+                acc[1] += v.y;
+                acc[2] += v.z;
             }
         }
 #pragma unroll
@@ -115,31 +119,39 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_step_kernel(
     }
 }
 
-// Resident backward sweep: one launch runs steps T-1 .. 0 for all voxels. A
-// workgroup owns a voxel slab [vb, ve) (at most kResMaxVoxels voxels) for the
-// whole launch: the slab's ybar (78 per voxel) stays in LDS, its consumer lists
-// (consumer connection 22 bits | delay 10 bits) and coefficients in registers.
-// Per step the arithmetic is that of adjoint_step_kernel (same expressions and
-// summation orders, so the same bits); q goes through the ring in memory and
-// a barrier over the workgroups separates the steps.
+// Resident backward sweep: one launch runs steps T-1 .. 0 for all voxels,
+// with each connection's ybar recurrence owned by the workgroup of the voxel
+// that CONSUMES its q. Connection c = (v, d) reading w with delay D and
+// coefficient a contributes q_tau = a ybar_tau + ybar_{tau+1} to xbar_s[w] at
+// tau = s + D, and ybar_tau = G_c^T xbar_tau[v] - a ybar_{tau+1}. So at sweep
+// step s the lane owning (w, j) (c = the j-th consumer of w) reads
+// xbar_{s+D}[v] from the xbar history (written D steps earlier in the sweep),
+// advances its ybar (kept in registers) and leaves q in LDS; then one lane
+// per voxel sums its consumers' q in list order. There is no q ring: the
+// gathers hit the last Dmax rows of xbar, a small footprint, instead of a
+// ring of 78 values per voxel and step. The values and summation orders are
+// those of adjoint_step_kernel, so the results are the same bits.
+//
+// A workgroup owns voxels [vb, ve) (at most kResMaxVoxels). Registers per
+// item (w, j): ybar (3) and the producer voxel and delay (17 | 15 bits); per
+// phase-2 voxel lane its consumers' delays, three per word. The gains and
+// coefficient of each item come from gcons (N, 26, 10), consumer-ordered.
 constexpr int kSweepThreads = 512;
 constexpr int kSweepItems = (kResMaxVoxels * 26 + kSweepThreads - 1) / kSweepThreads;
-constexpr int kSweepConsumerBits = 22;
+constexpr int kSweepBatch = 4;
+constexpr int kConsGainStride = 10;  // gcons floats per (voxel, consumer) slot
 constexpr uint32_t kSweepNoEntry = 0xFFFFFFFFu;
 
 __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
-    const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
-    const float *__restrict__ gains_rows, const int32_t *__restrict__ cons_ptr,
-    const int32_t *__restrict__ cons_idx, const int32_t *__restrict__ cons_delay,
-    const float *__restrict__ x, const float *__restrict__ E, float *__restrict__ qring,
-    float *__restrict__ xbar, GridBarrier *__restrict__ gb, int32_t N, int32_t T, uint32_t rmask,
-    AdjointParams p, uint64_t wait_ticks) {
+    const int32_t *__restrict__ cons_ptr, const int32_t *__restrict__ cons_idx,
+    const int32_t *__restrict__ delays, const float *__restrict__ gcons,
+    const float *__restrict__ x, const float *__restrict__ E,
+    float *__restrict__ xbar, GridBarrier *__restrict__ gb, int32_t N, int32_t T, AdjointParams p,
+    uint64_t wait_ticks) {
     extern __shared__ float lds[];
-    float *yb_lds = lds;                                   // [voxel][78]
-    float *xb_lds = lds + kResMaxVoxels * 78;              // [voxel][3]
-    int *ok_lds = reinterpret_cast<int *>(xb_lds + kResMaxVoxels * 3);
+    float *q_lds = lds;  // [voxel][26][3]
+    int *ok_lds = reinterpret_cast<int *>(lds + kResMaxVoxels * 78);
     const int64_t n = 3LL * N;
-    const int64_t M = int64_t(N) * 78;  // one ring slot
     const int32_t slab = xcd_block(blockIdx.x, gridDim.x);
     const int32_t vb = int32_t(int64_t(slab) * N / gridDim.x);
     const int32_t ve = int32_t(int64_t(slab + 1) * N / gridDim.x);
@@ -147,87 +159,127 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
     const int tid = threadIdx.x;
     uint32_t gen = 0;
 
-    // consumers of the voxel this lane owns in phase A (lane < nv)
-    uint32_t cons[26];
+    // phase-2 lane (tid < nv): its voxel's consumer count and delays
+    uint32_t dj[9];
     int32_t ncons = 0;
 #pragma unroll
-    for (int j = 0; j < 26; ++j) cons[j] = kSweepNoEntry;
+    for (int q = 0; q < 9; ++q) dj[q] = 0u;
     if (tid < nv) {
-        const int32_t w = vb + tid;
-        const int32_t j0 = cons_ptr[w], j1 = cons_ptr[w + 1];
-        ncons = j1 - j0;
+        const int32_t j0 = cons_ptr[vb + tid];
+        ncons = cons_ptr[vb + tid + 1] - j0;
 #pragma unroll
         for (int j = 0; j < 26; ++j)
-            if (j < ncons)
-                cons[j] = uint32_t(cons_idx[j0 + j]) |
-                          (uint32_t(min(cons_delay[j0 + j], 1023)) << kSweepConsumerBits);
+            if (j < ncons) dj[j / 3] |= uint32_t(min(delays[cons_idx[j0 + j]], 1023)) << (10 * (j % 3));
     }
-    // coefficients of the connections this lane owns in phase B
-    float ak[kSweepItems];
-    uint32_t has = 0;
+    // phase-1 items (w, j): producer voxel, delay, coefficient, ybar state
+    uint32_t pk[kSweepItems];
+    float yb[kSweepItems][3];
 #pragma unroll
     for (int k = 0; k < kSweepItems; ++k) {
         const int it = tid + k * kSweepThreads;
-        ak[k] = 0.0f;
+        pk[k] = kSweepNoEntry;
+#pragma unroll
+        for (int o = 0; o < 3; ++o) yb[k][o] = 0.0f;
         if (it < items) {
-            const int64_t c = int64_t(vb) * 26 + it;
-            if (nbr[c] >= 0) {
-                has |= 1u << k;
-                ak[k] = coefs[c];
+            const int32_t w = vb + it / 26, j = it % 26;
+            const int32_t j0 = cons_ptr[w];
+            if (j < cons_ptr[w + 1] - j0) {
+                const int32_t c = cons_idx[j0 + j];
+                pk[k] = uint32_t(c / 26) | (uint32_t(min(delays[c], 32767)) << 17);
             }
         }
     }
-    for (int idx = tid; idx < nv * 78; idx += kSweepThreads) yb_lds[idx] = 0.0f;
     __syncthreads();
 
     for (int32_t s = T - 1; s >= 0; --s) {
-        // phase A: xbar_s of the slab's states
-        if (tid < nv) {
-            const int32_t w = vb + tid;
-            const int64_t row = int64_t(s) * n + 3LL * w;
-            const float xv[3] = {x[row], x[row + 1], x[row + 2]};
-            float acc[3];
+        // phase 1: each item's q for step s (its ybar at tau = s + D). The
+        // empty asm statement hides the lane index from the optimiser, so the
+        // per-item addresses are recomputed each step instead of being kept
+        // live across the time loop (which would not fit the register file).
+        int32_t lane = tid;
+        asm volatile("" : "+v"(lane));
 #pragma unroll
-            for (int o = 0; o < 3; ++o) acc[o] = step_factor(xv, o, E[row + o], p);
+        for (int k0 = 0; k0 < kSweepItems; k0 += kSweepBatch) {
+            float3_a4 xv[kSweepBatch], g[kSweepBatch][3];
+            float av[kSweepBatch];
+            bool on[kSweepBatch];
 #pragma unroll
-            for (int j = 0; j < 26; ++j) {
-                if (j >= ncons) break;
-                const uint32_t e = cons[j];
-                const int32_t t = s + int32_t(e >> kSweepConsumerBits);
-                if (t < T) {
-                    const float *q = qring + int64_t(uint32_t(t) & rmask) * M +
-                                     int64_t(e & ((1u << kSweepConsumerBits) - 1)) * 3;
+            for (int kk = 0; kk < kSweepBatch; ++kk) {
+                const int k = k0 + kk < kSweepItems ? k0 + kk : kSweepItems - 1;
+                const int it = lane + k * kSweepThreads;
+                const int32_t tau = s + int32_t(pk[k] >> 17);
+                on[kk] = k0 + kk < kSweepItems && pk[k] != kSweepNoEntry && tau < T;
+                // absent items read valid dummy addresses (row 0, gains of item 0)
+                const float *xs = on[kk] ? xbar + int64_t(tau) * n + 3LL * (pk[k] & 0x1FFFFu) : xbar;
+                const float *gs = gcons + (on[kk] ? (int64_t(vb) * 26 + it) * kConsGainStride : 0);
+                xv[kk] = *reinterpret_cast<const float3_a4 *>(xs);
 #pragma unroll
-                    for (int o = 0; o < 3; ++o) acc[o] += q[o];
-                }
+                for (int i = 0; i < 3; ++i) g[kk][i] = *reinterpret_cast<const float3_a4 *>(gs + 3 * i);
+                av[kk] = gs[9];
             }
 #pragma unroll
-            for (int o = 0; o < 3; ++o) {
-                xb_lds[tid * 3 + o] = acc[o];
-                xbar[row + o] = acc[o];
+            for (int kk = 0; kk < kSweepBatch; ++kk) {
+                const int k = k0 + kk;
+                if (!on[kk]) continue;
+                const int it = lane + k * kSweepThreads;
+#pragma unroll
+                for (int o = 0; o < 3; ++o) {
+                    const float next = yb[k][o];
+                    const float ny = g[kk][0][o] * xv[kk].x + g[kk][1][o] * xv[kk].y +
+                                     g[kk][2][o] * xv[kk].z - av[kk] * next;
+                    q_lds[it * 3 + o] = av[kk] * ny + next;
+                    yb[k][o] = ny;
+                }
             }
         }
         __syncthreads();
-        // phase B: ybar_s and q_s of the slab's connections
-        float *qs = qring + int64_t(uint32_t(s) & rmask) * M;
+        // phase 2: xbar_s of the slab's states, consumers in list order
+        if (tid < nv) {
+            const int32_t w = vb + tid;
+            const int64_t row = int64_t(s) * n + 3LL * w;
+            const float xs3[3] = {x[row], x[row + 1], x[row + 2]};
+            float acc[3];
 #pragma unroll
-        for (int k = 0; k < kSweepItems; ++k) {
-            if (!((has >> k) & 1u)) continue;
-            const int it = tid + k * kSweepThreads;
-            const int vv = it / 26, d = it - 26 * vv;
-            const int64_t c = int64_t(vb) * 26 + it;
-            const float *g = gains_rows + 3LL * (vb + vv) * 78 + 3 * d;
-            const float x0 = xb_lds[3 * vv], x1 = xb_lds[3 * vv + 1], x2 = xb_lds[3 * vv + 2];
+            for (int o = 0; o < 3; ++o) acc[o] = step_factor(xs3, o, E[row + o], p);
 #pragma unroll
-            for (int oo = 0; oo < 3; ++oo) {
-                const float next = yb_lds[it * 3 + oo];
-                const float yb = g[oo] * x0 + g[78 + oo] * x1 + g[156 + oo] * x2 - ak[k] * next;
-                qs[c * 3 + oo] = ak[k] * yb + next;
-                yb_lds[it * 3 + oo] = yb;
+            for (int j = 0; j < 26; ++j) {
+                const int32_t D = int32_t((dj[j / 3] >> (10 * (j % 3))) & 1023u);
+                if (j < ncons && s + D < T) {
+#pragma unroll
+                    for (int o = 0; o < 3; ++o) acc[o] += q_lds[(tid * 26 + j) * 3 + o];
+                }
             }
+#pragma unroll
+            for (int o = 0; o < 3; ++o) xbar[row + o] = acc[o];
         }
         if (!team_barrier(gb, gridDim.x, blockIdx.x, gen, ok_lds, wait_ticks)) return;
     }
+}
+
+// gcons[w][j] = (G[3v+in][3d+o] for in, o; a) of the j-th consumer
+// c = (v, d) of w: 10 floats per slot.
+__global__ void consumer_gains_kernel(const int32_t *__restrict__ cons_ptr,
+                                      const int32_t *__restrict__ cons_idx,
+                                      const float *__restrict__ gains_rows,
+                                      const float *__restrict__ coefs, float *__restrict__ gcons,
+                                      int32_t N) {
+    const int64_t slot = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (slot >= int64_t(N) * 26) return;
+    const int32_t w = int32_t(slot / 26), j = int32_t(slot % 26);
+    const int32_t j0 = cons_ptr[w];
+    float *dst = gcons + slot * kConsGainStride;
+    if (j >= cons_ptr[w + 1] - j0) {
+#pragma unroll
+        for (int e = 0; e < kConsGainStride; ++e) dst[e] = 0.0f;
+        return;
+    }
+    const int32_t c = cons_idx[j0 + j];
+    const int64_t v = c / 26, d = c % 26;
+#pragma unroll
+    for (int in = 0; in < 3; ++in)
+#pragma unroll
+        for (int o = 0; o < 3; ++o) dst[3 * in + o] = gains_rows[(3 * v + in) * 78 + 3 * d + o];
+    dst[9] = coefs[c];
 }
 
 // Gradient pass: one lane per connection. For each K-step segment, from the
@@ -411,7 +463,7 @@ hipError_t launch_adjoint_step(hipStream_t s, const int32_t *nbr, const float *c
     return hipGetLastError();
 }
 
-size_t sweep_lds_bytes() { return sizeof(float) * kResMaxVoxels * 81 + 16; }
+size_t sweep_lds_bytes() { return sizeof(float) * kResMaxVoxels * 78 + 16; }
 
 hipError_t sweep_capacity(int32_t *blocks_per_cu) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&adjoint_sweep_resident_kernel),
@@ -426,19 +478,27 @@ hipError_t sweep_capacity(int32_t *blocks_per_cu) {
     return e;
 }
 
-hipError_t launch_adjoint_sweep_resident(hipStream_t s, int32_t blocks, const int32_t *nbr,
-                                         const float *coefs, const float *gains_rows,
-                                         const int32_t *cons_ptr, const int32_t *cons_idx,
-                                         const int32_t *cons_delay, const float *x,
-                                         const float *E, float *qring, float *xbar,
-                                         GridBarrier *gb, int32_t N, int32_t T, uint32_t rmask,
+hipError_t launch_consumer_gains(hipStream_t s, const int32_t *cons_ptr, const int32_t *cons_idx,
+                                 const float *gains_rows, const float *coefs, float *gcons,
+                                 int32_t N) {
+    if (N <= 0) return hipSuccess;
+    const int64_t slots = int64_t(N) * 26;
+    consumer_gains_kernel<<<unsigned((slots + 255) / 256), 256, 0, s>>>(cons_ptr, cons_idx,
+                                                                        gains_rows, coefs, gcons, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_sweep_resident(hipStream_t s, int32_t blocks, const int32_t *cons_ptr,
+                                         const int32_t *cons_idx, const int32_t *delays,
+                                         const float *gcons,
+                                         const float *x, const float *E, float *xbar,
+                                         GridBarrier *gb, int32_t N, int32_t T,
                                          const AdjointParams &p, uint64_t wait_ticks) {
     if (N <= 0 || T <= 0) return hipSuccess;
     hipError_t e = launch_barrier_reset(s, gb, 1);
     if (e != hipSuccess) return e;
     adjoint_sweep_resident_kernel<<<blocks, kSweepThreads, sweep_lds_bytes(), s>>>(
-        nbr, coefs, gains_rows, cons_ptr, cons_idx, cons_delay, x, E, qring, xbar, gb, N, T, rmask,
-        p, wait_ticks);
+        cons_ptr, cons_idx, delays, gcons, x, E, xbar, gb, N, T, p, wait_ticks);
     return hipGetLastError();
 }
 

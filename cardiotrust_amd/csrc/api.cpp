@@ -447,6 +447,13 @@ int build_consumers(ct_results_gpu &r) {
 
 // Reverse-mode gradient of the forward pass held on the device (adjoint.hip);
 // adds dJ/dG and dJ/da of this beat to the derivative accumulators.
+// The one-launch backward sweep runs when the resident propagation does and
+// its tables fit: delays below 1024 (ring of `slots` >= Dmax + 1 entries)
+// and at most 26 consumers per voxel.
+static bool resident_sweep(const ct_results_gpu &r, int64_t slots) {
+    return r.sweep_ok && r.propagation_mode != 1 && slots <= 1024 && r.max_consumers <= 26;
+}
+
 // Buffers and q-ring size of the reverse-mode gradient (checkpoints every K).
 static int adjoint_prepare(ct_results_gpu &r, int32_t K, int64_t &slots) {
     if (K < 1) return fail(CT_ERR_INVALID, "checkpoint interval must be >= 1");
@@ -469,9 +476,13 @@ static int adjoint_prepare(ct_results_gpu &r, int32_t K, int64_t &slots) {
         slots = 1;
         while (slots < int64_t(dmax) + 1) slots <<= 1;
     }
-    int64_t have_q = r.qring_slots * M;
-    CT_CHECK(ensure(r, &r.qring, have_q, slots * M, 1));
-    r.qring_slots = std::max(r.qring_slots, slots);
+    if (resident_sweep(r, slots)) {
+        CT_CHECK(ensure(r, &r.gcons, r.gcons_have, int64_t(r.N) * 26 * 10, 1));
+    } else {
+        int64_t have_q = r.qring_slots * M;
+        CT_CHECK(ensure(r, &r.qring, have_q, slots * M, 1));
+        r.qring_slots = std::max(r.qring_slots, slots);
+    }
     int64_t have_y = r.yseg_steps * M;
     CT_CHECK(ensure(r, &r.yseg, have_y, int64_t(K) * M, 1));
     r.yseg_steps = std::max(r.yseg_steps, int64_t(K));
@@ -511,16 +522,17 @@ static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K,
     p.threshold = c.maximum_regularization_threshold;
     {
         Span sp(k, kAdjointSweep);
-        CT_HIP(ct::launch_consumer_delays(s, r.cons_idx, r.delays, r.cons_delay, r.cons_len));
-        if (r.sweep_ok && r.propagation_mode != 1 && slots <= 1024 && r.max_consumers <= 26) {
-            // all steps in one launch (delays < 1024 and at most 26 consumers
-            // per voxel fit its register tables)
-            CT_HIP(ct::launch_adjoint_sweep_resident(
-                s, r.resident_blocks, r.nbr, r.coefs, r.staging, r.cons_ptr, r.cons_idx,
-                r.cons_delay, r.x, r.E, r.qring, r.xbar, r.barrier, r.N, r.T,
-                uint32_t(slots - 1), p, wait_ticks(*r.gpu, 2.0)));
+        if (resident_sweep(r, slots)) {
+            // all steps in one launch, recurrences owned by the consuming voxels
+            CT_HIP(ct::launch_consumer_gains(s, r.cons_ptr, r.cons_idx, r.staging, r.coefs,
+                                             r.gcons, r.N));
+            CT_HIP(ct::launch_adjoint_sweep_resident(s, r.resident_blocks, r.cons_ptr, r.cons_idx,
+                                                     r.delays, r.gcons, r.x, r.E, r.xbar,
+                                                     r.barrier, r.N, r.T, p,
+                                                     wait_ticks(*r.gpu, 2.0)));
             r.resident_used = true;
         } else {
+            CT_HIP(ct::launch_consumer_delays(s, r.cons_idx, r.delays, r.cons_delay, r.cons_len));
             CT_HIP(ct::launch_fill(s, r.ybar, 0.0f, M));
             for (int32_t step = r.T - 1; step >= 0; --step)
                 CT_HIP(ct::launch_adjoint_step(s, r.nbr, r.coefs, r.staging, r.cons_ptr,
