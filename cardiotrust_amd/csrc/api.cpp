@@ -676,7 +676,7 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
     const int64_t n = r->n, T = r->T, S = r->S, Sp = r->Sp, B = r->B;
 
     // GEMM split-K plan: about 2048 workgroups for the projection.
-    const int64_t tiles = (T + 63) / 64 * (Sp / 128);
+    const int64_t tiles = (T + ct::kProjTile - 1) / ct::kProjTile * (Sp / 128);
     int64_t splits = std::max<int64_t>(1, std::min<int64_t>((n + 31) / 32, 2048 / std::max<int64_t>(tiles, 1)));
     int64_t kps = ((n + splits - 1) / splits + ct::kGemmTileK - 1) / ct::kGemmTileK * ct::kGemmTileK;
     kps = std::max<int64_t>(kps, ct::kGemmTileK);

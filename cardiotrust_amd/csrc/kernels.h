@@ -14,6 +14,7 @@
 namespace ct {
 
 constexpr int kGemmTileK = 32;
+constexpr int kProjTile = 128;  // steps per project workgroup (H·Xᵀ)
 
 struct DeriveParams {
     float mse_scaling;

@@ -15,6 +15,8 @@ namespace ct {
 namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float float4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+constexpr int kMapTile = 128;  // steps per map_residuals workgroup
 
 // ---------------------------------------------------------------------------
 // Propagation: one launch per time step (prediction.rs:40-105).
@@ -314,54 +316,75 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
                                                       const float *__restrict__ H,
                                                       float *__restrict__ partial, int32_t T,
                                                       int64_t n, int32_t Sp, int64_t kps) {
-    __shared__ float As[64][kGemmTileK + 1];
-    __shared__ float Bs[128][kGemmTileK + 1];
+    // k-major tiles (rows padded by 4 floats): the MFMA operand reads of a
+    // wave are 32 consecutive floats, free of bank conflicts
+    __shared__ float As[kGemmTileK][kProjTile + 4];  // X[t][k], kProjTile steps
+    __shared__ float Bs[kGemmTileK][128 + 4];        // H[s][k], 128 sensors
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int32_t t0 = blockIdx.x * 64;
+    const int32_t t0 = blockIdx.x * kProjTile;
     const int32_t split = blockIdx.y;
     const int32_t s0 = blockIdx.z * 128;
     const int64_t k0 = int64_t(split) * kps;
     const int64_t k1 = min(n, k0 + kps);
-    floatx16 acc0 = {0}, acc1 = {0};
+    floatx16 acc[kProjTile / 32];
+#pragma unroll
+    for (int q = 0; q < kProjTile / 32; ++q) acc[q] = floatx16{0};
+    // loader: 8 lanes cover one row's 32 k values (one 128-byte line)
+    const int lrow = tid >> 3, lk = (tid & 7) * 4;
     for (int64_t kb = k0; kb < k1; kb += kGemmTileK) {
-        {
-            const int row = tid >> 2, kq = (tid & 3) * 8;
+        const bool full = kb + kGemmTileK <= k1;
+#pragma unroll
+        for (int pass = 0; pass < kProjTile / 32; ++pass) {
+            const int row = pass * 32 + lrow;
             const int32_t t = t0 + row;
+            float4_a4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (t < T) {
+                const float *src = x + int64_t(t) * n + kb + lk;
+                if (full) {
+                    v = *reinterpret_cast<const float4_a4 *>(src);
+                } else {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int64_t k = kb + kq + j;
-                As[row][kq + j] = (t < T && k < k1) ? x[int64_t(t) * n + k] : 0.0f;
+                    for (int j = 0; j < 4; ++j) v[j] = (kb + lk + j < k1) ? src[j] : 0.0f;
+                }
             }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) As[lk + j][row] = v[j];
         }
-        {
-            const int row = tid >> 1, kq = (tid & 1) * 16;
-            const float *hr = H + int64_t(s0 + row) * n;
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const int64_t k = kb + kq + j;
-                Bs[row][kq + j] = (k < k1) ? hr[k] : 0.0f;
+        for (int pass = 0; pass < 4; ++pass) {
+            const int row = pass * 32 + lrow;
+            const float *src = H + int64_t(s0 + row) * n + kb + lk;
+            float4_a4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (full) {
+                v = *reinterpret_cast<const float4_a4 *>(src);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) v[j] = (kb + lk + j < k1) ? src[j] : 0.0f;
             }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) Bs[lk + j][row] = v[j];
         }
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < kGemmTileK; kk += 2) {
             const int kr = kk + (lane >> 5);
-            const float a0 = As[lane & 31][kr];
-            const float a1 = As[32 + (lane & 31)][kr];
-            const float bv = Bs[wave * 32 + (lane & 31)][kr];
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, bv, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bv, acc1, 0, 0, 0);
+            const float bv = Bs[kr][wave * 32 + (lane & 31)];
+#pragma unroll
+            for (int q = 0; q < kProjTile / 32; ++q) {
+                const float a = As[kr][q * 32 + (lane & 31)];
+                acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv, acc[q], 0, 0, 0);
+            }
         }
         __syncthreads();
     }
     const int32_t s = s0 + wave * 32 + (lane & 31);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int row = (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-        const int32_t ta = t0 + row, tb = t0 + 32 + row;
-        if (ta < T) partial[(int64_t(split) * T + ta) * Sp + s] = acc0[r];
-        if (tb < T) partial[(int64_t(split) * T + tb) * Sp + s] = acc1[r];
-    }
+    for (int q = 0; q < kProjTile / 32; ++q)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int32_t t = t0 + q * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+            if (t < T) partial[(int64_t(split) * T + t) * Sp + s] = acc[q][r];
+        }
 }
 
 // Residuals and per-step MSE (estimation.rs:146-152, metrics.rs:189-190).
@@ -505,51 +528,68 @@ __global__ __launch_bounds__(256) void map_residuals_kernel(const float *__restr
                                                             const float *__restrict__ R,
                                                             float *__restrict__ eT, int64_t n,
                                                             int32_t T, int32_t Sp) {
-    __shared__ float Hs[kGemmTileK][128 + 4];
-    __shared__ float Rs[64][kGemmTileK + 1];
+    // tile: 128 states x kMapTile steps; k (sensors) in chunks of 32, k-major
+    // LDS tiles (rows padded by 4 floats)
+    __shared__ float Hs[kGemmTileK][128 + 4];       // H[s][i]
+    __shared__ float Rs[kGemmTileK][kMapTile + 4];  // R[t][s]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t i0 = int64_t(blockIdx.x) * 128;
-    const int32_t t0 = blockIdx.y * 64;
-    floatx16 acc0 = {0}, acc1 = {0};
-    for (int32_t sb = 0; sb < Sp; sb += kGemmTileK) {
-        {
-            const int row = tid >> 3, cq = (tid & 7) * 16;
-            const float *hr = H + int64_t(sb + row) * n;
+    const int32_t t0 = blockIdx.y * kMapTile;
+    floatx16 acc[kMapTile / 32];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const int64_t i = i0 + cq + j;
-                Hs[row][cq + j] = (i < n) ? hr[i] : 0.0f;
+    for (int q = 0; q < kMapTile / 32; ++q) acc[q] = floatx16{0};
+    const bool full_i = i0 + 128 <= n;
+    for (int32_t sb = 0; sb < Sp; sb += kGemmTileK) {
+        {   // H rows sb..sb+31 over states i0..i0+127: 32 lanes cover one row
+            const int row = tid >> 5, cq = (tid & 31) * 4;
+#pragma unroll
+            for (int pass = 0; pass < 4; ++pass) {
+                const int rr = pass * 8 + row;
+                const float *src = H + int64_t(sb + rr) * n + i0 + cq;
+                float4_a4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+                if (full_i) {
+                    v = *reinterpret_cast<const float4_a4 *>(src);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) v[j] = (i0 + cq + j < n) ? src[j] : 0.0f;
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) Hs[rr][cq + j] = v[j];
             }
         }
-        {
-            const int row = tid >> 2, kq = (tid & 3) * 8;
-            const int32_t t = t0 + row;
+        {   // R rows t0.., sensors sb..sb+31: 8 lanes per row
+            const int row = tid >> 3, kq = (tid & 7) * 4;
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-                Rs[row][kq + j] = (t < T) ? R[int64_t(t) * Sp + sb + kq + j] : 0.0f;
+            for (int pass = 0; pass < kMapTile / 32; ++pass) {
+                const int rr = pass * 32 + row;
+                const int32_t t = t0 + rr;
+                float4_a4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+                if (t < T) v = *reinterpret_cast<const float4_a4 *>(R + int64_t(t) * Sp + sb + kq);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) Rs[kq + j][rr] = v[j];
+            }
         }
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < kGemmTileK; kk += 2) {
             const int kr = kk + (lane >> 5);
             const float a = Hs[kr][wave * 32 + (lane & 31)];
-            const float b0 = Rs[lane & 31][kr];
-            const float b1 = Rs[32 + (lane & 31)][kr];
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1, acc1, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < kMapTile / 32; ++q) {
+                const float b = Rs[kr][q * 32 + (lane & 31)];
+                acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[q], 0, 0, 0);
+            }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int row = (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-        const int64_t i = i0 + wave * 32 + row;
-        const int32_t ta = t0 + (lane & 31), tb = t0 + 32 + (lane & 31);
-        if (i < n) {
-            if (ta < T) eT[i * T + ta] = acc0[r];
-            if (tb < T) eT[i * T + tb] = acc1[r];
+    for (int q = 0; q < kMapTile / 32; ++q)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int64_t i = i0 + wave * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+            const int32_t t = t0 + q * 32 + (lane & 31);
+            if (i < n && t < T) eT[i * T + t] = acc[q][r];
         }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -992,7 +1032,7 @@ hipError_t launch_propagate_batch(hipStream_t s, int32_t slots, const ResidentJo
 
 hipError_t launch_project(hipStream_t s, const float *x, const float *H, float *partial,
                           int32_t T, int64_t n, int32_t Sp, int32_t splits, int64_t kps) {
-    dim3 grid(blocks_for(T, 64), splits, Sp / 128);
+    dim3 grid(blocks_for(T, kProjTile), splits, Sp / 128);
     project_kernel<<<grid, 256, 0, s>>>(x, H, partial, T, n, Sp, kps);
     return hipGetLastError();
 }
@@ -1034,7 +1074,7 @@ hipError_t launch_transpose(hipStream_t s, const float *in, float *out, int64_t 
 hipError_t launch_map_residuals(hipStream_t s, const float *H, const float *R, float *eT,
                                 int64_t n, int32_t T, int32_t Sp) {
     if (n <= 0) return hipSuccess;
-    dim3 grid(blocks_for(n, 128), blocks_for(T, 64));
+    dim3 grid(blocks_for(n, 128), blocks_for(T, kMapTile));
     map_residuals_kernel<<<grid, 256, 0, s>>>(H, R, eT, n, T, Sp);
     return hipGetLastError();
 }
