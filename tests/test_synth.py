@@ -1,0 +1,96 @@
+"""Seeded synthetic workloads (no GPU): exact sizes, the reference's
+connectivity form, the grid sensor motion of sensors.rs:194-225 and the slow
+sub-regions of the batched adjoint workload."""
+import numpy as np
+import pytest
+
+from cardiotrust_amd import synth
+from oracle import oracle
+
+
+@pytest.mark.parametrize("n", [1, 7, 300, 1234])
+def test_blob_has_exactly_the_requested_voxels(n):
+    g = synth.blob(11, n)
+    assert g.num_voxels == n
+    # neighbours are mutual: w is a neighbour of v in direction d iff v is
+    # one of w's in the opposite direction (index 25 - d)
+    nb = g.neighbours
+    for v in range(min(n, 200)):
+        for d in range(26):
+            w = nb[v, d]
+            if w >= 0:
+                assert nb[w, 25 - d] == v
+
+
+def test_blob_is_seeded():
+    a, b = synth.blob(3, 500), synth.blob(3, 500)
+    assert np.array_equal(a.coords, b.coords)
+    assert not np.array_equal(a.coords, synth.blob(4, 500).coords)
+
+
+def test_grid_motion_offsets_follow_the_reference_loop_order():
+    off = synth.grid_motion_offsets((2, 3, 2), (10.0, 20.0, 5.0))
+    assert off.shape == (12, 3)
+    want = [[10.0 * x, 10.0 * y, 5.0 * z] for x in range(2) for y in range(3) for z in range(2)]
+    assert np.allclose(off, want)
+    assert np.allclose(synth.grid_motion_offsets((4, 1, 1), (30.0, 0.0, 0.0))[:, 0],
+                       [0.0, 10.0, 20.0, 30.0])
+
+
+def test_beat_offsets_move_the_sensor_array():
+    g = synth.cube(3)
+    off = np.array([[0.0, 0.0, 0.0], [12.5, -3.0, 4.0]])
+    H = synth.measurement_matrix(g, synth.sensor_grid(g, num_beats=2, beat_offsets_mm=off))
+    s = synth.sensor_grid(g)
+    s.positions_mm = s.positions_mm + off[1]
+    H1 = synth.measurement_matrix(g, s)
+    assert np.array_equal(H[1], H1[0])
+    assert not np.array_equal(H[0], H[1])
+
+
+def test_slow_region_doubles_the_true_delays_only():
+    g = synth.blob(5, 400)
+    region = synth.sphere_region(g, 5, 0.3)
+    assert 0 < region.sum() < g.num_voxels
+    p = synth.make_problem(g, num_steps=20, max_delay=10, seed=5, slow_region=region)
+    q = synth.make_problem(g, num_steps=20, max_delay=10, seed=5)
+    conn = g.neighbours >= 0
+    inside = conn & region[:, None]
+    assert np.array_equal(p.true_delays[inside], 2 * q.true_delays[inside])
+    assert np.array_equal(p.true_delays[conn & ~region[:, None]],
+                          q.true_delays[conn & ~region[:, None]])
+    assert np.array_equal(p.delays, q.delays)  # the estimate starts undoubled
+
+
+def test_uneven_workload_sizes():
+    ps = synth.config3(count=6, max_voxels=2000, min_steps=40, max_steps=80, max_beats=3)
+    for p in ps:
+        assert 1000 <= p.num_voxels <= 2000
+        assert 40 <= p.num_steps <= 80
+        assert 1 <= p.num_beats <= 3
+        assert p.measurement_matrix.shape == (p.num_beats, 108, 3 * p.num_voxels)
+        assert p.num_steps == p.control_function.shape[0]
+
+
+def test_permuted_numbering_gives_the_same_measurements():
+    # the epoch does not depend on how voxels are numbered: renumbering every
+    # array consistently leaves the oracle's measurements unchanged (up to
+    # the order of the projection sums)
+    g = synth.blob(8, 150)
+    p = synth.make_problem(g, num_steps=30, max_delay=6, seed=8)
+    N = p.num_voxels
+    perm = np.random.default_rng(0).permutation(N)  # perm[v] = new index of v
+    inv = np.empty(N, np.int64)
+    inv[perm] = np.arange(N)
+    rows3 = (3 * inv[:, None] + np.arange(3)[None, :]).reshape(-1)
+    osi = p.output_state_indices[rows3]
+    osi = np.where(osi >= 0, 3 * perm[np.maximum(osi, 0) // 3] + osi % 3, -1).astype(np.int32)
+    q = synth.Problem(p.geometry, p.num_sensors, p.num_steps, p.num_beats,
+                      np.ascontiguousarray(p.gains[rows3]), np.ascontiguousarray(osi),
+                      np.ascontiguousarray(p.coefs[inv]), np.ascontiguousarray(p.delays[inv]),
+                      np.ascontiguousarray(p.initial_delays[inv]),
+                      np.ascontiguousarray(p.control_matrix[rows3]), p.control_function,
+                      np.ascontiguousarray(p.measurement_matrix[:, :, rows3]))
+    a = oracle.OracleResults(p, np.float64).simulate()
+    b = oracle.OracleResults(q, np.float64).simulate()
+    assert np.allclose(a, b, rtol=1e-12, atol=1e-18)
