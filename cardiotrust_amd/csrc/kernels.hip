@@ -137,7 +137,12 @@ typedef float float3_a4 __attribute__((ext_vector_type(3), aligned(4)));
 constexpr int kResThreads = 512;
 constexpr int kResItems = (kResMaxVoxels * 26 + kResThreads - 1) / kResThreads;
 constexpr int kResStates = (kResMaxVoxels * 3 + kResThreads - 1) / kResThreads;
-constexpr int kResBatch = 9;
+// Gathers issued per group in phase 1, and connections per gains group in
+// phase 2 (26 = 9 + 9 + 8): measured on the 120k-voxel shell, one gather at
+// a time and gains in thirds (83.7 ms per 2000 steps) beat larger groups
+// (91.7 ms with 9 gathers and halves), which hold more registers.
+constexpr int kResBatch = 1;
+constexpr int kResGainConns = 9;
 constexpr uint32_t kNoConnection = 0xFFFFFFFFu;
 constexpr int kResNeighbourBits = 17;  // voxel index field; delay in the upper 15 bits
 
@@ -244,13 +249,16 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
             const uint32_t m = mask[vl];
             float acc = 0.0f;
 #pragma unroll 1
-            for (int h = 0; h < 2; ++h) {  // gains in two halves of 13 connections
-                float g[39];
+            for (int h = 0; h < (26 + kResGainConns - 1) / kResGainConns; ++h) {
+                float g[3 * kResGainConns];
 #pragma unroll
-                for (int q = 0; q < 39; ++q) g[q] = gT[uint32_t(39 * h + q) * uint32_t(n) + uint32_t(i)];
+                for (int q = 0; q < 3 * kResGainConns; ++q)
+                    if (3 * kResGainConns * h + q < 78)
+                        g[q] = gT[uint32_t(3 * kResGainConns * h + q) * uint32_t(n) + uint32_t(i)];
 #pragma unroll
-                for (int dd = 0; dd < 13; ++dd) {
-                    const int d = 13 * h + dd;
+                for (int dd = 0; dd < kResGainConns; ++dd) {
+                    const int d = kResGainConns * h + dd;
+                    if (d >= 26) break;
                     if (!((m >> d) & 1u)) continue;
 #pragma unroll
                     for (int o = 0; o < 3; ++o)
