@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_step_kernel(
 // coefficient of each item come from gcons (N, 26, 10), consumer-ordered.
 constexpr int kSweepThreads = 512;
 constexpr int kSweepItems = (kResMaxVoxels * 26 + kSweepThreads - 1) / kSweepThreads;
-constexpr int kSweepBatch = 4;
+constexpr int kSweepBatch = 1;  // items per load group: 1 measured fastest (102 ms vs 112 at 4)
 constexpr int kConsGainStride = 10;  // gcons floats per (voxel, consumer) slot
 constexpr uint32_t kSweepNoEntry = 0xFFFFFFFFu;
 
@@ -283,10 +283,12 @@ __global__ void consumer_gains_kernel(const int32_t *__restrict__ cons_ptr,
 }
 
 // Gradient pass: one lane per connection. For each K-step segment, from the
-// last to the first: recompute the filter outputs from the checkpoint the
-// forward pass stored (y_{t0-1}), then walk the segment backwards in chunks of
-// kGradChunk steps (own-voxel xbar staged in LDS, the neighbour's states in
-// registers), recomputing ybar and accumulating dJ/dG and dJ/da.
+// last to the first: run the filter forward from the checkpoint the forward
+// pass stored (y_{t0-1}), keeping only its state at every kGradChunk-step chunk
+// start; then, chunk by chunk from the last, recompute the chunk's filter
+// outputs into registers (from the same neighbour rows the walk reads) and
+// walk it backwards (own-voxel xbar staged in LDS), recomputing ybar and
+// accumulating dJ/dG and dJ/da.
 constexpr int kGradThreads = 256;
 constexpr int kGradChunk = 16;
 
@@ -312,7 +314,7 @@ __global__ __launch_bounds__(kGradThreads) void adjoint_gradient_kernel(
     const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
     const int32_t *__restrict__ delays, const float *__restrict__ gT,
     const float *__restrict__ xT, const float *__restrict__ xbarT, const float *__restrict__ ckpt,
-    float *__restrict__ yseg, float *__restrict__ dGT, float *__restrict__ dA, int32_t N,
+    float *__restrict__ ychunk, float *__restrict__ dGT, float *__restrict__ dA, int32_t N,
     int32_t T, int32_t K) {
     __shared__ float xs[kGradMaxVoxels][3][kGradChunk];
     const int64_t total = int64_t(N) * 26;
@@ -332,6 +334,9 @@ __global__ __launch_bounds__(kGradThreads) void adjoint_gradient_kernel(
     const int32_t D = delays[it];
     const float *xw = xT + (has ? 3LL * w : 0) * T;
     const int32_t nseg = (T + K - 1) / K;
+    // per segment, the filter state (y, x[t-D-1]) at the start of each chunk
+    float *ys = ychunk + it * 6;
+    const int64_t ystride = total * 6;
 
     float G[3][3], g[3][3];
 #pragma unroll
@@ -344,44 +349,47 @@ __global__ __launch_bounds__(kGradThreads) void adjoint_gradient_kernel(
     float A = 0.0f, ybn[3] = {0.0f, 0.0f, 0.0f};
     for (int32_t k = nseg - 1; k >= 0; --k) {
         const int32_t t0 = k * K, t1 = min(T, t0 + K);
-        float yc[3] = {0.0f, 0.0f, 0.0f}, ycur[3] = {0.0f, 0.0f, 0.0f};
+        const int32_t nchunks = (t1 - t0 + kGradChunk - 1) / kGradChunk;
+        // pass 1: from the segment's checkpoint forward, keeping only the
+        // state at each chunk start
         if (has) {
-            float pin[3];
+            float ycur[3], pin[3];
 #pragma unroll
             for (int o = 0; o < 3; ++o) {
-                yc[o] = k == 0 ? 0.0f : ckpt[int64_t(k - 1) * M + it * 3 + o];
-                ycur[o] = yc[o];
+                ycur[o] = k == 0 ? 0.0f : ckpt[int64_t(k - 1) * M + it * 3 + o];
                 pin[o] = D <= t0 - 1 ? xw[int64_t(o) * T + t0 - 1 - D] : 0.0f;
             }
-            for (int32_t tc = t0; tc < t1; tc += kGradChunk) {
+            for (int32_t c = 0; c < nchunks; ++c) {
+                const int32_t tc = t0 + c * kGradChunk;
+#pragma unroll
+                for (int o = 0; o < 3; ++o) {
+                    ys[c * ystride + o] = ycur[o];
+                    ys[c * ystride + 3 + o] = pin[o];
+                }
+                if (c == nchunks - 1) break;  // the last chunk's steps are not needed here
                 float xc[3][kGradChunk];
-                if (tc >= D && tc + kGradChunk <= t1) {
-                    load_rows16(xw + (tc - D), T, xc);
+                if (tc >= D) {
+                    load_rows16(xw + (tc - D), T, xc);  // a full chunk: tc + 16 <= t1
                 } else {
 #pragma unroll
                     for (int o = 0; o < 3; ++o)
 #pragma unroll
-                        for (int j = 0; j < kGradChunk; ++j) {
-                            const int32_t t = tc + j;
-                            xc[o][j] = (t < t1 && D <= t) ? xw[int64_t(o) * T + t - D] : 0.0f;
-                        }
+                        for (int j = 0; j < kGradChunk; ++j)
+                            xc[o][j] = D <= tc + j ? xw[int64_t(o) * T + tc + j - D] : 0.0f;
                 }
 #pragma unroll
-                for (int j = 0; j < kGradChunk; ++j) {
-                    const int32_t t = tc + j;
-                    if (t >= t1) break;
+                for (int j = 0; j < kGradChunk; ++j)
 #pragma unroll
                     for (int o = 0; o < 3; ++o) {
                         ycur[o] = fmaf(a, xc[o][j] - ycur[o], pin[o]);
                         pin[o] = xc[o][j];
-                        yseg[int64_t(t - t0) * M + it * 3 + o] = ycur[o];
                     }
-                }
             }
         }
-        // ycur = y_{t1-1}; walk back
-        for (int32_t cend = t1; cend > t0; cend -= kGradChunk) {
-            const int32_t cbeg = max(t0, cend - kGradChunk), len = cend - cbeg;
+        // pass 2: chunks from the last to the first; each recomputes its
+        // filter outputs into registers from its start state and walks back
+        for (int32_t c = nchunks - 1; c >= 0; --c) {
+            const int32_t cbeg = t0 + c * kGradChunk, len = min(kGradChunk, t1 - cbeg);
             for (int idx = threadIdx.x; idx < nvox * len; idx += kGradThreads) {
                 const int vv = idx / len, j = idx - vv * len;
                 const int64_t row = 3LL * (vbeg + vv) * T + cbeg + j;
@@ -400,24 +408,39 @@ __global__ __launch_bounds__(kGradThreads) void adjoint_gradient_kernel(
                         xi[o][j] = (has && j < len && src >= 0) ? xw[int64_t(o) * T + src] : 0.0f;
                     }
             }
+            float y0[3], yv[3][kGradChunk];
+            if (has) {
+                float pin[3];
+#pragma unroll
+                for (int o = 0; o < 3; ++o) {
+                    y0[o] = ys[c * ystride + o];
+                    pin[o] = ys[c * ystride + 3 + o];
+                }
+#pragma unroll
+                for (int o = 0; o < 3; ++o) {
+                    float yc = y0[o];
+#pragma unroll
+                    for (int j = 0; j < kGradChunk; ++j) {
+                        yc = fmaf(a, xi[o][j] - yc, j == 0 ? pin[o] : xi[o][j - 1]);
+                        yv[o][j] = yc;
+                    }
+                }
+            }
             __syncthreads();
             if (has) {
 #pragma unroll
                 for (int j = kGradChunk - 1; j >= 0; --j) {
                     if (j >= len) continue;
-                    const int32_t s = cbeg + j;
                     const float xb[3] = {xs[vl][0][j], xs[vl][1][j], xs[vl][2][j]};
 #pragma unroll
                     for (int o = 0; o < 3; ++o) {
                         const float yb =
                             G[0][o] * xb[0] + G[1][o] * xb[1] + G[2][o] * xb[2] - a * ybn[o];
-                        const float ym =
-                            s > t0 ? yseg[int64_t(s - 1 - t0) * M + it * 3 + o] : yc[o];
+                        const float ym = j > 0 ? yv[o][j - 1] : y0[o];
 #pragma unroll
-                        for (int in = 0; in < 3; ++in) g[in][o] += xb[in] * ycur[o];
+                        for (int in = 0; in < 3; ++in) g[in][o] += xb[in] * yv[o][j];
                         A += yb * (xi[o][j] - ym);
                         ybn[o] = yb;
-                        ycur[o] = ym;
                     }
                 }
             }
@@ -504,12 +527,12 @@ hipError_t launch_adjoint_sweep_resident(hipStream_t s, int32_t blocks, const in
 
 hipError_t launch_adjoint_gradient(hipStream_t s, const int32_t *nbr, const float *coefs,
                                    const int32_t *delays, const float *gT, const float *xT,
-                                   const float *xbarT, const float *ckpt, float *yseg, float *dGT,
+                                   const float *xbarT, const float *ckpt, float *ychunk, float *dGT,
                                    float *dA, int32_t N, int32_t T, int32_t K) {
     if (N <= 0) return hipSuccess;
     const unsigned blocks = unsigned((int64_t(N) * 26 + kGradThreads - 1) / kGradThreads);
     adjoint_gradient_kernel<<<blocks, kGradThreads, 0, s>>>(nbr, coefs, delays, gT, xT, xbarT,
-                                                            ckpt, yseg, dGT, dA, N, T, K);
+                                                            ckpt, ychunk, dGT, dA, N, T, K);
     return hipGetLastError();
 }
 

@@ -483,9 +483,9 @@ static int adjoint_prepare(ct_results_gpu &r, int32_t K, int64_t &slots) {
         CT_CHECK(ensure(r, &r.qring, have_q, slots * M, 1));
         r.qring_slots = std::max(r.qring_slots, slots);
     }
-    int64_t have_y = r.yseg_steps * M;
-    CT_CHECK(ensure(r, &r.yseg, have_y, int64_t(K) * M, 1));
-    r.yseg_steps = std::max(r.yseg_steps, int64_t(K));
+    // gradient pass: filter state (6 values) per connection at each 16-step
+    // chunk start of a segment
+    CT_CHECK(ensure(r, &r.yseg, r.yseg_have, int64_t((K + 15) / 16) * int64_t(r.N) * 26 * 6, 1));
     return CT_OK;
 }
 

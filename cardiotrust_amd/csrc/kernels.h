@@ -142,7 +142,7 @@ hipError_t launch_adjoint_sweep_resident(hipStream_t s, int32_t blocks, const in
 hipError_t launch_barrier_reset(hipStream_t s, GridBarrier *gb, int32_t count);
 hipError_t launch_adjoint_gradient(hipStream_t s, const int32_t *nbr, const float *coefs,
                                    const int32_t *delays, const float *gT, const float *xT,
-                                   const float *xbarT, const float *ckpt, float *yseg, float *dGT,
+                                   const float *xbarT, const float *ckpt, float *ychunk, float *dGT,
                                    float *dA, int32_t N, int32_t T, int32_t K);
 
 hipError_t launch_sgd(hipStream_t s, float *v, const float *d, int64_t len, float scale);
