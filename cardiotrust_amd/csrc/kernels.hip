@@ -143,6 +143,11 @@ constexpr int kResStates = (kResMaxVoxels * 3 + kResThreads - 1) / kResThreads;
 // (91.7 ms with 9 gathers and halves), which hold more registers.
 constexpr int kResBatch = 1;
 constexpr int kResGainConns = 9;
+// The batched kernel also holds its job's fields: its own measured values
+// (gains in halves: 649 ms per epoch of a 24-scenario batch, against 699 ms
+// in thirds).
+constexpr int kBatchGather = 1;
+constexpr int kBatchGainConns = 13;
 constexpr uint32_t kNoConnection = 0xFFFFFFFFu;
 constexpr int kResNeighbourBits = 17;  // voxel index field; delay in the upper 15 bits
 
@@ -150,6 +155,7 @@ constexpr int kResNeighbourBits = 17;  // voxel index field; delay in the upper 
 // the job's team. The first barrier waits up to `first_wait` ticks (team
 // members may still be finishing earlier jobs), the others `step_wait`.
 // Returns false if the job's barrier aborted.
+template <int kGather, int kGainConns>
 __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, int32_t ve,
                                              uint32_t member, int32_t t_begin, int32_t t_end,
                                              float *lds, uint64_t first_wait, uint64_t step_wait) {
@@ -204,10 +210,10 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
     for (int32_t t = t_begin; t < t_end; ++t) {
         const bool last = (t == T - 1);
 #pragma unroll
-        for (int k0 = 0; k0 < kResItems; k0 += kResBatch) {
-            float in0[kResBatch][3];
+        for (int k0 = 0; k0 < kResItems; k0 += kGather) {
+            float in0[kGather][3];
 #pragma unroll
-            for (int q = 0; q < kResBatch; ++q) {
+            for (int q = 0; q < kGather; ++q) {
                 const int k = k0 + q;
 #pragma unroll
                 for (int o = 0; o < 3; ++o) in0[q][o] = 0.0f;
@@ -225,7 +231,7 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
                 }
             }
 #pragma unroll
-            for (int q = 0; q < kResBatch; ++q) {
+            for (int q = 0; q < kGather; ++q) {
                 const int k = k0 + q;
                 if (k >= kResItems || pk[k] == kNoConnection) continue;
                 const int it = tid + k * kResThreads;
@@ -249,15 +255,15 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
             const uint32_t m = mask[vl];
             float acc = 0.0f;
 #pragma unroll 1
-            for (int h = 0; h < (26 + kResGainConns - 1) / kResGainConns; ++h) {
-                float g[3 * kResGainConns];
+            for (int h = 0; h < (26 + kGainConns - 1) / kGainConns; ++h) {
+                float g[3 * kGainConns];
 #pragma unroll
-                for (int q = 0; q < 3 * kResGainConns; ++q)
-                    if (3 * kResGainConns * h + q < 78)
-                        g[q] = gT[uint32_t(3 * kResGainConns * h + q) * uint32_t(n) + uint32_t(i)];
+                for (int q = 0; q < 3 * kGainConns; ++q)
+                    if (3 * kGainConns * h + q < 78)
+                        g[q] = gT[uint32_t(3 * kGainConns * h + q) * uint32_t(n) + uint32_t(i)];
 #pragma unroll
-                for (int dd = 0; dd < kResGainConns; ++dd) {
-                    const int d = kResGainConns * h + dd;
+                for (int dd = 0; dd < kGainConns; ++dd) {
+                    const int d = kGainConns * h + dd;
                     if (d >= 26) break;
                     if (!((m >> d) & 1u)) continue;
 #pragma unroll
@@ -288,7 +294,8 @@ __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
     const int64_t slab = xcd_block(blockIdx.x, gridDim.x);
     const int32_t vb = int32_t(slab * job.N / gridDim.x);
     const int32_t ve = int32_t((slab + 1) * job.N / gridDim.x);
-    resident_run(job, vb, ve, blockIdx.x, t_begin, t_end, lds, wait_ticks, wait_ticks);
+    resident_run<kResBatch, kResGainConns>(job, vb, ve, blockIdx.x, t_begin, t_end, lds,
+                                           wait_ticks, wait_ticks);
 }
 
 // Many jobs in one persistent launch. Logical slot = xcd_block(blockIdx.x), so
@@ -309,7 +316,8 @@ __global__ __launch_bounds__(kResThreads) void propagate_batch_kernel(
         const int32_t member = slot - job.slot0;
         const int32_t vb = int32_t(int64_t(member) * job.N / job.width);
         const int32_t ve = int32_t(int64_t(member + 1) * job.N / job.width);
-        if (!resident_run(job, vb, ve, uint32_t(member), 0, job.T, lds, first_wait, step_wait))
+        if (!resident_run<kBatchGather, kBatchGainConns>(job, vb, ve, uint32_t(member), 0, job.T,
+                                                         lds, first_wait, step_wait))
             return;
         __syncthreads();  // LDS is reused by the next job
     }
