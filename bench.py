@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "epochs/s at 120k voxels x 2000 steps (fwd+grad+update), HBM GB/s vs peak"
 HBM_PEAK_GBS = 8000.0
+MFMA_FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
 # Algorithmic bytes of one propagation step per active voxel: 4 B x (234 gains
 # + 26 coefs + 26 delays + 2 x 78 filter outputs read/write + 3 states written
 # + 3 states read).
@@ -300,6 +301,20 @@ def main() -> int:
             "us_per_step": 1e3 * launch_ms / steps_per_launch,
             "bytes_per_launch": bytes_per_launch,
         },
+    }
+    # MFMA utilisation of the two deferred projections (FP32 MFMA, per epoch)
+    flop = 2.0 * S * 3 * N * T
+    out["mfma"] = {
+        "peak_tflops": MFMA_FP32_PEAK_TFLOPS,
+        "project": {"kernel": "project_kernel (H·Xᵀ)", "gflop": flop / 1e9,
+                    "ms": prof["project_ms"],
+                    "tflops": flop / (prof["project_ms"] * 1e-3) / 1e12,
+                    "frac": flop / (prof["project_ms"] * 1e-3) / 1e12 / MFMA_FP32_PEAK_TFLOPS},
+        "map_residuals": {"kernel": "map_residuals_kernel (Hᵀ·R)", "gflop": flop / 1e9,
+                          "ms": prof["map_residuals_ms"],
+                          "tflops": flop / (prof["map_residuals_ms"] * 1e-3) / 1e12,
+                          "frac": flop / (prof["map_residuals_ms"] * 1e-3) / 1e12
+                          / MFMA_FP32_PEAK_TFLOPS},
     }
     kernel.close()
     actual.close()
