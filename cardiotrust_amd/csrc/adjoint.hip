@@ -39,7 +39,7 @@ __global__ void max_delay_kernel(const int32_t *__restrict__ nbr, const int32_t 
     red[threadIdx.x] = m;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + s]);
+        if (int(threadIdx.x) < s) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + s]);
         __syncthreads();
     }
     if (threadIdx.x == 0) atomicMax(out, red[0]);
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_step_kernel(
     const int32_t v0 = xcd_block(blockIdx.x, gridDim.x) * kAdjVoxels;
     const int32_t nv = min(kAdjVoxels, N - v0);
 
-    if (threadIdx.x < nv) {
+    if (int(threadIdx.x) < nv) {
         const int32_t w = v0 + threadIdx.x;
         const int64_t row = int64_t(s) * n + 3LL * w;
         const float xv[3] = {x[row], x[row + 1], x[row + 2]};

@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void objective_terms_f64_kernel(
     red[threadIdx.x] = 0.5 * s_mse * mse + 0.5 * s_reg * reg;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) terms[t] = red[0];

@@ -472,7 +472,7 @@ __global__ __launch_bounds__(256) void maxreg_partials_kernel(const float *__res
     red[threadIdx.x] = f2;
     __syncthreads();
     for (int stride = 128; stride > 0; stride >>= 1) {
-        if (threadIdx.x < stride) red[threadIdx.x] += red[threadIdx.x + stride];
+        if (int(threadIdx.x) < stride) red[threadIdx.x] += red[threadIdx.x + stride];
         __syncthreads();
     }
     if (threadIdx.x == 0) partial[int64_t(t) * gridDim.x + blockIdx.x] = red[0];
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(256) void maxreg_step_sums_kernel(const float *__re
     red[threadIdx.x] = s;
     __syncthreads();
     for (int stride = 128; stride > 0; stride >>= 1) {
-        if (threadIdx.x < stride) red[threadIdx.x] += red[threadIdx.x + stride];
+        if (int(threadIdx.x) < stride) red[threadIdx.x] += red[threadIdx.x + stride];
         __syncthreads();
     }
     if (threadIdx.x == 0) step_mr[t] = red[0];

@@ -217,3 +217,14 @@ def test_batch_rejects_bad_members(gpu, ps):
         b.set_voxels_per_group(1000)
     b.close()
     m.close()
+
+
+def test_batch_with_an_empty_scenario(gpu, ps):
+    note("case: batch with an empty scenario (N = 0) among others")
+    g = synth.grid_geometry(np.zeros((2, 2, 2), dtype=bool), 2.5)
+    empty = synth.make_problem(g, num_steps=30, max_delay=4, seed=1, simulate=oracle.simulate_f32)
+    mixed = [ps[2], empty, ps[3]]
+    alone = run_alone(gpu, mixed, epochs=2)
+    got, infos = run_batched(gpu, mixed, epochs=2, vpg=64)
+    assert_same(alone, got)
+    assert infos[-1]["scenarios"] == 3
