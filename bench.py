@@ -36,7 +36,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(problem, measurements, sample_steps: int):
+def cpu_baseline(problem, measurements, sample_steps: int, threads_cap: int = 16):
     """Oracle FP32 epoch (single thread) on the first `sample_steps` steps of
     the same problem, scaled to epochs/s of the full T."""
     from oracle import oracle
@@ -48,20 +48,50 @@ def cpu_baseline(problem, measurements, sample_steps: int):
                 problem.initial_delays, problem.control_matrix,
                 problem.control_function[:sample_steps], problem.measurement_matrix,
                 np.ascontiguousarray(measurements[:, :sample_steps]))
-    r = oracle.OracleResults(p, np.float32)
     cfg = Algorithm(learning_rate=1e-3, freeze_delays=False)
+    r = oracle.OracleResults(p, np.float32)
     t0 = time.perf_counter()
     r.run_epoch(cfg)
     dt = time.perf_counter() - t0
     per_epoch = dt * problem.num_steps / sample_steps
-    return {
+    out = {
         "value": 1.0 / per_epoch,
         "unit": "epochs/s",
         "cores": 1,
         "kind": "port",
         "sample": f"oracle FP32 run_epoch on the first {sample_steps} of {problem.num_steps} steps "
                   f"of the same problem ({dt:.1f} s), scaled by T/{sample_steps}",
+        "cpu_model": cpu_model(),
     }
+    # throughput variant: one independent scenario epoch per core, concurrently
+    # (the reference runs one scenario per thread); the oracle's ctypes calls
+    # release the GIL
+    import threading
+    threads = max(1, min(os.cpu_count() or 1, threads_cap))
+    copies = [oracle.OracleResults(p, np.float32) for _ in range(threads)]
+    workers = [threading.Thread(target=c.run_epoch, args=(cfg,)) for c in copies]
+    t0 = time.perf_counter()
+    for w in workers:
+        w.start()
+    for w in workers:
+        w.join()
+    dt_all = time.perf_counter() - t0
+    out["throughput"] = {
+        "value": threads / (dt_all * problem.num_steps / sample_steps),
+        "unit": "epochs/s", "cores": threads,
+        "sample": f"{threads} concurrent copies of the same sample epoch ({dt_all:.1f} s)",
+    }
+    return out
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def bench_batched(args) -> int:
