@@ -50,7 +50,8 @@ def problem(seed, beats=3, T=60, n=400):
                               simulate=oracle.simulate_f32)
 
 
-def run(gpu, p, cfg, share, epochs=2, gradient=None, optimizer=0, order=None, batch=False):
+def run(gpu, p, cfg, share, epochs=2, gradient=None, optimizer=0, order=None, batch=False,
+        graph=False):
     from cardiotrust_amd.gpu import Batch, Buffer, EpochKernel, ResultsGPU
     nb = epochs * (p.num_beats if cfg.batch_size else 1)
     r = ResultsGPU(gpu, p, num_batch_entries=nb, optimizer=optimizer)
@@ -61,7 +62,7 @@ def run(gpu, p, cfg, share, epochs=2, gradient=None, optimizer=0, order=None, ba
     if order is not None:
         k.set_beat_order(order)
     k.set_share_propagation(share)
-    b = Batch(gpu, [k], voxels_per_group=96) if batch else None
+    b = Batch(gpu, [k], voxels_per_group=96, graph=graph) if batch else None
     for _ in range(epochs):
         (b or k).execute()
     if b:
@@ -114,3 +115,15 @@ def test_shared_propagation_in_a_batch_equals_alone(gpu):
     for f in FIELDS:
         x, y = np.asarray(a[f]), np.asarray(b[f])
         assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), f
+
+
+def test_shared_propagation_in_a_graph_replayed_batch(gpu):
+    note("case: shared propagation, reverse mode, batch captured in a graph, 3 epochs")
+    p = problem(4, beats=3)
+    cfg = Algorithm(learning_rate=0.05, freeze_delays=False)
+    a = run(gpu, p, cfg, True, epochs=3, gradient=(1, 16))
+    b = run(gpu, p, cfg, True, epochs=3, gradient=(1, 16), batch=True, graph=True)
+    for f in FIELDS:
+        x, y = np.asarray(a[f]), np.asarray(b[f])
+        assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), f
+
