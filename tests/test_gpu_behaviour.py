@@ -1,0 +1,92 @@
+"""Behaviour the reference's own (ignored) tests assert, on the GPU path:
+loss_batch decreases over epochs (algorithm/tests/loss_decreases.rs:36-89),
+the simulated maximum state is about 1 (data/simulation/tests.rs:34-45), and a
+diverging or NaN-fed epoch ends with a non-finite loss instead of a failure
+(the reference stops its epoch loop on !loss.is_normal(), scenario.rs:778-780)."""
+import sys
+
+import numpy as np
+import pytest
+
+from cardiotrust_amd import Algorithm, synth
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from cardiotrust_amd.gpu import GPU
+    g = GPU(0)
+    yield g
+    g.close()
+
+
+@pytest.fixture(scope="module")
+def blob():
+    mask = synth.uniform(21, 99, 7 * 6 * 5).reshape(7, 6, 5) < 0.6
+    mask[0, 0, 0] = True
+    g = synth.grid_geometry(mask, 2.5)
+    return synth.make_problem(g, num_steps=80, max_delay=10, seed=21, gain_sigma=0.2,
+                              simulate=oracle.simulate_f32)
+
+
+def epochs(gpu, p, cfg, n, gradient=None, measurements=None):
+    from cardiotrust_amd.gpu import Buffer, EpochKernel, ResultsGPU
+    r = ResultsGPU(gpu, p, num_batch_entries=n)
+    buf = Buffer(gpu, p.measurements if measurements is None else measurements)
+    k = EpochKernel(gpu, r, buf, cfg, 3 * p.num_voxels, p.num_sensors, p.num_steps)
+    if gradient:
+        k.set_gradient(*gradient)
+    for _ in range(n):
+        k.execute()
+    out = r.update_from_gpu(["loss_batch", "delays"])
+    k.close()
+    buf.close()
+    r.close()
+    return out
+
+
+@pytest.mark.parametrize("gradient", [None, (1, 16)])
+def test_loss_decreases(gpu, blob, gradient):
+    note(f"case: 4 epochs, lr 1, freeze_delays false, gradient {gradient}")
+    cfg = Algorithm(learning_rate=1.0, freeze_delays=False)
+    lb = epochs(gpu, blob, cfg, 4, gradient)["loss_batch"]
+    note(f"  loss_batch {lb.tolist()}")
+    assert np.all(np.isfinite(lb))
+    assert np.all(np.diff(lb) < 0), lb
+    if gradient is None:  # the reference derivation: also the oracle's numbers
+        o = oracle.OracleResults(blob, np.float32, num_batch_entries=4)
+        for _ in range(4):
+            o.run_epoch(cfg)
+        assert np.allclose(lb, o.loss_batch, rtol=1e-4)
+
+
+def test_simulated_maximum_state_is_about_one(gpu):
+    from cardiotrust_amd.gpu import ResultsGPU
+    note("case: simulate the 10^3 cube's ground truth, max |x|")
+    p = synth.config0()
+    r = ResultsGPU(gpu, p.truth())
+    r.simulate()
+    x = r.update_from_gpu(["system_states"])["system_states"]
+    r.close()
+    note(f"  max |x| = {np.abs(x).max():.6f}")
+    assert 0.99 < np.abs(x).max() < 1.05
+
+
+def test_divergence_gives_a_non_finite_loss(gpu, blob):
+    note("case: lr 1000 diverges; NaN measurements")
+    lb = epochs(gpu, blob, Algorithm(learning_rate=1000.0, freeze_delays=False), 4)
+    assert np.isfinite(lb["loss_batch"][0])
+    assert not np.all(np.isfinite(lb["loss_batch"]))
+    assert lb["delays"].min() >= 1 and lb["delays"].max() <= 1000  # rolls stay in range
+    bad = blob.measurements.copy()
+    bad[0, 5, 3] = np.nan
+    lb = epochs(gpu, blob, Algorithm(learning_rate=1.0, freeze_delays=False), 2,
+                measurements=bad)
+    assert np.isnan(lb["loss_batch"]).all()
