@@ -16,7 +16,7 @@ from typing import Optional
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle_ct.so")
+_LIB_PATH = os.environ.get("CT_ORACLE_LIB") or os.path.join(_HERE, "liboracle_ct.so")
 _lib = None
 
 
