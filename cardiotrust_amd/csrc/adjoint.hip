@@ -20,6 +20,7 @@
 
 #include "barrier.h"
 #include "device.h"
+#include "lds_sort.h"
 #include "kernels.h"
 
 namespace ct {
@@ -133,24 +134,37 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_step_kernel(
 // those of adjoint_step_kernel, so the results are the same bits.
 //
 // A workgroup owns voxels [vb, ve) (at most kResMaxVoxels). Registers per
-// item (w, j): ybar (3) and the producer voxel and delay (17 | 15 bits); per
-// phase-2 voxel lane its consumers' delays, three per word. The gains and
-// coefficient of each item come from gcons (N, 26, 10), consumer-ordered.
+// item (w, j): ybar (3) and one packed word; per phase-2 voxel lane its
+// consumers' delays, three per word. The set-up writes each item's consumer
+// gains and coefficient (prediction.rs gains G_c, coefficient a_c) to gcons
+// (N, 26, 10) at the item's position in the slab's phase-1 order, which the
+// step loop then streams contiguously. When every delay of the slab is at
+// most kSortedMaxDelay, phase 1 runs the items sorted by (delay, consumer):
+// the lanes of one gather read one xbar row at nearby voxels (the packed
+// word is producer voxel 17 bits | item 14 bits, the delay sits in an LDS
+// byte table). Otherwise items keep (voxel, consumer) order and the word is
+// consumer voxel 17 bits | delay 15 bits.
 constexpr int kSweepThreads = 512;
 constexpr int kSweepItems = (kResMaxVoxels * 26 + kSweepThreads - 1) / kSweepThreads;
 constexpr int kSweepBatch = 1;  // items per load group: 1 measured fastest (102 ms vs 112 at 4)
 constexpr int kConsGainStride = 10;  // gcons floats per (voxel, consumer) slot
 constexpr uint32_t kSweepNoEntry = 0xFFFFFFFFu;
+static_assert(kSortSize >= kSweepItems * kSweepThreads || kSortSize >= kResMaxVoxels * 26,
+              "every phase-1 position has a key");
+static_assert(kSortSize * 8 <= kResMaxVoxels * 78 * 4, "sort keys must fit over q");
 
 __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
     const int32_t *__restrict__ cons_ptr, const int32_t *__restrict__ cons_idx,
-    const int32_t *__restrict__ delays, const float *__restrict__ gcons,
+    const int32_t *__restrict__ delays, const float *__restrict__ gains_rows,
+    const float *__restrict__ coefs, float *__restrict__ gcons,
     const float *__restrict__ x, const float *__restrict__ E,
     float *__restrict__ xbar, GridBarrier *__restrict__ gb, int32_t N, int32_t T, AdjointParams p,
     uint64_t wait_ticks) {
     extern __shared__ float lds[];
     float *q_lds = lds;  // [voxel][26][3]
-    int *ok_lds = reinterpret_cast<int *>(lds + kResMaxVoxels * 78);
+    int *ok_lds = reinterpret_cast<int *>(lds + kResMaxVoxels * 78);  // [4]
+    uint8_t *dtab = reinterpret_cast<uint8_t *>(ok_lds + 4);          // [phase-1 position]
+    uint64_t *keys = reinterpret_cast<uint64_t *>(lds);                 // set-up only, over q
     const int64_t n = 3LL * N;
     const int32_t slab = xcd_block(blockIdx.x, gridDim.x);
     const int32_t vb = int32_t(int64_t(slab) * N / gridDim.x);
@@ -171,25 +185,60 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
         for (int j = 0; j < 26; ++j)
             if (j < ncons) dj[j / 3] |= uint32_t(min(delays[cons_idx[j0 + j]], 1023)) << (10 * (j % 3));
     }
-    // phase-1 items (w, j): producer voxel, delay, coefficient, ybar state
-    uint32_t pk[kSweepItems];
-    float yb[kSweepItems][3];
-#pragma unroll
-    for (int k = 0; k < kSweepItems; ++k) {
-        const int it = tid + k * kSweepThreads;
-        pk[k] = kSweepNoEntry;
-#pragma unroll
-        for (int o = 0; o < 3; ++o) yb[k][o] = 0.0f;
+    // Keys D << 40 | c << 14 | item, c = the consuming connection; absent
+    // items sort last. keys[position] is the item at that phase-1 position.
+    if (tid == 0) ok_lds[1] = 0;
+    __syncthreads();
+    for (int it = tid; it < kSortSize; it += kSweepThreads) {
+        uint64_t key = ~0ull;
         if (it < items) {
             const int32_t w = vb + it / 26, j = it % 26;
             const int32_t j0 = cons_ptr[w];
             if (j < cons_ptr[w + 1] - j0) {
                 const int32_t c = cons_idx[j0 + j];
-                pk[k] = uint32_t(c / 26) | (uint32_t(min(delays[c], 32767)) << 17);
+                const int32_t D = min(delays[c], 32767);
+                key = (uint64_t(D) << 40) | (uint64_t(c) << 14) | uint64_t(it);
+                atomicMax(&ok_lds[1], D);
+            }
+        }
+        keys[it] = key;
+    }
+    __syncthreads();
+    const bool sorted = ok_lds[1] <= kSortedMaxDelay;
+    if (sorted) sort_keys_lds<kSweepThreads, kSortSize>(keys);
+    // phase-1 items: packed word, ybar state; their gains go to gcons here
+    uint32_t pk[kSweepItems];
+    float yb[kSweepItems][3];
+#pragma unroll
+    for (int k = 0; k < kSweepItems; ++k) {
+        const int pos = tid + k * kSweepThreads;
+        pk[k] = kSweepNoEntry;
+#pragma unroll
+        for (int o = 0; o < 3; ++o) yb[k][o] = 0.0f;
+        if (pos < items) {
+            const uint64_t key = keys[pos];
+            if (key != ~0ull) {
+                const int32_t D = int32_t(key >> 40);
+                const int32_t c = int32_t((key >> 14) & 0x3FFFFFFu);
+                const uint32_t v = uint32_t(c / 26), d = uint32_t(c % 26);
+                const uint32_t it = uint32_t(key & 0x3FFFu);
+                if (sorted) {
+                    dtab[pos] = uint8_t(D);
+                    pk[k] = v | (it << 17);
+                } else {
+                    pk[k] = v | (uint32_t(D) << 17);
+                }
+                float *dst = gcons + (int64_t(vb) * 26 + pos) * kConsGainStride;
+#pragma unroll
+                for (int in = 0; in < 3; ++in)
+#pragma unroll
+                    for (int o = 0; o < 3; ++o)
+                        dst[3 * in + o] = gains_rows[(3 * int64_t(v) + in) * 78 + 3 * d + o];
+                dst[9] = coefs[c];
             }
         }
     }
-    __syncthreads();
+    __syncthreads();  // the keys are dead: q is written over them; gcons rows are this lane's own
 
     for (int32_t s = T - 1; s >= 0; --s) {
         // phase 1: each item's q for step s (its ybar at tau = s + D). The
@@ -206,12 +255,15 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
 #pragma unroll
             for (int kk = 0; kk < kSweepBatch; ++kk) {
                 const int k = k0 + kk < kSweepItems ? k0 + kk : kSweepItems - 1;
-                const int it = lane + k * kSweepThreads;
-                const int32_t tau = s + int32_t(pk[k] >> 17);
-                on[kk] = k0 + kk < kSweepItems && pk[k] != kSweepNoEntry && tau < T;
+                const int pos = lane + k * kSweepThreads;
+                const bool present = k0 + kk < kSweepItems && pk[k] != kSweepNoEntry;
+                const int32_t D = !present ? 0
+                                  : sorted ? int32_t(dtab[pos]) : int32_t(pk[k] >> 17);
+                const int32_t tau = s + D;
+                on[kk] = present && tau < T;
                 // absent items read valid dummy addresses (row 0, gains of item 0)
                 const float *xs = on[kk] ? xbar + int64_t(tau) * n + 3LL * (pk[k] & 0x1FFFFu) : xbar;
-                const float *gs = gcons + (on[kk] ? (int64_t(vb) * 26 + it) * kConsGainStride : 0);
+                const float *gs = gcons + (on[kk] ? (int64_t(vb) * 26 + pos) * kConsGainStride : 0);
                 xv[kk] = *reinterpret_cast<const float3_a4 *>(xs);
 #pragma unroll
                 for (int i = 0; i < 3; ++i) g[kk][i] = *reinterpret_cast<const float3_a4 *>(gs + 3 * i);
@@ -221,7 +273,7 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
             for (int kk = 0; kk < kSweepBatch; ++kk) {
                 const int k = k0 + kk;
                 if (!on[kk]) continue;
-                const int it = lane + k * kSweepThreads;
+                const int it = sorted ? int(pk[k] >> 17) : lane + k * kSweepThreads;
 #pragma unroll
                 for (int o = 0; o < 3; ++o) {
                     const float next = yb[k][o];
@@ -258,29 +310,6 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
 
 // gcons[w][j] = (G[3v+in][3d+o] for in, o; a) of the j-th consumer
 // c = (v, d) of w: 10 floats per slot.
-__global__ void consumer_gains_kernel(const int32_t *__restrict__ cons_ptr,
-                                      const int32_t *__restrict__ cons_idx,
-                                      const float *__restrict__ gains_rows,
-                                      const float *__restrict__ coefs, float *__restrict__ gcons,
-                                      int32_t N) {
-    const int64_t slot = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (slot >= int64_t(N) * 26) return;
-    const int32_t w = int32_t(slot / 26), j = int32_t(slot % 26);
-    const int32_t j0 = cons_ptr[w];
-    float *dst = gcons + slot * kConsGainStride;
-    if (j >= cons_ptr[w + 1] - j0) {
-#pragma unroll
-        for (int e = 0; e < kConsGainStride; ++e) dst[e] = 0.0f;
-        return;
-    }
-    const int32_t c = cons_idx[j0 + j];
-    const int64_t v = c / 26, d = c % 26;
-#pragma unroll
-    for (int in = 0; in < 3; ++in)
-#pragma unroll
-        for (int o = 0; o < 3; ++o) dst[3 * in + o] = gains_rows[(3 * v + in) * 78 + 3 * d + o];
-    dst[9] = coefs[c];
-}
 
 // Gradient pass: one lane per connection. For each K-step segment, from the
 // last to the first: run the filter forward from the checkpoint the forward
@@ -486,7 +515,7 @@ hipError_t launch_adjoint_step(hipStream_t s, const int32_t *nbr, const float *c
     return hipGetLastError();
 }
 
-size_t sweep_lds_bytes() { return sizeof(float) * kResMaxVoxels * 78 + 16; }
+size_t sweep_lds_bytes() { return sizeof(float) * kResMaxVoxels * 78 + 16 + kResMaxVoxels * 26; }
 
 hipError_t sweep_capacity(int32_t *blocks_per_cu) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&adjoint_sweep_resident_kernel),
@@ -501,27 +530,17 @@ hipError_t sweep_capacity(int32_t *blocks_per_cu) {
     return e;
 }
 
-hipError_t launch_consumer_gains(hipStream_t s, const int32_t *cons_ptr, const int32_t *cons_idx,
-                                 const float *gains_rows, const float *coefs, float *gcons,
-                                 int32_t N) {
-    if (N <= 0) return hipSuccess;
-    const int64_t slots = int64_t(N) * 26;
-    consumer_gains_kernel<<<unsigned((slots + 255) / 256), 256, 0, s>>>(cons_ptr, cons_idx,
-                                                                        gains_rows, coefs, gcons, N);
-    return hipGetLastError();
-}
-
 hipError_t launch_adjoint_sweep_resident(hipStream_t s, int32_t blocks, const int32_t *cons_ptr,
                                          const int32_t *cons_idx, const int32_t *delays,
-                                         const float *gcons,
-                                         const float *x, const float *E, float *xbar,
-                                         GridBarrier *gb, int32_t N, int32_t T,
+                                         const float *gains_rows, const float *coefs,
+                                         float *gcons, const float *x, const float *E,
+                                         float *xbar, GridBarrier *gb, int32_t N, int32_t T,
                                          const AdjointParams &p, uint64_t wait_ticks) {
     if (N <= 0 || T <= 0) return hipSuccess;
     hipError_t e = launch_barrier_reset(s, gb, 1);
     if (e != hipSuccess) return e;
     adjoint_sweep_resident_kernel<<<blocks, kSweepThreads, sweep_lds_bytes(), s>>>(
-        cons_ptr, cons_idx, delays, gcons, x, E, xbar, gb, N, T, p, wait_ticks);
+        cons_ptr, cons_idx, delays, gains_rows, coefs, gcons, x, E, xbar, gb, N, T, p, wait_ticks);
     return hipGetLastError();
 }
 

@@ -524,10 +524,9 @@ static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K,
         Span sp(k, kAdjointSweep);
         if (resident_sweep(r, slots)) {
             // all steps in one launch, recurrences owned by the consuming voxels
-            CT_HIP(ct::launch_consumer_gains(s, r.cons_ptr, r.cons_idx, r.staging, r.coefs,
-                                             r.gcons, r.N));
             CT_HIP(ct::launch_adjoint_sweep_resident(s, r.resident_blocks, r.cons_ptr, r.cons_idx,
-                                                     r.delays, r.gcons, r.x, r.E, r.xbar,
+                                                     r.delays, r.staging, r.coefs, r.gcons, r.x,
+                                                     r.E, r.xbar,
                                                      r.barrier, r.N, r.T, p,
                                                      wait_ticks(*r.gpu, 2.0)));
             r.resident_used = true;

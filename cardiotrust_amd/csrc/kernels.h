@@ -124,19 +124,18 @@ hipError_t launch_adjoint_step(hipStream_t s, const int32_t *nbr, const float *c
 // forward propagation.
 // Resident backward sweep (all steps in one launch; the workgroups stay
 // resident and meet at `gb` after every step). Each connection's ybar
-// recurrence runs in the workgroup of the voxel that consumes it, from the
-// consumer-ordered gains and coefficients in gcons (N, 26, 10; built by
-// launch_consumer_gains from the gains rows). Needs the resident
-// propagation's limits (N < 2^17 - 1, at most kResMaxVoxels voxels per
-// workgroup), delays below 1024 and at most 26 consumers per voxel.
+// recurrence runs in the workgroup of the voxel that consumes it. The launch
+// first copies each connection's gains (from the gains rows, (n, 78)) and
+// coefficient into gcons (N, 26, 10 floats of scratch) in its phase-1 order.
+// Needs the resident propagation's limits (N < 2^17 - 1, at most
+// kResMaxVoxels voxels per workgroup), delays below 1024 and at most 26
+// consumers per voxel.
 size_t sweep_lds_bytes();
 hipError_t sweep_capacity(int32_t *blocks_per_cu);
-hipError_t launch_consumer_gains(hipStream_t s, const int32_t *cons_ptr, const int32_t *cons_idx,
-                                 const float *gains_rows, const float *coefs, float *gcons,
-                                 int32_t N);
 hipError_t launch_adjoint_sweep_resident(hipStream_t s, int32_t blocks, const int32_t *cons_ptr,
                                          const int32_t *cons_idx, const int32_t *delays,
-                                         const float *gcons, const float *x, const float *E,
+                                         const float *gains_rows, const float *coefs,
+                                         float *gcons, const float *x, const float *E,
                                          float *xbar, GridBarrier *gb, int32_t N, int32_t T,
                                          const AdjointParams &p, uint64_t wait_ticks);
 hipError_t launch_barrier_reset(hipStream_t s, GridBarrier *gb, int32_t count);

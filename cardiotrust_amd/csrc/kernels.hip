@@ -10,6 +10,7 @@
 
 #include "barrier.h"
 #include "device.h"
+#include "lds_sort.h"
 
 namespace ct {
 namespace {
@@ -138,18 +139,20 @@ constexpr int kResThreads = 512;
 constexpr int kResItems = (kResMaxVoxels * 26 + kResThreads - 1) / kResThreads;
 constexpr int kResStates = (kResMaxVoxels * 3 + kResThreads - 1) / kResThreads;
 // Gathers issued per group in phase 1, and connections per gains group in
-// phase 2 (26 = 9 + 9 + 8): measured on the 120k-voxel shell, one gather at
-// a time and gains in thirds (83.7 ms per 2000 steps) beat larger groups
-// (91.7 ms with 9 gathers and halves), which hold more registers.
-constexpr int kResBatch = 1;
+// phase 2 (26 = 9 + 9 + 8), measured on the 120k-voxel shell with sorted
+// connections (ms per 2000 steps): 2 gathers / thirds 71.6, 1 / thirds 73.9,
+// 3 / thirds 72.0, 4 / thirds 72.6, 2 / halves 72.9, 2 / sixths 77.8.
+constexpr int kResBatch = 2;
 constexpr int kResGainConns = 9;
 // The batched kernel also holds its job's fields: its own measured values
-// (gains in halves: 649 ms per epoch of a 24-scenario batch, against 699 ms
-// in thirds).
-constexpr int kBatchGather = 1;
+// (ms per epoch of the 64-scenario uneven batch): 3 gathers / halves 1751,
+// 2 / halves 1775, 1 / halves 1984, 2 / thirds 1830, 1 / thirds 1893.
+constexpr int kBatchGather = 3;
 constexpr int kBatchGainConns = 13;
 constexpr uint32_t kNoConnection = 0xFFFFFFFFu;
-constexpr int kResNeighbourBits = 17;  // voxel index field; delay in the upper 15 bits
+constexpr int kResNeighbourBits = 17;  // voxel index field; delay or connection in the upper 15 bits
+static_assert(kSortSize >= kResMaxVoxels * 26 && kSortSize < (1 << 14) + 1, "sort size");
+static_assert(kSortSize * 8 <= kResMaxVoxels * 78 * 4, "sort keys must fit over the filter outputs");
 
 // Steps [t_begin, t_end) of voxels [vb, ve) of job `j`, as member `member` of
 // the job's team. The first barrier waits up to `first_wait` ticks (team
@@ -161,7 +164,9 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
                                              float *lds, uint64_t first_wait, uint64_t step_wait) {
     float *ys = lds;                                                       // [voxel][78]
     uint32_t *mask = reinterpret_cast<uint32_t *>(lds + kResMaxVoxels * 78);  // [voxel]
-    int *ok_lds = reinterpret_cast<int *>(mask + kResMaxVoxels);
+    int *ok_lds = reinterpret_cast<int *>(mask + kResMaxVoxels);           // [4]
+    uint8_t *dtab = reinterpret_cast<uint8_t *>(ok_lds + 4);                // [sorted position]
+    uint64_t *keys = reinterpret_cast<uint64_t *>(lds);  // set-up only, over ys
     const int32_t N = j.N, T = j.T;
     const int32_t *__restrict__ nbr = j.nbr;
     const float *__restrict__ gT = j.gT;
@@ -174,8 +179,33 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
     const int32_t items = nv * 26;
     const int tid = threadIdx.x;
 
+    // Phase 1 runs the slab's connections in (delay, neighbour) order when
+    // every delay fits the byte table: the lanes of one gather instruction
+    // then read one state row at nearby voxels, about 29 cache lines per wave
+    // instead of about 61 (one per lane). Key = D << 40 | w << 14 | local
+    // connection; a missing connection sorts last.
     for (int k = tid; k < kResMaxVoxels; k += kResThreads) mask[k] = 0u;
+    if (tid == 0) ok_lds[1] = 0;
     __syncthreads();
+    for (int p = tid; p < kSortSize; p += kResThreads) {
+        uint64_t key = ~0ull;
+        if (p < items) {
+            const int64_t c = int64_t(vb) * 26 + p;
+            const int32_t w = nbr[c];
+            if (w >= 0) {
+                const int32_t D = min(j.delays[c], 32767);
+                key = (uint64_t(D) << 40) | (uint64_t(w) << 14) | uint64_t(p);
+                atomicMax(&ok_lds[1], D);
+            }
+        }
+        keys[p] = key;
+    }
+    __syncthreads();
+    const bool sorted = ok_lds[1] <= kSortedMaxDelay;
+    if (sorted) sort_keys_lds<kResThreads, kSortSize>(keys);
+    // pk = neighbour (17 bits) | delay (15 bits) in connection order, or
+    // neighbour | local connection (14 bits) in sorted order with the delay in
+    // dtab[position].
     uint32_t pk[kResItems];
     float ak[kResItems];
     // pv = the neighbour states this connection read one step earlier, i.e.
@@ -189,13 +219,27 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
 #pragma unroll
         for (int o = 0; o < 3; ++o) pv[k][o] = 0.0f;
         if (it < items) {
-            const int64_t c = int64_t(vb) * 26 + it;
-            const int32_t w = nbr[c];
+            int32_t w = -1, D = 0, cl = it;
+            if (sorted) {
+                const uint64_t key = keys[it];
+                if (key != ~0ull) {
+                    D = int32_t(key >> 40);
+                    w = int32_t((key >> 14) & 0x3FFFFFFu);
+                    cl = int32_t(key & 0x3FFFu);
+                    dtab[it] = uint8_t(D);
+                    pk[k] = uint32_t(w) | (uint32_t(cl) << kResNeighbourBits);
+                }
+            } else {
+                const int64_t c = int64_t(vb) * 26 + it;
+                w = nbr[c];
+                if (w >= 0) {
+                    D = min(j.delays[c], 32767);
+                    pk[k] = uint32_t(w) | (uint32_t(D) << kResNeighbourBits);
+                }
+            }
             if (w >= 0) {
-                const int32_t D = min(j.delays[c], 32767);
-                pk[k] = uint32_t(w) | (uint32_t(D) << kResNeighbourBits);
-                ak[k] = j.coefs[c];
-                atomicOr(&mask[it / 26], 1u << (it % 26));
+                ak[k] = j.coefs[int64_t(vb) * 26 + cl];
+                atomicOr(&mask[cl / 26], 1u << (cl % 26));
                 if (D < t_begin) {
                     const int64_t r1 = int64_t(t_begin - D - 1) * n + 3LL * w;
 #pragma unroll
@@ -204,6 +248,7 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
             }
         }
     }
+    __syncthreads();  // the keys are dead: ys is loaded over them
     for (int idx = tid; idx < nv * 78; idx += kResThreads) ys[idx] = j.y[int64_t(vb) * 78 + idx];
     __syncthreads();
 
@@ -218,9 +263,14 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
 #pragma unroll
                 for (int o = 0; o < 3; ++o) in0[q][o] = 0.0f;
                 if (k >= kResItems || pk[k] == kNoConnection) continue;
+                // An opaque copy: otherwise the compiler keeps every value derived
+                // from pk (addresses, delays) live across the step loop and spills.
+                uint32_t pko = pk[k];
+                asm volatile("" : "+v"(pko));
                 // 32-bit element offsets: the host only picks this kernel when T * n < 2^32
-                const uint32_t w = pk[k] & ((1u << kResNeighbourBits) - 1);
-                const int32_t D = int32_t(pk[k] >> kResNeighbourBits);
+                const uint32_t w = pko & ((1u << kResNeighbourBits) - 1);
+                const int32_t D = sorted ? int32_t(dtab[tid + k * kResThreads])
+                                         : int32_t(pko >> kResNeighbourBits);
                 if (D <= t) {
                     // the neighbour's three states: one 12-byte load
                     const uint32_t r0 = uint32_t(t - D) * uint32_t(n) + 3u * w;
@@ -234,7 +284,9 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
             for (int q = 0; q < kGather; ++q) {
                 const int k = k0 + q;
                 if (k >= kResItems || pk[k] == kNoConnection) continue;
-                const int it = tid + k * kResThreads;
+                uint32_t pko = pk[k];
+                asm volatile("" : "+v"(pko));
+                const int it = sorted ? int(pko >> kResNeighbourBits) : tid + k * kResThreads;
 #pragma unroll
                 for (int o = 0; o < 3; ++o) {
                     const float prev = ys[it * 3 + o];
@@ -982,7 +1034,9 @@ hipError_t launch_propagate_step(hipStream_t s, const int32_t *nbr, const float 
     return hipGetLastError();
 }
 
-size_t resident_lds_bytes() { return sizeof(float) * kResMaxVoxels * 78 + 4 * kResMaxVoxels + 16; }
+size_t resident_lds_bytes() {
+    return sizeof(float) * kResMaxVoxels * 78 + 4 * kResMaxVoxels + 16 + kResMaxVoxels * 26;
+}
 
 hipError_t resident_capacity(int32_t *blocks_per_cu) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&propagate_resident_kernel),

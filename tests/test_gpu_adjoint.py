@@ -151,3 +151,28 @@ def test_resident_sweep_matches_per_step_sweep_on_many_workgroups(gpu):
     assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
     assert np.array_equal(a[2].view(np.uint32), b[2].view(np.uint32))
 
+
+
+def test_resident_sweep_long_delays_bitwise(gpu):
+    """Sweep slabs whose delays all fit a byte run their items sorted by
+    (delay, consumer); a slab with a delay above 255 keeps item order. Both in
+    one launch must equal the per-step sweep bit for bit."""
+    note("case: adjoint on a 24^3 shell, T=420, delays 256-400 on the first 1500 voxels, "
+         "mode 1 vs mode 2")
+    g0 = synth.ellipsoid_shell(shape=(24, 24, 24), outer=(12, 12, 12), inner=(6, 6, 6))
+    p = synth.make_problem(g0, num_steps=420, max_delay=40, seed=13)
+    long = 256 + (synth.uniform(13, 40, 1500 * 26) * 145).astype(np.int32).reshape(1500, 26)
+    p.delays[:1500] = np.where(g0.neighbours[:1500] >= 0, long, p.delays[:1500])
+    from cardiotrust_amd.gpu import ResultsGPU
+    truth = ResultsGPU(gpu, p.truth())
+    truth.set_propagation(1)
+    p.measurements = truth.simulate()
+    truth.close()
+    p.gains = (p.gains * 1.1).astype(np.float32)  # a non-zero residual
+    cfg = Algorithm(maximum_regularization_threshold=0.05)
+    a = gpu_grad(gpu, p, cfg, K=64, mode=1)
+    b = gpu_grad(gpu, p, cfg, K=64, mode=2)
+    assert a[0] == b[0] and a[0] > 0
+    assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+    assert np.array_equal(a[2].view(np.uint32), b[2].view(np.uint32))
+    assert np.abs(a[1]).max() > 0
