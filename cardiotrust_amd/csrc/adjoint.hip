@@ -209,6 +209,9 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
     // phase-1 items: packed word, ybar state; their gains go to gcons here
     uint32_t pk[kSweepItems];
     float yb[kSweepItems][3];
+    // bit k: item k has delay 1, so it reads the xbar row of the sweep step
+    // just before and must wait for that step's barrier
+    uint32_t late = 0u;
 #pragma unroll
     for (int k = 0; k < kSweepItems; ++k) {
         const int pos = tid + k * kSweepThreads;
@@ -222,6 +225,7 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
                 const int32_t c = int32_t((key >> 14) & 0x3FFFFFFu);
                 const uint32_t v = uint32_t(c / 26), d = uint32_t(c % 26);
                 const uint32_t it = uint32_t(key & 0x3FFFu);
+                if (D == 1) late |= 1u << k;
                 if (sorted) {
                     dtab[pos] = uint8_t(D);
                     pk[k] = v | (it << 17);
@@ -240,7 +244,13 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
     }
     __syncthreads();  // the keys are dead: q is written over them; gcons rows are this lane's own
 
+    // Step s: phase 1 of the items with delay >= 2 (rows published by the
+    // barrier of step s + 2), the wait for step s + 1's barrier, phase 1 of
+    // the delay-1 items, phase 2, and the arrival at step s's barrier.
     for (int32_t s = T - 1; s >= 0; --s) {
+#pragma unroll 1
+        for (uint32_t pass = 0; pass < 2; ++pass) {
+        if (pass == 1 && s < T - 1 && !team_wait(gb, gen, ok_lds, wait_ticks)) return;
         // phase 1: each item's q for step s (its ybar at tau = s + D). The
         // empty asm statement hides the lane index from the optimiser, so the
         // per-item addresses are recomputed each step instead of being kept
@@ -256,14 +266,15 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
             for (int kk = 0; kk < kSweepBatch; ++kk) {
                 const int k = k0 + kk < kSweepItems ? k0 + kk : kSweepItems - 1;
                 const int pos = lane + k * kSweepThreads;
-                const bool present = k0 + kk < kSweepItems && pk[k] != kSweepNoEntry;
+                const bool present = k0 + kk < kSweepItems && pk[k] != kSweepNoEntry &&
+                                     ((late >> k) & 1u) == pass;
                 const int32_t D = !present ? 0
                                   : sorted ? int32_t(dtab[pos]) : int32_t(pk[k] >> 17);
                 const int32_t tau = s + D;
                 on[kk] = present && tau < T;
-                // absent items read valid dummy addresses (row 0, gains of item 0)
-                const float *xs = on[kk] ? xbar + int64_t(tau) * n + 3LL * (pk[k] & 0x1FFFFu) : xbar;
-                const float *gs = gcons + (on[kk] ? (int64_t(vb) * 26 + pos) * kConsGainStride : 0);
+                if (!on[kk]) continue;  // not this pass, absent, or tau beyond the beat
+                const float *xs = xbar + int64_t(tau) * n + 3LL * (pk[k] & 0x1FFFFu);
+                const float *gs = gcons + (int64_t(vb) * 26 + pos) * kConsGainStride;
                 xv[kk] = *reinterpret_cast<const float3_a4 *>(xs);
 #pragma unroll
                 for (int i = 0; i < 3; ++i) g[kk][i] = *reinterpret_cast<const float3_a4 *>(gs + 3 * i);
@@ -283,6 +294,7 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
                     yb[k][o] = ny;
                 }
             }
+        }
         }
         __syncthreads();
         // phase 2: xbar_s of the slab's states, consumers in list order
@@ -304,7 +316,7 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
 #pragma unroll
             for (int o = 0; o < 3; ++o) xbar[row + o] = acc[o];
         }
-        if (!team_barrier(gb, gridDim.x, blockIdx.x, gen, ok_lds, wait_ticks)) return;
+        team_arrive(gb, gridDim.x, blockIdx.x, gen);
     }
 }
 

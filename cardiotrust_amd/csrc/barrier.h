@@ -8,16 +8,18 @@
 
 namespace ct {
 
-// Two-level counting barrier over `members` workgroups. Member m first counts
-// into group m % 8, the last arrival of a group counts into the top counter,
-// and the last group arrival publishes the next generation. Counters only grow
-// within a launch (they are zeroed before it), so "last" is recognised from
-// the running count and nothing has to be reset between steps.
-__device__ __forceinline__ bool team_barrier(GridBarrier *gb, uint32_t members, uint32_t member,
-                                             uint32_t &gen, int *ok_lds, uint64_t wait_ticks) {
+// Two-level counting barrier over `members` workgroups, in two halves so
+// that work independent of the other members can run between them:
+// team_arrive counts this workgroup in (after its writes), team_wait returns
+// once every member has arrived. Member m first counts into group m % 8, the
+// last arrival of a group counts into the top counter, and the last group
+// arrival publishes the next generation. Counters only grow within a launch
+// (they are zeroed before it), so "last" is recognised from the running
+// count and nothing has to be reset between steps.
+__device__ __forceinline__ void team_arrive(GridBarrier *gb, uint32_t members, uint32_t member,
+                                            uint32_t &gen) {
     __syncthreads();
     if (threadIdx.x == 0) {
-        int ok = 1;
         const uint32_t next = gen + 1;
         const uint32_t grp = member % kBarrierGroups;
         const uint32_t gsize = (members - grp + kBarrierGroups - 1) / kBarrierGroups;
@@ -26,7 +28,6 @@ __device__ __forceinline__ bool team_barrier(GridBarrier *gb, uint32_t members, 
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const uint32_t a1 = __hip_atomic_fetch_add(&gb->group[grp].count, 1u, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_AGENT);
-        bool released = false;
         if (a1 == next * gsize - 1) {
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
             const uint32_t a2 = __hip_atomic_fetch_add(&gb->top.count, 1u, __ATOMIC_RELAXED,
@@ -35,34 +36,48 @@ __device__ __forceinline__ bool team_barrier(GridBarrier *gb, uint32_t members, 
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
                 __hip_atomic_store(&gb->generation.count, next, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
-                released = true;
             }
         }
-        if (!released) {
-            const uint64_t t0 = uint64_t(wall_clock64());
-            uint32_t polls = 0;
-            while (__hip_atomic_load(&gb->generation.count, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT) < next) {
-                if (__hip_atomic_load(&gb->abort.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    ok = 0;
-                    break;
-                }
-                if ((++polls & 63u) == 0 && uint64_t(wall_clock64()) - t0 > wait_ticks) {
-                    __hip_atomic_store(&gb->abort.count, 1u, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    ok = 0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
+    }
+    gen += 1;
+}
+
+// Waits (at most `wait_ticks` of wall clock) until generation `gen` is
+// published; false if the team aborted or the wait timed out (which raises
+// the team's sticky abort word).
+__device__ __forceinline__ bool team_wait(GridBarrier *gb, uint32_t gen, int *ok_lds,
+                                          uint64_t wait_ticks) {
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const uint64_t t0 = uint64_t(wall_clock64());
+        uint32_t polls = 0;
+        while (__hip_atomic_load(&gb->generation.count, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT) < gen) {
+            if (__hip_atomic_load(&gb->abort.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ok = 0;
+                break;
             }
+            if ((++polls & 63u) == 0 && uint64_t(wall_clock64()) - t0 > wait_ticks) {
+                __hip_atomic_store(&gb->abort.count, 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
         }
         // acquire: drop stale cached copies of the other workgroups' rows
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         *ok_lds = ok;
     }
-    gen += 1;
     __syncthreads();
     return *ok_lds != 0;
+}
+
+// Both halves back to back.
+__device__ __forceinline__ bool team_barrier(GridBarrier *gb, uint32_t members, uint32_t member,
+                                             uint32_t &gen, int *ok_lds, uint64_t wait_ticks) {
+    team_arrive(gb, members, member, gen);
+    return team_wait(gb, gen, ok_lds, wait_ticks);
 }
 
 }  // namespace ct
