@@ -163,8 +163,7 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
                                              uint32_t member, int32_t t_begin, int32_t t_end,
                                              float *lds, uint64_t first_wait, uint64_t step_wait) {
     float *ys = lds;                                                       // [voxel][78]
-    uint32_t *mask = reinterpret_cast<uint32_t *>(lds + kResMaxVoxels * 78);  // [voxel]
-    int *ok_lds = reinterpret_cast<int *>(mask + kResMaxVoxels);           // [4]
+    int *ok_lds = reinterpret_cast<int *>(lds + kResMaxVoxels * 78);     // [4]
     uint8_t *dtab = reinterpret_cast<uint8_t *>(ok_lds + 4);                // [sorted position]
     uint64_t *keys = reinterpret_cast<uint64_t *>(lds);  // set-up only, over ys
     const int32_t N = j.N, T = j.T;
@@ -184,7 +183,6 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
     // then read one state row at nearby voxels, about 29 cache lines per wave
     // instead of about 61 (one per lane). Key = D << 40 | w << 14 | local
     // connection; a missing connection sorts last.
-    for (int k = tid; k < kResMaxVoxels; k += kResThreads) mask[k] = 0u;
     if (tid == 0) ok_lds[1] = 0;
     __syncthreads();
     for (int p = tid; p < kSortSize; p += kResThreads) {
@@ -239,7 +237,6 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
             }
             if (w >= 0) {
                 ak[k] = j.coefs[int64_t(vb) * 26 + cl];
-                atomicOr(&mask[cl / 26], 1u << (cl % 26));
                 if (D < t_begin) {
                     const int64_t r1 = int64_t(t_begin - D - 1) * n + 3LL * w;
 #pragma unroll
@@ -297,31 +294,29 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
             }
         }
         __syncthreads();
+        // Phase 2 (a lane per state): x_t[i] = sum over the 78 (connection,
+        // output) pairs in order of G[i][.] * y, then + u_t b[i]. A missing
+        // connection's y is +0 (reset per beat, never written) and its term a
+        // signed zero, which leaves the sum (started at +0) unchanged: the
+        // sum over all 78 equals the reference's sum over the existing ones
+        // bit for bit whenever the missing connections' gains are finite.
         const float ut = u[t];
 #pragma unroll 1
         for (int k = 0; k < kResStates; ++k) {
             const int sidx = tid + k * kResThreads;
             if (sidx >= nv * 3) break;
-            const int vl = sidx / 3;
-            const int64_t i = 3LL * vb + sidx;
-            const uint32_t m = mask[vl];
+            const float *yv = ys + (sidx / 3) * 78;
+            const uint32_t i = uint32_t(3 * vb + sidx);
             float acc = 0.0f;
-#pragma unroll 1
-            for (int h = 0; h < (26 + kGainConns - 1) / kGainConns; ++h) {
+#pragma unroll
+            for (int h = 0; h < 78; h += 3 * kGainConns) {
                 float g[3 * kGainConns];
 #pragma unroll
                 for (int q = 0; q < 3 * kGainConns; ++q)
-                    if (3 * kGainConns * h + q < 78)
-                        g[q] = gT[uint32_t(3 * kGainConns * h + q) * uint32_t(n) + uint32_t(i)];
+                    if (h + q < 78) g[q] = (gT + size_t(h + q) * size_t(n))[i];  // row h+q, uniform
 #pragma unroll
-                for (int dd = 0; dd < kGainConns; ++dd) {
-                    const int d = kGainConns * h + dd;
-                    if (d >= 26) break;
-                    if (!((m >> d) & 1u)) continue;
-#pragma unroll
-                    for (int o = 0; o < 3; ++o)
-                        acc = acc + g[3 * dd + o] * ys[(vl * 26 + d) * 3 + o];
-                }
+                for (int q = 0; q < 3 * kGainConns; ++q)
+                    if (h + q < 78) acc = acc + g[q] * yv[h + q];
             }
             x[int64_t(t) * n + i] = acc + ut * b[i];
         }
@@ -1035,7 +1030,7 @@ hipError_t launch_propagate_step(hipStream_t s, const int32_t *nbr, const float 
 }
 
 size_t resident_lds_bytes() {
-    return sizeof(float) * kResMaxVoxels * 78 + 4 * kResMaxVoxels + 16 + kResMaxVoxels * 26;
+    return sizeof(float) * kResMaxVoxels * 78 + 16 + kResMaxVoxels * 26;
 }
 
 hipError_t resident_capacity(int32_t *blocks_per_cu) {
