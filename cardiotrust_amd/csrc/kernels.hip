@@ -144,6 +144,10 @@ constexpr int kResStates = (kResMaxVoxels * 3 + kResThreads - 1) / kResThreads;
 // 3 / thirds 72.0, 4 / thirds 72.6, 2 / halves 72.9, 2 / sixths 77.8.
 constexpr int kResBatch = 2;
 constexpr int kResGainConns = 9;
+// Gain rows of each lane's first state held in registers for the launch: one
+// group of 27 (68.2-68.9 ms per 2000 steps against 70.4-71.1 without; two
+// groups exceed the register file).
+constexpr int kResCachedRows = 27;
 // The batched kernel also holds its job's fields: its own measured values
 // (ms per epoch of the 64-scenario uneven batch): 3 gathers / halves 1751,
 // 2 / halves 1775, 1 / halves 1984, 2 / thirds 1830, 1 / thirds 1893.
@@ -158,7 +162,7 @@ static_assert(kSortSize * 8 <= kResMaxVoxels * 78 * 4, "sort keys must fit over 
 // the job's team. The first barrier waits up to `first_wait` ticks (team
 // members may still be finishing earlier jobs), the others `step_wait`.
 // Returns false if the job's barrier aborted.
-template <int kGather, int kGainConns>
+template <int kGather, int kGainConns, int kCachedRows>
 __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, int32_t ve,
                                              uint32_t member, int32_t t_begin, int32_t t_end,
                                              float *lds, uint64_t first_wait, uint64_t step_wait) {
@@ -245,6 +249,12 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
             }
         }
     }
+    // The first kCachedRows gains of the lane's first state stay in registers
+    // for the whole launch (gains do not change within a beat).
+    float gc[kCachedRows > 0 ? kCachedRows : 1];
+#pragma unroll
+    for (int q = 0; q < kCachedRows; ++q)
+        gc[q] = tid < nv * 3 ? gT[int64_t(q) * n + 3LL * vb + tid] : 0.0f;
     __syncthreads();  // the keys are dead: ys is loaded over them
     for (int idx = tid; idx < nv * 78; idx += kResThreads) ys[idx] = j.y[int64_t(vb) * 78 + idx];
     __syncthreads();
@@ -311,9 +321,15 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
 #pragma unroll
             for (int h = 0; h < 78; h += 3 * kGainConns) {
                 float g[3 * kGainConns];
+                if (h + 3 * kGainConns <= kCachedRows && k == 0) {
 #pragma unroll
-                for (int q = 0; q < 3 * kGainConns; ++q)
-                    if (h + q < 78) g[q] = (gT + size_t(h + q) * size_t(n))[i];  // row h+q, uniform
+                    for (int q = 0; q < 3 * kGainConns; ++q)
+                        g[q] = gc[h + q < kCachedRows ? h + q : 0];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 3 * kGainConns; ++q)
+                        if (h + q < 78) g[q] = (gT + size_t(h + q) * size_t(n))[i];  // row h+q, uniform
+                }
 #pragma unroll
                 for (int q = 0; q < 3 * kGainConns; ++q)
                     if (h + q < 78) acc = acc + g[q] * yv[h + q];
@@ -341,7 +357,8 @@ __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
     const int64_t slab = xcd_block(blockIdx.x, gridDim.x);
     const int32_t vb = int32_t(slab * job.N / gridDim.x);
     const int32_t ve = int32_t((slab + 1) * job.N / gridDim.x);
-    resident_run<kResBatch, kResGainConns>(job, vb, ve, blockIdx.x, t_begin, t_end, lds,
+    resident_run<kResBatch, kResGainConns, kResCachedRows>(job, vb, ve, blockIdx.x, t_begin, t_end,
+                                                           lds,
                                            wait_ticks, wait_ticks);
 }
 
@@ -363,7 +380,7 @@ __global__ __launch_bounds__(kResThreads) void propagate_batch_kernel(
         const int32_t member = slot - job.slot0;
         const int32_t vb = int32_t(int64_t(member) * job.N / job.width);
         const int32_t ve = int32_t(int64_t(member + 1) * job.N / job.width);
-        if (!resident_run<kBatchGather, kBatchGainConns>(job, vb, ve, uint32_t(member), 0, job.T,
+        if (!resident_run<kBatchGather, kBatchGainConns, 0>(job, vb, ve, uint32_t(member), 0, job.T,
                                                          lds, first_wait, step_wait))
             return;
         __syncthreads();  // LDS is reused by the next job
