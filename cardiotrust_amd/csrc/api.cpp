@@ -116,6 +116,7 @@ ct::ResidentJob resident_job(ct_results_gpu &r, int32_t ckpt_interval) {
     j.u = r.u;
     j.ckpt = r.ckpt;
     j.gb = r.barrier;
+    j.flags = r.resident_blocks <= ct::kMaxSlabFlags ? r.flags : nullptr;
     j.N = r.N;
     j.T = r.T;
     j.ckpt_interval = ckpt_interval;
@@ -731,6 +732,7 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
     ALLOC(r->dmax, 1);
     ALLOC(r->u, T);
     ALLOC(r->barrier, 1);
+    ALLOC(r->flags, ct::kMaxSlabFlags);
     ALLOC(r->scal, std::max<int64_t>(B, 1));
 #undef ALLOC
     if (hipHostMalloc(reinterpret_cast<void **>(&r->scal_host),
@@ -754,6 +756,7 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
     for (float *p : {r->loss_b, r->loss_mse_b, r->loss_mr_b})
         if ((rc = zero(p, sizeof(float) * r->nbatch)) != CT_OK) return cleanup(rc);
     if ((rc = zero(r->barrier, sizeof(ct::GridBarrier))) != CT_OK) return cleanup(rc);
+    if ((rc = zero(r->flags, sizeof(uint32_t) * ct::kMaxSlabFlags)) != CT_OK) return cleanup(rc);
     if ((rc = zero(r->y, sizeof(float) * N * 78)) != CT_OK) return cleanup(rc);
     if ((rc = zero(r->y_last, sizeof(float) * N * 78)) != CT_OK) return cleanup(rc);
     if ((rc = zero(r->avg, sizeof(float) * N)) != CT_OK) return cleanup(rc);

@@ -73,6 +73,59 @@ __device__ __forceinline__ bool team_wait(GridBarrier *gb, uint32_t gen, int *ok
     return *ok_lds != 0;
 }
 
+// Slab flags: in a single-job resident launch, flags[s] counts the steps
+// slab s has completed. A slab waits only for the slabs that own its
+// neighbours ([smin, smax] but itself), not for the whole grid.
+// Publishing follows the slab's writes of the step (after __syncthreads).
+__device__ __forceinline__ void slab_publish(uint32_t *flags, int32_t me, uint32_t steps) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // release: this slab's state rows reach device-coherent memory
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(&flags[me], steps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Waits (at most `wait_ticks` of wall clock) until every slab in [smin, smax]
+// but `me` has completed `steps` steps; wave 0 polls 64 flags per round. False
+// if the job aborted or the wait timed out (which raises the sticky abort).
+__device__ __forceinline__ bool slab_wait(const uint32_t *flags, GridBarrier *gb, int32_t smin,
+                                          int32_t smax, int32_t me, uint32_t steps, int *ok_lds,
+                                          uint64_t wait_ticks) {
+    if (threadIdx.x < 64) {
+        int ok = 1;
+        const uint64_t t0 = uint64_t(wall_clock64());
+        uint32_t polls = 0;
+        for (;;) {
+            int done = 1;
+            for (int32_t s = smin + int32_t(threadIdx.x); s <= smax; s += 64)
+                if (s != me &&
+                    __hip_atomic_load(&flags[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < steps)
+                    done = 0;
+            if (__all(done)) break;
+            const uint32_t ab = __builtin_amdgcn_readfirstlane(
+                __hip_atomic_load(&gb->abort.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            if (ab) {
+                ok = 0;
+                break;
+            }
+            if ((++polls & 63u) == 0 && uint64_t(wall_clock64()) - t0 > wait_ticks) {
+                if (threadIdx.x == 0)
+                    __hip_atomic_store(&gb->abort.count, 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        // acquire: drop stale cached copies of the other slabs' rows
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (threadIdx.x == 0) *ok_lds = ok;
+    }
+    __syncthreads();
+    return *ok_lds != 0;
+}
+
 // Both halves back to back.
 __device__ __forceinline__ bool team_barrier(GridBarrier *gb, uint32_t members, uint32_t member,
                                              uint32_t &gen, int *ok_lds, uint64_t wait_ticks) {

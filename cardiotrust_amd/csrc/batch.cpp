@@ -219,6 +219,7 @@ int plan(ct_batch &b, int32_t vpg) {
         for (int32_t s : R.resident) {
             ct_results_gpu &r = *b.kernels[s]->r;
             jobs.push_back(resident_job(r, b.plan_checkpoints[s]));
+            jobs.back().flags = nullptr;  // batched teams meet at their barrier
         }
         std::vector<std::vector<int32_t>> tasks;
         R.predicted_ms = rs.empty() ? 0.0 : place_round(rs, b.slots, vpg, jobs, tasks);

@@ -61,13 +61,16 @@ struct ResidentJob {
     const float *b;
     const float *u;
     float *ckpt;          // filter outputs every ckpt_interval steps (or unused)
-    GridBarrier *gb;
+    GridBarrier *gb;        // abort word; the team barrier when flags is null
+    uint32_t *flags;        // per-slab completed steps (single-job launches), or null
     int32_t N;
     int32_t T;
     int32_t ckpt_interval;  // 0 = no checkpoints
     int32_t slot0;
     int32_t width;          // workgroups in the team
 };
+// Most slabs a single-job resident launch can synchronise with flags.
+constexpr int kMaxSlabFlags = 4096;
 size_t resident_lds_bytes();
 hipError_t resident_capacity(int32_t *blocks_per_cu);
 hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const ResidentJob &job,

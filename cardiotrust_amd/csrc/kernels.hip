@@ -187,7 +187,11 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
     // then read one state row at nearby voxels, about 29 cache lines per wave
     // instead of about 61 (one per lane). Key = D << 40 | w << 14 | local
     // connection; a missing connection sorts last.
-    if (tid == 0) ok_lds[1] = 0;
+    if (tid == 0) {
+        ok_lds[1] = 0;        // largest delay
+        ok_lds[2] = N;        // smallest neighbour
+        ok_lds[3] = -1;       // largest neighbour
+    }
     __syncthreads();
     for (int p = tid; p < kSortSize; p += kResThreads) {
         uint64_t key = ~0ull;
@@ -198,12 +202,21 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
                 const int32_t D = min(j.delays[c], 32767);
                 key = (uint64_t(D) << 40) | (uint64_t(w) << 14) | uint64_t(p);
                 atomicMax(&ok_lds[1], D);
+                atomicMin(&ok_lds[2], w);
+                atomicMax(&ok_lds[3], w);
             }
         }
         keys[p] = key;
     }
     __syncthreads();
     const bool sorted = ok_lds[1] <= kSortedMaxDelay;
+    // With slab flags, the slabs this one waits for: those owning its
+    // neighbours (slab s holds voxels [s N / G, (s+1) N / G)).
+    const int32_t G = j.width;
+    const auto slab_of = [N, G](int32_t w) { return int32_t((int64_t(w + 1) * G - 1) / N); };
+    const int32_t me = slab_of(vb);
+    const int32_t smin = ok_lds[3] >= 0 ? slab_of(ok_lds[2]) : 1;
+    const int32_t smax = ok_lds[3] >= 0 ? slab_of(ok_lds[3]) : 0;
     if (sorted) sort_keys_lds<kResThreads, kSortSize>(keys);
     // pk = neighbour (17 bits) | delay (15 bits) in connection order, or
     // neighbour | local connection (14 bits) in sorted order with the delay in
@@ -261,6 +274,9 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
 
     for (int32_t t = t_begin; t < t_end; ++t) {
         const bool last = (t == T - 1);
+        if (j.flags && t > t_begin &&
+            !slab_wait(j.flags, j.gb, smin, smax, me, uint32_t(t - t_begin), ok_lds, step_wait))
+            return false;  // host reports the abort
 #pragma unroll
         for (int k0 = 0; k0 < kResItems; k0 += kGather) {
             float in0[kGather][3];
@@ -342,9 +358,12 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
             float *dst = j.ckpt + int64_t((t + 1) / K - 1) * N * 78 + int64_t(vb) * 78;
             for (int idx = tid; idx < nv * 78; idx += kResThreads) dst[idx] = ys[idx];
         }
-        if (!team_barrier(j.gb, uint32_t(j.width), member, gen, ok_lds,
-                          t == t_begin ? first_wait : step_wait))
+        if (j.flags) {
+            slab_publish(j.flags, me, uint32_t(t + 1 - t_begin));
+        } else if (!team_barrier(j.gb, uint32_t(j.width), member, gen, ok_lds,
+                                 t == t_begin ? first_wait : step_wait)) {
             return false;  // host reports the abort
+        }
     }
     for (int idx = tid; idx < nv * 78; idx += kResThreads) j.y[int64_t(vb) * 78 + idx] = ys[idx];
     return true;
@@ -1030,6 +1049,14 @@ __global__ void barrier_reset_kernel(GridBarrier *gb, int32_t count) {
     reinterpret_cast<BarrierWord *>(gb + j)[w].count = 0u;
 }
 
+// Zeroes one job's barrier counters (not its abort word) and `nflags` slab flags.
+__global__ void resident_reset_kernel(GridBarrier *gb, uint32_t *flags, int32_t nflags) {
+    constexpr int kWords = int(sizeof(GridBarrier) / sizeof(BarrierWord));
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < kWords - 1) reinterpret_cast<BarrierWord *>(gb)[k].count = 0u;
+    if (flags && k < nflags) flags[k] = 0u;
+}
+
 inline unsigned blocks_for(int64_t len, int threads) {
     return unsigned((len + threads - 1) / threads);
 }
@@ -1065,8 +1092,10 @@ hipError_t resident_capacity(int32_t *blocks_per_cu) {
 
 hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const ResidentJob &job,
                                      int32_t t_begin, int32_t t_end, uint64_t wait_ticks) {
-    // the barrier starts from zero: a kernel rather than a memset node
-    barrier_reset_kernel<<<1, 64, 0, s>>>(job.gb, 1);
+    // the barrier and the slab flags start from zero: a kernel rather than a memset node
+    if (job.flags && blocks > kMaxSlabFlags) return hipErrorInvalidValue;
+    resident_reset_kernel<<<blocks_for(blocks > 64 ? blocks : 64, 256), 256, 0, s>>>(
+        job.gb, job.flags, job.flags ? blocks : 0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     propagate_resident_kernel<<<blocks, kResThreads, resident_lds_bytes(), s>>>(job, t_begin, t_end,

@@ -89,6 +89,7 @@ struct ct_results_gpu {
     float *E = nullptr, *xbar = nullptr, *xbarT = nullptr, *ybar = nullptr;
     float *qring = nullptr, *ckpt = nullptr, *yseg = nullptr;
     float *gcons = nullptr;  // (N, 26, 10) consumer-ordered gains and coefficients
+    uint32_t *flags = nullptr;  // kMaxSlabFlags per-slab step counters (resident launches)
     int64_t qring_slots = 0, ckpt_segments = 0, yseg_have = 0;
     int64_t ring_slots_fixed = 0;  // > 0: q-ring size fixed for a captured epoch
     int64_t pending_slots = 0;     // q-ring size of the update batch being differentiated
