@@ -142,6 +142,8 @@ hipError_t launch_adjoint_sweep_resident(hipStream_t s, int32_t blocks, const in
                                          float *xbar, GridBarrier *gb, int32_t N, int32_t T,
                                          const AdjointParams &p, uint64_t wait_ticks);
 hipError_t launch_barrier_reset(hipStream_t s, GridBarrier *gb, int32_t count);
+// One job's barrier counters and its first `nflags` slab flags (flags may be null).
+hipError_t launch_resident_reset(hipStream_t s, GridBarrier *gb, uint32_t *flags, int32_t nflags);
 hipError_t launch_adjoint_gradient(hipStream_t s, const int32_t *nbr, const float *coefs,
                                    const int32_t *delays, const float *gT, const float *xT,
                                    const float *xbarT, const float *ckpt, float *ychunk, float *dGT,

@@ -1093,13 +1093,17 @@ hipError_t resident_capacity(int32_t *blocks_per_cu) {
 hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const ResidentJob &job,
                                      int32_t t_begin, int32_t t_end, uint64_t wait_ticks) {
     // the barrier and the slab flags start from zero: a kernel rather than a memset node
-    if (job.flags && blocks > kMaxSlabFlags) return hipErrorInvalidValue;
-    resident_reset_kernel<<<blocks_for(blocks > 64 ? blocks : 64, 256), 256, 0, s>>>(
-        job.gb, job.flags, job.flags ? blocks : 0);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_resident_reset(s, job.gb, job.flags, blocks);
     if (e != hipSuccess) return e;
     propagate_resident_kernel<<<blocks, kResThreads, resident_lds_bytes(), s>>>(job, t_begin, t_end,
                                                                                 wait_ticks);
+    return hipGetLastError();
+}
+
+hipError_t launch_resident_reset(hipStream_t s, GridBarrier *gb, uint32_t *flags, int32_t nflags) {
+    if (flags && nflags > kMaxSlabFlags) return hipErrorInvalidValue;
+    resident_reset_kernel<<<blocks_for(nflags > 64 ? nflags : 64, 256), 256, 0, s>>>(
+        gb, flags, flags ? nflags : 0);
     return hipGetLastError();
 }
 
