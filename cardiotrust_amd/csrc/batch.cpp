@@ -35,6 +35,7 @@ struct ct_batch {
         std::vector<int32_t> propagate;   // those that propagate in this round
         std::vector<int32_t> resident;    // of those, the ones in the batched launch (job order)
         int32_t job0 = 0;                 // first job / barrier of the round
+        int32_t flag0 = 0, nflags = 0;    // the round's slab flags (one per team member)
         int32_t task0 = 0;                // first task of the round
         int32_t ptr0 = 0;                 // first task_ptr entry (slots + 1 per round)
         double predicted_ms = 0.0;
@@ -46,6 +47,7 @@ struct ct_batch {
     ct::ResidentJob *jobs = nullptr;
     int32_t *task_ptr = nullptr, *task_job = nullptr;
     ct::GridBarrier *barriers = nullptr;
+    uint32_t *flags = nullptr;  // slab flags of every job, job after job
     int32_t num_jobs = 0;
 
     // graph of one epoch
@@ -219,7 +221,6 @@ int plan(ct_batch &b, int32_t vpg) {
         for (int32_t s : R.resident) {
             ct_results_gpu &r = *b.kernels[s]->r;
             jobs.push_back(resident_job(r, b.plan_checkpoints[s]));
-            jobs.back().flags = nullptr;  // batched teams meet at their barrier
         }
         std::vector<std::vector<int32_t>> tasks;
         R.predicted_ms = rs.empty() ? 0.0 : place_round(rs, b.slots, vpg, jobs, tasks);
@@ -242,15 +243,29 @@ int plan(ct_batch &b, int32_t vpg) {
     batch_free(b, b.task_ptr);
     batch_free(b, b.task_job);
     batch_free(b, b.barriers);
+    batch_free(b, b.flags);
     b.jobs = nullptr;
     b.task_ptr = b.task_job = nullptr;
     b.barriers = nullptr;
+    b.flags = nullptr;
     b.num_jobs = int32_t(all_jobs.size());
+    // each job's members publish their completed steps in its own flag range
+    std::vector<int32_t> flag_of(all_jobs.size() + 1, 0);
+    for (size_t j = 0; j < all_jobs.size(); ++j) flag_of[j + 1] = flag_of[j] + all_jobs[j].width;
+    for (ct_batch::Round &R : b.rounds) {
+        const size_t j1 = size_t(R.job0) + R.resident.size();
+        R.flag0 = flag_of[size_t(R.job0)];
+        R.nflags = flag_of[j1] - R.flag0;
+    }
     CT_CHECK(batch_alloc(b, &b.jobs, b.num_jobs));
     CT_CHECK(batch_alloc(b, &b.task_ptr, int64_t(all_ptr.size())));
     CT_CHECK(batch_alloc(b, &b.task_job, int64_t(all_task.size())));
     CT_CHECK(batch_alloc(b, &b.barriers, b.num_jobs));
-    for (size_t j = 0; j < all_jobs.size(); ++j) all_jobs[j].gb = b.barriers + j;
+    CT_CHECK(batch_alloc(b, &b.flags, std::max<int64_t>(flag_of.back(), 1)));
+    for (size_t j = 0; j < all_jobs.size(); ++j) {
+        all_jobs[j].gb = b.barriers + j;
+        all_jobs[j].flags = b.flags + flag_of[j];
+    }
     CT_HIP(hipMemset(b.barriers, 0, sizeof(ct::GridBarrier) * std::max(b.num_jobs, 1)));
     CT_CHECK(upload(b.jobs, all_jobs.data(), sizeof(ct::ResidentJob) * all_jobs.size()));
     CT_CHECK(upload(b.task_ptr, all_ptr.data(), sizeof(int32_t) * all_ptr.size()));
@@ -294,8 +309,8 @@ int propagate_round(ct_batch &b, int32_t rd, bool timed) {
     }
     CT_HIP(ct::launch_propagate_batch(s, b.slots, b.jobs + R.job0, b.task_ptr + R.ptr0,
                                       b.task_job, b.barriers + R.job0,
-                                      int32_t(R.resident.size()), wait_ticks(*b.gpu, 60.0),
-                                      wait_ticks(*b.gpu, 2.0)));
+                                      int32_t(R.resident.size()), b.flags + R.flag0, R.nflags,
+                                      wait_ticks(*b.gpu, 60.0), wait_ticks(*b.gpu, 2.0)));
     if (timed) CT_HIP(hipEventRecord(e1, s));
     return CT_OK;
 }

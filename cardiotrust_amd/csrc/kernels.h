@@ -62,7 +62,7 @@ struct ResidentJob {
     const float *u;
     float *ckpt;          // filter outputs every ckpt_interval steps (or unused)
     GridBarrier *gb;        // abort word; the team barrier when flags is null
-    uint32_t *flags;        // per-slab completed steps (single-job launches), or null
+    uint32_t *flags;        // completed steps per team member (width words), or null
     int32_t N;
     int32_t T;
     int32_t ckpt_interval;  // 0 = no checkpoints
@@ -81,8 +81,8 @@ hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const Reside
 hipError_t batch_capacity(int32_t *blocks_per_cu);
 hipError_t launch_propagate_batch(hipStream_t s, int32_t slots, const ResidentJob *jobs,
                                   const int32_t *task_ptr, const int32_t *task_job,
-                                  GridBarrier *barriers, int32_t num_jobs, uint64_t first_wait,
-                                  uint64_t step_wait);
+                                  GridBarrier *barriers, int32_t num_jobs, uint32_t *flags,
+                                  int32_t num_flags, uint64_t first_wait, uint64_t step_wait);
 
 hipError_t launch_project(hipStream_t s, const float *x, const float *H, float *partial,
                           int32_t T, int64_t n, int32_t Sp, int32_t splits, int64_t k_per_split);

@@ -275,7 +275,8 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
     for (int32_t t = t_begin; t < t_end; ++t) {
         const bool last = (t == T - 1);
         if (j.flags && t > t_begin &&
-            !slab_wait(j.flags, j.gb, smin, smax, me, uint32_t(t - t_begin), ok_lds, step_wait))
+            !slab_wait(j.flags, j.gb, smin, smax, me, uint32_t(t - t_begin), ok_lds,
+                       t == t_begin + 1 ? first_wait : step_wait))
             return false;  // host reports the abort
 #pragma unroll
         for (int k0 = 0; k0 < kResItems; k0 += kGather) {
@@ -1049,6 +1050,11 @@ __global__ void barrier_reset_kernel(GridBarrier *gb, int32_t count) {
     reinterpret_cast<BarrierWord *>(gb + j)[w].count = 0u;
 }
 
+__global__ void fill_words_kernel(uint32_t *p, uint32_t v, int64_t len) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < len) p[k] = v;
+}
+
 // Zeroes one job's barrier counters (not its abort word) and `nflags` slab flags.
 __global__ void resident_reset_kernel(GridBarrier *gb, uint32_t *flags, int32_t nflags) {
     constexpr int kWords = int(sizeof(GridBarrier) / sizeof(BarrierWord));
@@ -1134,12 +1140,17 @@ hipError_t batch_capacity(int32_t *blocks_per_cu) {
 
 hipError_t launch_propagate_batch(hipStream_t s, int32_t slots, const ResidentJob *jobs,
                                   const int32_t *task_ptr, const int32_t *task_job,
-                                  GridBarrier *barriers, int32_t num_jobs, uint64_t first_wait,
-                                  uint64_t step_wait) {
+                                  GridBarrier *barriers, int32_t num_jobs, uint32_t *flags,
+                                  int32_t num_flags, uint64_t first_wait, uint64_t step_wait) {
     if (slots <= 0 || num_jobs <= 0) return hipSuccess;
     barrier_reset_kernel<<<blocks_for(int64_t(num_jobs) * 10, 256), 256, 0, s>>>(barriers, num_jobs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (flags && num_flags > 0) {
+        fill_words_kernel<<<blocks_for(num_flags, 256), 256, 0, s>>>(flags, 0u, num_flags);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     propagate_batch_kernel<<<slots, kResThreads, resident_lds_bytes(), s>>>(jobs, task_ptr, task_job,
                                                                             first_wait, step_wait);
     return hipGetLastError();
