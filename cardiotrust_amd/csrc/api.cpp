@@ -863,6 +863,26 @@ int ct_results_simulate(ct_results_gpu *results, float *measurements_out) {
     return CT_OK;
 }
 
+int ct_results_simulate_noisy(ct_results_gpu *results, const float *sensor_std, uint64_t seed,
+                              float *measurements_out) {
+    if (!results || !sensor_std) return fail(CT_ERR_INVALID, "null argument");
+    ct_results_gpu &r = *results;
+    CT_CHECK(check_gpu(r.gpu));
+    // Normal::new rejects a negative or non-finite standard deviation
+    for (int32_t s = 0; s < r.S; ++s)
+        if (!(sensor_std[s] >= 0.0f) || !std::isfinite(sensor_std[s]))
+            return fail(CT_ERR_INVALID, "measurement noise standard deviation of sensor " +
+                                            std::to_string(s) + " is negative or not finite");
+    CT_CHECK(ensure(r, &r.sensor_std, r.sensor_std_have, std::max<int64_t>(r.S, 1), 1));
+    CT_CHECK(upload(r.sensor_std, sensor_std, sizeof(float) * r.S));
+    CT_CHECK(ct_results_simulate(results, nullptr));
+    CT_HIP(ct::launch_measurement_noise(r.gpu->stream, r.m, r.sensor_std, r.B, r.T, r.S, seed));
+    CT_HIP(hipStreamSynchronize(r.gpu->stream));
+    if (measurements_out)
+        CT_CHECK(download(measurements_out, r.m, sizeof(float) * r.B * r.T * r.S));
+    return CT_OK;
+}
+
 int ct_epoch_kernel_new(ct_gpu *gpu, ct_results_gpu *results, const ct_buffer *actual,
                         const ct_algorithm *config, int32_t number_of_states,
                         int32_t number_of_sensors, int32_t number_of_steps, ct_epoch_kernel **out) {

@@ -177,6 +177,10 @@ hipError_t launch_expand_rows(hipStream_t s, const float *in, float *out, int64_
 
 hipError_t launch_fill(hipStream_t s, float *p, float value, int64_t len);
 hipError_t launch_add(hipStream_t s, float *a, const float *b, int64_t len);  // a += b
+// Measurement noise: m[b][t][s] += sd[s] * z_k with k = (s B + b) T + t and
+// z_k the k-th standard normal of the counter-based generator keyed by `seed`.
+hipError_t launch_measurement_noise(hipStream_t s, float *m, const float *sd, int32_t B,
+                                    int32_t T, int32_t S, uint64_t seed);
 
 // FP64 objective for gradient checking (gradcheck.hip). kind 0 = none,
 // 1 = gains[index] in the (3N, 78) row-major layout, 2 = coefs[index] (N, 26).

@@ -1050,6 +1050,39 @@ __global__ void barrier_reset_kernel(GridBarrier *gb, int32_t count) {
     reinterpret_cast<BarrierWord *>(gb + j)[w].count = 0u;
 }
 
+// splitmix64 finaliser (Steele, Lea and Flood 2014).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Uniform in [0, 1) from the top 53 bits of splitmix64(counter ^ key).
+__device__ __forceinline__ double counter_uniform(uint64_t counter, uint64_t key) {
+    return double(splitmix64(counter ^ key) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Measurement noise (the noise step of data/simulation.rs:152-169, with this
+// library's generator): element (b, t, s) of m (B, T, S) gets
+// sd[s] * z_k, k = (s B + b) T + t -- the reference's sensor, beat, step
+// sampling order -- where z_k = sqrt(-2 ln(1 - U(2k))) cos(2 pi U(2k+1)) is the
+// Box-Muller deviate of two counter uniforms. The sum is taken in FP32 after
+// rounding sd[s] * z_k (FP64) to FP32.
+__global__ void measurement_noise_kernel(float *__restrict__ m, const float *__restrict__ sd,
+                                         int32_t B, int32_t T, int32_t S, uint64_t key) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= int64_t(B) * T * S) return;
+    const int32_t s = int32_t(e % S);
+    const int32_t t = int32_t((e / S) % T);
+    const int32_t b = int32_t(e / (int64_t(S) * T));
+    const uint64_t k = (uint64_t(s) * uint64_t(B) + uint64_t(b)) * uint64_t(T) + uint64_t(t);
+    const double u1 = 1.0 - counter_uniform(2 * k, key);
+    const double u2 = counter_uniform(2 * k + 1, key);
+    const double z = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+    m[e] = m[e] + float(double(sd[s]) * z);
+}
+
 __global__ void fill_words_kernel(uint32_t *p, uint32_t v, int64_t len) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k < len) p[k] = v;
@@ -1061,6 +1094,13 @@ __global__ void resident_reset_kernel(GridBarrier *gb, uint32_t *flags, int32_t 
     const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < kWords - 1) reinterpret_cast<BarrierWord *>(gb)[k].count = 0u;
     if (flags && k < nflags) flags[k] = 0u;
+}
+
+inline uint64_t splitmix64_host(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
 }
 
 inline unsigned blocks_for(int64_t len, int threads) {
@@ -1110,6 +1150,16 @@ hipError_t launch_resident_reset(hipStream_t s, GridBarrier *gb, uint32_t *flags
     if (flags && nflags > kMaxSlabFlags) return hipErrorInvalidValue;
     resident_reset_kernel<<<blocks_for(nflags > 64 ? nflags : 64, 256), 256, 0, s>>>(
         gb, flags, flags ? nflags : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_measurement_noise(hipStream_t s, float *m, const float *sd, int32_t B,
+                                    int32_t T, int32_t S, uint64_t seed) {
+    const int64_t len = int64_t(B) * T * S;
+    if (len <= 0) return hipSuccess;
+    // key of stream 1000 of `seed`, as the host-side generator derives it
+    const uint64_t key = splitmix64_host((seed << 20) ^ 1000u);
+    measurement_noise_kernel<<<blocks_for(len, 256), 256, 0, s>>>(m, sd, B, T, S, key);
     return hipGetLastError();
 }
 

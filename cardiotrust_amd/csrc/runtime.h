@@ -68,6 +68,8 @@ struct ct_results_gpu {
     float *loss = nullptr, *loss_mse = nullptr, *loss_mr = nullptr;
     float *loss_b = nullptr, *loss_mse_b = nullptr, *loss_mr_b = nullptr;
     float *u = nullptr;  // control function on the device (resident propagation)
+    float *sensor_std = nullptr;  // (S) measurement noise standard deviations
+    int64_t sensor_std_have = 0;
     // propagation path: resident (one launch per beat) when the
     // problem fits on chip, else one launch per step
     int32_t propagation_mode = 0;  // 0 auto, 1 per-step launches, 2 resident

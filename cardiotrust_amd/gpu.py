@@ -23,7 +23,8 @@ LIB_PATH = os.path.join(_HERE, "libcardiotrust_hip.so")
 EXPORTS = (
     "ct_last_error", "ct_gpu_new", "ct_gpu_free", "ct_gpu_synchronize",
     "ct_measurements_to_gpu", "ct_buffer_free", "ct_results_to_gpu", "ct_results_gpu_free",
-    "ct_results_update_from_gpu", "ct_results_simulate", "ct_results_set_propagation",
+    "ct_results_update_from_gpu", "ct_results_simulate", "ct_results_simulate_noisy",
+    "ct_results_set_propagation",
     "ct_results_forward", "ct_results_backward", "ct_epoch_kernel_set_gradient",
     "ct_epoch_kernel_new",
     "ct_epoch_kernel_free", "ct_epoch_kernel_execute", "ct_epoch_kernel_set_freeze_delays",
@@ -116,6 +117,7 @@ def lib() -> ctypes.CDLL:
         "ct_results_gpu_free": ([vp], None),
         "ct_results_update_from_gpu": ([vp, P(_Host)], ctypes.c_int),
         "ct_results_simulate": ([vp, vp], ctypes.c_int),
+        "ct_results_simulate_noisy": ([vp, vp, ctypes.c_uint64, vp], ctypes.c_int),
         "ct_results_set_propagation": ([vp, i32, P(i32)], ctypes.c_int),
         "ct_results_forward": ([vp, vp, i32, P(CAlgorithm), i32, P(ctypes.c_double)],
                                ctypes.c_int),
@@ -304,10 +306,19 @@ class ResultsGPU:
                "backward")
         return self.update_from_gpu(["d_gains", "d_coefs"])
 
-    def simulate(self) -> np.ndarray:
-        """Noise-free forward simulation of every beat; returns (B, T, S)."""
+    def simulate(self, noise_std=None, seed: int = 0) -> np.ndarray:
+        """Forward simulation of every beat; returns (B, T, S). With
+        `noise_std` (S standard deviations) the measurements also get the
+        library's seeded Gaussian noise (ct_results_simulate_noisy)."""
         out = np.zeros((self.B, self.T, self.S), dtype=np.float32)
-        _check(lib().ct_results_simulate(self.handle, _ptr(out)), "simulate")
+        if noise_std is None:
+            _check(lib().ct_results_simulate(self.handle, _ptr(out)), "simulate")
+        else:
+            sd = np.ascontiguousarray(noise_std, dtype=np.float32)
+            if sd.shape != (self.S,):
+                raise ValueError(f"noise_std must have shape ({self.S},)")
+            _check(lib().ct_results_simulate_noisy(self.handle, _ptr(sd), ctypes.c_uint64(seed),
+                                                   _ptr(out)), "simulate_noisy")
         return out
 
     def close(self) -> None:

@@ -68,6 +68,25 @@ def normal(seed: int, stream: int, count: int) -> np.ndarray:
     u2 = u[1::2]
     return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
 
+def measurement_noise(seed: int, sensor_std: np.ndarray, num_beats: int,
+                      num_steps: int) -> np.ndarray:
+    """(B, T, S) float32 noise of ct_results_simulate_noisy for `seed`: element
+    (b, t, s) is float32(sensor_std[s] * z_k), k = (s B + b) T + t (the
+    reference's sensor, beat, step order, data/simulation.rs:152-169), z_k
+    the Box-Muller deviate of counter uniforms 2k and 2k+1 of stream 1000 of
+    `seed` (key taken modulo 2^64)."""
+    sd = np.asarray(sensor_std, dtype=np.float32)
+    S, B, T = sd.shape[0], int(num_beats), int(num_steps)
+    key = splitmix64(np.array([((int(seed) << 20) ^ 1000) & 0xFFFFFFFFFFFFFFFF],
+                              dtype=np.uint64))[0]
+    ctr = np.arange(0, 2 * S * B * T, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        bits = splitmix64(ctr ^ key)
+    u = (bits >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    z = np.sqrt(-2.0 * np.log(1.0 - u[0::2])) * np.cos(2.0 * np.pi * u[1::2])
+    noise = (sd.astype(np.float64)[:, None, None] * z.reshape(S, B, T)).astype(np.float32)
+    return np.ascontiguousarray(noise.transpose(1, 2, 0))
+
 @dataclass
 class Geometry:
     """Active voxels of a regular grid, numbered in C order."""

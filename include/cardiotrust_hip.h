@@ -148,11 +148,26 @@ int ct_results_update_from_gpu(ct_results_gpu *results, const ct_results_host *o
  * and, when `measurements_out` is not NULL, copies them there (B,T,S). */
 int ct_results_simulate(ct_results_gpu *results, float *measurements_out);
 
+/* Simulation::run with its measurement noise (src/core/data/simulation.rs:
+ * 129-173): ct_results_simulate, then m[b][t][s] += sensor_std[s] * z with z
+ * standard normal, drawn in the reference's order (sensor, then beat, then
+ * step). The reference passes the covariance diagonal entry of each sensor as
+ * the standard deviation (simulation.rs:156-160); pass the same S values.
+ * CT_ERR_INVALID for a negative or non-finite value, as Normal::new fails.
+ * The reference draws z from ChaCha8Rng::seed_from_u64(42) through rand_distr;
+ * this library draws it from its own counter-based generator keyed by `seed`
+ * (splitmix64 counters, Box-Muller), so the noise is reproducible for a seed
+ * but is not the reference's stream. The device measurements keep the noise;
+ * `measurements_out` (may be NULL) receives them (B,T,S). */
+int ct_results_simulate_noisy(ct_results_gpu *results, const float *sensor_std, uint64_t seed,
+                              float *measurements_out);
+
 /* Propagation path (no reference counterpart; a tuning switch of this
  * library): 0 = automatic (resident when the problem fits on chip), 1 = one
  * launch per time step, 2 = resident (one launch per beat whose workgroups
- * all stay resident, keep the filter outputs in LDS and meet at a grid
- * barrier after every step; CT_ERR_INVALID if the problem does not fit).
+ * all stay resident, keep the filter outputs in LDS and wait after every step
+ * for the workgroups owning their neighbours; CT_ERR_INVALID if the problem
+ * does not fit).
  * `resident` (may be NULL) receives 1 if the resident path will run. */
 int ct_results_set_propagation(ct_results_gpu *results, int32_t mode, int32_t *resident);
 

@@ -271,3 +271,28 @@ def test_invalid_inputs_are_rejected(gpu):
     k.close()
     buf.close()
     r.close()
+
+
+def test_simulate_with_measurement_noise(gpu):
+    """ct_results_simulate_noisy = the noise-free simulation plus the seeded
+    noise that synth.measurement_noise mirrors on the host (FP64 Box-Muller
+    on both sides: equal up to one FP32 ulp where the two libm differ)."""
+    from cardiotrust_amd.gpu import CTError, ResultsGPU
+    note("case: simulate blob seed 4, 2 beats, with per-sensor noise, seed 99")
+    p = blob_problem(4, T=70, beats=2)
+    r = ResultsGPU(gpu, p)
+    clean = r.simulate()
+    sd = (0.01 * (1.0 + np.arange(p.num_sensors) % 5)).astype(np.float32)
+    noisy = r.simulate(noise_std=sd, seed=99)
+    want = (clean + synth.measurement_noise(99, sd, p.num_beats, p.num_steps)).astype(np.float32)
+    ulp = np.abs(noisy.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+    assert ulp.max() <= 1 and np.mean(ulp == 0) > 0.999
+    again = r.simulate(noise_std=sd, seed=99)
+    assert np.array_equal(again, noisy)
+    assert np.array_equal(r.simulate(noise_std=np.zeros_like(sd), seed=5), clean)
+    for bad in (-1.0, np.nan, np.inf):
+        sd2 = sd.copy()
+        sd2[3] = bad
+        with pytest.raises(CTError):
+            r.simulate(noise_std=sd2, seed=1)
+    r.close()

@@ -94,3 +94,23 @@ def test_permuted_numbering_gives_the_same_measurements():
     a = oracle.OracleResults(p, np.float64).simulate()
     b = oracle.OracleResults(q, np.float64).simulate()
     assert np.allclose(a, b, rtol=1e-12, atol=1e-18)
+
+
+def test_measurement_noise_order_and_statistics():
+    """ct_results_simulate_noisy's generator mirrored on the host: element
+    (b, t, s) is the k-th deviate of stream 1000, k = (s B + b) T + t (the
+    reference's sensor, beat, step sampling order); Box-Muller deviates have
+    mean 0 and variance 1; a zero standard deviation adds exact zeros."""
+    S, B, T = 5, 3, 7
+    sd = np.array([0.5, 1.0, 0.0, 2.0, 0.25], dtype=np.float32)
+    n = synth.measurement_noise(11, sd, B, T)
+    assert n.shape == (B, T, S) and n.dtype == np.float32
+    z = synth.normal(11, 1000, S * B * T)
+    for b, t, s in [(0, 0, 0), (2, 6, 4), (1, 3, 2), (0, 5, 1)]:
+        k = (s * B + b) * T + t
+        assert n[b, t, s] == np.float32(np.float64(sd[s]) * z[k])
+    assert np.all(n[:, :, 2] == 0.0)
+    assert np.array_equal(n, synth.measurement_noise(11, sd, B, T))
+    assert not np.array_equal(n, synth.measurement_noise(12, sd, B, T))
+    big = synth.measurement_noise(3, np.ones(4, np.float32), 5, 50_000).astype(np.float64)
+    assert abs(big.mean()) < 5e-3 and abs(big.std() - 1.0) < 5e-3
