@@ -863,6 +863,54 @@ int ct_results_simulate(ct_results_gpu *results, float *measurements_out) {
     return CT_OK;
 }
 
+int ct_measurement_matrix(ct_gpu *gpu, int32_t num_voxels, const float *voxel_positions_mm,
+                          int32_t num_sensors, const float *sensor_positions_mm,
+                          const float *sensor_orientations, int32_t num_beats,
+                          const float *sensor_offsets_mm, float voxel_size_mm, float *out) {
+    CT_CHECK(check_gpu(gpu));
+    if (num_voxels < 0 || num_sensors < 0 || num_beats < 0)
+        return fail(CT_ERR_INVALID, "negative voxel, sensor or beat count");
+    if (!out || (num_voxels > 0 && !voxel_positions_mm) ||
+        (num_sensors > 0 && (!sensor_positions_mm || !sensor_orientations)) ||
+        (num_beats > 0 && !sensor_offsets_mm))
+        return fail(CT_ERR_INVALID, "null array");
+    if (!(voxel_size_mm > 0.0f) || !std::isfinite(voxel_size_mm))
+        return fail(CT_ERR_INVALID, "voxel size must be positive and finite");
+    const int64_t len = int64_t(num_beats) * num_sensors * 3 * num_voxels;
+    if (len == 0) return CT_OK;
+    // the reference's common factor, in its f32 operations
+    const float pi = 3.14159265358979323846f;
+    const float side = voxel_size_mm / 1000.0f;
+    const float volume = side * (side * side);
+    const float common = (float(1.25663706212e-6) * volume) / (4.0f * pi) * 1e12f;
+    hipStream_t s = gpu->stream;
+    float *dev = nullptr;
+    const size_t in_floats = 3 * (size_t(num_voxels) + 2 * size_t(num_sensors) + size_t(num_beats));
+    CT_HIP(hipMalloc(&dev, sizeof(float) * (in_floats + size_t(len))));
+    float *vp = dev, *sp = vp + 3 * size_t(num_voxels), *so = sp + 3 * size_t(num_sensors);
+    float *off = so + 3 * size_t(num_sensors), *h = off + 3 * size_t(num_beats);
+    int rc = CT_OK;
+    auto step = [&](hipError_t e) {
+        if (rc == CT_OK && e != hipSuccess)
+            rc = fail(CT_ERR_DEVICE, std::string("measurement matrix: ") + hipGetErrorString(e));
+    };
+    step(hipMemcpyAsync(vp, voxel_positions_mm, sizeof(float) * 3 * num_voxels,
+                        hipMemcpyHostToDevice, s));
+    step(hipMemcpyAsync(sp, sensor_positions_mm, sizeof(float) * 3 * num_sensors,
+                        hipMemcpyHostToDevice, s));
+    step(hipMemcpyAsync(so, sensor_orientations, sizeof(float) * 3 * num_sensors,
+                        hipMemcpyHostToDevice, s));
+    step(hipMemcpyAsync(off, sensor_offsets_mm, sizeof(float) * 3 * num_beats,
+                        hipMemcpyHostToDevice, s));
+    if (rc == CT_OK)
+        step(ct::launch_measurement_matrix(s, vp, sp, so, off, common, num_voxels, num_sensors,
+                                           num_beats, h));
+    if (rc == CT_OK) step(hipMemcpyAsync(out, h, sizeof(float) * len, hipMemcpyDeviceToHost, s));
+    step(hipStreamSynchronize(s));
+    (void)hipFree(dev);
+    return rc;
+}
+
 int ct_results_simulate_noisy(ct_results_gpu *results, const float *sensor_std, uint64_t seed,
                               float *measurements_out) {
     if (!results || !sensor_std) return fail(CT_ERR_INVALID, "null argument");

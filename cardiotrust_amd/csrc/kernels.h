@@ -177,6 +177,11 @@ hipError_t launch_expand_rows(hipStream_t s, const float *in, float *out, int64_
 
 hipError_t launch_fill(hipStream_t s, float *p, float value, int64_t len);
 hipError_t launch_add(hipStream_t s, float *a, const float *b, int64_t len);  // a += b
+// Biot-Savart matrix (B, S, 3N) from voxel positions (N,3), sensor positions
+// and orientations (S,3), beat offsets (B,3); `common` = mu0 V / (4 pi) 1e12.
+hipError_t launch_measurement_matrix(hipStream_t s, const float *vp, const float *sp,
+                                     const float *so, const float *off, float common, int32_t N,
+                                     int32_t S, int32_t B, float *out);
 // Measurement noise: m[b][t][s] += sd[s] * z_k with k = (s B + b) T + t and
 // z_k the k-th standard normal of the counter-based generator keyed by `seed`.
 hipError_t launch_measurement_noise(hipStream_t s, float *m, const float *sd, int32_t B,

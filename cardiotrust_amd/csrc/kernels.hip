@@ -1083,6 +1083,34 @@ __global__ void measurement_noise_kernel(float *__restrict__ m, const float *__r
     m[e] = m[e] + float(double(sd[s]) * z);
 }
 
+// Biot-Savart measurement matrix, measurement.rs:66-123, in the reference's
+// f32 operation order: d = (sensor + offset - voxel) / 1000, |d|^3 = r (r r)
+// with r = sqrt((0 + d0^2) + d1^2) + d2^2), entries common * o_a.mul_add(d_b,
+// -o_b d_c) / |d|^3. A thread per (beat, sensor, voxel) writes its 3 entries.
+__global__ void measurement_matrix_kernel(const float *__restrict__ vp, const float *__restrict__ sp,
+                                          const float *__restrict__ so, const float *__restrict__ off,
+                                          float common, int32_t N, int32_t S, int32_t B,
+                                          float *__restrict__ out) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= int64_t(B) * S * N) return;
+    const int32_t v = int32_t(e % N);
+    const int64_t bs = e / N;
+    const int32_t s = int32_t(bs % S), b = int32_t(bs / S);
+    float d[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d[k] = ((sp[3 * s + k] + off[3 * b + k]) - vp[3LL * v + k]) / 1000.0f;
+    float sq = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sq = sq + d[k] * d[k];
+    const float r = sqrtf(sq);
+    const float r3 = r * (r * r);
+    const float *o = so + 3 * s;
+    float *row = out + bs * (3LL * N) + 3LL * v;
+    row[0] = common * fmaf(o[2], d[1], -o[1] * d[2]) / r3;
+    row[1] = common * fmaf(o[0], d[2], -o[2] * d[0]) / r3;
+    row[2] = common * fmaf(o[1], d[0], -o[0] * d[1]) / r3;
+}
+
 __global__ void fill_words_kernel(uint32_t *p, uint32_t v, int64_t len) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k < len) p[k] = v;
@@ -1160,6 +1188,16 @@ hipError_t launch_measurement_noise(hipStream_t s, float *m, const float *sd, in
     // key of stream 1000 of `seed`, as the host-side generator derives it
     const uint64_t key = splitmix64_host((seed << 20) ^ 1000u);
     measurement_noise_kernel<<<blocks_for(len, 256), 256, 0, s>>>(m, sd, B, T, S, key);
+    return hipGetLastError();
+}
+
+hipError_t launch_measurement_matrix(hipStream_t s, const float *vp, const float *sp,
+                                     const float *so, const float *off, float common, int32_t N,
+                                     int32_t S, int32_t B, float *out) {
+    const int64_t len = int64_t(B) * S * N;
+    if (len <= 0) return hipSuccess;
+    measurement_matrix_kernel<<<blocks_for(len, 256), 256, 0, s>>>(vp, sp, so, off, common, N, S,
+                                                                   B, out);
     return hipGetLastError();
 }
 

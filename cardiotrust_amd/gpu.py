@@ -24,6 +24,7 @@ EXPORTS = (
     "ct_last_error", "ct_gpu_new", "ct_gpu_free", "ct_gpu_synchronize",
     "ct_measurements_to_gpu", "ct_buffer_free", "ct_results_to_gpu", "ct_results_gpu_free",
     "ct_results_update_from_gpu", "ct_results_simulate", "ct_results_simulate_noisy",
+    "ct_measurement_matrix",
     "ct_results_set_propagation",
     "ct_results_forward", "ct_results_backward", "ct_epoch_kernel_set_gradient",
     "ct_epoch_kernel_new",
@@ -118,6 +119,8 @@ def lib() -> ctypes.CDLL:
         "ct_results_update_from_gpu": ([vp, P(_Host)], ctypes.c_int),
         "ct_results_simulate": ([vp, vp], ctypes.c_int),
         "ct_results_simulate_noisy": ([vp, vp, ctypes.c_uint64, vp], ctypes.c_int),
+        "ct_measurement_matrix": ([vp, ctypes.c_int32, vp, ctypes.c_int32, vp, vp, ctypes.c_int32,
+                                   vp, ctypes.c_float, vp], ctypes.c_int),
         "ct_results_set_propagation": ([vp, i32, P(i32)], ctypes.c_int),
         "ct_results_forward": ([vp, vp, i32, P(CAlgorithm), i32, P(ctypes.c_double)],
                                ctypes.c_int),
@@ -403,6 +406,21 @@ class EpochKernel:
             self.close()
         except Exception:
             pass
+
+
+def measurement_matrix(gpu: GPU, voxel_positions_mm, sensor_positions_mm, sensor_orientations,
+                       sensor_offsets_mm, voxel_size_mm: float) -> np.ndarray:
+    """(B, S, 3N) Biot-Savart matrix built on the GPU (ct_measurement_matrix)."""
+    vp = np.ascontiguousarray(voxel_positions_mm, dtype=np.float32).reshape(-1, 3)
+    sp = np.ascontiguousarray(sensor_positions_mm, dtype=np.float32).reshape(-1, 3)
+    so = np.ascontiguousarray(sensor_orientations, dtype=np.float32).reshape(-1, 3)
+    off = np.ascontiguousarray(sensor_offsets_mm, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros((off.shape[0], sp.shape[0], 3 * vp.shape[0]), dtype=np.float32)
+    _check(lib().ct_measurement_matrix(gpu.handle, vp.shape[0], _ptr(vp), sp.shape[0], _ptr(sp),
+                                       _ptr(so), off.shape[0], _ptr(off),
+                                       ctypes.c_float(voxel_size_mm), _ptr(out)),
+           "measurement_matrix")
+    return out
 
 
 def simulate(gpu: GPU, problem) -> np.ndarray:

@@ -208,7 +208,7 @@ def sensor_grid(geometry: Geometry, per_axis: int = 6, pitch_mm: float = 30.0,
 
 def measurement_matrix(geometry: Geometry, sensors: Sensors) -> np.ndarray:
     """(B, S, 3N) float32 Biot-Savart matrix, `measurement.rs:85-117`:
-    H[b,s,3v+k] = c * (o x d)_k / |d|^3 with d = (sensor + offset_b - voxel)/1000
+    H[b,s,3v+k] = c * (d x o)_k / |d|^3 with d = (sensor + offset_b - voxel)/1000
     and c = mu0 * V / (4 pi) * 1e12 (V in m^3, result in pT per A m^2)."""
     V = (geometry.voxel_size_mm / 1000.0) ** 3
     c = (MU0 * V) / (4.0 * np.pi) * 1e12

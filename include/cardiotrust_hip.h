@@ -148,6 +148,18 @@ int ct_results_update_from_gpu(ct_results_gpu *results, const ct_results_host *o
  * and, when `measurements_out` is not NULL, copies them there (B,T,S). */
 int ct_results_simulate(ct_results_gpu *results, float *measurements_out);
 
+/* MeasurementMatrix::from_model_spatial_description (src/core/model/
+ * functional/measurement.rs:66-123) on the device, in the reference's f32
+ * operations: out (B, S, 3N) with out[b][s][3v+k] the Biot-Savart factor of
+ * voxel v's moment component k at sensor s of beat b. voxel_positions_mm
+ * (N, 3) by voxel number, sensor_positions_mm and sensor_orientations (S, 3),
+ * sensor_offsets_mm (B, 3) (SensorArrayMotion offsets), voxel_size_mm > 0.
+ * Blocks until `out` is written. */
+int ct_measurement_matrix(ct_gpu *gpu, int32_t num_voxels, const float *voxel_positions_mm,
+                          int32_t num_sensors, const float *sensor_positions_mm,
+                          const float *sensor_orientations, int32_t num_beats,
+                          const float *sensor_offsets_mm, float voxel_size_mm, float *out);
+
 /* Simulation::run with its measurement noise (src/core/data/simulation.rs:
  * 129-173): ct_results_simulate, then m[b][t][s] += sensor_std[s] * z with z
  * standard normal, drawn in the reference's order (sensor, then beat, then

@@ -816,4 +816,40 @@ float ct_oracle_unrolled_dot_f32(const float *a, const float *b, int64_t len) {
 
 float ct_oracle_unrolled_sum_f32(const float *a, int64_t len) { return unrolled_sum<float>(a, len); }
 
+// measurement.rs:66-123. Every operation is the reference's f32 operation:
+// voxel volume (size/1000).powi(3); common factor (mu0 as f32 * V) / (4 pi)
+// * 1e12 with mu0 = 1.25663706212e-6 (physical_constants 0.5.0, CODATA 2018);
+// distance (sensor + offset - voxel) / 1000; |d|^3 = sqrt of the ndarray sum
+// ((0 + d0^2) + d1^2) + d2^2, then powi(3) = |d| * (|d| * |d|); each entry
+// common_factor * o_a.mul_add(d_b, -o_b * d_c) / |d|^3.
+void ct_oracle_measurement_matrix_f32(int32_t num_voxels, const float *voxel_positions_mm,
+                                      int32_t num_sensors, const float *sensor_positions_mm,
+                                      const float *sensor_orientations, int32_t num_beats,
+                                      const float *sensor_offsets_mm, float voxel_size_mm,
+                                      float *out) {
+    const float pi = 3.14159265358979323846f;
+    const float side = voxel_size_mm / 1000.0f;
+    const float volume = side * (side * side);
+    const float common = (float(1.25663706212e-6) * volume) / (4.0f * pi) * 1e12f;
+    const int64_t n = 3LL * num_voxels;
+    for (int32_t b = 0; b < num_beats; ++b)
+        for (int32_t v = 0; v < num_voxels; ++v)
+            for (int32_t s = 0; s < num_sensors; ++s) {
+                float d[3];
+                for (int k = 0; k < 3; ++k) {
+                    const float sp = sensor_positions_mm[3 * s + k] + sensor_offsets_mm[3 * b + k];
+                    d[k] = (sp - voxel_positions_mm[3 * int64_t(v) + k]) / 1000.0f;
+                }
+                float sq = 0.0f;
+                for (int k = 0; k < 3; ++k) sq = sq + d[k] * d[k];
+                const float r = std::sqrt(sq);
+                const float r3 = r * (r * r);
+                const float *o = sensor_orientations + 3 * s;
+                float *row = out + (int64_t(b) * num_sensors + s) * n + 3LL * v;
+                row[0] = common * std::fma(o[2], d[1], -o[1] * d[2]) / r3;
+                row[1] = common * std::fma(o[0], d[2], -o[2] * d[0]) / r3;
+                row[2] = common * std::fma(o[1], d[0], -o[0] * d[1]) / r3;
+            }
+}
+
 }  // extern "C"

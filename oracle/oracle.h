@@ -136,6 +136,16 @@ void ct_oracle_mapped_residuals_f32(const float *measurement_matrix, const float
 float ct_oracle_unrolled_dot_f32(const float *a, const float *b, int64_t len);
 float ct_oracle_unrolled_sum_f32(const float *a, int64_t len);
 
+/* MeasurementMatrix::from_model_spatial_description (measurement.rs:66-123)
+ * in the reference's f32 operation order: out (B, S, 3N), voxel positions
+ * (N, 3) by voxel number, sensor positions and orientations (S, 3), per-beat
+ * sensor-array offsets (B, 3), all in mm. */
+void ct_oracle_measurement_matrix_f32(int32_t num_voxels, const float *voxel_positions_mm,
+                                      int32_t num_sensors, const float *sensor_positions_mm,
+                                      const float *sensor_orientations, int32_t num_beats,
+                                      const float *sensor_offsets_mm, float voxel_size_mm,
+                                      float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,6 +110,10 @@ def lib():
         L.ct_oracle_unrolled_dot_f32.restype = ctypes.c_float
         L.ct_oracle_unrolled_sum_f32.argtypes = [ctypes.c_void_p, ctypes.c_int64]
         L.ct_oracle_unrolled_sum_f32.restype = ctypes.c_float
+        L.ct_oracle_measurement_matrix_f32.argtypes = [
+            ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+            ctypes.c_int32, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p]
+        L.ct_oracle_measurement_matrix_f32.restype = None
         _lib = L
     return _lib
 
@@ -228,3 +232,20 @@ class OracleResults:
 def simulate_f32(problem) -> np.ndarray:
     """Noise-free measurements of `problem`'s parameters (float32 oracle)."""
     return OracleResults(problem, np.float32).simulate().copy()
+
+
+def measurement_matrix_f32(voxel_positions_mm, sensor_positions_mm, sensor_orientations,
+                           sensor_offsets_mm, voxel_size_mm):
+    """(B, S, 3N) Biot-Savart matrix in the reference's f32 operation order
+    (measurement.rs:66-123); inputs are cast to float32 as the reference holds
+    them."""
+    vp = np.ascontiguousarray(voxel_positions_mm, dtype=np.float32).reshape(-1, 3)
+    sp = np.ascontiguousarray(sensor_positions_mm, dtype=np.float32).reshape(-1, 3)
+    so = np.ascontiguousarray(sensor_orientations, dtype=np.float32).reshape(-1, 3)
+    off = np.ascontiguousarray(sensor_offsets_mm, dtype=np.float32).reshape(-1, 3)
+    N, S, B = vp.shape[0], sp.shape[0], off.shape[0]
+    out = np.zeros((B, S, 3 * N), dtype=np.float32)
+    lib().ct_oracle_measurement_matrix_f32(N, vp.ctypes.data, S, sp.ctypes.data, so.ctypes.data,
+                                           B, off.ctypes.data, float(voxel_size_mm),
+                                           out.ctypes.data)
+    return out

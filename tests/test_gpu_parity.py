@@ -296,3 +296,27 @@ def test_simulate_with_measurement_noise(gpu):
         with pytest.raises(CTError):
             r.simulate(noise_std=sd2, seed=1)
     r.close()
+
+
+def test_measurement_matrix_on_device_equals_the_oracle(gpu):
+    """ct_measurement_matrix runs measurement.rs:66-123 in the reference's f32
+    operations (correctly rounded division and square root on gfx950), so it
+    equals the oracle's restatement bit for bit: a blob with two beats and the
+    120k-voxel shell with one."""
+    import time
+    from cardiotrust_amd.gpu import CTError, measurement_matrix
+    for g, offs in ((synth.blob(9, 500), np.array([[0.0, 0.0, 0.0], [12.5, -3.0, 4.0]])),
+                    (synth.ellipsoid_shell(), np.zeros((1, 3)))):
+        note(f"case: Biot-Savart matrix, {g.num_voxels} voxels, {len(offs)} beats")
+        s = synth.sensor_grid(g, num_beats=len(offs), beat_offsets_mm=offs)
+        t0 = time.perf_counter()
+        H = measurement_matrix(gpu, g.positions_mm, s.positions_mm, s.orientations, offs,
+                               g.voxel_size_mm)
+        dt = time.perf_counter() - t0
+        want = oracle.measurement_matrix_f32(g.positions_mm, s.positions_mm, s.orientations, offs,
+                                             g.voxel_size_mm)
+        note(f"  {H.nbytes / 1e6:.0f} MB in {dt * 1e3:.1f} ms (with the copy back)")
+        assert np.array_equal(H.view(np.uint32), want.view(np.uint32))
+    with pytest.raises(CTError):
+        measurement_matrix(gpu, np.zeros((2, 3)), np.ones((1, 3)), np.eye(3)[:1], np.zeros((1, 3)),
+                           -1.0)
