@@ -911,6 +911,68 @@ int ct_measurement_matrix(ct_gpu *gpu, int32_t num_voxels, const float *voxel_po
     return rc;
 }
 
+int ct_model_connections(ct_gpu *gpu, int32_t nx, int32_t ny, int32_t nz,
+                         const int32_t *state_numbers, const float *positions_mm,
+                         const float *velocities_m_per_s, float sample_rate_hz,
+                         int32_t num_voxels, int32_t *output_state_indices,
+                         float *delay_samples) {
+    CT_CHECK(check_gpu(gpu));
+    if (nx < 0 || ny < 0 || nz < 0 || num_voxels < 0)
+        return fail(CT_ERR_INVALID, "negative grid size or voxel count");
+    const int64_t cells = int64_t(nx) * ny * nz;
+    if (cells > 0 && (!state_numbers || !positions_mm || !velocities_m_per_s))
+        return fail(CT_ERR_INVALID, "null grid array");
+    if (num_voxels > 0 && (!output_state_indices || !delay_samples))
+        return fail(CT_ERR_INVALID, "null output array");
+    if (!(sample_rate_hz > 0.0f) || !std::isfinite(sample_rate_hz))
+        return fail(CT_ERR_INVALID, "sample rate must be positive and finite");
+    if (cells * 27 >= (int64_t(1) << 62)) return fail(CT_ERR_INVALID, "grid too large");
+    // every number is a distinct first state index 3v, v < num_voxels
+    std::vector<uint8_t> seen(size_t(num_voxels), 0);
+    for (int64_t c = 0; c < cells; ++c) {
+        const int32_t v = state_numbers[c];
+        if (v < 0) continue;
+        if (v % 3 != 0 || v / 3 >= num_voxels || seen[size_t(v / 3)])
+            return fail(CT_ERR_INVALID, "state numbers must be distinct multiples of 3 below 3N");
+        seen[size_t(v / 3)] = 1;
+    }
+    hipStream_t s = gpu->stream;
+    const size_t n_osi = size_t(num_voxels) * 3 * 78, n_d = size_t(num_voxels) * 26;
+    const size_t bytes = sizeof(int32_t) * size_t(cells) + sizeof(float) * size_t(cells) * 4 +
+                         sizeof(int32_t) * n_osi + sizeof(float) * n_d + 256;
+    char *dev = nullptr;
+    CT_HIP(hipMalloc(&dev, bytes));
+    int32_t *num = reinterpret_cast<int32_t *>(dev);
+    float *pos = reinterpret_cast<float *>(num + cells);
+    float *vel = pos + 3 * cells;
+    int32_t *osi = reinterpret_cast<int32_t *>(vel + cells);
+    float *dly = reinterpret_cast<float *>(osi + n_osi);
+    int32_t *flag = reinterpret_cast<int32_t *>(dly + n_d);
+    int rc = CT_OK;
+    auto step = [&](hipError_t e) {
+        if (rc == CT_OK && e != hipSuccess)
+            rc = fail(CT_ERR_DEVICE, std::string("model connections: ") + hipGetErrorString(e));
+    };
+    int32_t too_small = 0;
+    step(hipMemcpyAsync(num, state_numbers, sizeof(int32_t) * cells, hipMemcpyHostToDevice, s));
+    step(hipMemcpyAsync(pos, positions_mm, sizeof(float) * 3 * cells, hipMemcpyHostToDevice, s));
+    step(hipMemcpyAsync(vel, velocities_m_per_s, sizeof(float) * cells, hipMemcpyHostToDevice, s));
+    if (rc == CT_OK)
+        step(ct::launch_model_connections(s, nx, ny, nz, num, pos, vel, sample_rate_hz, num_voxels,
+                                          osi, dly, flag));
+    if (rc == CT_OK && num_voxels > 0) {
+        step(hipMemcpyAsync(output_state_indices, osi, sizeof(int32_t) * n_osi,
+                            hipMemcpyDeviceToHost, s));
+        step(hipMemcpyAsync(delay_samples, dly, sizeof(float) * n_d, hipMemcpyDeviceToHost, s));
+    }
+    if (rc == CT_OK) step(hipMemcpyAsync(&too_small, flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    step(hipStreamSynchronize(s));
+    (void)hipFree(dev);
+    if (rc == CT_OK && too_small)
+        rc = fail(CT_ERR_INVALID, "can not configure delays below 1 sample");
+    return rc;
+}
+
 int ct_results_simulate_noisy(ct_results_gpu *results, const float *sensor_std, uint64_t seed,
                               float *measurements_out) {
     if (!results || !sensor_std) return fail(CT_ERR_INVALID, "null argument");

@@ -177,6 +177,13 @@ hipError_t launch_expand_rows(hipStream_t s, const float *in, float *out, int64_
 
 hipError_t launch_fill(hipStream_t s, float *p, float value, int64_t len);
 hipError_t launch_add(hipStream_t s, float *a, const float *b, int64_t len);  // a += b
+// Connections of a grid of cells: output state indices (3N, 78; -1 = None)
+// and delays in samples (N, 26; 0 without a connection); *too_small = 1 if a
+// delay is below one sample.
+hipError_t launch_model_connections(hipStream_t s, int32_t nx, int32_t ny, int32_t nz,
+                                   const int32_t *numbers, const float *pos, const float *vel,
+                                   float rate, int32_t num_voxels, int32_t *osi, float *delays,
+                                   int32_t *too_small);
 // Biot-Savart matrix (B, S, 3N) from voxel positions (N,3), sensor positions
 // and orientations (S,3), beat offsets (B,3); `common` = mu0 V / (4 pi) 1e12.
 hipError_t launch_measurement_matrix(hipStream_t s, const float *vp, const float *sp,

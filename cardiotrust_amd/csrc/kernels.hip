@@ -1111,6 +1111,46 @@ __global__ void measurement_matrix_kernel(const float *__restrict__ vp, const fl
     row[2] = common * fmaf(o[1], d[0], -o[0] * d[1]) / r3;
 }
 
+// Connections of a voxel grid (allpass.rs:212-289 and allpass/delay.rs:35-111),
+// a thread per (cell, offset): for a connectable cell and a connectable
+// neighbour inside the grid, the nine output state indices of the offset and
+// its delay in samples, |(in - out) / 1000| / v_in * rate in the reference's
+// f32 operations. A delay below one sample raises *too_small.
+__global__ void model_connections_kernel(int32_t nx, int32_t ny, int32_t nz,
+                                         const int32_t *__restrict__ numbers,
+                                         const float *__restrict__ pos,
+                                         const float *__restrict__ vel, float rate,
+                                         int32_t *__restrict__ osi, float *__restrict__ delays,
+                                         int32_t *__restrict__ too_small) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t cells = int64_t(nx) * ny * nz;
+    if (e >= cells * 27) return;
+    const int64_t ci = e / 27;
+    const int32_t o = int32_t(e % 27);
+    const int32_t ox = o / 9 - 1, oy = (o / 3) % 3 - 1, oz = o % 3 - 1;
+    if (o == 13) return;  // the cell itself
+    const int32_t in_num = numbers[ci];
+    if (in_num < 0) return;
+    const int32_t z = int32_t(ci % nz), y = int32_t((ci / nz) % ny), x = int32_t(ci / (int64_t(ny) * nz));
+    const int32_t X = x + ox, Y = y + oy, Z = z + oz;
+    if (X < 0 || X >= nx || Y < 0 || Y >= ny || Z < 0 || Z >= nz) return;
+    const int64_t co = (int64_t(X) * ny + Y) * nz + Z;
+    const int32_t out_num = numbers[co];
+    if (out_num < 0) return;
+    // offset_to_delay_index (allpass.rs:582-592); gain index = 3 * delay index + out
+    const int32_t di = o > 13 ? o - 1 : o;
+    for (int32_t in = 0; in < 3; ++in)
+        for (int32_t od = 0; od < 3; ++od) osi[int64_t(in_num + in) * 78 + 3 * di + od] = out_num + od;
+    float sq = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        const float d = (pos[3 * ci + k] - pos[3 * co + k]) / 1000.0f;
+        sq = sq + d * d;
+    }
+    const float delay = sqrtf(sq) / vel[ci] * rate;
+    if (delay < 1.0f) atomicOr(too_small, 1);  // the reference's test (delay.rs:94)
+    delays[int64_t(in_num / 3) * 26 + di] = delay;
+}
+
 __global__ void fill_words_kernel(uint32_t *p, uint32_t v, int64_t len) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k < len) p[k] = v;
@@ -1198,6 +1238,24 @@ hipError_t launch_measurement_matrix(hipStream_t s, const float *vp, const float
     if (len <= 0) return hipSuccess;
     measurement_matrix_kernel<<<blocks_for(len, 256), 256, 0, s>>>(vp, sp, so, off, common, N, S,
                                                                    B, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_model_connections(hipStream_t s, int32_t nx, int32_t ny, int32_t nz,
+                                   const int32_t *numbers, const float *pos, const float *vel,
+                                   float rate, int32_t num_voxels, int32_t *osi, float *delays,
+                                   int32_t *too_small) {
+    const int64_t n_osi = int64_t(num_voxels) * 3 * 78, n_d = int64_t(num_voxels) * 26;
+    if (n_osi > 0) {
+        fill_words_kernel<<<blocks_for(n_osi, 256), 256, 0, s>>>(
+            reinterpret_cast<uint32_t *>(osi), 0xFFFFFFFFu, n_osi);
+        fill_kernel<<<blocks_for(n_d, 256), 256, 0, s>>>(delays, 0.0f, n_d);
+    }
+    fill_words_kernel<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t *>(too_small), 0u, 1);
+    const int64_t threads = int64_t(nx) * ny * nz * 27;
+    if (threads > 0)
+        model_connections_kernel<<<blocks_for(threads, 256), 256, 0, s>>>(
+            nx, ny, nz, numbers, pos, vel, rate, osi, delays, too_small);
     return hipGetLastError();
 }
 

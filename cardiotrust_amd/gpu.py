@@ -24,7 +24,7 @@ EXPORTS = (
     "ct_last_error", "ct_gpu_new", "ct_gpu_free", "ct_gpu_synchronize",
     "ct_measurements_to_gpu", "ct_buffer_free", "ct_results_to_gpu", "ct_results_gpu_free",
     "ct_results_update_from_gpu", "ct_results_simulate", "ct_results_simulate_noisy",
-    "ct_measurement_matrix",
+    "ct_measurement_matrix", "ct_model_connections",
     "ct_results_set_propagation",
     "ct_results_forward", "ct_results_backward", "ct_epoch_kernel_set_gradient",
     "ct_epoch_kernel_new",
@@ -121,6 +121,8 @@ def lib() -> ctypes.CDLL:
         "ct_results_simulate_noisy": ([vp, vp, ctypes.c_uint64, vp], ctypes.c_int),
         "ct_measurement_matrix": ([vp, ctypes.c_int32, vp, ctypes.c_int32, vp, vp, ctypes.c_int32,
                                    vp, ctypes.c_float, vp], ctypes.c_int),
+        "ct_model_connections": ([vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp, vp, vp,
+                                  ctypes.c_float, ctypes.c_int32, vp, vp], ctypes.c_int),
         "ct_results_set_propagation": ([vp, i32, P(i32)], ctypes.c_int),
         "ct_results_forward": ([vp, vp, i32, P(CAlgorithm), i32, P(ctypes.c_double)],
                                ctypes.c_int),
@@ -421,6 +423,22 @@ def measurement_matrix(gpu: GPU, voxel_positions_mm, sensor_positions_mm, sensor
                                        ctypes.c_float(voxel_size_mm), _ptr(out)),
            "measurement_matrix")
     return out
+
+
+def model_connections(gpu: GPU, state_numbers, positions_mm, velocities_m_per_s,
+                      sample_rate_hz: float, num_voxels: int):
+    """(output_state_indices (3N, 78) int32, delay_samples (N, 26) float32) of a
+    voxel grid, built on the GPU (ct_model_connections)."""
+    num = np.ascontiguousarray(state_numbers, dtype=np.int32)
+    nx, ny, nz = num.shape
+    pos = np.ascontiguousarray(positions_mm, dtype=np.float32).reshape(nx, ny, nz, 3)
+    vel = np.ascontiguousarray(velocities_m_per_s, dtype=np.float32).reshape(nx, ny, nz)
+    osi = np.empty((3 * num_voxels, 78), dtype=np.int32)
+    dly = np.empty((num_voxels, 26), dtype=np.float32)
+    _check(lib().ct_model_connections(gpu.handle, nx, ny, nz, _ptr(num), _ptr(pos), _ptr(vel),
+                                      ctypes.c_float(sample_rate_hz), num_voxels, _ptr(osi),
+                                      _ptr(dly)), "model_connections")
+    return osi, dly
 
 
 def simulate(gpu: GPU, problem) -> np.ndarray:

@@ -160,6 +160,22 @@ int ct_measurement_matrix(ct_gpu *gpu, int32_t num_voxels, const float *voxel_po
                           const float *sensor_orientations, int32_t num_beats,
                           const float *sensor_offsets_mm, float voxel_size_mm, float *out);
 
+/* init_output_state_indicies (src/core/model/functional/allpass.rs:212-289)
+ * and calculate_delay_samples_array (allpass/delay.rs:35-111) on the device,
+ * in the reference's f32 operations. The grid has nx*ny*nz cells in C order;
+ * per cell, state_numbers holds the reference's voxel number (the first state
+ * index 3v) or -1 where the cell is not connectable, positions_mm its centre
+ * (cells, 3) and velocities_m_per_s the propagation velocity of its voxel type.
+ * Fills output_state_indices (3N, 78; -1 = None) and delay_samples (N, 26; 0
+ * where there is no connection). CT_ERR_INVALID for numbers that are not
+ * distinct multiples of 3 below 3N, and, as the reference errors, for a delay
+ * below one sample. Blocks until the outputs are written. */
+int ct_model_connections(ct_gpu *gpu, int32_t nx, int32_t ny, int32_t nz,
+                         const int32_t *state_numbers, const float *positions_mm,
+                         const float *velocities_m_per_s, float sample_rate_hz,
+                         int32_t num_voxels, int32_t *output_state_indices,
+                         float *delay_samples);
+
 /* Simulation::run with its measurement noise (src/core/data/simulation.rs:
  * 129-173): ct_results_simulate, then m[b][t][s] += sensor_std[s] * z with z
  * standard normal, drawn in the reference's order (sensor, then beat, then

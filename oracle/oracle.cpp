@@ -28,6 +28,7 @@
 
 #include "oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -850,6 +851,58 @@ void ct_oracle_measurement_matrix_f32(int32_t num_voxels, const float *voxel_pos
                 row[1] = common * std::fma(o[0], d[2], -o[2] * d[0]) / r3;
                 row[2] = common * std::fma(o[1], d[0], -o[0] * d[1]) / r3;
             }
+}
+
+// allpass/delay.rs:14-23: |(in - out) / 1000| / v, the norm the sqrt of the
+// ndarray sum ((0 + d0^2) + d1^2) + d2^2 in f32.
+float ct_oracle_delay_s_f32(const float *in, const float *out, float velocity) {
+    float sq = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        const float d = (in[k] - out[k]) / 1000.0f;
+        sq = sq + d * d;
+    }
+    return std::sqrt(sq) / velocity;
+}
+
+// allpass.rs:212-289 and allpass/delay.rs:35-111: the same loops (cells in C
+// order, offsets x, then y, then z, in -1..=1), the same validity test
+// (voxels.rs:117-150: inside the grid and connectable) and the same indices.
+int ct_oracle_model_connections_f32(int32_t nx, int32_t ny, int32_t nz, const int32_t *numbers,
+                                    const float *positions_mm, const float *velocities,
+                                    float sample_rate_hz, int32_t num_voxels, int32_t *osi,
+                                    float *delay_samples) {
+    std::fill(osi, osi + int64_t(num_voxels) * 3 * 78, -1);
+    std::fill(delay_samples, delay_samples + int64_t(num_voxels) * 26, 0.0f);
+    auto cell = [&](int32_t x, int32_t y, int32_t z) { return (int64_t(x) * ny + y) * nz + z; };
+    for (int32_t x = 0; x < nx; ++x)
+        for (int32_t y = 0; y < ny; ++y)
+            for (int32_t z = 0; z < nz; ++z) {
+                const int64_t ci = cell(x, y, z);
+                const int32_t in_num = numbers[ci];
+                if (in_num < 0) continue;
+                for (int32_t ox = -1; ox <= 1; ++ox)
+                    for (int32_t oy = -1; oy <= 1; ++oy)
+                        for (int32_t oz = -1; oz <= 1; ++oz) {
+                            if (ox == 0 && oy == 0 && oz == 0) continue;
+                            const int32_t X = x + ox, Y = y + oy, Z = z + oz;
+                            if (X < 0 || X >= nx || Y < 0 || Y >= ny || Z < 0 || Z >= nz) continue;
+                            const int64_t co = cell(X, Y, Z);
+                            const int32_t out_num = numbers[co];
+                            if (out_num < 0) continue;
+                            for (int32_t in = 0; in < 3; ++in)
+                                for (int32_t od = 0; od < 3; ++od)
+                                    osi[int64_t(in_num + in) * 78 +
+                                        ct_oracle_offset_to_gain_index(ox, oy, oz, od)] = out_num + od;
+                            const float delay = ct_oracle_delay_s_f32(positions_mm + 3 * ci,
+                                                                      positions_mm + 3 * co,
+                                                                      velocities[ci]) *
+                                                sample_rate_hz;
+                            if (delay < 1.0f) return -1;
+                            delay_samples[int64_t(in_num / 3) * 26 +
+                                          ct_oracle_offset_to_delay_index(ox, oy, oz)] = delay;
+                        }
+            }
+    return 0;
 }
 
 }  // extern "C"

@@ -146,6 +146,22 @@ void ct_oracle_measurement_matrix_f32(int32_t num_voxels, const float *voxel_pos
                                       const float *sensor_offsets_mm, float voxel_size_mm,
                                       float *out);
 
+/* init_output_state_indicies (allpass.rs:212-289) and
+ * calculate_delay_samples_array (allpass/delay.rs:35-111) over a grid of
+ * nx*ny*nz cells in C order: numbers[c] is the cell's first state index (a
+ * multiple of 3) or -1 where it is not connectable, positions_mm (cells, 3),
+ * velocities_m_per_s the propagation velocity of the cell's voxel type.
+ * Fills output_state_indices (3N, 78; -1 = None) and delay_samples (N, 26;
+ * 0 without a connection). Returns -1, as the reference errors, when a delay
+ * is below one sample. */
+int ct_oracle_model_connections_f32(int32_t nx, int32_t ny, int32_t nz, const int32_t *numbers,
+                                    const float *positions_mm, const float *velocities_m_per_s,
+                                    float sample_rate_hz, int32_t num_voxels,
+                                    int32_t *output_state_indices, float *delay_samples);
+/* calculate_delay_s (allpass/delay.rs:14-23). */
+float ct_oracle_delay_s_f32(const float *input_position_mm, const float *output_position_mm,
+                            float propagation_velocity_m_per_s);
+
 #ifdef __cplusplus
 }
 #endif

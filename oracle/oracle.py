@@ -114,6 +114,12 @@ def lib():
             ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
             ctypes.c_int32, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p]
         L.ct_oracle_measurement_matrix_f32.restype = None
+        L.ct_oracle_model_connections_f32.argtypes = [
+            ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+            ctypes.c_void_p, ctypes.c_float, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
+        L.ct_oracle_model_connections_f32.restype = ctypes.c_int
+        L.ct_oracle_delay_s_f32.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float]
+        L.ct_oracle_delay_s_f32.restype = ctypes.c_float
         _lib = L
     return _lib
 
@@ -249,3 +255,27 @@ def measurement_matrix_f32(voxel_positions_mm, sensor_positions_mm, sensor_orien
                                            B, off.ctypes.data, float(voxel_size_mm),
                                            out.ctypes.data)
     return out
+
+
+def delay_s_f32(input_position_mm, output_position_mm, velocity_m_per_s):
+    a = np.ascontiguousarray(input_position_mm, dtype=np.float32)
+    b = np.ascontiguousarray(output_position_mm, dtype=np.float32)
+    return float(lib().ct_oracle_delay_s_f32(a.ctypes.data, b.ctypes.data, float(velocity_m_per_s)))
+
+
+def model_connections_f32(numbers, positions_mm, velocities_m_per_s, sample_rate_hz, num_voxels):
+    """(output_state_indices (3N, 78) int32, delay_samples (N, 26) float32) of a
+    grid (nx, ny, nz): numbers = first state index per cell or -1; raises
+    ValueError where the reference errors (a delay below one sample)."""
+    num = np.ascontiguousarray(numbers, dtype=np.int32)
+    nx, ny, nz = num.shape
+    pos = np.ascontiguousarray(positions_mm, dtype=np.float32).reshape(nx, ny, nz, 3)
+    vel = np.ascontiguousarray(velocities_m_per_s, dtype=np.float32).reshape(nx, ny, nz)
+    osi = np.empty((3 * num_voxels, 78), dtype=np.int32)
+    dly = np.empty((num_voxels, 26), dtype=np.float32)
+    rc = lib().ct_oracle_model_connections_f32(nx, ny, nz, num.ctypes.data, pos.ctypes.data,
+                                               vel.ctypes.data, float(sample_rate_hz), num_voxels,
+                                               osi.ctypes.data, dly.ctypes.data)
+    if rc != 0:
+        raise ValueError("a delay is below one sample")
+    return osi, dly

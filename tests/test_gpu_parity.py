@@ -320,3 +320,34 @@ def test_measurement_matrix_on_device_equals_the_oracle(gpu):
     with pytest.raises(CTError):
         measurement_matrix(gpu, np.zeros((2, 3)), np.ones((1, 3)), np.eye(3)[:1], np.zeros((1, 3)),
                            -1.0)
+
+
+def test_model_connections_on_device_equal_the_oracle(gpu):
+    """ct_model_connections builds output_state_indices and the delays in
+    samples with the reference's f32 operations: equal to the oracle bit for
+    bit on a ragged blob (with per-cell velocities) and the 120k-voxel shell;
+    a sub-sample delay and malformed numbers are errors."""
+    import time
+    from cardiotrust_amd.gpu import CTError, model_connections
+    for g in (synth.blob(6, 900), synth.ellipsoid_shell()):
+        note(f"case: model connections, grid {g.shape}, {g.num_voxels} voxels")
+        numbers = np.where(g.numbers >= 0, 3 * g.numbers, -1).astype(np.int32)
+        idx = np.indices(g.shape).transpose(1, 2, 3, 0).astype(np.float64)
+        pos = ((idx + 0.5) * g.voxel_size_mm).astype(np.float32)
+        vel = (0.6 + 0.4 * synth.uniform(6, 3, int(np.prod(g.shape)))).astype(np.float32)
+        t0 = time.perf_counter()
+        osi, dly = model_connections(gpu, numbers, pos, vel.reshape(g.shape), 2000.0, g.num_voxels)
+        note(f"  {(time.perf_counter() - t0) * 1e3:.1f} ms with the copies")
+        o_osi, o_dly = oracle.model_connections_f32(numbers, pos, vel.reshape(g.shape), 2000.0,
+                                                    g.num_voxels)
+        assert np.array_equal(osi, o_osi)
+        assert np.array_equal(dly.view(np.uint32), o_dly.view(np.uint32))
+    g = synth.cube(3, voxel_size_mm=0.1)
+    numbers = (3 * g.numbers).astype(np.int32)
+    pos = ((np.indices(g.shape).transpose(1, 2, 3, 0) + 0.5) * 0.1).astype(np.float32)
+    with pytest.raises(CTError):
+        model_connections(gpu, numbers, pos, np.ones(g.shape, np.float32), 2000.0, g.num_voxels)
+    bad = numbers.copy()
+    bad[0, 0, 0] = 1  # not a multiple of 3
+    with pytest.raises(CTError):
+        model_connections(gpu, bad, pos, np.ones(g.shape, np.float32), 2.0, g.num_voxels)
