@@ -973,6 +973,56 @@ int ct_model_connections(ct_gpu *gpu, int32_t nx, int32_t ny, int32_t nz,
     return rc;
 }
 
+int ct_results_spherical(ct_results_gpu *results, float sample_rate_hz, float *magnitude,
+                         float *theta, float *phi, float *magnitude_max, float *theta_max,
+                         float *phi_max, float *activation_time_ms) {
+    if (!results) return fail(CT_ERR_INVALID, "null argument");
+    ct_results_gpu &r = *results;
+    CT_CHECK(check_gpu(r.gpu));
+    if (!(sample_rate_hz > 0.0f) || !std::isfinite(sample_rate_hz))
+        return fail(CT_ERR_INVALID, "sample rate must be positive and finite");
+    if (r.N == 0) return CT_OK;
+    if (r.T == 0) return fail(CT_ERR_INVALID, "no steps: the maxima are undefined");
+    hipStream_t s = r.gpu->stream;
+    const size_t full = size_t(r.T) * r.N, N = size_t(r.N);
+    const size_t nfull = size_t(magnitude != nullptr) + (theta != nullptr) + (phi != nullptr);
+    char *dev = nullptr;
+    CT_HIP(hipMalloc(&dev, sizeof(float) * (nfull * full + 4 * N) + 256));
+    float *p = reinterpret_cast<float *>(dev);
+    float *d_mag = magnitude ? p : nullptr;
+    if (magnitude) p += full;
+    float *d_theta = theta ? p : nullptr;
+    if (theta) p += full;
+    float *d_phi = phi ? p : nullptr;
+    if (phi) p += full;
+    float *d_mm = p, *d_tm = p + N, *d_pm = p + 2 * N, *d_act = p + 3 * N;
+    int32_t *d_flag = reinterpret_cast<int32_t *>(p + 4 * N);
+    int rc = CT_OK;
+    auto step = [&](hipError_t e) {
+        if (rc == CT_OK && e != hipSuccess)
+            rc = fail(CT_ERR_DEVICE, std::string("spherical states: ") + hipGetErrorString(e));
+    };
+    int32_t no_max = 0;
+    step(ct::launch_spherical(s, r.x, r.T, r.N, sample_rate_hz, d_mag, d_theta, d_phi, d_mm, d_tm,
+                              d_pm, d_act, d_flag));
+    const std::pair<float *, float *> copies[] = {
+        {magnitude, d_mag}, {theta, d_theta}, {phi, d_phi}};
+    for (const auto &c : copies)
+        if (c.first && rc == CT_OK)
+            step(hipMemcpyAsync(c.first, c.second, sizeof(float) * full, hipMemcpyDeviceToHost, s));
+    const std::pair<float *, float *> small[] = {
+        {magnitude_max, d_mm}, {theta_max, d_tm}, {phi_max, d_pm}, {activation_time_ms, d_act}};
+    for (const auto &c : small)
+        if (c.first && rc == CT_OK)
+            step(hipMemcpyAsync(c.first, c.second, sizeof(float) * N, hipMemcpyDeviceToHost, s));
+    if (rc == CT_OK) step(hipMemcpyAsync(&no_max, d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    step(hipStreamSynchronize(s));
+    (void)hipFree(dev);
+    if (rc == CT_OK && no_max)
+        rc = fail(CT_ERR_INVALID, "a voxel has no non-NaN magnitude (argmax_skipnan fails)");
+    return rc;
+}
+
 int ct_results_simulate_noisy(ct_results_gpu *results, const float *sensor_std, uint64_t seed,
                               float *measurements_out) {
     if (!results || !sensor_std) return fail(CT_ERR_INVALID, "null argument");

@@ -177,6 +177,12 @@ hipError_t launch_expand_rows(hipStream_t s, const float *in, float *out, int64_
 
 hipError_t launch_fill(hipStream_t s, float *p, float value, int64_t len);
 hipError_t launch_add(hipStream_t s, float *a, const float *b, int64_t len);  // a += b
+// Spherical states of x (T, 3N): (T, N) magnitude/theta/phi when not null,
+// their values at each voxel's maximum magnitude and the activation time (N);
+// *no_max = 1 if a voxel has no non-NaN magnitude.
+hipError_t launch_spherical(hipStream_t s, const float *x, int32_t T, int32_t N, float rate,
+                            float *mag, float *theta, float *phi, float *mag_max,
+                            float *theta_max, float *phi_max, float *act_ms, int32_t *no_max);
 // Connections of a grid of cells: output state indices (3N, 78; -1 = None)
 // and delays in samples (N, 26; 0 without a connection); *too_small = 1 if a
 // delay is below one sample.

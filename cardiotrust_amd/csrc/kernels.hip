@@ -1151,6 +1151,47 @@ __global__ void model_connections_kernel(int32_t nx, int32_t ny, int32_t nz,
     delays[int64_t(in_num / 3) * 26 + di] = delay;
 }
 
+// Spherical states, their values at the maximum magnitude and the activation
+// time (data/shapes.rs:171-191, 255-270, 303-322), a thread per voxel walking
+// the states x (T, 3N) once. The maximum is the first index whose magnitude
+// exceeds every earlier non-NaN one (argmax_skipnan); a voxel with no non-NaN
+// magnitude raises *no_max. The (T, N) outputs are written when not null.
+__global__ void spherical_kernel(const float *__restrict__ x, int32_t T, int32_t N, float rate,
+                                 float *__restrict__ mag, float *__restrict__ theta,
+                                 float *__restrict__ phi, float *__restrict__ mag_max,
+                                 float *__restrict__ theta_max, float *__restrict__ phi_max,
+                                 float *__restrict__ act_ms, int32_t *__restrict__ no_max) {
+    const int32_t v = int32_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const int64_t n = 3LL * N;
+    int32_t best = -1;
+    float bm = 0.0f, bt = 0.0f, bp = 0.0f;
+    for (int32_t t = 0; t < T; ++t) {
+        const float *s = x + int64_t(t) * n + 3LL * v;
+        const float s0 = s[0], s1 = s[1], s2 = s[2];
+        const float m = fabsf(s0) + fabsf(s1) + fabsf(s2);
+        const float th = acosf(s2 / m);
+        const float ph = atan2f(s1, s0);
+        if (mag) mag[int64_t(t) * N + v] = m;
+        if (theta) theta[int64_t(t) * N + v] = th;
+        if (phi) phi[int64_t(t) * N + v] = ph;
+        if (!isnan(m) && (best < 0 || m > bm)) {
+            best = t;
+            bm = m;
+            bt = th;
+            bp = ph;
+        }
+    }
+    if (best < 0) {
+        atomicOr(no_max, 1);
+        return;
+    }
+    mag_max[v] = bm;
+    theta_max[v] = bt;
+    phi_max[v] = bp;
+    act_ms[v] = float(best) / rate * 1000.0f;
+}
+
 __global__ void fill_words_kernel(uint32_t *p, uint32_t v, int64_t len) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k < len) p[k] = v;
@@ -1256,6 +1297,16 @@ hipError_t launch_model_connections(hipStream_t s, int32_t nx, int32_t ny, int32
     if (threads > 0)
         model_connections_kernel<<<blocks_for(threads, 256), 256, 0, s>>>(
             nx, ny, nz, numbers, pos, vel, rate, osi, delays, too_small);
+    return hipGetLastError();
+}
+
+hipError_t launch_spherical(hipStream_t s, const float *x, int32_t T, int32_t N, float rate,
+                            float *mag, float *theta, float *phi, float *mag_max,
+                            float *theta_max, float *phi_max, float *act_ms, int32_t *no_max) {
+    fill_words_kernel<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t *>(no_max), 0u, 1);
+    if (N > 0)
+        spherical_kernel<<<blocks_for(N, 256), 256, 0, s>>>(x, T, N, rate, mag, theta, phi, mag_max,
+                                                            theta_max, phi_max, act_ms, no_max);
     return hipGetLastError();
 }
 

@@ -24,7 +24,7 @@ EXPORTS = (
     "ct_last_error", "ct_gpu_new", "ct_gpu_free", "ct_gpu_synchronize",
     "ct_measurements_to_gpu", "ct_buffer_free", "ct_results_to_gpu", "ct_results_gpu_free",
     "ct_results_update_from_gpu", "ct_results_simulate", "ct_results_simulate_noisy",
-    "ct_measurement_matrix", "ct_model_connections",
+    "ct_measurement_matrix", "ct_model_connections", "ct_results_spherical",
     "ct_results_set_propagation",
     "ct_results_forward", "ct_results_backward", "ct_epoch_kernel_set_gradient",
     "ct_epoch_kernel_new",
@@ -119,6 +119,7 @@ def lib() -> ctypes.CDLL:
         "ct_results_update_from_gpu": ([vp, P(_Host)], ctypes.c_int),
         "ct_results_simulate": ([vp, vp], ctypes.c_int),
         "ct_results_simulate_noisy": ([vp, vp, ctypes.c_uint64, vp], ctypes.c_int),
+        "ct_results_spherical": ([vp, ctypes.c_float] + [vp] * 7, ctypes.c_int),
         "ct_measurement_matrix": ([vp, ctypes.c_int32, vp, ctypes.c_int32, vp, vp, ctypes.c_int32,
                                    vp, ctypes.c_float, vp], ctypes.c_int),
         "ct_model_connections": ([vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp, vp, vp,
@@ -324,6 +325,22 @@ class ResultsGPU:
                 raise ValueError(f"noise_std must have shape ({self.S},)")
             _check(lib().ct_results_simulate_noisy(self.handle, _ptr(sd), ctypes.c_uint64(seed),
                                                    _ptr(out)), "simulate_noisy")
+        return out
+
+    def spherical(self, sample_rate_hz: float, full: bool = False) -> dict:
+        """Spherical states of the device states (ct_results_spherical): the
+        values at each voxel's maximum magnitude and the activation time in ms
+        (N each), and with `full` the (T, N) magnitude, theta and phi."""
+        N = self.N
+        out = {k: np.zeros(N, np.float32) for k in ("magnitude_max", "theta_max", "phi_max",
+                                                    "activation_time_ms")}
+        if full:
+            out.update({k: np.zeros((self.T, N), np.float32) for k in ("magnitude", "theta", "phi")})
+        _check(lib().ct_results_spherical(
+            self.handle, ctypes.c_float(sample_rate_hz), _ptr(out.get("magnitude")),
+            _ptr(out.get("theta")), _ptr(out.get("phi")), _ptr(out["magnitude_max"]),
+            _ptr(out["theta_max"]), _ptr(out["phi_max"]), _ptr(out["activation_time_ms"])),
+            "spherical")
         return out
 
     def close(self) -> None:

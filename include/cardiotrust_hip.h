@@ -176,6 +176,20 @@ int ct_model_connections(ct_gpu *gpu, int32_t nx, int32_t ny, int32_t nz,
                          int32_t num_voxels, int32_t *output_state_indices,
                          float *delay_samples);
 
+/* SystemStatesSpherical, SystemStatesSphericalMax and
+ * ActivationTimePerStateMs ::calculate (src/core/data/shapes.rs:150-330) over
+ * the device states of the results (the last propagated beat, as the
+ * reference's Simulation holds them), without copying the states back:
+ * magnitude = |x0| + |x1| + |x2|, theta = acos(x2 / magnitude),
+ * phi = atan2(x1, x0), each (T, N); at each voxel's first maximum magnitude
+ * (NaN skipped) the three values and the activation time index / rate * 1000
+ * ms, each (N). Any output may be NULL (not copied; the (T, N) arrays are then
+ * not formed). CT_ERR_INVALID where the reference errors: a voxel whose
+ * magnitudes are all NaN, or T = 0. */
+int ct_results_spherical(ct_results_gpu *results, float sample_rate_hz, float *magnitude,
+                         float *theta, float *phi, float *magnitude_max, float *theta_max,
+                         float *phi_max, float *activation_time_ms);
+
 /* Simulation::run with its measurement noise (src/core/data/simulation.rs:
  * 129-173): ct_results_simulate, then m[b][t][s] += sensor_std[s] * z with z
  * standard normal, drawn in the reference's order (sensor, then beat, then

@@ -853,6 +853,38 @@ void ct_oracle_measurement_matrix_f32(int32_t num_voxels, const float *voxel_pos
             }
 }
 
+// data/shapes.rs:171-191 (spherical), 255-270 (max), 303-322 (activation
+// time). acos and atan2 are the C library's float functions, which Rust's
+// f32::acos / f32::atan2 call. argmax_skipnan keeps the first index whose
+// value is greater than every earlier non-NaN value.
+int ct_oracle_spherical_f32(const float *x, int32_t T, int32_t N, float rate, float *mag,
+                            float *theta, float *phi, float *mag_max, float *theta_max,
+                            float *phi_max, float *act_ms) {
+    const int64_t n = 3LL * N;
+    for (int32_t t = 0; t < T; ++t)
+        for (int32_t v = 0; v < N; ++v) {
+            const float *s = x + t * n + 3LL * v;
+            const float m = std::fabs(s[0]) + std::fabs(s[1]) + std::fabs(s[2]);
+            mag[int64_t(t) * N + v] = m;
+            theta[int64_t(t) * N + v] = std::acos(s[2] / m);
+            phi[int64_t(t) * N + v] = std::atan2(s[1], s[0]);
+        }
+    for (int32_t v = 0; v < N; ++v) {
+        int32_t best = -1;
+        for (int32_t t = 0; t < T; ++t) {
+            const float m = mag[int64_t(t) * N + v];
+            if (std::isnan(m)) continue;
+            if (best < 0 || m > mag[int64_t(best) * N + v]) best = t;
+        }
+        if (best < 0) return -1;
+        mag_max[v] = mag[int64_t(best) * N + v];
+        theta_max[v] = theta[int64_t(best) * N + v];
+        phi_max[v] = phi[int64_t(best) * N + v];
+        act_ms[v] = float(best) / rate * 1000.0f;
+    }
+    return 0;
+}
+
 // allpass/delay.rs:14-23: |(in - out) / 1000| / v, the norm the sqrt of the
 // ndarray sum ((0 + d0^2) + d1^2) + d2^2 in f32.
 float ct_oracle_delay_s_f32(const float *in, const float *out, float velocity) {

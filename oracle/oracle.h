@@ -158,6 +158,16 @@ int ct_oracle_model_connections_f32(int32_t nx, int32_t ny, int32_t nz, const in
                                     const float *positions_mm, const float *velocities_m_per_s,
                                     float sample_rate_hz, int32_t num_voxels,
                                     int32_t *output_state_indices, float *delay_samples);
+/* SystemStatesSpherical / SystemStatesSphericalMax / ActivationTimePerStateMs
+ * ::calculate (data/shapes.rs:150-330) over states x (T, 3N): magnitude,
+ * theta, phi (T, N); at each voxel's first maximum magnitude (NaN skipped)
+ * the three values (N) and the activation time index / rate * 1000 ms (N).
+ * Returns -1 where the reference errors (a voxel whose magnitudes are all NaN
+ * or T = 0). */
+int ct_oracle_spherical_f32(const float *x, int32_t num_steps, int32_t num_voxels,
+                            float sample_rate_hz, float *magnitude, float *theta, float *phi,
+                            float *magnitude_max, float *theta_max, float *phi_max,
+                            float *activation_time_ms);
 /* calculate_delay_s (allpass/delay.rs:14-23). */
 float ct_oracle_delay_s_f32(const float *input_position_mm, const float *output_position_mm,
                             float propagation_velocity_m_per_s);

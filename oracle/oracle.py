@@ -118,6 +118,9 @@ def lib():
             ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
             ctypes.c_void_p, ctypes.c_float, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
         L.ct_oracle_model_connections_f32.restype = ctypes.c_int
+        L.ct_oracle_spherical_f32.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.c_float] + [ctypes.c_void_p] * 7
+        L.ct_oracle_spherical_f32.restype = ctypes.c_int
         L.ct_oracle_delay_s_f32.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float]
         L.ct_oracle_delay_s_f32.restype = ctypes.c_float
         _lib = L
@@ -279,3 +282,23 @@ def model_connections_f32(numbers, positions_mm, velocities_m_per_s, sample_rate
     if rc != 0:
         raise ValueError("a delay is below one sample")
     return osi, dly
+
+
+def spherical_f32(x, sample_rate_hz):
+    """dict of magnitude/theta/phi (T, N), their values at each voxel's
+    maximum magnitude and activation_time_ms (N) for states x (T, 3N);
+    ValueError where the reference errors."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    T, n = x.shape
+    N = n // 3
+    out = {k: np.zeros((T, N), np.float32) for k in ("magnitude", "theta", "phi")}
+    out.update({k: np.zeros(N, np.float32) for k in ("magnitude_max", "theta_max", "phi_max",
+                                                     "activation_time_ms")})
+    rc = lib().ct_oracle_spherical_f32(
+        x.ctypes.data, T, N, float(sample_rate_hz), out["magnitude"].ctypes.data,
+        out["theta"].ctypes.data, out["phi"].ctypes.data, out["magnitude_max"].ctypes.data,
+        out["theta_max"].ctypes.data, out["phi_max"].ctypes.data,
+        out["activation_time_ms"].ctypes.data)
+    if rc != 0:
+        raise ValueError("a voxel has no non-NaN magnitude")
+    return out

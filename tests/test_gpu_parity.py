@@ -351,3 +351,29 @@ def test_model_connections_on_device_equal_the_oracle(gpu):
     bad[0, 0, 0] = 1  # not a multiple of 3
     with pytest.raises(CTError):
         model_connections(gpu, bad, pos, np.ones(g.shape, np.float32), 2.0, g.num_voxels)
+
+
+def test_spherical_states_on_device(gpu):
+    """ct_results_spherical over the device states of a simulation against the
+    oracle's restatement on the same states: magnitudes, the argmax-based
+    activation times and maxima bit for bit; theta and phi within 4 FP32 ulp
+    (acosf / atan2f of the device library against the C library's)."""
+    from cardiotrust_amd.gpu import ResultsGPU
+    note("case: spherical states of a 2-beat blob simulation and of the 120k-voxel shell")
+    for p in (blob_problem(4, T=70, beats=2),
+              synth.make_problem(synth.ellipsoid_shell(), num_steps=300, max_delay=40, seed=3)):
+        r = ResultsGPU(gpu, p)
+        r.simulate()
+        x = r.update_from_gpu(["system_states"])["system_states"]
+        got = r.spherical(2000.0, full=True)
+        r.close()
+        want = oracle.spherical_f32(x, 2000.0)
+        for k in ("magnitude", "magnitude_max", "activation_time_ms"):
+            assert np.array_equal(got[k].view(np.uint32), want[k].view(np.uint32)), k
+        for k in ("theta", "phi", "theta_max", "phi_max"):
+            a, b = got[k], want[k]
+            assert np.array_equal(np.isnan(a), np.isnan(b)), k
+            ok = ~np.isnan(a)
+            ulp = np.abs(a[ok].view(np.int32).astype(np.int64) - b[ok].view(np.int32).astype(np.int64))
+            assert ulp.max(initial=0) <= 4, (k, ulp.max())
+        note(f"  N={p.num_voxels}: activation times {np.unique(got['activation_time_ms']).size} distinct")
