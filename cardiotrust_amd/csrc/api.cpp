@@ -1276,6 +1276,26 @@ int ct_epoch_kernel_get_batch_index(const ct_epoch_kernel *k, int32_t *batch_ind
     return CT_OK;
 }
 
+int ct_epoch_kernel_last_loss(ct_epoch_kernel *k, float *loss, float *loss_mse,
+                              float *loss_maximum_regularization, int32_t *is_normal) {
+    if (!k) return fail(CT_ERR_INVALID, "null argument");
+    CT_CHECK(check_gpu(k->gpu));
+    if (k->batch_index < 1) return fail(CT_ERR_INVALID, "no batch has been recorded yet");
+    ct_results_gpu &r = *k->r;
+    const int64_t i = int64_t(k->batch_index) - 1;
+    if (i >= r.nbatch) return fail(CT_ERR_INVALID, "batch index beyond the batch arrays");
+    float v[3] = {0.0f, 0.0f, 0.0f};
+    CT_HIP(hipStreamSynchronize(r.gpu->stream));
+    CT_CHECK(download(&v[0], r.loss_b + i, sizeof(float)));
+    CT_CHECK(download(&v[1], r.loss_mse_b + i, sizeof(float)));
+    CT_CHECK(download(&v[2], r.loss_mr_b + i, sizeof(float)));
+    if (loss) *loss = v[0];
+    if (loss_mse) *loss_mse = v[1];
+    if (loss_maximum_regularization) *loss_maximum_regularization = v[2];
+    if (is_normal) *is_normal = std::isnormal(v[0]) ? 1 : 0;
+    return CT_OK;
+}
+
 int ct_epoch_kernel_set_profiling(ct_epoch_kernel *k, int32_t enabled) {
     if (!k) return fail(CT_ERR_INVALID, "null argument");
     k->profiling = enabled != 0;

@@ -25,6 +25,7 @@ EXPORTS = (
     "ct_measurements_to_gpu", "ct_buffer_free", "ct_results_to_gpu", "ct_results_gpu_free",
     "ct_results_update_from_gpu", "ct_results_simulate", "ct_results_simulate_noisy",
     "ct_measurement_matrix", "ct_model_connections", "ct_results_spherical",
+    "ct_epoch_kernel_last_loss",
     "ct_results_set_propagation",
     "ct_results_forward", "ct_results_backward", "ct_epoch_kernel_set_gradient",
     "ct_epoch_kernel_new",
@@ -120,6 +121,7 @@ def lib() -> ctypes.CDLL:
         "ct_results_simulate": ([vp, vp], ctypes.c_int),
         "ct_results_simulate_noisy": ([vp, vp, ctypes.c_uint64, vp], ctypes.c_int),
         "ct_results_spherical": ([vp, ctypes.c_float] + [vp] * 7, ctypes.c_int),
+        "ct_epoch_kernel_last_loss": ([vp, vp, vp, vp, vp], ctypes.c_int),
         "ct_measurement_matrix": ([vp, ctypes.c_int32, vp, ctypes.c_int32, vp, vp, ctypes.c_int32,
                                    vp, ctypes.c_float, vp], ctypes.c_int),
         "ct_model_connections": ([vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp, vp, vp,
@@ -405,6 +407,16 @@ class EpochKernel:
         _check(lib().ct_epoch_kernel_get_batch_index(self.handle, ctypes.byref(v)),
                "get_batch_index")
         return int(v.value)
+
+    def last_loss(self) -> dict:
+        """The last recorded batch's losses and whether the loss is a normal
+        number (the reference's divergence stop; ct_epoch_kernel_last_loss)."""
+        v = [ctypes.c_float() for _ in range(3)]
+        ok = ctypes.c_int32()
+        _check(lib().ct_epoch_kernel_last_loss(self.handle, *map(ctypes.byref, v),
+                                               ctypes.byref(ok)), "last_loss")
+        return {"loss": v[0].value, "loss_mse": v[1].value,
+                "loss_maximum_regularization": v[2].value, "is_normal": bool(ok.value)}
 
     def set_profiling(self, enabled: bool) -> None:
         _check(lib().ct_epoch_kernel_set_profiling(self.handle, int(bool(enabled))),

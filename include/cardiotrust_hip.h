@@ -290,6 +290,15 @@ int ct_epoch_kernel_set_beat_order(ct_epoch_kernel *kernel, const int32_t *order
 int ct_epoch_kernel_set_batch_index(ct_epoch_kernel *kernel, int32_t batch_index);
 int ct_epoch_kernel_get_batch_index(const ct_epoch_kernel *kernel, int32_t *batch_index);
 
+/* The divergence check of the reference's epoch loop (src/core/scenario.rs:
+ * 750-780 and 846-884): the batch losses of the last recorded batch,
+ * loss_batch[batch_index - 1] and its MSE and maximum-regularisation parts,
+ * read back after the kernel's work; *is_normal = 0 when the loss is not a
+ * normal number (NaN, infinite, zero or subnormal: the reference stops its
+ * epochs). Any output may be NULL. CT_ERR_INVALID before the first batch. */
+int ct_epoch_kernel_last_loss(ct_epoch_kernel *kernel, float *loss, float *loss_mse,
+                              float *loss_maximum_regularization, int32_t *is_normal);
+
 /* Per-phase HIP-event timing of ct_epoch_kernel_execute (no reference
  * counterpart; the reference times epochs with Criterion,
  * benches/gpu_epoch_benches.rs:40-53). Reading the profile synchronises. */

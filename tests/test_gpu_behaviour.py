@@ -37,15 +37,24 @@ def blob():
 
 
 def epochs(gpu, p, cfg, n, gradient=None, measurements=None):
-    from cardiotrust_amd.gpu import Buffer, EpochKernel, ResultsGPU
+    from cardiotrust_amd.gpu import Buffer, CTError, EpochKernel, ResultsGPU
     r = ResultsGPU(gpu, p, num_batch_entries=n)
     buf = Buffer(gpu, p.measurements if measurements is None else measurements)
     k = EpochKernel(gpu, r, buf, cfg, 3 * p.num_voxels, p.num_sensors, p.num_steps)
     if gradient:
         k.set_gradient(*gradient)
+    with pytest.raises(CTError):
+        k.last_loss()  # nothing recorded yet
+    flags = []
     for _ in range(n):
         k.execute()
+        flags.append(k.last_loss())
     out = r.update_from_gpu(["loss_batch", "delays"])
+    out["last_loss"] = flags
+    # the divergence flag is the reference's is_normal() of loss_batch[batch_index - 1]
+    for f, lb in zip(flags, out["loss_batch"]):
+        assert f["loss"] == lb or (np.isnan(f["loss"]) and np.isnan(lb))
+        assert f["is_normal"] == bool(np.isfinite(lb) and abs(lb) >= np.finfo(np.float32).tiny)
     k.close()
     buf.close()
     r.close()
@@ -84,6 +93,7 @@ def test_divergence_gives_a_non_finite_loss(gpu, blob):
     lb = epochs(gpu, blob, Algorithm(learning_rate=1000.0, freeze_delays=False), 4)
     assert np.isfinite(lb["loss_batch"][0])
     assert not np.all(np.isfinite(lb["loss_batch"]))
+    assert lb["last_loss"][0]["is_normal"] and not lb["last_loss"][-1]["is_normal"]
     assert lb["delays"].min() >= 1 and lb["delays"].max() <= 1000  # rolls stay in range
     bad = blob.measurements.copy()
     bad[0, 5, 3] = np.nan
