@@ -747,6 +747,12 @@ __device__ __forceinline__ void gain_factors(const float xo[3], const float e[3]
     }
 }
 
+// One step of one connection. A missing connection (has = false) reads
+// in0 = +0 and starts from y = prev_in = fir = iir = +0, so updating its
+// filter and FIR unconditionally keeps them +0 (for a finite coefficient),
+// as the reference's guarded updates do. With kSteady the caller has checked
+// t >= D for every lane of the wave.
+template <bool kSteady>
 __device__ __forceinline__ void derive_step(ConnState &c, const float in0[3], const float e[3],
                                             const float cg[3], bool has, float a, int32_t D,
                                             int32_t t, float reg, const DeriveParams &p) {
@@ -754,7 +760,7 @@ __device__ __forceinline__ void derive_step(ConnState &c, const float in0[3], co
 #pragma unroll
     for (int o = 0; o < 3; ++o) {
         yl[o] = c.y[o];
-        if (has) c.y[o] = fmaf(a, in0[o] - c.y[o], c.prev_in[o]);
+        c.y[o] = fmaf(a, in0[o] - c.y[o], c.prev_in[o]);
     }
     if (!p.freeze_gains) {
 #pragma unroll
@@ -763,21 +769,21 @@ __device__ __forceinline__ void derive_step(ConnState &c, const float in0[3], co
             for (int o = 0; o < 3; ++o) c.g[in][o] += c.y[o] * cg[in];
     }
     if (!p.freeze_delays) {
+        const bool on = kSteady || t >= D;
         if (!p.simple) {
-            if (has && t >= D) {
+            if (on) {
 #pragma unroll
-                for (int o = 0; o < 3; ++o) c.f[o] = fmaf(-a, c.f[o], in0[o]);
-            }
-            if (t >= D) {
-#pragma unroll
-                for (int o = 0; o < 3; ++o) c.h[o] = fmaf(-a, c.h[o], yl[o]);
+                for (int o = 0; o < 3; ++o) {
+                    c.f[o] = fmaf(-a, c.f[o], in0[o]);
+                    c.h[o] = fmaf(-a, c.h[o], yl[o]);
+                }
             }
 #pragma unroll
             for (int in = 0; in < 3; ++in)
 #pragma unroll
                 for (int o = 0; o < 3; ++o)
                     c.A += fmaf((c.f[o] - c.h[o]) * c.G[in][o] * e[in], p.mse_scaling, reg);
-        } else if (has && t >= D) {
+        } else if (has && on) {
 #pragma unroll
             for (int in = 0; in < 3; ++in)
 #pragma unroll
@@ -870,14 +876,24 @@ __global__ __launch_bounds__(kDeriveThreads, 4) void derive_kernel(
                 }
         }
         __syncthreads();
-        if (active) {
+        // steady chunk: whole, and every lane of the wave past its delay
+        const bool steady = __all(!active || t0 >= D) && t0 + kDeriveChunk <= T;
+        if (active && steady) {
+#pragma unroll
+            for (int j = 0; j < kDeriveChunk; ++j) {
+                const float in0[3] = {xin[0][j], xin[1][j], xin[2][j]};
+                const float e[3] = {es[vl][0][j], es[vl][1][j], es[vl][2][j]};
+                const float cg[3] = {cs[vl][0][j], cs[vl][1][j], cs[vl][2][j]};
+                derive_step<true>(c, in0, e, cg, has, a, D, t0 + j, reg, p);
+            }
+        } else if (active) {
 #pragma unroll
             for (int j = 0; j < kDeriveChunk; ++j) {
                 if (t0 + j < T) {
                     const float in0[3] = {xin[0][j], xin[1][j], xin[2][j]};
                     const float e[3] = {es[vl][0][j], es[vl][1][j], es[vl][2][j]};
                     const float cg[3] = {cs[vl][0][j], cs[vl][1][j], cs[vl][2][j]};
-                    derive_step(c, in0, e, cg, has, a, D, t0 + j, reg, p);
+                    derive_step<false>(c, in0, e, cg, has, a, D, t0 + j, reg, p);
                 }
             }
         }
