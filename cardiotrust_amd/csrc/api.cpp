@@ -116,7 +116,7 @@ ct::ResidentJob resident_job(ct_results_gpu &r, int32_t ckpt_interval) {
     j.u = r.u;
     j.ckpt = r.ckpt;
     j.gb = r.barrier;
-    j.flags = r.resident_blocks <= ct::kMaxSlabFlags ? r.flags : nullptr;
+    j.flags = r.flags;
     j.N = r.N;
     j.T = r.T;
     j.ckpt_interval = ckpt_interval;
@@ -351,7 +351,7 @@ int plan_resident(ct_results_gpu &r) {
     }
     const int64_t capacity = int64_t(cus) * per_cu;
     const int64_t blocks = std::min<int64_t>(capacity, std::max<int64_t>(1, (r.N + 127) / 128));
-    if ((r.N + blocks - 1) / blocks > ct::kResMaxVoxels) return CT_OK;
+    if ((r.N + blocks - 1) / blocks > ct::kResMaxVoxels || blocks > ct::kMaxSlabFlags) return CT_OK;
     if (r.N >= (1 << 17) - 1 || r.T > 32767 || int64_t(r.T) * r.n >= (int64_t(1) << 32))
         return CT_OK;
     r.resident_blocks = int32_t(blocks);

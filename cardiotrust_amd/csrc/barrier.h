@@ -1,4 +1,5 @@
-// Barrier over a team of co-resident workgroups (resident kernels).
+// Synchronisation of co-resident workgroups (resident kernels): a counting
+// barrier in two halves, and per-slab step flags.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -124,13 +125,6 @@ __device__ __forceinline__ bool slab_wait(const uint32_t *flags, GridBarrier *gb
     }
     __syncthreads();
     return *ok_lds != 0;
-}
-
-// Both halves back to back.
-__device__ __forceinline__ bool team_barrier(GridBarrier *gb, uint32_t members, uint32_t member,
-                                             uint32_t &gen, int *ok_lds, uint64_t wait_ticks) {
-    team_arrive(gb, members, member, gen);
-    return team_wait(gb, gen, ok_lds, wait_ticks);
 }
 
 }  // namespace ct

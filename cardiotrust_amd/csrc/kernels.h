@@ -61,15 +61,15 @@ struct ResidentJob {
     const float *b;
     const float *u;
     float *ckpt;          // filter outputs every ckpt_interval steps (or unused)
-    GridBarrier *gb;        // abort word; the team barrier when flags is null
-    uint32_t *flags;        // completed steps per team member (width words), or null
+    GridBarrier *gb;        // the job's sticky abort word
+    uint32_t *flags;        // completed steps per team member (width words)
     int32_t N;
     int32_t T;
     int32_t ckpt_interval;  // 0 = no checkpoints
     int32_t slot0;
     int32_t width;          // workgroups in the team
 };
-// Most slabs a single-job resident launch can synchronise with flags.
+// Most workgroups of a resident launch (one flag word each).
 constexpr int kMaxSlabFlags = 4096;
 size_t resident_lds_bytes();
 hipError_t resident_capacity(int32_t *blocks_per_cu);

@@ -158,13 +158,14 @@ constexpr int kResNeighbourBits = 17;  // voxel index field; delay or connection
 static_assert(kSortSize >= kResMaxVoxels * 26 && kSortSize < (1 << 14) + 1, "sort size");
 static_assert(kSortSize * 8 <= kResMaxVoxels * 78 * 4, "sort keys must fit over the filter outputs");
 
-// Steps [t_begin, t_end) of voxels [vb, ve) of job `j`, as member `member` of
-// the job's team. The first barrier waits up to `first_wait` ticks (team
-// members may still be finishing earlier jobs), the others `step_wait`.
-// Returns false if the job's barrier aborted.
+// Steps [t_begin, t_end) of voxels [vb, ve), one slab of job `j`. After each
+// step the slab publishes its progress in j.flags; before the next it waits
+// for the slabs owning its neighbours, the first wait up to `first_wait`
+// ticks (they may still be finishing earlier jobs), the others `step_wait`.
+// Returns false if the job aborted (its sticky abort word is set).
 template <int kGather, int kGainConns, int kCachedRows>
 __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, int32_t ve,
-                                             uint32_t member, int32_t t_begin, int32_t t_end,
+                                             int32_t t_begin, int32_t t_end,
                                              float *lds, uint64_t first_wait, uint64_t step_wait) {
     float *ys = lds;                                                       // [voxel][78]
     int *ok_lds = reinterpret_cast<int *>(lds + kResMaxVoxels * 78);     // [4]
@@ -176,7 +177,6 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
     const float *__restrict__ b = j.b;
     const float *__restrict__ u = j.u;
     float *__restrict__ x = j.x;
-    uint32_t gen = 0;
     const int64_t n = 3LL * N;
     const int32_t nv = ve - vb;
     const int32_t items = nv * 26;
@@ -274,7 +274,7 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
 
     for (int32_t t = t_begin; t < t_end; ++t) {
         const bool last = (t == T - 1);
-        if (j.flags && t > t_begin &&
+        if (t > t_begin &&
             !slab_wait(j.flags, j.gb, smin, smax, me, uint32_t(t - t_begin), ok_lds,
                        t == t_begin + 1 ? first_wait : step_wait))
             return false;  // host reports the abort
@@ -359,12 +359,7 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
             float *dst = j.ckpt + int64_t((t + 1) / K - 1) * N * 78 + int64_t(vb) * 78;
             for (int idx = tid; idx < nv * 78; idx += kResThreads) dst[idx] = ys[idx];
         }
-        if (j.flags) {
-            slab_publish(j.flags, me, uint32_t(t + 1 - t_begin));
-        } else if (!team_barrier(j.gb, uint32_t(j.width), member, gen, ok_lds,
-                                 t == t_begin ? first_wait : step_wait)) {
-            return false;  // host reports the abort
-        }
+        slab_publish(j.flags, me, uint32_t(t + 1 - t_begin));
     }
     for (int idx = tid; idx < nv * 78; idx += kResThreads) j.y[int64_t(vb) * 78 + idx] = ys[idx];
     return true;
@@ -377,8 +372,7 @@ __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
     const int64_t slab = xcd_block(blockIdx.x, gridDim.x);
     const int32_t vb = int32_t(slab * job.N / gridDim.x);
     const int32_t ve = int32_t((slab + 1) * job.N / gridDim.x);
-    resident_run<kResBatch, kResGainConns, kResCachedRows>(job, vb, ve, blockIdx.x, t_begin, t_end,
-                                                           lds,
+    resident_run<kResBatch, kResGainConns, kResCachedRows>(job, vb, ve, t_begin, t_end, lds,
                                            wait_ticks, wait_ticks);
 }
 
@@ -400,7 +394,7 @@ __global__ __launch_bounds__(kResThreads) void propagate_batch_kernel(
         const int32_t member = slot - job.slot0;
         const int32_t vb = int32_t(int64_t(member) * job.N / job.width);
         const int32_t ve = int32_t(int64_t(member + 1) * job.N / job.width);
-        if (!resident_run<kBatchGather, kBatchGainConns, 0>(job, vb, ve, uint32_t(member), 0, job.T,
+        if (!resident_run<kBatchGather, kBatchGainConns, 0>(job, vb, ve, 0, job.T,
                                                          lds, first_wait, step_wait))
             return;
         __syncthreads();  // LDS is reused by the next job
@@ -1263,7 +1257,9 @@ hipError_t resident_capacity(int32_t *blocks_per_cu) {
 
 hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const ResidentJob &job,
                                      int32_t t_begin, int32_t t_end, uint64_t wait_ticks) {
-    // the barrier and the slab flags start from zero: a kernel rather than a memset node
+    // the abort word's neighbours and the slab flags start from zero: a
+    // kernel rather than a memset node
+    if (!job.flags || blocks > kMaxSlabFlags) return hipErrorInvalidValue;
     hipError_t e = launch_resident_reset(s, job.gb, job.flags, blocks);
     if (e != hipSuccess) return e;
     propagate_resident_kernel<<<blocks, kResThreads, resident_lds_bytes(), s>>>(job, t_begin, t_end,
