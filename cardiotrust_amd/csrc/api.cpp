@@ -136,7 +136,7 @@ int propagate_beat(ct_results_gpu &r, ct_epoch_kernel *k, int32_t ckpt_interval)
     if (resident && r.N > 0) {
         Span sp(k, kPropagate, r.T);
         CT_HIP(ct::launch_propagate_resident(s, r.resident_blocks, resident_job(r, ckpt_interval),
-                                             0, r.T, wait_ticks(*r.gpu, 2.0)));
+                                             0, r.T, wait_ticks(*r.gpu, r.wait_seconds)));
         r.resident_used = true;
     } else {
         for (int32_t t = 0; t < r.T; ++t) {
@@ -363,9 +363,10 @@ int plan_resident(ct_results_gpu &r) {
     return CT_OK;
 }
 
-// After resident launches: waits for the stream and reports a grid-barrier
-// timeout (workgroups that were not all resident), which leaves the results of
-// that launch invalid. The path is then disabled for this problem.
+// After resident launches: waits for the stream and reports a timed-out wait
+// (workgroups not all resident, or a wait limit too short), which leaves the
+// results of that launch invalid. The resident forward and backward paths are
+// then disabled for this problem; the abort word stays set.
 int check_resident(ct_results_gpu &r) {
     if (!r.resident_used) return CT_OK;
     r.resident_used = false;
@@ -374,8 +375,10 @@ int check_resident(ct_results_gpu &r) {
     CT_HIP(hipMemcpy(&gb, r.barrier, sizeof(gb), hipMemcpyDeviceToHost));
     if (gb.abort.count) {
         r.resident_ok = false;
-        return fail(CT_ERR_DEVICE, "resident propagation: grid barrier timed out (workgroups not "
-                                   "all resident); the per-step path is used from now on");
+        r.sweep_ok = false;
+        return fail(CT_ERR_DEVICE, "resident propagation: a wait timed out (workgroups not all "
+                                   "resident, or the wait limit too short); the per-step path "
+                                   "is used from now on");
     }
     return CT_OK;
 }
@@ -529,7 +532,7 @@ static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K,
                                                      r.delays, r.staging, r.coefs, r.gcons, r.x,
                                                      r.E, r.xbar,
                                                      r.barrier, r.N, r.T, p,
-                                                     wait_ticks(*r.gpu, 2.0)));
+                                                     wait_ticks(*r.gpu, r.wait_seconds)));
             r.resident_used = true;
         } else {
             CT_HIP(ct::launch_consumer_delays(s, r.cons_idx, r.delays, r.cons_delay, r.cons_len));
@@ -1121,6 +1124,14 @@ int ct_results_set_propagation(ct_results_gpu *results, int32_t mode, int32_t *r
         return fail(CT_ERR_INVALID, "the resident propagation kernel cannot hold this problem");
     results->propagation_mode = mode;
     if (resident) *resident = (results->resident_ok && mode != 1) ? 1 : 0;
+    return CT_OK;
+}
+
+int ct_results_set_wait_limit(ct_results_gpu *results, double seconds) {
+    if (!results) return fail(CT_ERR_INVALID, "null argument");
+    if (!(seconds >= 0.0) || !std::isfinite(seconds))
+        return fail(CT_ERR_INVALID, "wait limit must be finite and >= 0 seconds");
+    results->wait_seconds = seconds;
     return CT_OK;
 }
 

@@ -58,7 +58,7 @@ __device__ __forceinline__ bool team_wait(GridBarrier *gb, uint32_t gen, int *ok
                 ok = 0;
                 break;
             }
-            if ((++polls & 63u) == 0 && uint64_t(wall_clock64()) - t0 > wait_ticks) {
+            if ((polls++ & 63u) == 0 && uint64_t(wall_clock64()) - t0 > wait_ticks) {
                 __hip_atomic_store(&gb->abort.count, 1u, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
@@ -88,8 +88,9 @@ __device__ __forceinline__ void slab_publish(uint32_t *flags, int32_t me, uint32
 }
 
 // Waits (at most `wait_ticks` of wall clock) until every slab in [smin, smax]
-// but `me` has completed `steps` steps; wave 0 polls 64 flags per round. False
-// if the job aborted or the wait timed out (which raises the sticky abort).
+// but `me` has completed `steps` steps; wave 0 polls 64 flags per round and
+// reads the clock on the first unsatisfied round and every 64th. False if the
+// job aborted or the wait timed out (which raises the sticky abort).
 __device__ __forceinline__ bool slab_wait(const uint32_t *flags, GridBarrier *gb, int32_t smin,
                                           int32_t smax, int32_t me, uint32_t steps, int *ok_lds,
                                           uint64_t wait_ticks) {
@@ -110,7 +111,7 @@ __device__ __forceinline__ bool slab_wait(const uint32_t *flags, GridBarrier *gb
                 ok = 0;
                 break;
             }
-            if ((++polls & 63u) == 0 && uint64_t(wall_clock64()) - t0 > wait_ticks) {
+            if ((polls++ & 63u) == 0 && uint64_t(wall_clock64()) - t0 > wait_ticks) {
                 if (threadIdx.x == 0)
                     __hip_atomic_store(&gb->abort.count, 1u, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);

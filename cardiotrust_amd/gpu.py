@@ -25,7 +25,7 @@ EXPORTS = (
     "ct_measurements_to_gpu", "ct_buffer_free", "ct_results_to_gpu", "ct_results_gpu_free",
     "ct_results_update_from_gpu", "ct_results_simulate", "ct_results_simulate_noisy",
     "ct_measurement_matrix", "ct_model_connections", "ct_results_spherical",
-    "ct_epoch_kernel_last_loss",
+    "ct_epoch_kernel_last_loss", "ct_results_set_wait_limit",
     "ct_results_set_propagation",
     "ct_results_forward", "ct_results_backward", "ct_epoch_kernel_set_gradient",
     "ct_epoch_kernel_new",
@@ -122,6 +122,7 @@ def lib() -> ctypes.CDLL:
         "ct_results_simulate_noisy": ([vp, vp, ctypes.c_uint64, vp], ctypes.c_int),
         "ct_results_spherical": ([vp, ctypes.c_float] + [vp] * 7, ctypes.c_int),
         "ct_epoch_kernel_last_loss": ([vp, vp, vp, vp, vp], ctypes.c_int),
+        "ct_results_set_wait_limit": ([vp, ctypes.c_double], ctypes.c_int),
         "ct_measurement_matrix": ([vp, ctypes.c_int32, vp, ctypes.c_int32, vp, vp, ctypes.c_int32,
                                    vp, ctypes.c_float, vp], ctypes.c_int),
         "ct_model_connections": ([vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp, vp, vp,
@@ -328,6 +329,11 @@ class ResultsGPU:
             _check(lib().ct_results_simulate_noisy(self.handle, _ptr(sd), ctypes.c_uint64(seed),
                                                    _ptr(out)), "simulate_noisy")
         return out
+
+    def set_wait_limit(self, seconds: float) -> None:
+        """Bound on every resident wait (ct_results_set_wait_limit)."""
+        _check(lib().ct_results_set_wait_limit(self.handle, ctypes.c_double(seconds)),
+               "set_wait_limit")
 
     def spherical(self, sample_rate_hz: float, full: bool = False) -> dict:
         """Spherical states of the device states (ct_results_spherical): the

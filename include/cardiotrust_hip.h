@@ -213,6 +213,13 @@ int ct_results_simulate_noisy(ct_results_gpu *results, const float *sensor_std, 
  * `resident` (may be NULL) receives 1 if the resident path will run. */
 int ct_results_set_propagation(ct_results_gpu *results, int32_t mode, int32_t *resident);
 
+/* Bound on every wait of the resident kernels, in seconds of wall clock
+ * (default 2; no reference counterpart: a diagnostic switch of this library).
+ * A wait that exceeds it sets the launch's sticky abort word, every workgroup
+ * leaves at its next wait, the call returns CT_ERR_DEVICE, and the results
+ * object uses the per-step kernels from then on. */
+int ct_results_set_wait_limit(ct_results_gpu *results, double seconds);
+
 /* EpochKernel::new (src/core/algorithm/gpu/epoch.rs:29-93). The sizes must
  * match the results; `actual_measurements` must be (B,T,S). */
 int ct_epoch_kernel_new(ct_gpu *gpu, ct_results_gpu *results, const ct_buffer *actual_measurements,

@@ -377,3 +377,43 @@ def test_spherical_states_on_device(gpu):
             ulp = np.abs(a[ok].view(np.int32).astype(np.int64) - b[ok].view(np.int32).astype(np.int64))
             assert ulp.max(initial=0) <= 4, (k, ulp.max())
         note(f"  N={p.num_voxels}: activation times {np.unique(got['activation_time_ms']).size} distinct")
+
+
+def test_wait_timeout_aborts_and_falls_back_to_per_step(gpu):
+    """A failure forced from the host through the kernel's own bounded wait: a
+    wait limit of 0 s makes a resident launch time out at the first wait that
+    is not already satisfied. Every workgroup then leaves at its next wait, the
+    call reports CT_ERR_DEVICE, and the results object runs the per-step
+    kernels, whose states equal the per-step path's bit for bit. (A launch in
+    which no wait has to wait completes normally; one of three attempts must
+    time out.)"""
+    from cardiotrust_amd.gpu import CTError, ResultsGPU
+    note("case: 24^3 shell, T=600, resident wait limit 0 s, then the per-step fallback")
+    g0 = synth.ellipsoid_shell(shape=(24, 24, 24), outer=(12, 12, 12), inner=(6, 6, 6))
+    p = synth.make_problem(g0, num_steps=600, max_delay=40, seed=14)
+    ref = ResultsGPU(gpu, p)
+    ref.set_propagation(1)
+    m_ref = ref.simulate()
+    x_ref = ref.update_from_gpu(["system_states"])["system_states"]
+    ref.close()
+    r = ResultsGPU(gpu, p)
+    assert r.set_propagation(0)
+    r.set_wait_limit(0.0)
+    timed_out = False
+    for _ in range(3):
+        try:
+            r.simulate()
+        except CTError as e:
+            assert "timed out" in str(e)
+            timed_out = True
+            break
+    assert timed_out
+    assert not r.set_propagation(0)  # the resident path is off for this problem now
+    r.set_wait_limit(2.0)
+    m = r.simulate()
+    x = r.update_from_gpu(["system_states"])["system_states"]
+    r.close()
+    assert np.array_equal(x.view(np.uint32), x_ref.view(np.uint32))
+    assert np.array_equal(m, m_ref)
+    with pytest.raises(CTError):
+        ResultsGPU(gpu, p).set_wait_limit(-1.0)
