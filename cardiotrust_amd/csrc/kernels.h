@@ -34,7 +34,9 @@ hipError_t launch_propagate_step(hipStream_t s, const int32_t *nbr, const float 
 // Resident propagation (one launch for steps [t_begin, t_end)). A workgroup
 // holds at most kResMaxVoxels voxels; requires N < 2^17 - 1,
 // blocks * kResMaxVoxels >= N and blocks <= what fits on the device at once.
-// `abort` != 0 after the launch means the grid barrier timed out.
+// `abort` != 0 after the launch means a wait timed out. The counting barrier
+// words serve the resident adjoint sweep; the propagation kernels wait on
+// per-slab flags and use only `abort`.
 constexpr int kResMaxVoxels = 472;
 constexpr int kBarrierGroups = 8;
 struct alignas(128) BarrierWord {
@@ -48,8 +50,8 @@ struct GridBarrier {
     BarrierWord abort;  // sticky: set on a timed-out wait, cleared only by the host
 };
 static_assert(sizeof(GridBarrier) == 11 * sizeof(BarrierWord), "barrier layout");
-// One job of the resident propagation: one scenario's beat. `gb` is the
-// job's barrier; slot0 / width place its team in a batched launch.
+// One job of the resident propagation: one scenario's beat. slot0 / width
+// place its team in a batched launch.
 struct ResidentJob {
     const int32_t *nbr;
     const float *coefs;
@@ -76,8 +78,9 @@ hipError_t resident_capacity(int32_t *blocks_per_cu);
 hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const ResidentJob &job,
                                      int32_t t_begin, int32_t t_end, uint64_t wait_ticks);
 // Batched resident propagation: `slots` workgroups (all resident at once) run
-// the task lists task_job[task_ptr[q] .. task_ptr[q+1]) of slot q; `jobs` and
-// the `num_jobs` barriers are device arrays.
+// the task lists task_job[task_ptr[q] .. task_ptr[q+1]) of slot q; `jobs`, the
+// `num_jobs` abort words and the jobs' `num_flags` slab flags are device
+// arrays, cleared here before the launch.
 hipError_t batch_capacity(int32_t *blocks_per_cu);
 hipError_t launch_propagate_batch(hipStream_t s, int32_t slots, const ResidentJob *jobs,
                                   const int32_t *task_ptr, const int32_t *task_job,

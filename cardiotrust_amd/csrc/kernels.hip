@@ -124,12 +124,13 @@ __global__ __launch_bounds__(kPropThreads) void propagate_step_kernel(
 // and its connection tables (neighbour, delay, coefficient) in registers, so
 // per step only the gains and the delayed neighbour states are read from
 // memory. Phase 1 (one lane per connection) and phase 2 (one lane per state)
-// are the same arithmetic as propagate_step_kernel; a barrier over the
-// workgroups of the job (its team) separates the steps.
+// are the same arithmetic as propagate_step_kernel. Between steps each slab
+// publishes its progress in the job's flags and waits only for the slabs that
+// own its neighbours.
 //
 // The host launches no more workgroups than fit on the device at once. Every
-// barrier wait is bounded in wall-clock time: a workgroup that waits longer
-// raises `abort`, every workgroup of the job leaves at its next barrier, and
+// wait is bounded in wall-clock time: a workgroup that waits longer raises the
+// job's `abort` word, every workgroup of the job leaves at its next wait, and
 // the host reports the failure instead of the kernel waiting forever.
 // ---------------------------------------------------------------------------
 
