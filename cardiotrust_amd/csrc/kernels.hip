@@ -424,14 +424,15 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
     floatx16 acc[kProjTile / 32];
 #pragma unroll
     for (int q = 0; q < kProjTile / 32; ++q) acc[q] = floatx16{0};
-    // loader: 8 lanes cover one row's 32 k values (one 128-byte line)
+    // loader: 8 lanes cover one row's 32 k values (one 128-byte line). The
+    // next k tile is loaded into registers while the MFMAs run on this one.
     const int lrow = tid >> 3, lk = (tid & 7) * 4;
-    for (int64_t kb = k0; kb < k1; kb += kGemmTileK) {
+    float4_a4 xa[kProjTile / 32], ha[4];
+    auto load_tile = [&](int64_t kb) __attribute__((always_inline)) {
         const bool full = kb + kGemmTileK <= k1;
 #pragma unroll
         for (int pass = 0; pass < kProjTile / 32; ++pass) {
-            const int row = pass * 32 + lrow;
-            const int32_t t = t0 + row;
+            const int32_t t = t0 + pass * 32 + lrow;
             float4_a4 v = {0.0f, 0.0f, 0.0f, 0.0f};
             if (t < T) {
                 const float *src = x + int64_t(t) * n + kb + lk;
@@ -442,13 +443,11 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
                     for (int j = 0; j < 4; ++j) v[j] = (kb + lk + j < k1) ? src[j] : 0.0f;
                 }
             }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) As[lk + j][row] = v[j];
+            xa[pass] = v;
         }
 #pragma unroll
         for (int pass = 0; pass < 4; ++pass) {
-            const int row = pass * 32 + lrow;
-            const float *src = H + int64_t(s0 + row) * n + kb + lk;
+            const float *src = H + int64_t(s0 + pass * 32 + lrow) * n + kb + lk;
             float4_a4 v = {0.0f, 0.0f, 0.0f, 0.0f};
             if (full) {
                 v = *reinterpret_cast<const float4_a4 *>(src);
@@ -456,18 +455,29 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
 #pragma unroll
                 for (int j = 0; j < 4; ++j) v[j] = (kb + lk + j < k1) ? src[j] : 0.0f;
             }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) Bs[lk + j][row] = v[j];
+            ha[pass] = v;
         }
+    };
+    if (k0 < k1) load_tile(k0);
+    for (int64_t kb = k0; kb < k1; kb += kGemmTileK) {
+#pragma unroll
+        for (int pass = 0; pass < kProjTile / 32; ++pass)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) As[lk + j][pass * 32 + lrow] = xa[pass][j];
+#pragma unroll
+        for (int pass = 0; pass < 4; ++pass)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) Bs[lk + j][pass * 32 + lrow] = ha[pass][j];
         __syncthreads();
+        if (kb + kGemmTileK < k1) load_tile(kb + kGemmTileK);
 #pragma unroll
         for (int kk = 0; kk < kGemmTileK; kk += 2) {
             const int kr = kk + (lane >> 5);
             const float bv = Bs[kr][wave * 32 + (lane & 31)];
 #pragma unroll
             for (int q = 0; q < kProjTile / 32; ++q) {
-                const float a = As[kr][q * 32 + (lane & 31)];
-                acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv, acc[q], 0, 0, 0);
+                const float av = As[kr][q * 32 + (lane & 31)];
+                acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[q], 0, 0, 0);
             }
         }
         __syncthreads();
@@ -634,37 +644,44 @@ __global__ __launch_bounds__(256) void map_residuals_kernel(const float *__restr
 #pragma unroll
     for (int q = 0; q < kMapTile / 32; ++q) acc[q] = floatx16{0};
     const bool full_i = i0 + 128 <= n;
+    // H rows of a k tile: 32 lanes cover one row; R rows: 8 lanes per row. The
+    // next k tile is loaded into registers while the MFMAs run on this one.
+    const int hrow = tid >> 5, hq = (tid & 31) * 4;
+    const int rrow = tid >> 3, rq = (tid & 7) * 4;
+    float4_a4 hv[4], rv[kMapTile / 32];
+    auto load_tile = [&](int32_t sb) __attribute__((always_inline)) {
+#pragma unroll
+        for (int pass = 0; pass < 4; ++pass) {
+            const float *src = H + int64_t(sb + pass * 8 + hrow) * n + i0 + hq;
+            float4_a4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (full_i) {
+                v = *reinterpret_cast<const float4_a4 *>(src);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) v[j] = (i0 + hq + j < n) ? src[j] : 0.0f;
+            }
+            hv[pass] = v;
+        }
+#pragma unroll
+        for (int pass = 0; pass < kMapTile / 32; ++pass) {
+            const int32_t t = t0 + pass * 32 + rrow;
+            float4_a4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (t < T) v = *reinterpret_cast<const float4_a4 *>(R + int64_t(t) * Sp + sb + rq);
+            rv[pass] = v;
+        }
+    };
+    if (Sp > 0) load_tile(0);
     for (int32_t sb = 0; sb < Sp; sb += kGemmTileK) {
-        {   // H rows sb..sb+31 over states i0..i0+127: 32 lanes cover one row
-            const int row = tid >> 5, cq = (tid & 31) * 4;
 #pragma unroll
-            for (int pass = 0; pass < 4; ++pass) {
-                const int rr = pass * 8 + row;
-                const float *src = H + int64_t(sb + rr) * n + i0 + cq;
-                float4_a4 v = {0.0f, 0.0f, 0.0f, 0.0f};
-                if (full_i) {
-                    v = *reinterpret_cast<const float4_a4 *>(src);
-                } else {
+        for (int pass = 0; pass < 4; ++pass)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) v[j] = (i0 + cq + j < n) ? src[j] : 0.0f;
-                }
+            for (int j = 0; j < 4; ++j) Hs[pass * 8 + hrow][hq + j] = hv[pass][j];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) Hs[rr][cq + j] = v[j];
-            }
-        }
-        {   // R rows t0.., sensors sb..sb+31: 8 lanes per row
-            const int row = tid >> 3, kq = (tid & 7) * 4;
+        for (int pass = 0; pass < kMapTile / 32; ++pass)
 #pragma unroll
-            for (int pass = 0; pass < kMapTile / 32; ++pass) {
-                const int rr = pass * 32 + row;
-                const int32_t t = t0 + rr;
-                float4_a4 v = {0.0f, 0.0f, 0.0f, 0.0f};
-                if (t < T) v = *reinterpret_cast<const float4_a4 *>(R + int64_t(t) * Sp + sb + kq);
-#pragma unroll
-                for (int j = 0; j < 4; ++j) Rs[kq + j][rr] = v[j];
-            }
-        }
+            for (int j = 0; j < 4; ++j) Rs[rq + j][pass * 32 + rrow] = rv[pass][j];
         __syncthreads();
+        if (sb + kGemmTileK < Sp) load_tile(sb + kGemmTileK);
 #pragma unroll
         for (int kk = 0; kk < kGemmTileK; kk += 2) {
             const int kr = kk + (lane >> 5);
