@@ -98,6 +98,7 @@ int prepare_propagation(ct_results_gpu &r, int32_t ckpt_interval) {
     hipStream_t s = r.gpu->stream;
     if (ckpt_interval > 0) CT_CHECK(ensure_checkpoints(r, ckpt_interval));
     r.ckpt_interval = ckpt_interval;
+    r.xT_fresh = false;  // the states are about to change
     CT_HIP(ct::launch_fill(s, r.y, 0.0f, int64_t(r.N) * 78));
     CT_HIP(ct::launch_fill(s, r.y_last, 0.0f, int64_t(r.N) * 78));
     return CT_OK;
@@ -158,7 +159,8 @@ int project_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_ke
     r.forward_beat = beat;
     Span sp(k, kProject);
     const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
-    CT_HIP(ct::launch_project(s, r.x, Hb, r.partial, r.T, r.n, r.Sp, r.splits, r.kps));
+    CT_HIP(ct::launch_project(s, r.x, Hb, r.partial, r.xT, r.T, r.n, r.Sp, r.splits, r.kps));
+    r.xT_fresh = true;
     CT_HIP(ct::launch_residuals(s, r.partial, r.splits,
                                 mhat ? mhat + int64_t(beat) * r.T * r.S : nullptr,
                                 r.m + int64_t(beat) * r.T * r.S, r.R, r.loss_mse, r.T, r.S, r.Sp));
@@ -194,9 +196,10 @@ int derive_reference(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, ct_
         CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.eT, r.n, r.T, r.Sp));
     }
     if (c.freeze_gains && c.freeze_delays) return CT_OK;
-    {
+    if (!r.xT_fresh) {
         Span sp(k, kTranspose);
         CT_HIP(ct::launch_transpose(s, r.x, r.xT, r.T, r.n));
+        r.xT_fresh = true;
     }
     Span sp(k, kDerive);
     ct::DeriveParams p;
@@ -546,7 +549,10 @@ static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K,
     }
     Span sp(k, kDerive);
     CT_HIP(ct::launch_transpose(s, r.xbar, r.xbarT, T, n));
-    CT_HIP(ct::launch_transpose(s, r.x, r.xT, T, n));
+    if (!r.xT_fresh) {
+        CT_HIP(ct::launch_transpose(s, r.x, r.xT, T, n));
+        r.xT_fresh = true;
+    }
     CT_HIP(ct::launch_adjoint_gradient(s, r.nbr, r.coefs, r.delays, r.gT, r.xT, r.xbarT, r.ckpt,
                                        r.yseg, r.dGT, r.dA, r.N, r.T, K));
     return CT_OK;

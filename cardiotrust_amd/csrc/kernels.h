@@ -87,8 +87,11 @@ hipError_t launch_propagate_batch(hipStream_t s, int32_t slots, const ResidentJo
                                   GridBarrier *barriers, int32_t num_jobs, uint32_t *flags,
                                   int32_t num_flags, uint64_t first_wait, uint64_t step_wait);
 
+// H·Xᵀ split-K partial sums; with xT not null the same pass also writes the
+// states transposed, xT (n, T).
 hipError_t launch_project(hipStream_t s, const float *x, const float *H, float *partial,
-                          int32_t T, int64_t n, int32_t Sp, int32_t splits, int64_t k_per_split);
+                          float *xT, int32_t T, int64_t n, int32_t Sp, int32_t splits,
+                          int64_t k_per_split);
 
 hipError_t launch_residuals(hipStream_t s, const float *partial, int32_t splits, const float *mhat,
                             float *m, float *R, float *loss_mse, int32_t T, int32_t S, int32_t Sp);

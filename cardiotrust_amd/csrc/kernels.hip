@@ -409,7 +409,8 @@ __global__ __launch_bounds__(kResThreads) void propagate_batch_kernel(
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ x,
                                                       const float *__restrict__ H,
-                                                      float *__restrict__ partial, int32_t T,
+                                                      float *__restrict__ partial,
+                                                      float *__restrict__ xT, int32_t T,
                                                       int64_t n, int32_t Sp, int64_t kps) {
     // k-major tiles (rows padded by 4 floats): the MFMA operand reads of a
     // wave are 32 consecutive floats, free of bank conflicts
@@ -470,6 +471,26 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
             for (int j = 0; j < 4; ++j) Bs[lk + j][pass * 32 + lrow] = ha[pass][j];
         __syncthreads();
         if (kb + kGemmTileK < k1) load_tile(kb + kGemmTileK);
+        if (xT && blockIdx.z == 0) {
+            // the k-major tile is the states-by-step layout: one pass over x
+            // also writes xT (row k: 16 consecutive steps per thread)
+            const int kr = tid >> 3, c0 = (tid & 7) * 16;
+            const int64_t k = kb + kr;
+            if (k < k1) {
+                float *dst = xT + k * T + t0 + c0;
+                if (t0 + c0 + 16 <= T) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        float4_a4 v;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) v[j] = As[kr][c0 + 4 * q + j];
+                        *reinterpret_cast<float4_a4 *>(dst + 4 * q) = v;
+                    }
+                } else {
+                    for (int c = 0; c < 16 && t0 + c0 + c < T; ++c) dst[c] = As[kr][c0 + c];
+                }
+            }
+        }
 #pragma unroll
         for (int kk = 0; kk < kGemmTileK; kk += 2) {
             const int kr = kk + (lane >> 5);
@@ -1384,9 +1405,10 @@ hipError_t launch_propagate_batch(hipStream_t s, int32_t slots, const ResidentJo
 }
 
 hipError_t launch_project(hipStream_t s, const float *x, const float *H, float *partial,
-                          int32_t T, int64_t n, int32_t Sp, int32_t splits, int64_t kps) {
+                          float *xT, int32_t T, int64_t n, int32_t Sp, int32_t splits,
+                          int64_t kps) {
     dim3 grid(blocks_for(T, kProjTile), splits, Sp / 128);
-    project_kernel<<<grid, 256, 0, s>>>(x, H, partial, T, n, Sp, kps);
+    project_kernel<<<grid, 256, 0, s>>>(x, H, partial, xT, T, n, Sp, kps);
     return hipGetLastError();
 }
 

@@ -74,6 +74,7 @@ struct ct_results_gpu {
     // problem fits on chip, else one launch per step
     int32_t propagation_mode = 0;  // 0 auto, 1 per-step launches, 2 resident
     double wait_seconds = 2.0;     // bound on every resident wait (ct_results_set_wait_limit)
+    bool xT_fresh = false;         // xT holds the current states (written by the projection)
     bool resident_ok = false;
     bool resident_used = false;  // a resident launch is in flight since the last check
     int32_t resident_blocks = 0;
