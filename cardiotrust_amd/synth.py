@@ -285,7 +285,8 @@ def make_problem(geometry: Geometry, num_steps: int, max_delay: int, seed: int,
                  perturbation: float = 0.2,
                  beat_offsets_mm: Optional[np.ndarray] = None,
                  slow_region: Optional[np.ndarray] = None,
-                 simulate: Optional[Callable[["Problem"], np.ndarray]] = None) -> Problem:
+                 simulate: Optional[Callable[["Problem"], np.ndarray]] = None,
+                 sensors_per_axis: int = 6) -> Problem:
     """Random ground-truth parameters, their perturbed initial estimate and,
     when `simulate` is given, the noise-free measurements of the truth.
 
@@ -319,7 +320,8 @@ def make_problem(geometry: Geometry, num_steps: int, max_delay: int, seed: int,
     a0 = np.clip(a.astype(np.float64) * (1.0 + perturbation * ea), 1e-4, 1.0 - 1e-4)
     a0 = np.where(exists, a0, 0.5).astype(np.float32)
 
-    sensors = sensor_grid(geometry, num_beats=num_beats, beat_offsets_mm=beat_offsets_mm)
+    sensors = sensor_grid(geometry, per_axis=sensors_per_axis, num_beats=num_beats,
+                          beat_offsets_mm=beat_offsets_mm)
     H = measurement_matrix(geometry, sensors)
     u = gaussian_derivative_pulse(num_steps)
     b = np.zeros(3 * N, dtype=np.float32)

@@ -417,3 +417,28 @@ def test_wait_timeout_aborts_and_falls_back_to_per_step(gpu):
     assert np.array_equal(m, m_ref)
     with pytest.raises(CTError):
         ResultsGPU(gpu, p).set_wait_limit(-1.0)
+
+
+@pytest.mark.parametrize("mode", [1, 2], ids=["per-step", "resident"])
+def test_more_than_128_sensors(gpu, mode):
+    """243 sensors (a 9 x 9 grid of 3-axis magnetometers) pad to two 128-row
+    sensor tiles: H·Xᵀ runs two sensor tiles (only the first writes xT), Hᵀ·R
+    eight k tiles. Two epochs against the oracle; states and delays bit-exact
+    in the first."""
+    note(f"case: blob seed 11, 243 sensors, T=90, 2 epochs, mode {mode}")
+    rng_u = synth.uniform(11, 99, 7 * 6 * 5).reshape(7, 6, 5)
+    mask = rng_u < 0.6
+    mask[0, 0, 0] = True
+    g = synth.grid_geometry(mask, 2.5)
+    p = synth.make_problem(g, num_steps=90, max_delay=14, seed=11, gain_sigma=0.12,
+                           simulate=oracle.simulate_f32, sensors_per_axis=9)
+    assert p.num_sensors == 243
+    cfg = Algorithm(learning_rate=0.3, freeze_delays=False, maximum_regularization_threshold=0.05)
+    g1, _ = run_gpu_epochs(gpu, p, cfg, mode=mode)
+    o32 = run_oracle_epochs(p, cfg, np.float32)
+    o64 = run_oracle_epochs(p, cfg, np.float64)
+    compare(g1, o32, o64)
+    g2, _ = run_gpu_epochs(gpu, p, cfg, epochs=2, nbatch=2, mode=mode)
+    o32 = run_oracle_epochs(p, cfg, np.float32, epochs=2, nbatch=2)
+    o64 = run_oracle_epochs(p, cfg, np.float64, epochs=2, nbatch=2)
+    compare(g2, o32, o64, exact=("delays",), close=CLOSE + ("system_states",))
