@@ -210,7 +210,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10, help="timed epochs")
     ap.add_argument("--warmup", type=int, default=2, help="untimed epochs")
     ap.add_argument("--num-steps", type=int, default=2000, help="time steps per beat")
-    ap.add_argument("--cpu-sample-steps", type=int, default=40)
+    ap.add_argument("--cpu-sample-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--propagation", choices=["auto", "per-step", "resident"], default="auto")
     ap.add_argument("--gradient", choices=["reference", "adjoint"], default="reference",
