@@ -18,6 +18,8 @@
 // walks it backwards, accumulating dG and da.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "barrier.h"
 #include "device.h"
 #include "lds_sort.h"
@@ -556,11 +558,19 @@ hipError_t launch_adjoint_sweep_resident(hipStream_t s, int32_t blocks, const in
     return hipGetLastError();
 }
 
+int64_t adjoint_gradient_state_len(int32_t N, int32_t T, int32_t K) {
+    // a segment spans min(K, T) steps; 6 floats per connection and chunk start
+    const int64_t steps = std::max<int64_t>(1, std::min(K, T));
+    return (steps + kGradChunk - 1) / kGradChunk * int64_t(N) * 26 * 6;
+}
+
 hipError_t launch_adjoint_gradient(hipStream_t s, const int32_t *nbr, const float *coefs,
                                    const int32_t *delays, const float *gT, const float *xT,
-                                   const float *xbarT, const float *ckpt, float *ychunk, float *dGT,
-                                   float *dA, int32_t N, int32_t T, int32_t K) {
+                                   const float *xbarT, const float *ckpt, float *ychunk,
+                                   int64_t ychunk_len, float *dGT, float *dA, int32_t N, int32_t T,
+                                   int32_t K) {
     if (N <= 0) return hipSuccess;
+    if (K < 1 || ychunk_len < adjoint_gradient_state_len(N, T, K)) return hipErrorInvalidValue;
     const unsigned blocks = unsigned((int64_t(N) * 26 + kGradThreads - 1) / kGradThreads);
     adjoint_gradient_kernel<<<blocks, kGradThreads, 0, s>>>(nbr, coefs, delays, gT, xT, xbarT,
                                                             ckpt, ychunk, dGT, dA, N, T, K);

@@ -490,9 +490,9 @@ static int adjoint_prepare(ct_results_gpu &r, int32_t K, int64_t &slots) {
         CT_CHECK(ensure(r, &r.qring, have_q, slots * M, 1));
         r.qring_slots = std::max(r.qring_slots, slots);
     }
-    // gradient pass: filter state (6 values) per connection at each 16-step
-    // chunk start of a segment
-    CT_CHECK(ensure(r, &r.yseg, r.yseg_have, int64_t((K + 15) / 16) * int64_t(r.N) * 26 * 6, 1));
+    // gradient pass: filter state (6 values) per connection at each chunk
+    // start of a segment
+    CT_CHECK(ensure(r, &r.yseg, r.yseg_have, ct::adjoint_gradient_state_len(r.N, r.T, K), 1));
     return CT_OK;
 }
 
@@ -554,7 +554,7 @@ static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K,
         r.xT_fresh = true;
     }
     CT_HIP(ct::launch_adjoint_gradient(s, r.nbr, r.coefs, r.delays, r.gT, r.xT, r.xbarT, r.ckpt,
-                                       r.yseg, r.dGT, r.dA, r.N, r.T, K));
+                                       r.yseg, r.yseg_have, r.dGT, r.dA, r.N, r.T, K));
     return CT_OK;
 }
 

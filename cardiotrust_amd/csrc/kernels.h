@@ -150,10 +150,16 @@ hipError_t launch_adjoint_sweep_resident(hipStream_t s, int32_t blocks, const in
 hipError_t launch_barrier_reset(hipStream_t s, GridBarrier *gb, int32_t count);
 // One job's barrier counters and its first `nflags` slab flags (flags may be null).
 hipError_t launch_resident_reset(hipStream_t s, GridBarrier *gb, uint32_t *flags, int32_t nflags);
+// Floats of the gradient pass's chunk-state buffer (`ychunk`) for N voxels,
+// T steps and checkpoint interval K.
+int64_t adjoint_gradient_state_len(int32_t N, int32_t T, int32_t K);
+// ychunk_len: floats the caller allocated at ychunk; fewer than
+// adjoint_gradient_state_len(N, T, K) gives hipErrorInvalidValue.
 hipError_t launch_adjoint_gradient(hipStream_t s, const int32_t *nbr, const float *coefs,
                                    const int32_t *delays, const float *gT, const float *xT,
-                                   const float *xbarT, const float *ckpt, float *ychunk, float *dGT,
-                                   float *dA, int32_t N, int32_t T, int32_t K);
+                                   const float *xbarT, const float *ckpt, float *ychunk,
+                                   int64_t ychunk_len, float *dGT, float *dA, int32_t N, int32_t T,
+                                   int32_t K);
 
 hipError_t launch_sgd(hipStream_t s, float *v, const float *d, int64_t len, float scale);
 // Per-update scalars of the parameter update, read by the kernels from
