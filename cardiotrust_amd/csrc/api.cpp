@@ -17,6 +17,13 @@ int fail(int code, const std::string &msg) {
     return code;
 }
 
+void gpu_release(ct_gpu *gpu) {
+    if (gpu->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+    (void)hipSetDevice(gpu->device);
+    if (gpu->stream) (void)hipStreamDestroy(gpu->stream);
+    delete gpu;
+}
+
 int check_gpu(ct_gpu *gpu) {
     if (!gpu) return fail(CT_ERR_INVALID, "null ct_gpu");
     CT_HIP(hipSetDevice(gpu->device));
@@ -602,10 +609,7 @@ int ct_gpu_new(int32_t device, ct_gpu **out) {
 }
 
 void ct_gpu_free(ct_gpu *gpu) {
-    if (!gpu) return;
-    (void)hipSetDevice(gpu->device);
-    if (gpu->stream) (void)hipStreamDestroy(gpu->stream);
-    delete gpu;
+    if (gpu) gpu_release(gpu);
 }
 
 int ct_gpu_synchronize(ct_gpu *gpu) {
@@ -1030,6 +1034,73 @@ int ct_results_spherical(ct_results_gpu *results, float sample_rate_hz, float *m
     if (rc == CT_OK && no_max)
         rc = fail(CT_ERR_INVALID, "a voxel has no non-NaN magnitude (argmax_skipnan fails)");
     return rc;
+}
+
+int ct_results_calculate_final_metrics(ct_results_gpu *results, const uint32_t *voxel_types,
+                                       const int32_t *voxel_numbers, int64_t cells, float *dice,
+                                       float *iou, float *precision, float *recall) {
+    if (!results || (cells > 0 && (!voxel_types || !voxel_numbers)))
+        return fail(CT_ERR_INVALID, "null argument");
+    if (cells < 0 || cells > INT32_MAX)
+        return fail(CT_ERR_INVALID, "cell count must be in [0, 2^31 - 1]");
+    ct_results_gpu &r = *results;
+    CT_CHECK(check_gpu(r.gpu));
+    // predict_voxeltype indexes the states at number, +1, +2 (metrics.rs:429-433)
+    const int64_t n = 3LL * r.N;
+    for (int64_t c = 0; c < cells; ++c) {
+        const int32_t v = voxel_numbers[c];
+        if (v < -1 || (v >= 0 && (v % 3 != 0 || v >= n)))
+            return fail(CT_ERR_INVALID, "voxel number " + std::to_string(v) + " of cell " +
+                                            std::to_string(c) +
+                                            " is neither -1 nor a multiple of 3 below 3N");
+    }
+    constexpr int kThresholds = ct::kSegmentationThresholds, kBins = kThresholds + 1;
+    ct::SegmentationThresholds th;
+    for (int i = 0; i < kThresholds; ++i) th.value[i] = float(i) / 100.0f;  // metrics.rs:233
+    hipStream_t s = r.gpu->stream;
+    const int64_t nvox = std::max<int32_t>(r.N, 1), ncell = std::max<int64_t>(cells, 1);
+    char *dev = nullptr;
+    CT_HIP(hipMalloc(&dev, sizeof(uint32_t) * 2 * kBins + sizeof(float) * nvox +
+                               (sizeof(int32_t) + sizeof(uint32_t)) * ncell));
+    uint32_t *d_hist = reinterpret_cast<uint32_t *>(dev);
+    float *d_max = reinterpret_cast<float *>(d_hist + 2 * kBins);
+    int32_t *d_num = reinterpret_cast<int32_t *>(d_max + nvox);
+    uint32_t *d_types = reinterpret_cast<uint32_t *>(d_num + ncell);
+    uint32_t hist[2 * kBins] = {};
+    int rc = CT_OK;
+    auto step = [&](hipError_t e) {
+        if (rc == CT_OK && e != hipSuccess)
+            rc = fail(CT_ERR_DEVICE, std::string("final metrics: ") + hipGetErrorString(e));
+    };
+    if (cells > 0) {
+        step(hipMemcpyAsync(d_num, voxel_numbers, sizeof(int32_t) * cells, hipMemcpyHostToDevice, s));
+        step(hipMemcpyAsync(d_types, voxel_types, sizeof(uint32_t) * cells, hipMemcpyHostToDevice,
+                            s));
+    }
+    if (rc == CT_OK)
+        step(ct::launch_segmentation_histogram(s, r.x, r.T, r.N, d_num, d_types, cells, th, d_max,
+                                               d_hist));
+    if (rc == CT_OK) step(hipMemcpyAsync(hist, d_hist, sizeof(hist), hipMemcpyDeviceToHost, s));
+    step(hipStreamSynchronize(s));
+    (void)hipFree(dev);
+    if (rc != CT_OK) return rc;
+    // The counts of calculate_dice / _iou / _precision / _recall
+    // (metrics.rs:253-404) at each threshold, from the histograms' prefix
+    // sums: a cell predicted Pathological at threshold i has bin <= i.
+    int64_t gt_pos = 0;
+    for (int k = 0; k < kBins; ++k) gt_pos += hist[kBins + k];
+    int64_t tp = 0, pred_pos = 0;
+    for (int i = 0; i < kThresholds; ++i) {
+        tp += hist[kBins + i];
+        pred_pos += int64_t(hist[i]) + hist[kBins + i];
+        const int64_t fp = pred_pos - tp, fn = gt_pos - tp;
+        const int64_t den = 2 * tp + fp + fn, uni = tp + fp + fn;
+        if (dice) dice[i] = den == 0 ? 0.0f : float(2 * tp) / float(den);
+        if (iou) iou[i] = uni == 0 ? 0.0f : float(tp) / float(uni);
+        if (precision) precision[i] = pred_pos == 0 ? 0.0f : float(tp) / float(pred_pos);
+        if (recall) recall[i] = gt_pos == 0 ? 1.0f : float(tp) / float(gt_pos);
+    }
+    return CT_OK;
 }
 
 int ct_results_simulate_noisy(ct_results_gpu *results, const float *sensor_std, uint64_t seed,

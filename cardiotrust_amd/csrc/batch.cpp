@@ -22,7 +22,7 @@
 using namespace ctrt;
 
 struct ct_batch {
-    ct_gpu *gpu = nullptr;
+    GpuRef gpu;
     std::vector<ct_epoch_kernel *> kernels;
     int32_t slots = 0;              // resident workgroups of the batched launch
     int32_t voxels_per_group = 0;   // 0 = autotune on the first epoch

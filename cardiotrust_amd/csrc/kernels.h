@@ -195,6 +195,24 @@ hipError_t launch_add(hipStream_t s, float *a, const float *b, int64_t len);  //
 hipError_t launch_spherical(hipStream_t s, const float *x, int32_t T, int32_t N, float rate,
                             float *mag, float *theta, float *phi, float *mag_max,
                             float *theta_max, float *phi_max, float *act_ms, int32_t *no_max);
+// Segmentation metrics over thresholds i / 100, i = 0..100
+// (algorithm/metrics.rs:225-241); VoxelType::Pathological's ordinal, as the
+// reference's voxel_types.npy stores it (model/spatial/voxels.rs:318, 664-676).
+constexpr int kSegmentationThresholds = 101;
+constexpr uint32_t kVoxelTypePathological = 6;
+struct SegmentationThresholds {
+    float value[kSegmentationThresholds];
+};
+// max_l1 (N): each voxel's maximum over steps of |x0| + |x1| + |x2| (NaN
+// skipped; NaN if none). hist (2, kSegmentationThresholds + 1), zeroed here:
+// per ground truth class (not Pathological, Pathological) the number of cells
+// first predicted Pathological at each threshold index, the last bin for
+// cells never predicted Pathological. numbers (cells): -1 or a multiple of 3
+// below 3N (checked by the caller).
+hipError_t launch_segmentation_histogram(hipStream_t s, const float *x, int32_t T, int32_t N,
+                                         const int32_t *numbers, const uint32_t *types,
+                                         int64_t cells, const SegmentationThresholds &th,
+                                         float *max_l1, uint32_t *hist);
 // Connections of a grid of cells: output state indices (3N, 78; -1 = None)
 // and delays in samples (N, 26; 0 without a connection); *too_small = 1 if a
 // delay is below one sample.

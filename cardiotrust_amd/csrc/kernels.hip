@@ -1241,6 +1241,78 @@ __global__ void spherical_kernel(const float *__restrict__ x, int32_t T, int32_t
     act_ms[v] = float(best) / rate * 1000.0f;
 }
 
+// Per voxel, the maximum over steps of |x0| + |x1| + |x2| with NaN values
+// skipped, NaN when every step is NaN: predict_voxeltype's abs array and its
+// max_skipnan (algorithm/metrics.rs:409-436). Thread (lane, slice) walks
+// steps slice, slice + kMaxL1Slices, ... of voxel blockIdx.x * 64 + lane; the
+// slices combine in LDS. The maximum of exact values does not depend on that
+// order.
+constexpr int kMaxL1Voxels = 64;
+constexpr int kMaxL1Slices = 4;
+__global__ __launch_bounds__(kMaxL1Voxels *kMaxL1Slices) void voxel_max_l1_kernel(
+    const float *__restrict__ x, int32_t T, int32_t N, float *__restrict__ out) {
+    __shared__ float part[kMaxL1Slices][kMaxL1Voxels];
+    const int lane = threadIdx.x % kMaxL1Voxels, slice = threadIdx.x / kMaxL1Voxels;
+    const int32_t v = int32_t(blockIdx.x) * kMaxL1Voxels + lane;
+    float m = __builtin_nanf("");
+    if (v < N) {
+        const int64_t n = 3LL * N;
+        for (int32_t t = slice; t < T; t += kMaxL1Slices) {
+            const float *s = x + int64_t(t) * n + 3LL * v;
+            const float a = fabsf(s[0]) + fabsf(s[1]) + fabsf(s[2]);
+            if (a == a && !(m >= a)) m = a;  // NaN skipped; a NaN maximum takes a
+        }
+    }
+    part[slice][lane] = m;
+    __syncthreads();
+    if (slice != 0 || v >= N) return;
+#pragma unroll
+    for (int k = 1; k < kMaxL1Slices; ++k) {
+        const float a = part[k][lane];
+        if (a == a && !(m >= a)) m = a;
+    }
+    out[v] = m;
+}
+
+// Two histograms over the grid cells (algorithm/metrics.rs:225-441): bin k of
+// hist[0] (ground truth not Pathological) or hist[1] (Pathological) counts the
+// cells first predicted Pathological at threshold index k, i.e. the smallest
+// k with max <= thresholds[k]; k = kSegmentationThresholds for cells without a
+// voxel or with a NaN maximum, which are never predicted Pathological.
+__global__ void segmentation_histogram_kernel(const int32_t *__restrict__ numbers,
+                                              const uint32_t *__restrict__ types, int64_t cells,
+                                              const float *__restrict__ max_l1,
+                                              SegmentationThresholds th,
+                                              uint32_t *__restrict__ hist) {
+    constexpr int kBins = kSegmentationThresholds + 1;
+    __shared__ uint32_t h[2 * kBins];
+    for (int i = threadIdx.x; i < 2 * kBins; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < cells;
+         c += int64_t(gridDim.x) * blockDim.x) {
+        const int32_t num = numbers[c];
+        int k = kSegmentationThresholds;
+        if (num >= 0) {
+            const float m = max_l1[num / 3];
+            if (m == m) {  // lower bound over the increasing thresholds
+                int lo = 0, hi = kSegmentationThresholds;
+                while (lo < hi) {
+                    const int mid = (lo + hi) / 2;
+                    if (m <= th.value[mid])
+                        hi = mid;
+                    else
+                        lo = mid + 1;
+                }
+                k = lo;
+            }
+        }
+        atomicAdd(&h[(types[c] == kVoxelTypePathological ? kBins : 0) + k], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 2 * kBins; i += blockDim.x)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
 __global__ void fill_words_kernel(uint32_t *p, uint32_t v, int64_t len) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k < len) p[k] = v;
@@ -1358,6 +1430,23 @@ hipError_t launch_spherical(hipStream_t s, const float *x, int32_t T, int32_t N,
     if (N > 0)
         spherical_kernel<<<blocks_for(N, 256), 256, 0, s>>>(x, T, N, rate, mag, theta, phi, mag_max,
                                                             theta_max, phi_max, act_ms, no_max);
+    return hipGetLastError();
+}
+
+hipError_t launch_segmentation_histogram(hipStream_t s, const float *x, int32_t T, int32_t N,
+                                         const int32_t *numbers, const uint32_t *types,
+                                         int64_t cells, const SegmentationThresholds &th,
+                                         float *max_l1, uint32_t *hist) {
+    constexpr int kWords = 2 * (kSegmentationThresholds + 1);
+    fill_words_kernel<<<1, 256, 0, s>>>(hist, 0u, kWords);
+    if (N > 0)
+        voxel_max_l1_kernel<<<blocks_for(N, kMaxL1Voxels), kMaxL1Voxels * kMaxL1Slices, 0, s>>>(
+            x, T, N, max_l1);
+    if (cells > 0) {
+        const unsigned blocks = blocks_for(cells, 256);  // grid-stride beyond 2048 blocks
+        segmentation_histogram_kernel<<<blocks < 2048u ? blocks : 2048u, 256, 0, s>>>(
+            numbers, types, cells, max_l1, th, hist);
+    }
     return hipGetLastError();
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -32,20 +33,55 @@ int fail(int code, const std::string &msg);
         if (rc_ != CT_OK) return rc_; \
     } while (0)
 
+// Reference-counted as the reference's GPU (gpu.rs: a Clone of the OpenCL
+// context and queue that every buffer and kernel keeps alive): the creator
+// holds one reference and every object built on it one more, so objects may
+// be freed before or after ct_gpu_free.
 struct ct_gpu {
     int32_t device = 0;
     hipStream_t stream = nullptr;
     uint64_t wall_ticks_per_second = 100000000;  // wall_clock64() rate
+    std::atomic<int32_t> refs{1};
 };
 
+namespace ctrt {
+// Drops one reference; the last one destroys the stream and the handle.
+void gpu_release(ct_gpu *gpu);
+
+// An object's reference to its ct_gpu: taken on assignment, dropped on
+// destruction (after the owning object's destructor body has run).
+class GpuRef {
+  public:
+    GpuRef() = default;
+    GpuRef(const GpuRef &) = delete;
+    GpuRef &operator=(const GpuRef &) = delete;
+    ~GpuRef() { reset(nullptr); }
+    GpuRef &operator=(ct_gpu *g) {
+        reset(g);
+        return *this;
+    }
+    ct_gpu *operator->() const { return g_; }
+    ct_gpu &operator*() const { return *g_; }
+    operator ct_gpu *() const { return g_; }
+
+  private:
+    void reset(ct_gpu *g) {
+        if (g) g->refs.fetch_add(1, std::memory_order_relaxed);
+        if (g_) gpu_release(g_);
+        g_ = g;
+    }
+    ct_gpu *g_ = nullptr;
+};
+}  // namespace ctrt
+
 struct ct_buffer {
-    ct_gpu *gpu = nullptr;
+    ctrt::GpuRef gpu;
     float *data = nullptr;
     int32_t beats = 0, steps = 0, sensors = 0;
 };
 
 struct ct_results_gpu {
-    ct_gpu *gpu = nullptr;
+    ctrt::GpuRef gpu;
     int32_t N = 0, S = 0, Sp = 0, T = 0, B = 0, nbatch = 0, optimizer = 0;
     int64_t n = 0;
     std::vector<void *> allocations;
@@ -127,7 +163,7 @@ struct ct_results_gpu {
 };
 
 struct ct_epoch_kernel {
-    ct_gpu *gpu = nullptr;
+    ctrt::GpuRef gpu;
     ct_results_gpu *r = nullptr;
     const ct_buffer *mhat = nullptr;
     ct_algorithm cfg{};

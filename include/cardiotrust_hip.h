@@ -122,7 +122,11 @@ typedef struct {
 const char *ct_last_error(void);
 
 /* GPU::new (src/core/algorithm/gpu.rs:31-63): open `device` and create the
- * library's stream (default priority). */
+ * library's stream (default priority). The handle is reference-counted like
+ * the reference's GPU (a Clone of the context that every buffer and kernel
+ * holds): every object created on it keeps it alive, so ct_gpu_free may come
+ * before or after the frees of those objects; the stream is destroyed with
+ * the last of them. */
 int ct_gpu_new(int32_t device, ct_gpu **out);
 void ct_gpu_free(ct_gpu *gpu);
 int ct_gpu_synchronize(ct_gpu *gpu);
@@ -189,6 +193,24 @@ int ct_model_connections(ct_gpu *gpu, int32_t nx, int32_t ny, int32_t nz,
 int ct_results_spherical(ct_results_gpu *results, float sample_rate_hz, float *magnitude,
                          float *theta, float *phi, float *magnitude_max, float *theta_max,
                          float *phi_max, float *activation_time_ms);
+
+/* metrics::calculate_final (src/core/algorithm/metrics.rs:225-441) over the
+ * device states of the results (the last propagated beat, as the reference's
+ * Estimations hold them): at threshold i / 100 (i = 0..100) a cell is
+ * predicted Pathological when it holds a voxel whose maximum over steps of
+ * |x0| + |x1| + |x2| (NaN values skipped, as ndarray-stats max_skipnan) is at
+ * most the threshold, and the Dice score, IoU, precision and recall of those
+ * predictions against the ground truth are written to dice, iou, precision
+ * and recall (101 each; any may be NULL), with the reference's values for
+ * empty denominators (0, 0, 0 and 1). The grid arrays (`cells` each) take the
+ * reference's voxel_types.npy / voxel_numbers.npy forms: VoxelType ordinals
+ * (Pathological = 6) and each cell's first state index, -1 for None. One
+ * pass over the states replaces the reference's 101. CT_ERR_INVALID for a
+ * voxel number that is neither -1 nor a multiple of 3 below 3N (the reference
+ * indexes out of bounds). Blocks until the outputs are written. */
+int ct_results_calculate_final_metrics(ct_results_gpu *results, const uint32_t *voxel_types,
+                                       const int32_t *voxel_numbers, int64_t cells, float *dice,
+                                       float *iou, float *precision, float *recall);
 
 /* Simulation::run with its measurement noise (src/core/data/simulation.rs:
  * 129-173): ct_results_simulate, then m[b][t][s] += sensor_std[s] * z with z

@@ -885,6 +885,80 @@ int ct_oracle_spherical_f32(const float *x, int32_t T, int32_t N, float rate, fl
     return 0;
 }
 
+// algorithm/metrics.rs:225-441 (calculate_final, calculate_for_threshold,
+// the four metric functions, predict_voxeltype), loop for loop: for each
+// threshold i as f32 / 100.0, each cell holding a voxel number k gets the
+// series |x[t][k]| + |x[t][k+1]| + |x[t][k+2]| and is Pathological when its
+// max_skipnan is <= threshold, else Ventricle; cells without a number stay
+// None. max_skipnan is ndarray-stats 0.6.0's (pinned in the reference's
+// Cargo.lock; not part of the reference): a fold over the non-NaN values
+// keeping the larger, NaN when there is none, so such a cell is never
+// Pathological. Each metric counts over all cells as its function does and
+// converts the usize counts to f32 before dividing.
+namespace {
+constexpr uint8_t kPredNone = 0, kPredVentricle = 1, kPredPathological = 2;
+constexpr uint32_t kTypePathological = 6;  // VoxelType ordinal (model/spatial/voxels.rs:664-676)
+}  // namespace
+
+void ct_oracle_calculate_final_f32(const float *x, int32_t T, int64_t n, const uint32_t *types,
+                                   const int32_t *numbers, int64_t cells, float *dice, float *iou,
+                                   float *precision, float *recall) {
+    std::vector<uint8_t> pred(size_t(std::max<int64_t>(cells, 1)));
+    std::vector<float> abs_series(size_t(std::max<int32_t>(T, 1)));
+    for (int i = 0; i <= 100; ++i) {
+        const float threshold = float(i) / 100.0f;
+        for (int64_t c = 0; c < cells; ++c) {
+            pred[c] = kPredNone;
+            if (numbers[c] < 0) continue;
+            const int64_t k = numbers[c];
+            for (int32_t t = 0; t < T; ++t)
+                abs_series[t] = std::fabs(x[t * n + k]) + std::fabs(x[t * n + k + 1]) +
+                                std::fabs(x[t * n + k + 2]);
+            bool have = false;
+            float mx = 0.0f;
+            for (int32_t t = 0; t < T; ++t) {
+                if (std::isnan(abs_series[t])) continue;
+                if (!have || abs_series[t] > mx) mx = abs_series[t];
+                have = true;
+            }
+            const float max_skipnan = have ? mx : std::nanf("");
+            pred[c] = max_skipnan <= threshold ? kPredPathological : kPredVentricle;
+        }
+        auto count = [&](auto keep) {
+            int64_t m = 0;
+            for (int64_t c = 0; c < cells; ++c)
+                m += keep(pred[c] == kPredPathological, types[c] == kTypePathological) ? 1 : 0;
+            return m;
+        };
+        // calculate_dice (metrics.rs:373-404)
+        {
+            const int64_t tp = count([](bool p, bool g) { return g && p; });
+            const int64_t fp = count([](bool p, bool g) { return !g && p; });
+            const int64_t fn = count([](bool p, bool g) { return g && !p; });
+            const int64_t den = 2 * tp + fp + fn;
+            dice[i] = den == 0 ? 0.0f : float(2 * tp) / float(den);
+        }
+        // calculate_iou (metrics.rs:335-364)
+        {
+            const int64_t inter = count([](bool p, bool g) { return g && p; });
+            const int64_t uni = count([](bool p, bool g) { return g || p; });
+            iou[i] = uni == 0 ? 0.0f : float(inter) / float(uni);
+        }
+        // calculate_precision (metrics.rs:300-326)
+        {
+            const int64_t pp = count([](bool p, bool) { return p; });
+            const int64_t tp = count([](bool p, bool g) { return g && p; });
+            precision[i] = pp == 0 ? 0.0f : float(tp) / float(pp);
+        }
+        // calculate_recall (metrics.rs:266-292)
+        {
+            const int64_t gp = count([](bool, bool g) { return g; });
+            const int64_t tp = count([](bool p, bool g) { return g && p; });
+            recall[i] = gp == 0 ? 1.0f : float(tp) / float(gp);
+        }
+    }
+}
+
 // allpass/delay.rs:14-23: |(in - out) / 1000| / v, the norm the sqrt of the
 // ndarray sum ((0 + d0^2) + d1^2) + d2^2 in f32.
 float ct_oracle_delay_s_f32(const float *in, const float *out, float velocity) {

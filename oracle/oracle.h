@@ -168,6 +168,14 @@ int ct_oracle_spherical_f32(const float *x, int32_t num_steps, int32_t num_voxel
                             float sample_rate_hz, float *magnitude, float *theta, float *phi,
                             float *magnitude_max, float *theta_max, float *phi_max,
                             float *activation_time_ms);
+/* metrics::calculate_final (algorithm/metrics.rs:225-441) over states x
+ * (T, n): Dice, IoU, precision and recall (101 each) of the voxel-type
+ * predictions at thresholds i / 100 against the ground truth. types: VoxelType
+ * ordinals per cell (Pathological = 6); numbers: each cell's first state
+ * index, -1 for None (a multiple of 3 below n). */
+void ct_oracle_calculate_final_f32(const float *x, int32_t num_steps, int64_t num_states,
+                                   const uint32_t *types, const int32_t *numbers, int64_t cells,
+                                   float *dice, float *iou, float *precision, float *recall);
 /* calculate_delay_s (allpass/delay.rs:14-23). */
 float ct_oracle_delay_s_f32(const float *input_position_mm, const float *output_position_mm,
                             float propagation_velocity_m_per_s);

@@ -121,6 +121,10 @@ def lib():
         L.ct_oracle_spherical_f32.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                               ctypes.c_float] + [ctypes.c_void_p] * 7
         L.ct_oracle_spherical_f32.restype = ctypes.c_int
+        L.ct_oracle_calculate_final_f32.argtypes = [
+            ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+            ctypes.c_int64] + [ctypes.c_void_p] * 4
+        L.ct_oracle_calculate_final_f32.restype = None
         L.ct_oracle_delay_s_f32.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float]
         L.ct_oracle_delay_s_f32.restype = ctypes.c_float
         _lib = L
@@ -282,6 +286,31 @@ def model_connections_f32(numbers, positions_mm, velocities_m_per_s, sample_rate
     if rc != 0:
         raise ValueError("a delay is below one sample")
     return osi, dly
+
+
+FINAL_METRICS = ("dice_score_over_threshold", "iou_over_threshold",
+                 "precision_over_threshold", "recall_over_threshold")
+
+
+def calculate_final_f32(x, voxel_types, voxel_numbers):
+    """metrics::calculate_final over states x (T, n): dict of the four
+    (101,) arrays named as the reference's Metrics fields. voxel_types:
+    VoxelType ordinals per grid cell (Pathological = 6); voxel_numbers: each
+    cell's first state index, -1 for None (the reference's .npy forms)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    types = np.ascontiguousarray(voxel_types, dtype=np.uint32).ravel()
+    numbers = np.ascontiguousarray(voxel_numbers, dtype=np.int32).ravel()
+    if types.shape != numbers.shape:
+        raise ValueError("voxel_types and voxel_numbers differ in shape")
+    T, n = x.shape
+    if numbers.size and (numbers.max() + 2 >= n or numbers.min() < -1
+                         or np.any(numbers[numbers >= 0] % 3)):
+        raise ValueError("voxel numbers must be -1 or multiples of 3 below n")
+    out = {k: np.zeros(101, np.float32) for k in FINAL_METRICS}
+    lib().ct_oracle_calculate_final_f32(x.ctypes.data, T, n, types.ctypes.data,
+                                        numbers.ctypes.data, types.size,
+                                        *(out[k].ctypes.data for k in FINAL_METRICS))
+    return out
 
 
 def spherical_f32(x, sample_rate_hz):

@@ -100,3 +100,24 @@ def test_divergence_gives_a_non_finite_loss(gpu, blob):
     lb = epochs(gpu, blob, Algorithm(learning_rate=1.0, freeze_delays=False), 2,
                 measurements=bad)
     assert np.isnan(lb["loss_batch"]).all()
+
+
+def test_objects_outlive_the_gpu_handle(blob):
+    """The handle is reference-counted like the reference's GPU (gpu.rs: a
+    Clone of the context every buffer and kernel keeps): objects built on it
+    keep working, and can be freed, after ct_gpu_free."""
+    from cardiotrust_amd.gpu import GPU, Buffer, EpochKernel, ResultsGPU
+    note("case: free the GPU handle first, then execute and free its objects")
+    g = GPU(0)
+    r = ResultsGPU(g, blob, num_batch_entries=2)
+    buf = Buffer(g, blob.measurements)
+    k = EpochKernel(g, r, buf, Algorithm(learning_rate=1.0, freeze_delays=False),
+                    3 * blob.num_voxels, blob.num_sensors, blob.num_steps)
+    k.execute()
+    g.close()
+    k.execute()
+    lb = r.update_from_gpu(["loss_batch"])["loss_batch"]
+    assert np.all(np.isfinite(lb)) and lb[1] < lb[0]
+    r.close()  # in a different order from their creation
+    k.close()
+    buf.close()
