@@ -1257,6 +1257,8 @@ __global__ __launch_bounds__(kMaxL1Voxels *kMaxL1Slices) void voxel_max_l1_kerne
     float m = __builtin_nanf("");
     if (v < N) {
         const int64_t n = 3LL * N;
+        // one 12-byte load per step: 6.0 TB/s at 120k voxels x 500 steps;
+        // four steps in flight per lane measured slower (126-137 us vs 120)
         for (int32_t t = slice; t < T; t += kMaxL1Slices) {
             const float *s = x + int64_t(t) * n + 3LL * v;
             const float a = fabsf(s[0]) + fabsf(s[1]) + fabsf(s[2]);
