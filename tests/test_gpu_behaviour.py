@@ -121,3 +121,47 @@ def test_objects_outlive_the_gpu_handle(blob):
     r.close()  # in a different order from their creation
     k.close()
     buf.close()
+
+
+def empty_model(num_states=3000, num_sensors=300, num_steps=3, num_beats=10):
+    """Model::empty / Data::empty of the reference's no-crash tests
+    (algorithm/tests/no_crash.rs:13-53; model/functional/allpass.rs:50-60):
+    every array zero and no connection (output_state_indices all None)."""
+    N = num_states // 3
+    z = np.zeros
+    return synth.Problem(
+        synth.grid_geometry(np.ones((N, 1, 1), bool), 2.5), num_sensors, num_steps, num_beats,
+        gains=z((num_states, 78), np.float32),
+        output_state_indices=np.full((num_states, 78), -1, np.int32),
+        coefs=z((N, 26), np.float32), delays=z((N, 26), np.int32),
+        initial_delays=z((N, 26), np.float32), control_matrix=z(num_states, np.float32),
+        control_function=z(num_steps, np.float32),
+        measurement_matrix=z((num_beats, num_sensors, num_states), np.float32),
+        measurements=z((num_beats, num_steps, num_sensors), np.float32))
+
+
+def test_run_epoch_no_crash_on_an_empty_model(gpu):
+    """run_epoch_no_crash / run_no_crash (algorithm/tests/no_crash.rs:13-97):
+    3000 states, 300 sensors, 3 steps, 10 beats, the default configuration;
+    here also every array against the oracle's epochs on the same model."""
+    from cardiotrust_amd.gpu import Buffer, EpochKernel, ResultsGPU
+    note("case: empty model (no connections, all zeros), 300 sensors, 10 beats, 3 epochs")
+    p = empty_model()
+    cfg = Algorithm()
+    fields = ("system_states", "ap_outputs_now", "ap_outputs_last", "delays", "gains", "coefs",
+              "d_gains", "d_coefs", "loss_batch", "measurements")
+    r = ResultsGPU(gpu, p, num_batch_entries=3)
+    buf = Buffer(gpu, p.measurements)
+    k = EpochKernel(gpu, r, buf, cfg, 3 * p.num_voxels, p.num_sensors, p.num_steps)
+    for _ in range(3):
+        k.execute()
+    got = r.update_from_gpu(list(fields))
+    k.close()
+    buf.close()
+    r.close()
+    o = oracle.OracleResults(p, np.float32, num_batch_entries=3)
+    for _ in range(3):
+        o.run_epoch(cfg)
+    for f in fields:
+        np.testing.assert_array_equal(got[f], getattr(o, f), err_msg=f)
+    assert np.all(got["loss_batch"] == 0.0) and np.all(got["system_states"] == 0.0)
