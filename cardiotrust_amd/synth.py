@@ -236,6 +236,20 @@ def gaussian_derivative_pulse(num_steps: int, sample_rate_hz: float = 2000.0,
     g = -(t_ms - t0_ms) / sigma_ms**2 * np.exp(-((t_ms - t0_ms) ** 2) / (2.0 * sigma_ms**2))
     return (g / np.max(np.abs(g))).astype(np.float32)
 
+def tiled_control_function(raw: np.ndarray, num_steps: int, sample_rate_hz: float = 2000.0,
+                           raw_rate_hz: float = 2000.0) -> np.ndarray:
+    """(T,) float32 drive from a recorded control function, as
+    ControlFunction::from_model_config builds its Ohara drive
+    (model/functional/control.rs:174-223) at the recording's own rate: sample i
+    is raw[i % len(raw)]. The reference resamples other rates with a sinc
+    resampler (rubato), which is not provided here: ValueError."""
+    raw = np.asarray(raw, dtype=np.float32).ravel()
+    if raw.size == 0:
+        raise ValueError("empty control function")
+    if abs(sample_rate_hz - raw_rate_hz) > 1e-3 * max(abs(sample_rate_hz), abs(raw_rate_hz)):
+        raise ValueError("resampling a recorded control function is not provided")
+    return raw[np.arange(num_steps) % raw.size].copy()
+
 def coef_to_samples(coef: np.ndarray) -> np.ndarray:
     """allpass.rs:665-668 in float32."""
     coef = np.asarray(coef, dtype=np.float32)

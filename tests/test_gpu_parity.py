@@ -7,6 +7,7 @@ Bars (written per assertion):
     Hᵀr run on MFMA in a different summation order than ndarray/matrixmultiply);
   * every compared array also reports its error against the FP64 oracle.
 """
+import os
 import sys
 
 import numpy as np
@@ -111,6 +112,24 @@ def test_epoch_config0_parity(gpu, config0, mode):
     o64 = run_oracle_epochs(p, cfg, np.float64)
     compare(g, o32, o64)
     note(f"  profile: {prof}")
+
+
+@pytest.mark.parametrize("mode", [1, 2], ids=["per-step", "resident"])
+def test_epoch_with_the_reference_ohara_drive(gpu, mode):
+    """The reference's default drive (its assets/control_function_ohara.npy,
+    kept as a fixture and tiled modulo its 2001 samples as control.rs:215-221
+    does) over 2300 steps, so it wraps once: one epoch against the oracle."""
+    note(f"case: blob of 300 voxels, O'Hara drive, T=2300, delays 1-15, mode {mode}")
+    raw = np.load(os.path.join(os.path.dirname(__file__), "golden",
+                               "control_function_ohara.npy"), allow_pickle=False)
+    T = 2300
+    p = synth.make_problem(synth.blob(77, 300), num_steps=T, max_delay=15, seed=77,
+                           gain_sigma=0.1)
+    p.control_function = synth.tiled_control_function(raw, T)
+    p.measurements = oracle.simulate_f32(p.truth())
+    cfg = Algorithm(learning_rate=1e-3, freeze_delays=False)
+    g, _ = run_gpu_epochs(gpu, p, cfg, mode=mode)
+    compare(g, run_oracle_epochs(p, cfg, np.float32), run_oracle_epochs(p, cfg, np.float64))
 
 
 def blob_problem(seed, shape=(7, 6, 5), T=64, max_delay=12, beats=1, fill=0.6, gain_sigma=0.12):

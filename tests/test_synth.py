@@ -7,6 +7,9 @@ import pytest
 from cardiotrust_amd import synth
 from oracle import oracle
 
+# sha256 of the reference's assets/control_function_ohara.npy samples (float32 bytes)
+FIXTURE_SHA256_PREFIX = "2fe9023a2d5a975b"
+
 
 @pytest.mark.parametrize("n", [1, 7, 300, 1234])
 def test_blob_has_exactly_the_requested_voxels(n):
@@ -114,3 +117,22 @@ def test_measurement_noise_order_and_statistics():
     assert not np.array_equal(n, synth.measurement_noise(12, sd, B, T))
     big = synth.measurement_noise(3, np.ones(4, np.float32), 5, 50_000).astype(np.float64)
     assert abs(big.mean()) < 5e-3 and abs(big.std() - 1.0) < 5e-3
+
+
+def test_tiled_control_function_follows_the_reference_drive():
+    """The O'Hara fixture (the reference's assets/control_function_ohara.npy)
+    tiles modulo its length at its own 2 kHz (control.rs:215-221)."""
+    import hashlib
+    import os
+    raw = np.load(os.path.join(os.path.dirname(__file__), "golden", "control_function_ohara.npy"),
+                  allow_pickle=False)
+    assert raw.dtype == np.float32 and raw.shape == (2001,)
+    assert raw[0] == np.float32(2.5872957e-08) and raw.max() == np.float32(0.99999535)
+    assert hashlib.sha256(raw.tobytes()).hexdigest()[:16] == FIXTURE_SHA256_PREFIX
+    u = synth.tiled_control_function(raw, 4500)
+    assert u.dtype == np.float32 and u.shape == (4500,)
+    assert np.array_equal(u[:2001], raw) and np.array_equal(u[2001:4002], raw)
+    assert np.array_equal(u[4002:], raw[:498])
+    assert np.array_equal(synth.tiled_control_function(raw, 10), raw[:10])
+    with pytest.raises(ValueError):
+        synth.tiled_control_function(raw, 10, sample_rate_hz=3000.0)
