@@ -74,3 +74,38 @@ def test_full_size_resident_equals_per_step(gpu):
     nonzero = float(np.mean(x2[-1] != 0.0))
     note(f"  bit-identical; max |x| {peak:.4f}, states non-zero at the last step: {nonzero:.3f}")
     assert np.isfinite(peak) and nonzero > 0.5  # the pulse has crossed most of the shell
+
+
+def test_full_size_epoch_projections_against_float64(gpu):
+    """One epoch of the headline workload (lr 1e-3, freeze_delays false): its
+    H x_t for all 2000 steps (prediction.rs:112-130) against a float64 product
+    of the downloaded states, the last step's residuals (estimation.rs:146-152)
+    exactly, and its H^T r (derivation.rs:538-551) against float64. The split
+    reductions over 360,360 states are the full-size part."""
+    from cardiotrust_amd import Algorithm
+    from cardiotrust_amd.gpu import Buffer, EpochKernel, ResultsGPU, simulate
+    note("case: 120,120-voxel shell, 2000 steps, one epoch: projections vs float64")
+    p = synth.config1()
+    p.measurements = simulate(gpu, p.truth())
+    r = ResultsGPU(gpu, p)
+    buf = Buffer(gpu, p.measurements)
+    k = EpochKernel(gpu, r, buf, Algorithm(learning_rate=1e-3, freeze_delays=False),
+                    3 * p.num_voxels, p.num_sensors, p.num_steps)
+    k.execute()
+    got = r.update_from_gpu(["system_states", "measurements", "residuals", "mapped_residuals"])
+    k.close()
+    buf.close()
+    r.close()
+    x, m = got["system_states"], got["measurements"][0]
+    H = p.measurement_matrix[0].astype(np.float64)
+    want = np.empty(m.shape, np.float64)
+    for t0 in range(0, x.shape[0], 250):
+        want[t0:t0 + 250] = x[t0:t0 + 250].astype(np.float64) @ H.T
+    err_m = float(np.max(np.abs(m - want)) / np.max(np.abs(want)))
+    T = p.num_steps
+    r_last = m[T - 1] - p.measurements[0, T - 1]
+    assert np.array_equal(got["residuals"].view(np.uint32), r_last.view(np.uint32))
+    e = H.T @ r_last.astype(np.float64)
+    err_e = float(np.max(np.abs(got["mapped_residuals"] - e)) / np.max(np.abs(e)))
+    note(f"  H x rel err {err_m:.1e}; residuals exact; H^T r rel err {err_e:.1e}")
+    assert err_m <= 1e-4 and err_e <= 1e-4
