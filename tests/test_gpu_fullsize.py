@@ -10,6 +10,10 @@ slab flags across all XCDs).
 * Full N and full T: the resident path against the per-step path over all
   2000 steps of the pulse-driven ground truth, bit for bit (the per-step path
   is the one pinned to the oracle step by step at smaller sizes).
+* One full-size epoch: both projections against float64 products of the
+  downloaded states, and the derivatives of 64 sampled voxels against a
+  float64 recompute from those states (`derivatives_f64`, which agrees with the
+  FP64 oracle to about 1e-15 on small problems), all to 1e-4 relative.
 """
 import sys
 
@@ -92,7 +96,8 @@ def test_full_size_epoch_projections_against_float64(gpu):
     k = EpochKernel(gpu, r, buf, Algorithm(learning_rate=1e-3, freeze_delays=False),
                     3 * p.num_voxels, p.num_sensors, p.num_steps)
     k.execute()
-    got = r.update_from_gpu(["system_states", "measurements", "residuals", "mapped_residuals"])
+    got = r.update_from_gpu(["system_states", "measurements", "residuals", "mapped_residuals",
+                             "d_gains", "d_coefs"])
     k.close()
     buf.close()
     r.close()
@@ -109,3 +114,62 @@ def test_full_size_epoch_projections_against_float64(gpu):
     err_e = float(np.max(np.abs(got["mapped_residuals"] - e)) / np.max(np.abs(e)))
     note(f"  H x rel err {err_m:.1e}; residuals exact; H^T r rel err {err_e:.1e}")
     assert err_m <= 1e-4 and err_e <= 1e-4
+
+    # derivatives of 64 sampled voxels, recomputed in float64 from the
+    # downloaded states and measurements
+    vs, dG, dA = derivatives_f64(p, x, m, synth.uniform(12, 42, 64))
+    got_dG = got["d_gains"].reshape(p.num_voxels, 3, 26, 3)[vs]
+    got_dA = got["d_coefs"][vs]
+    err_g = float(np.max(np.abs(got_dG - dG)) / np.max(np.abs(dG)))
+    err_a = float(np.max(np.abs(got_dA - dA)) / np.max(np.abs(dA)))
+    note(f"  {vs.size} sampled voxels: d_gains rel err {err_g:.1e}, d_coefs rel err {err_a:.1e}")
+    assert err_g <= 1e-4 and err_a <= 1e-4
+
+
+def derivatives_f64(p, x, m, u):
+    """The voxels floor(u * N) and their d_gains (V, 3, 26, 3) and d_coefs
+    (V, 26) of one beat, in float64 from the states x (T, 3N) and predicted
+    measurements m (T, S): the filter outputs of prediction.rs:68-82, the gain
+    derivative of derivation.rs:303-329 and the Textbook coefficient
+    derivative of derivation.rs:408-501 (no difference regularisation)."""
+    cfg_theta, s_reg = 1.01, 1.0  # Algorithm defaults (maximum regularisation)
+    T, n = x.shape
+    S = m.shape[1]
+    s_mse = 1.0 / S
+    vs = np.floor(np.asarray(u) * p.num_voxels).astype(np.int64)
+    V = vs.size
+    nb = p.geometry.neighbours[vs].astype(np.int64)  # (V, 26), -1 = none
+    has = nb >= 0
+    a = p.coefs[vs].astype(np.float64)
+    D = p.delays[vs].astype(np.int64)
+    G = p.gains.reshape(p.num_voxels, 3, 26, 3)[vs].astype(np.float64)  # [v, in, d, out]
+    cols = 3 * np.where(has, nb, 0)[..., None] + np.arange(3)             # (V, 26, 3)
+    xin_all = x[:, cols.reshape(-1)].astype(np.float64).reshape(T, V, 26, 3)
+    xin_all *= has[None, :, :, None]
+    own = 3 * vs[:, None] + np.arange(3)                                 # (V, 3)
+    x_own = x[:, own.reshape(-1)].astype(np.float64).reshape(T, V, 3)
+    H = p.measurement_matrix[0][:, own.reshape(-1)].astype(np.float64)   # (S, 3V)
+    res = (m - p.measurements[0]).astype(np.float64)                     # f32 residuals
+    e = (res @ H).reshape(T, V, 3)
+    l1 = np.abs(x_own).sum(axis=2, keepdims=True)
+    sgn = np.where(np.signbit(x_own), -1.0, 1.0)                         # Rust signum
+    mr = np.where(l1 > cfg_theta, (l1 - cfg_theta) * sgn, 0.0)
+    c = e * s_mse + mr * s_reg                                           # (T, V, 3)
+    y = np.zeros((V, 26, 3))
+    fir = np.zeros((V, 26, 3))
+    iir = np.zeros((V, 26, 3))
+    dG = np.zeros((V, 3, 26, 3))
+    dA = np.zeros((V, 26))
+    vi, di = np.meshgrid(np.arange(V), np.arange(26), indexing="ij")
+    for t in range(T):
+        r0, r1 = t - D, t - D - 1
+        xin = np.where((r0 >= 0)[..., None], xin_all[np.maximum(r0, 0), vi, di], 0.0)
+        xin1 = np.where((r1 >= 0)[..., None], xin_all[np.maximum(r1, 0), vi, di], 0.0)
+        y_last = y
+        y = np.where(has[..., None], a[..., None] * (xin - y) + xin1, 0.0)
+        dG += c[t][:, :, None, None] * y[:, None, :, :]
+        on = (t >= D)[..., None]
+        fir = np.where(on & has[..., None], -a[..., None] * fir + xin, fir)
+        iir = np.where(on, -a[..., None] * iir + y_last, iir)
+        dA += np.einsum("vdo,vido,vi->vd", fir - iir, G, e[t]) * s_mse
+    return vs, dG, dA
