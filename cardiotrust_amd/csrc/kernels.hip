@@ -146,8 +146,9 @@ constexpr int kResStates = (kResMaxVoxels * 3 + kResThreads - 1) / kResThreads;
 constexpr int kResBatch = 2;
 constexpr int kResGainConns = 9;
 // Gain rows of each lane's first state held in registers for the launch: one
-// group of 27 (68.2-68.9 ms per 2000 steps against 70.4-71.1 without; two
-// groups exceed the register file).
+// group of 27 (68.2-68.9 ms per 2000 steps against 70.4-71.1 without). The
+// kernel then needs 219 VGPRs, but 36 cached rows (a partial second group)
+// already spill 177 and 54 spill 291.
 constexpr int kResCachedRows = 27;
 // The batched kernel also holds its job's fields: its own measured values
 // (ms per epoch of the 64-scenario uneven batch): 3 gathers / halves 1751,
