@@ -103,6 +103,7 @@ def bench_batched(args) -> int:
 
     t0 = time.perf_counter()
     adjoint = args.workload == "batched-adjoint"
+    shared = args.share_propagation == "on" or (args.share_propagation == "auto" and not adjoint)
     if adjoint:
         problems = synth.config4(count=args.scenarios or 8, num_steps=args.num_steps)
         label = (f"{len(problems)} ellipsoidal shells of ~120k voxels with slow sub-regions, "
@@ -128,7 +129,7 @@ def bench_batched(args) -> int:
         k = EpochKernel(gpu, r, b, cfg, 3 * p.num_voxels, p.num_sensors, p.num_steps)
         if adjoint:
             k.set_gradient(1, args.checkpoint)
-        k.set_share_propagation(args.share_propagation)
+        k.set_share_propagation({"auto": None, "on": True, "off": False}[args.share_propagation])
         members.append((r, b, k))
     voxel_steps = sum(p.num_voxels * p.num_steps * p.num_beats for p in problems)
 
@@ -164,7 +165,7 @@ def bench_batched(args) -> int:
     resident_vs = 0  # voxel-steps the batched launches propagated in one epoch
     for (r, _, _), p in zip(members, problems):
         if r.set_propagation(0):
-            launches = 1 if args.share_propagation else p.num_beats  # batch_size 0
+            launches = 1 if shared else p.num_beats  # batch_size 0
             resident_vs += p.num_voxels * p.num_steps * launches
     achieved = PROP_BYTES_PER_VOXEL_STEP * resident_vs / (prop_ms * 1e-3) / 1e9 if prop_ms else 0.0
     out = {
@@ -181,7 +182,7 @@ def bench_batched(args) -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": label, "scenarios": len(problems),
-                   "shared_propagation": bool(args.share_propagation),
+                   "shared_propagation": shared, "share_mode": args.share_propagation,
                    "voxel_steps_per_epoch": voxel_steps, "graph": bool(info["graph"]),
                    "voxels_per_group": info["voxels_per_group"], "rounds": info["rounds"],
                    "jobs": info["jobs"]},
@@ -226,9 +227,10 @@ def main() -> int:
     ap.add_argument("--voxels-per-group", type=int, default=0,
                     help="batched workloads: 0 = autotune")
     ap.add_argument("--no-graph", action="store_true", help="batched workloads: eager epochs")
-    ap.add_argument("--share-propagation", action="store_true",
+    ap.add_argument("--share-propagation", choices=["auto", "on", "off"], default="auto",
                     help="batched workloads: one propagation (and, with the reverse-mode "
-                         "gradient, one backward pass) per update batch of beats")
+                         "gradient, one backward pass) per update batch of beats; auto = "
+                         "the library default (shared under the reference derivation only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the epoch on a single MI355X; "

@@ -235,8 +235,12 @@ int32_t update_group(const ct_epoch_kernel &k) {
     return k.cfg.batch_size == 0 ? k.r->B : k.cfg.batch_size;
 }
 
+bool shares_propagation(const ct_epoch_kernel &k) {
+    return k.share_mode > 0 || (k.share_mode < 0 && !uses_adjoint(k));
+}
+
 bool propagates_at(const ct_epoch_kernel &k, int32_t pos) {
-    return !k.share_propagation || pos % update_group(k) == 0;
+    return !shares_propagation(k) || pos % update_group(k) == 0;
 }
 
 int finish_beat(ct_epoch_kernel &k, int32_t pos) {
@@ -245,7 +249,8 @@ int finish_beat(ct_epoch_kernel &k, int32_t pos) {
     CT_CHECK(project_beat(r, beat, k.mhat->data, &k));
     CT_CHECK(losses_beat(r, k.cfg, &k));
     if (!uses_adjoint(k)) return derive_reference(r, beat, k.cfg, &k);
-    if (!k.share_propagation) return derive_adjoint(r, beat, k.cfg, k.checkpoint_interval, &k);
+    if (!shares_propagation(k))
+        return derive_adjoint(r, beat, k.cfg, k.checkpoint_interval, &k);
     // one backward pass per update batch, for the batch's summed objective
     if (r.N <= 0) return CT_OK;
     const int32_t group = update_group(k), first = pos - pos % group;
@@ -1315,7 +1320,8 @@ int ct_results_backward(ct_results_gpu *results, const ct_buffer *actual, int32_
 
 int ct_epoch_kernel_set_share_propagation(ct_epoch_kernel *k, int32_t enabled) {
     if (!k) return fail(CT_ERR_INVALID, "null argument");
-    k->share_propagation = enabled != 0;
+    if (enabled < -1 || enabled > 1) return fail(CT_ERR_INVALID, "share mode must be -1, 0 or 1");
+    k->share_mode = enabled;
     return CT_OK;
 }
 

@@ -193,7 +193,7 @@ int plan(ct_batch &b, int32_t vpg) {
     for (int32_t s = 0; s < S; ++s) {
         ct_epoch_kernel &k = *b.kernels[s];
         b.plan_checkpoints[s] = batch_checkpoint(k);
-        b.plan_groups[s] = k.share_propagation ? update_group(k) : 0;
+        b.plan_groups[s] = shares_propagation(k) ? update_group(k) : 0;
         // checkpoint buffers are allocated now: the job table holds their address
         if (b.plan_checkpoints[s] > 0 && k.r->N > 0)
             CT_CHECK(ensure_checkpoints(*k.r, b.plan_checkpoints[s]));
@@ -279,7 +279,7 @@ bool plan_current(const ct_batch &b) {
     for (size_t s = 0; s < b.kernels.size(); ++s) {
         const ct_epoch_kernel &k = *b.kernels[s];
         if (b.plan_checkpoints[s] != batch_checkpoint(k) ||
-            b.plan_groups[s] != (k.share_propagation ? update_group(k) : 0))
+            b.plan_groups[s] != (shares_propagation(k) ? update_group(k) : 0))
             return false;
     }
     return true;
@@ -415,7 +415,7 @@ bool graph_current(const ct_batch &b) {
     for (size_t s = 0; s < b.kernels.size(); ++s) {
         const ct_epoch_kernel &k = *b.kernels[s];
         if (k.order != b.graph_orders[s] || !same_cfg(k.cfg, b.graph_cfgs[s]) ||
-            k.gradient * 100000 + k.checkpoint_interval + (k.share_propagation ? 1 << 30 : 0) !=
+            k.gradient * 100000 + k.checkpoint_interval + (shares_propagation(k) ? 1 << 30 : 0) !=
                 b.graph_gradients[s])
             return false;
     }
@@ -437,7 +437,7 @@ int capture(ct_batch &b) {
         b.graph_orders.push_back(k->order);
         b.graph_cfgs.push_back(k->cfg);
         b.graph_gradients.push_back(k->gradient * 100000 + k->checkpoint_interval +
-                                    (k->share_propagation ? 1 << 30 : 0));
+                                    (shares_propagation(*k) ? 1 << 30 : 0));
     }
     CT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const int rc = enqueue_epoch(b, false);

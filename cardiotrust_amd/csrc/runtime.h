@@ -174,8 +174,10 @@ struct ct_epoch_kernel {
     int32_t checkpoint_interval = 64;
     // beats of one update batch share one propagation (their states are
     // identical: same parameters and drive); with the reverse-mode gradient
-    // also one backward pass for the batch's summed objective
-    bool share_propagation = false;
+    // also one backward pass for the batch's summed objective. -1 automatic:
+    // shared when the epoch runs the reference derivation (bit-identical),
+    // not under the reverse-mode gradient; 0 never; 1 always.
+    int32_t share_mode = -1;
     std::vector<hipEvent_t> events;
     size_t used_events = 0;
     struct SpanRec {
@@ -240,6 +242,8 @@ int finish_beat(ct_epoch_kernel &k, int32_t pos);
 int32_t update_group(const ct_epoch_kernel &k);
 bool propagates_at(const ct_epoch_kernel &k, int32_t pos);
 bool uses_adjoint(const ct_epoch_kernel &k);
+// Whether the beats of one update batch share one propagation (share_mode).
+bool shares_propagation(const ct_epoch_kernel &k);
 int finish_batch(ct_epoch_kernel &k, int64_t num_beats, bool reset_after, int32_t update);
 int32_t updates_per_epoch(const ct_epoch_kernel &k);
 int begin_epoch(ct_epoch_kernel &k);

@@ -407,10 +407,14 @@ class EpochKernel:
         _check(lib().ct_epoch_kernel_set_gradient(self.handle, int(mode), int(checkpoint_interval)),
                "set_gradient")
 
-    def set_share_propagation(self, enabled: bool) -> None:
+    def set_share_propagation(self, enabled: Optional[bool]) -> None:
         """Propagate once per update batch (the beats' states are identical);
-        with the reverse-mode gradient, one backward pass per update batch."""
-        _check(lib().ct_epoch_kernel_set_share_propagation(self.handle, int(bool(enabled))),
+        with the reverse-mode gradient, one backward pass per update batch.
+        None = automatic (the default: shared under the reference derivation,
+        where the results are bit-identical, not under the reverse-mode
+        gradient)."""
+        mode = -1 if enabled is None else int(bool(enabled))
+        _check(lib().ct_epoch_kernel_set_share_propagation(self.handle, mode),
                "set_share_propagation")
 
     def set_freeze_delays(self, value: bool) -> None:

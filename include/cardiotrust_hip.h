@@ -267,15 +267,17 @@ int ct_epoch_kernel_execute(ct_epoch_kernel *kernel);
 int ct_epoch_kernel_set_gradient(ct_epoch_kernel *kernel, int32_t mode,
                                  int32_t checkpoint_interval);
 
-/* Shared propagation (no reference counterpart; off by default). The beats
- * of one update batch (all beats, or batch_size of them) use the same
- * parameters and drive, and the propagation does not depend on the beat
- * (prediction.rs:15-105), so their states are identical: with `enabled` the
- * epoch propagates once per update batch instead of once per beat, with
+/* Shared propagation (no reference counterpart). The beats of one update
+ * batch (all beats, or batch_size of them) use the same parameters and
+ * drive, and the propagation does not depend on the beat
+ * (prediction.rs:15-105), so their states are identical: shared, the epoch
+ * propagates once per update batch instead of once per beat, with
  * bit-identical results under gradient mode 0. Under gradient mode 1 the
  * batch's summed objective sum_b J_b is differentiated with one backward pass
  * (forcing summed over the beats), instead of one pass per beat: the same
- * gradient up to FP32 rounding. */
+ * gradient up to FP32 rounding. `enabled`: 1 shared, 0 one propagation per
+ * beat, -1 automatic (the default): shared when the epoch runs the reference
+ * derivation, not under gradient mode 1. CT_ERR_INVALID for other values. */
 int ct_epoch_kernel_set_share_propagation(ct_epoch_kernel *kernel, int32_t enabled);
 
 /* Forward / backward pair of one beat (the counterpart of an autograd

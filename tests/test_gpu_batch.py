@@ -89,6 +89,14 @@ class Member:
         self.r.close()
 
 
+def propagations(p, spec):
+    """Propagation jobs of one epoch under the default share mode: one per
+    update batch of beats (the reference derivation shares the identical
+    propagation of the beats of one update batch)."""
+    group = spec["cfg"].batch_size or p.num_beats
+    return -(-p.num_beats // group) if p.num_beats else 0
+
+
 def orders(ps):
     return [list(range(p.num_beats))[::-1] if p.num_beats > 1 else None for p in ps]
 
@@ -153,7 +161,7 @@ def test_batch_equals_each_scenario_alone(gpu, ps, alone2, vpg):
     assert_same(alone2, got)
     info = infos[-1]
     assert info["rounds"] == max(p.num_beats for p in ps)
-    assert info["jobs"] == sum(p.num_beats for p in ps)
+    assert info["jobs"] == sum(propagations(p, CFGS[i % len(CFGS)]) for i, p in enumerate(ps))
     assert info["slots"] > 0
     if vpg == 0:
         assert info["autotune"], info
@@ -187,7 +195,8 @@ def test_batch_with_a_per_step_member(gpu, ps):
     alone = run_alone(gpu, ps, epochs=2, modes=modes)
     got, infos = run_batched(gpu, ps, epochs=2, vpg=128, modes=modes)
     assert_same(alone, got)
-    assert infos[-1]["jobs"] == sum(p.num_beats for i, p in enumerate(ps) if modes[i] != 1)
+    assert infos[-1]["jobs"] == sum(propagations(p, CFGS[i % len(CFGS)])
+                                   for i, p in enumerate(ps) if modes[i] != 1)
 
 
 def test_batch_member_matches_oracle(gpu, ps):

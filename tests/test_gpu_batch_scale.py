@@ -3,8 +3,9 @@
 The first 12 scenarios of the seeded 64-scenario uneven workload (voxels
 log-uniform in 1k-40k, 1-8 beats, 600-2000 steps, delays 1-20) run two epochs
 as one batch (graph replay, autotuned slab size) and each on its own epoch
-kernel. Every result array must be the same bits: compared by SHA-256 so the
-host holds one member's arrays at a time.
+kernel, with one propagation per beat (share mode off) so that the batch
+carries one job per beat. Every result array must be the same bits: compared
+by SHA-256 so the host holds one member's arrays at a time.
 """
 import hashlib
 import sys
@@ -44,6 +45,7 @@ def member(gpu, p, cfg, epochs):
     r = ResultsGPU(gpu, p, num_batch_entries=epochs)
     buf = Buffer(gpu, p.measurements)
     k = EpochKernel(gpu, r, buf, cfg, 3 * p.num_voxels, p.num_sensors, p.num_steps)
+    k.set_share_propagation(False)  # one propagation job per beat: 61 jobs in all
     return r, buf, k
 
 

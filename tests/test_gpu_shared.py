@@ -51,7 +51,7 @@ def problem(seed, beats=3, T=60, n=400):
 
 
 def run(gpu, p, cfg, share, epochs=2, gradient=None, optimizer=0, order=None, batch=False,
-        graph=False):
+        graph=False, launches=None):
     from cardiotrust_amd.gpu import Batch, Buffer, EpochKernel, ResultsGPU
     nb = epochs * (p.num_beats if cfg.batch_size else 1)
     r = ResultsGPU(gpu, p, num_batch_entries=nb, optimizer=optimizer)
@@ -61,10 +61,14 @@ def run(gpu, p, cfg, share, epochs=2, gradient=None, optimizer=0, order=None, ba
         k.set_gradient(*gradient)
     if order is not None:
         k.set_beat_order(order)
-    k.set_share_propagation(share)
+    if share != "default":
+        k.set_share_propagation(share)
+    k.set_profiling(launches is not None)
     b = Batch(gpu, [k], voxels_per_group=96, graph=graph) if batch else None
     for _ in range(epochs):
         (b or k).execute()
+        if launches is not None:
+            launches.append(int(k.profile()["propagate_launches"]))
     if b:
         b.close()
     out = r.update_from_gpu(list(FIELDS))
@@ -127,3 +131,22 @@ def test_shared_propagation_in_a_graph_replayed_batch(gpu):
         x, y = np.asarray(a[f]), np.asarray(b[f])
         assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), f
 
+
+
+def test_default_shares_under_the_reference_derivation_only(gpu):
+    """The default (automatic) propagates once per update batch under the
+    reference derivation, bit-identical to one propagation per beat, and once
+    per beat under the reverse-mode gradient (the per-beat backward passes)."""
+    note("case: 3 beats, default share mode vs explicit, both gradient modes")
+    p = problem(5)
+    cfg = Algorithm(learning_rate=0.02, freeze_delays=False)
+    n_auto, n_off = [], []
+    auto = run(gpu, p, cfg, "default", launches=n_auto)
+    off = run(gpu, p, cfg, False, launches=n_off)
+    assert n_auto == [1, 1] and n_off == [3, 3], (n_auto, n_off)
+    for f in FIELDS:
+        assert np.array_equal(np.asarray(auto[f]).view(np.uint8), np.asarray(off[f]).view(np.uint8)), f
+    auto = run(gpu, p, cfg, "default", gradient=(1, 16))
+    off = run(gpu, p, cfg, False, gradient=(1, 16))
+    for f in FIELDS:
+        assert np.array_equal(np.asarray(auto[f]).view(np.uint8), np.asarray(off[f]).view(np.uint8)), f
