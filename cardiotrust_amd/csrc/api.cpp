@@ -84,7 +84,7 @@ int neighbours_from_indices(int32_t N, const int32_t *osi, const int32_t *delays
 }
 
 int reset_derivatives(ct_results_gpu &r) {
-    hipStream_t s = r.gpu->stream;
+    hipStream_t s = r.stream;
     // fill kernels, not memset nodes, so that the epoch can be captured in a graph
     CT_HIP(ct::launch_fill(s, r.dGT, 0.0f, r.n * 78));
     CT_HIP(ct::launch_fill(s, r.dA, 0.0f, int64_t(r.N) * 26));
@@ -102,7 +102,7 @@ uint64_t wait_ticks(const ct_gpu &g, double seconds) {
 // Clears y / y_last (estimation.rs:82-86: a beat starts from rest) and records
 // the checkpoint interval of the coming propagation.
 int prepare_propagation(ct_results_gpu &r, int32_t ckpt_interval) {
-    hipStream_t s = r.gpu->stream;
+    hipStream_t s = r.stream;
     if (ckpt_interval > 0) CT_CHECK(ensure_checkpoints(r, ckpt_interval));
     r.ckpt_interval = ckpt_interval;
     r.xT_fresh = false;  // the states are about to change
@@ -137,7 +137,7 @@ ct::ResidentJob resident_job(ct_results_gpu &r, int32_t ckpt_interval) {
 // ckpt_interval = K > 0 the pass also stores the filter outputs y_t at
 // t = kK - 1 (k = 1 .. ceil(T/K) - 1) for the reverse-mode gradient.
 int propagate_beat(ct_results_gpu &r, ct_epoch_kernel *k, int32_t ckpt_interval) {
-    hipStream_t s = r.gpu->stream;
+    hipStream_t s = r.stream;
     CT_CHECK(prepare_propagation(r, ckpt_interval));
     const int64_t M = int64_t(r.N) * 78;
     const bool resident = r.resident_ok && r.propagation_mode != 1;
@@ -162,7 +162,7 @@ int propagate_beat(ct_results_gpu &r, ct_epoch_kernel *k, int32_t ckpt_interval)
 // The projected measurements of the beat (prediction.rs:112-130 for every
 // step); with `mhat` also residuals and per-step MSE.
 int project_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_kernel *k) {
-    hipStream_t s = r.gpu->stream;
+    hipStream_t s = r.stream;
     r.forward_beat = beat;
     Span sp(k, kProject);
     const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
@@ -184,7 +184,7 @@ int forward_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_ke
 // losses (derivation.rs:509-535, metrics.rs:180-196), and the adjoint
 // objective J of the beat.
 int losses_beat(ct_results_gpu &r, const ct_algorithm &c, ct_epoch_kernel *k) {
-    hipStream_t s = r.gpu->stream;
+    hipStream_t s = r.stream;
     Span sp(k, kRegularize);
     CT_HIP(ct::launch_maxreg_partials(s, r.x, r.mr_partial, r.mr_last, r.N, r.T,
                                       c.maximum_regularization_threshold, r.mr_blocks));
@@ -196,7 +196,7 @@ int losses_beat(ct_results_gpu &r, const ct_algorithm &c, ct_epoch_kernel *k) {
 
 // The reference's per-step derivation over the whole beat (derivation.rs:173-229).
 int derive_reference(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, ct_epoch_kernel *k) {
-    hipStream_t s = r.gpu->stream;
+    hipStream_t s = r.stream;
     const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
     {
         Span sp(k, kMapResiduals);
@@ -278,7 +278,7 @@ static int run_beat(ct_epoch_kernel &k, int32_t pos) {
 // device scalars written by begin_epoch.
 int finish_batch(ct_epoch_kernel &k, int64_t num_beats, bool reset_after, int32_t update) {
     ct_results_gpu &r = *k.r;
-    hipStream_t s = r.gpu->stream;
+    hipStream_t s = r.stream;
     const ct_algorithm &c = k.cfg;
     Span sp(&k, kUpdate);
     const ct::UpdateScalars *sc = r.scal + update;
@@ -328,7 +328,7 @@ int begin_epoch(ct_epoch_kernel &k) {
     if (c.optimizer == 1 && ((!c.freeze_gains && !r.gm1) || (!c.freeze_delays && !r.cm1)))
         return fail(CT_ERR_STATE, "Adam update without Adam moments (results were uploaded with "
                                   "optimizer = Sgd)");
-    CT_HIP(hipStreamSynchronize(r.gpu->stream));
+    CT_HIP(hipStreamSynchronize(r.stream));
     for (int32_t u = 0; u < updates; ++u) {
         const float step = float(r.step + u);
         r.scal_host[u].c1 = 1.0f - std::pow(0.9f, step);
@@ -337,7 +337,7 @@ int begin_epoch(ct_epoch_kernel &k) {
         r.scal_host[u].pad = 0;
     }
     CT_HIP(hipMemcpyAsync(r.scal, r.scal_host, sizeof(ct::UpdateScalars) * updates,
-                          hipMemcpyHostToDevice, r.gpu->stream));
+                          hipMemcpyHostToDevice, r.stream));
     return CT_OK;
 }
 
@@ -386,7 +386,7 @@ int check_resident(ct_results_gpu &r) {
     if (!r.resident_used) return CT_OK;
     r.resident_used = false;
     ct::GridBarrier gb{};
-    CT_HIP(hipStreamSynchronize(r.gpu->stream));
+    CT_HIP(hipStreamSynchronize(r.stream));
     CT_HIP(hipMemcpy(&gb, r.barrier, sizeof(gb), hipMemcpyDeviceToHost));
     if (gb.abort.count) {
         r.resident_ok = false;
@@ -405,7 +405,7 @@ int ensure(ct_results_gpu &r, V **p, int64_t &have, int64_t count, int64_t unit)
     if (*p) {
         auto it = std::find(r.allocations.begin(), r.allocations.end(), static_cast<void *>(*p));
         if (it != r.allocations.end()) r.allocations.erase(it);
-        CT_HIP(hipStreamSynchronize(r.gpu->stream));
+        CT_HIP(hipStreamSynchronize(r.stream));
         CT_HIP(hipFree(*p));
         *p = nullptr;
     }
@@ -425,7 +425,7 @@ int ensure_checkpoints(ct_results_gpu &r, int32_t K) {
 }
 
 int fix_adjoint_ring(ct_results_gpu &r, int64_t headroom) {
-    hipStream_t s = r.gpu->stream;
+    hipStream_t s = r.stream;
     const int64_t M = int64_t(r.N) * 78;
     CT_HIP(ct::launch_max_delay(s, r.nbr, r.delays, int64_t(r.N) * 26, r.dmax));
     int32_t dmax = 0;
@@ -476,7 +476,7 @@ static bool resident_sweep(const ct_results_gpu &r, int64_t slots) {
 // Buffers and q-ring size of the reverse-mode gradient (checkpoints every K).
 static int adjoint_prepare(ct_results_gpu &r, int32_t K, int64_t &slots) {
     if (K < 1) return fail(CT_ERR_INVALID, "checkpoint interval must be >= 1");
-    hipStream_t s = r.gpu->stream;
+    hipStream_t s = r.stream;
     const int64_t n = r.n, T = r.T, M = int64_t(r.N) * 78;
     CT_CHECK(build_consumers(r));
     int64_t have_e = r.E ? n * T : 0, have_xb = r.xbar ? n * T : 0, have_xbt = r.xbarT ? n * T : 0,
@@ -511,7 +511,7 @@ static int adjoint_prepare(ct_results_gpu &r, int32_t K, int64_t &slots) {
 // Hᵀ r of `beat` for every step into eT (n, T); with `accumulate` added to
 // what eT holds (beats of one update batch, in beat order).
 static int adjoint_forcing(ct_results_gpu &r, int32_t beat, bool accumulate, ct_epoch_kernel *k) {
-    hipStream_t s = r.gpu->stream;
+    hipStream_t s = r.stream;
     const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
     Span sp(k, kMapResiduals);
     if (!accumulate) {
@@ -528,7 +528,7 @@ static int adjoint_forcing(ct_results_gpu &r, int32_t beat, bool accumulate, ct_
 // maximum-regularisation term once).
 static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K, int64_t slots,
                             int32_t num_beats, ct_epoch_kernel *k) {
-    hipStream_t s = r.gpu->stream;
+    hipStream_t s = r.stream;
     const int64_t n = r.n, T = r.T, M = int64_t(r.N) * 78;
     {
         Span sp(k, kTranspose);
@@ -681,6 +681,7 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
         return rc;
     };
     r->gpu = gpu;
+    r->stream = gpu->stream;
     r->N = N;
     r->n = 3LL * N;
     r->S = d.num_sensors;
@@ -818,7 +819,7 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
 void ct_results_gpu_free(ct_results_gpu *results) {
     if (!results) return;
     (void)hipSetDevice(results->gpu->device);
-    (void)hipStreamSynchronize(results->gpu->stream);
+    (void)hipStreamSynchronize(results->stream);
     delete results;
 }
 
@@ -826,7 +827,7 @@ int ct_results_update_from_gpu(ct_results_gpu *results, const ct_results_host *o
     if (!results || !o) return fail(CT_ERR_INVALID, "null argument");
     ct_results_gpu &r = *results;
     CT_CHECK(check_gpu(r.gpu));
-    hipStream_t s = r.gpu->stream;
+    hipStream_t s = r.stream;
     CT_HIP(hipStreamSynchronize(s));
     const int64_t n = r.n, N = r.N, T = r.T, S = r.S;
     auto via_staging = [&](float *dst, auto &&produce) -> int {
@@ -875,7 +876,7 @@ int ct_results_simulate(ct_results_gpu *results, float *measurements_out) {
     CT_CHECK(check_gpu(r.gpu));
     for (int32_t beat = 0; beat < r.B; ++beat) CT_CHECK(forward_beat(r, beat, nullptr, nullptr));
     CT_CHECK(check_resident(r));
-    CT_HIP(hipStreamSynchronize(r.gpu->stream));
+    CT_HIP(hipStreamSynchronize(r.stream));
     if (measurements_out)
         CT_CHECK(download(measurements_out, r.m, sizeof(float) * r.B * r.T * r.S));
     return CT_OK;
@@ -1001,7 +1002,7 @@ int ct_results_spherical(ct_results_gpu *results, float sample_rate_hz, float *m
         return fail(CT_ERR_INVALID, "sample rate must be positive and finite");
     if (r.N == 0) return CT_OK;
     if (r.T == 0) return fail(CT_ERR_INVALID, "no steps: the maxima are undefined");
-    hipStream_t s = r.gpu->stream;
+    hipStream_t s = r.stream;
     const size_t full = size_t(r.T) * r.N, N = size_t(r.N);
     const size_t nfull = size_t(magnitude != nullptr) + (theta != nullptr) + (phi != nullptr);
     char *dev = nullptr;
@@ -1062,7 +1063,7 @@ int ct_results_calculate_final_metrics(ct_results_gpu *results, const uint32_t *
     constexpr int kThresholds = ct::kSegmentationThresholds, kBins = kThresholds + 1;
     ct::SegmentationThresholds th;
     for (int i = 0; i < kThresholds; ++i) th.value[i] = float(i) / 100.0f;  // metrics.rs:233
-    hipStream_t s = r.gpu->stream;
+    hipStream_t s = r.stream;
     const int64_t nvox = std::max<int32_t>(r.N, 1), ncell = std::max<int64_t>(cells, 1);
     char *dev = nullptr;
     CT_HIP(hipMalloc(&dev, sizeof(uint32_t) * 2 * kBins + sizeof(float) * nvox +
@@ -1121,8 +1122,8 @@ int ct_results_simulate_noisy(ct_results_gpu *results, const float *sensor_std, 
     CT_CHECK(ensure(r, &r.sensor_std, r.sensor_std_have, std::max<int64_t>(r.S, 1), 1));
     CT_CHECK(upload(r.sensor_std, sensor_std, sizeof(float) * r.S));
     CT_CHECK(ct_results_simulate(results, nullptr));
-    CT_HIP(ct::launch_measurement_noise(r.gpu->stream, r.m, r.sensor_std, r.B, r.T, r.S, seed));
-    CT_HIP(hipStreamSynchronize(r.gpu->stream));
+    CT_HIP(ct::launch_measurement_noise(r.stream, r.m, r.sensor_std, r.B, r.T, r.S, seed));
+    CT_HIP(hipStreamSynchronize(r.stream));
     if (measurements_out)
         CT_CHECK(download(measurements_out, r.m, sizeof(float) * r.B * r.T * r.S));
     return CT_OK;
@@ -1235,12 +1236,12 @@ int ct_results_forward(ct_results_gpu *results, const ct_buffer *actual, int32_t
     if (beat < 0 || beat >= r.B) return fail(CT_ERR_INVALID, "beat out of range");
     if (actual->beats != r.B || actual->steps != r.T || actual->sensors != r.S)
         return fail(CT_ERR_INVALID, "actual measurements are not (B, T, S) of the results");
-    CT_HIP(hipMemsetAsync(r.mr_sum, 0, sizeof(float), r.gpu->stream));
+    CT_HIP(hipMemsetAsync(r.mr_sum, 0, sizeof(float), r.stream));
     if (checkpoint_interval < 0) return fail(CT_ERR_INVALID, "negative checkpoint interval");
     CT_CHECK(forward_beat(r, beat, actual->data, nullptr, checkpoint_interval));
     CT_CHECK(losses_beat(r, *config, nullptr));
     CT_CHECK(check_resident(r));
-    CT_HIP(hipStreamSynchronize(r.gpu->stream));
+    CT_HIP(hipStreamSynchronize(r.stream));
     if (objective) CT_CHECK(download(objective, r.objective, sizeof(double)));
     return CT_OK;
 }
@@ -1258,7 +1259,7 @@ int ct_results_objective_f64(ct_results_gpu *results, const ct_buffer *actual, i
     if ((param == 1 && (index < 0 || index >= r.n * 78)) ||
         (param == 2 && (index < 0 || index >= int64_t(r.N) * 26)))
         return fail(CT_ERR_INVALID, "parameter index out of range");
-    hipStream_t s = r.gpu->stream;
+    hipStream_t s = r.stream;
     const int64_t n = r.n, T = r.T;
     const int64_t tiles = (r.Sp / 64) * ((T + 63) / 64);
     const int32_t splits = int32_t(std::min<int64_t>(64, std::max<int64_t>(1, (2048 + tiles - 1) /
@@ -1314,7 +1315,7 @@ int ct_results_backward(ct_results_gpu *results, const ct_buffer *actual, int32_
         CT_CHECK(derive_adjoint(r, beat, *config, checkpoint_interval, nullptr));
     }
     CT_CHECK(check_resident(r));
-    CT_HIP(hipStreamSynchronize(r.gpu->stream));
+    CT_HIP(hipStreamSynchronize(r.stream));
     return CT_OK;
 }
 
@@ -1379,7 +1380,7 @@ int ct_epoch_kernel_last_loss(ct_epoch_kernel *k, float *loss, float *loss_mse,
     const int64_t i = int64_t(k->batch_index) - 1;
     if (i >= r.nbatch) return fail(CT_ERR_INVALID, "batch index beyond the batch arrays");
     float v[3] = {0.0f, 0.0f, 0.0f};
-    CT_HIP(hipStreamSynchronize(r.gpu->stream));
+    CT_HIP(hipStreamSynchronize(r.stream));
     CT_CHECK(download(&v[0], r.loss_b + i, sizeof(float)));
     CT_CHECK(download(&v[1], r.loss_mse_b + i, sizeof(float)));
     CT_CHECK(download(&v[2], r.loss_mr_b + i, sizeof(float)));

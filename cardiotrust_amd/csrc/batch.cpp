@@ -69,7 +69,22 @@ struct ct_batch {
 
     std::vector<void *> allocations;
 
+    // Side streams (default priority) for the scenarios' work after each
+    // round's propagation: scenario s runs on side[s % side.size()], after the
+    // `fork` event recorded behind the propagation; the next propagation waits
+    // for every side stream's `done` event. Used only while no member runs the
+    // reverse-mode gradient: its backward sweep is a persistent kernel whose
+    // workgroups must all be resident at once, which two sweeps on two streams
+    // would contend for. Otherwise everything runs on the GPU's stream.
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> done;
+    hipEvent_t fork = nullptr;
+    bool side_active = false;  // this epoch uses the side streams
+
     ~ct_batch() {
+        for (hipStream_t st : side) (void)hipStreamDestroy(st);
+        for (hipEvent_t e : done) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (auto &p : prop_events) {
@@ -83,6 +98,10 @@ struct ct_batch {
 };
 
 namespace {
+
+// Side streams per batch: with the GPU's own stream four, the hardware queues
+// a process gets by default.
+constexpr int kSideStreams = 3;
 
 int batch_checkpoint(const ct_epoch_kernel &k) {
     return uses_adjoint(k) ? k.checkpoint_interval : 0;
@@ -316,31 +335,72 @@ int propagate_round(ct_batch &b, int32_t rd, bool timed) {
 }
 
 // One epoch of every scenario, enqueued on the stream (no synchronisation).
+// Every side stream's work so far before the GPU stream's next work.
+int join_side(ct_batch &b) {
+    if (!b.side_active) return CT_OK;
+    for (size_t j = 0; j < b.side.size(); ++j) {
+        CT_HIP(hipEventRecord(b.done[j], b.side[j]));
+        CT_HIP(hipStreamWaitEvent(b.gpu->stream, b.done[j], 0));
+    }
+    return CT_OK;
+}
+
+// The GPU stream's work so far before every side stream's next work.
+int fork_side(ct_batch &b) {
+    if (!b.side_active) return CT_OK;
+    CT_HIP(hipEventRecord(b.fork, b.gpu->stream));
+    for (hipStream_t st : b.side) CT_HIP(hipStreamWaitEvent(st, b.fork, 0));
+    return CT_OK;
+}
+
+// Runs `fn` with scenario s's work on its side stream (if any).
+template <typename F>
+int on_side(ct_batch &b, int32_t s, F &&fn) {
+    ct_results_gpu &r = *b.kernels[s]->r;
+    const hipStream_t keep = r.stream;
+    if (b.side_active) r.stream = b.side[size_t(s) % b.side.size()];
+    const int rc = fn();
+    r.stream = keep;
+    return rc;
+}
+
 int enqueue_epoch(ct_batch &b, bool timed) {
     const int32_t S = int32_t(b.kernels.size());
     std::vector<int64_t> in_batch(S, 0);
     std::vector<int32_t> update(S, 0);
+    b.side_active = !b.side.empty() &&
+                    std::none_of(b.kernels.begin(), b.kernels.end(),
+                                 [](const ct_epoch_kernel *k) { return uses_adjoint(*k); });
     for (int32_t s = 0; s < S; ++s) CT_CHECK(reset_derivatives(*b.kernels[s]->r));
     for (int32_t rd = 0; rd < int32_t(b.rounds.size()); ++rd) {
+        // the round's propagation overwrites states the previous round's
+        // beats read on the side streams
+        if (rd > 0) CT_CHECK(join_side(b));
         CT_CHECK(propagate_round(b, rd, timed));
+        CT_CHECK(fork_side(b));
         for (int32_t s : b.rounds[rd].scenarios) {
             ct_epoch_kernel &k = *b.kernels[s];
-            CT_CHECK(finish_beat(k, rd));
-            if (k.cfg.batch_size != 0 && ++in_batch[s] == k.cfg.batch_size) {
-                CT_CHECK(finish_batch(k, in_batch[s], true, update[s]++));
-                in_batch[s] = 0;
-            }
+            CT_CHECK(on_side(b, s, [&] {
+                CT_CHECK(finish_beat(k, rd));
+                if (k.cfg.batch_size != 0 && ++in_batch[s] == k.cfg.batch_size) {
+                    CT_CHECK(finish_batch(k, in_batch[s], true, update[s]++));
+                    in_batch[s] = 0;
+                }
+                return CT_OK;
+            }));
         }
     }
     for (int32_t s = 0; s < S; ++s) {
         ct_epoch_kernel &k = *b.kernels[s];
-        if (k.cfg.batch_size != 0) {
-            if (in_batch[s] > 0) CT_CHECK(finish_batch(k, in_batch[s], false, update[s]++));
-        } else {
-            CT_CHECK(finish_batch(k, k.r->B, false, update[s]++));
-        }
+        CT_CHECK(on_side(b, s, [&] {
+            if (k.cfg.batch_size != 0) {
+                if (in_batch[s] > 0) CT_CHECK(finish_batch(k, in_batch[s], false, update[s]++));
+                return CT_OK;
+            }
+            return finish_batch(k, k.r->B, false, update[s]++);
+        }));
     }
-    return CT_OK;
+    return join_side(b);
 }
 
 // Reads the abort words of every barrier after an epoch.
@@ -490,6 +550,22 @@ int ct_batch_new(ct_gpu *gpu, ct_epoch_kernel *const *kernels, int32_t count, ct
         ct::batch_capacity(&per_cu) == hipSuccess && cus > 0 && per_cu > 0)
         b->slots = cus * per_cu;
     (void)hipGetLastError();
+    for (int j = 0; j < kSideStreams; ++j) {
+        hipStream_t st = nullptr;
+        hipEvent_t e = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) break;
+        b->side.push_back(st);
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            delete b;
+            return fail(CT_ERR_DEVICE, "hipEventCreateWithFlags failed");
+        }
+        b->done.push_back(e);
+    }
+    if (b->side.size() != size_t(kSideStreams) ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+        delete b;
+        return fail(CT_ERR_DEVICE, "creating the batch's side streams failed");
+    }
     *out = b;
     return CT_OK;
 }
@@ -602,6 +678,7 @@ int ct_batch_get_info(ct_batch *b, ct_batch_info *out) {
         out->tuned_voxels[i] = b->tuned[i];
         out->tuned_ms[i] = b->tuned_ms[i];
     }
+    out->side_streams = b->side_active ? int32_t(b->side.size()) : 0;
     return CT_OK;
 }
 

@@ -82,6 +82,10 @@ struct ct_buffer {
 
 struct ct_results_gpu {
     ctrt::GpuRef gpu;
+    // the stream this scenario's work goes to: the GPU's, or one of a batch's
+    // side streams while the batch runs the scenario's beats after a round's
+    // propagation
+    hipStream_t stream = nullptr;
     int32_t N = 0, S = 0, Sp = 0, T = 0, B = 0, nbatch = 0, optimizer = 0;
     int64_t n = 0;
     std::vector<void *> allocations;
@@ -210,11 +214,11 @@ struct Span {
         if (k->used_events + 2 > k->events.size()) return;
         start = k->used_events;
         k->used_events += 2;
-        on = hipEventRecord(k->events[start], k->r->gpu->stream) == hipSuccess;
+        on = hipEventRecord(k->events[start], k->r->stream) == hipSuccess;
     }
     ~Span() {
         if (!on) return;
-        if (hipEventRecord(k->events[start + 1], k->r->gpu->stream) == hipSuccess)
+        if (hipEventRecord(k->events[start + 1], k->r->stream) == hipSuccess)
             k->spans.push_back({phase, steps, start, start + 1});
     }
 };

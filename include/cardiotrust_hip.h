@@ -363,6 +363,10 @@ typedef struct {
     int32_t tuned_candidates;             /* autotuner: voxels-per-workgroup values timed */
     int32_t tuned_voxels[16];
     double tuned_ms[16];                  /* time of round 0's batched propagation for each */
+    int32_t side_streams;                 /* streams the last epoch ran the scenarios' beats on
+                                             besides the GPU's own (0 while a member runs the
+                                             reverse-mode gradient, whose backward sweep must
+                                             have the device to itself) */
 } ct_batch_info;
 
 /* All kernels must live on `gpu`, with distinct results objects. The batch

@@ -160,6 +160,7 @@ def test_batch_equals_each_scenario_alone(gpu, ps, alone2, vpg):
     got, infos = run_batched(gpu, ps, epochs=2, vpg=vpg)
     assert_same(alone2, got)
     info = infos[-1]
+    assert info["side_streams"] == 3  # reference derivation only: beats overlap on side streams
     assert info["rounds"] == max(p.num_beats for p in ps)
     assert info["jobs"] == sum(propagations(p, CFGS[i % len(CFGS)]) for i, p in enumerate(ps))
     assert info["slots"] > 0
@@ -186,6 +187,8 @@ def test_batch_adjoint_members_and_graph(gpu, ps):
     assert_same(alone, got)
     got_g, infos = run_batched(gpu, ps, epochs=3, vpg=96, graph=True, gradients=grads)
     assert infos[-1]["graph"] == 1
+    # a reverse-mode member's backward sweep needs the device to itself
+    assert infos[-1]["side_streams"] == 0
     assert_same(alone, got_g)
 
 
