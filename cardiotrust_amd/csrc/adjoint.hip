@@ -150,6 +150,9 @@ constexpr int kSweepThreads = 512;
 constexpr int kSweepItems = (kResMaxVoxels * 26 + kSweepThreads - 1) / kSweepThreads;
 constexpr int kSweepBatch = 1;  // items per load group: 1 measured fastest (102 ms vs 112 at 4)
 constexpr int kConsGainStride = 10;  // gcons floats per (voxel, consumer) slot
+// Items per lane whose consumer gains and coefficient stay in registers for
+// the whole sweep instead of being re-read from gcons every step.
+constexpr int kSweepCachedItems = 2;
 constexpr uint32_t kSweepNoEntry = 0xFFFFFFFFu;
 static_assert(kSortSize >= kSweepItems * kSweepThreads || kSortSize >= kResMaxVoxels * 26,
               "every phase-1 position has a key");
@@ -214,6 +217,14 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
     // bit k: item k has delay 1, so it reads the xbar row of the sweep step
     // just before and must wait for that step's barrier
     uint32_t late = 0u;
+    float3_a4 gkeep[kSweepCachedItems][3];
+    float akeep[kSweepCachedItems];
+#pragma unroll
+    for (int k = 0; k < kSweepCachedItems; ++k) {
+        akeep[k] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) gkeep[k][i] = float3_a4{0.0f, 0.0f, 0.0f};
+    }
 #pragma unroll
     for (int k = 0; k < kSweepItems; ++k) {
         const int pos = tid + k * kSweepThreads;
@@ -241,6 +252,13 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
                     for (int o = 0; o < 3; ++o)
                         dst[3 * in + o] = gains_rows[(3 * int64_t(v) + in) * 78 + 3 * d + o];
                 dst[9] = coefs[c];
+                if (k < kSweepCachedItems) {
+                    const int kc = k < kSweepCachedItems ? k : 0;
+#pragma unroll
+                    for (int in = 0; in < 3; ++in)
+                        gkeep[kc][in] = float3_a4{dst[3 * in], dst[3 * in + 1], dst[3 * in + 2]};
+                    akeep[kc] = dst[9];
+                }
             }
         }
     }
@@ -276,11 +294,19 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
                 on[kk] = present && tau < T;
                 if (!on[kk]) continue;  // not this pass, absent, or tau beyond the beat
                 const float *xs = xbar + int64_t(tau) * n + 3LL * (pk[k] & 0x1FFFFu);
-                const float *gs = gcons + (int64_t(vb) * 26 + pos) * kConsGainStride;
                 xv[kk] = *reinterpret_cast<const float3_a4 *>(xs);
+                if (k0 + kk < kSweepCachedItems) {
+                    const int kc = k0 + kk < kSweepCachedItems ? k0 + kk : 0;
 #pragma unroll
-                for (int i = 0; i < 3; ++i) g[kk][i] = *reinterpret_cast<const float3_a4 *>(gs + 3 * i);
-                av[kk] = gs[9];
+                    for (int i = 0; i < 3; ++i) g[kk][i] = gkeep[kc][i];
+                    av[kk] = akeep[kc];
+                } else {
+                    const float *gs = gcons + (int64_t(vb) * 26 + pos) * kConsGainStride;
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+                        g[kk][i] = *reinterpret_cast<const float3_a4 *>(gs + 3 * i);
+                    av[kk] = gs[9];
+                }
             }
 #pragma unroll
             for (int kk = 0; kk < kSweepBatch; ++kk) {
