@@ -702,7 +702,7 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
     splits = std::max<int64_t>(1, (n + kps - 1) / kps);
     r->splits = int32_t(splits);
     r->kps = kps;
-    r->mr_blocks = std::max<int32_t>(1, (N + 255) / 256);
+    r->mr_blocks = std::max<int32_t>(1, (N + ct::kMaxregVoxels - 1) / ct::kMaxregVoxels);
 
     int rc = CT_OK;
 #define ALLOC(ptr, count)                                  \

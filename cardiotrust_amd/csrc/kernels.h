@@ -96,6 +96,9 @@ hipError_t launch_project(hipStream_t s, const float *x, const float *H, float *
 hipError_t launch_residuals(hipStream_t s, const float *partial, int32_t splits, const float *mhat,
                             float *m, float *R, float *loss_mse, int32_t T, int32_t S, int32_t Sp);
 
+// Voxels per maxreg_partials workgroup (four per thread): the partial sums
+// per step number ceil(N / kMaxregVoxels).
+constexpr int kMaxregVoxels = 1024;
 hipError_t launch_maxreg_partials(hipStream_t s, const float *x, float *partial, float *mr_last,
                                   int32_t N, int32_t T, float threshold, int32_t blocks);
 

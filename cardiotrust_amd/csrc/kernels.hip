@@ -553,31 +553,41 @@ __global__ void residuals_kernel(const float *__restrict__ partial, int32_t spli
 }
 
 // Maximum regularisation (derivation.rs:509-535): per step, the sum over
-// voxels of (|x|_1 - threshold)^2 where positive; partial sums per workgroup.
+// voxels of (|x|_1 - threshold)^2 where positive; partial sums per workgroup
+// of kMaxregVoxels voxels, four per thread 256 apart (each load instruction
+// stays coalesced), summed in the thread in that order, then in a fixed tree.
 __global__ __launch_bounds__(256) void maxreg_partials_kernel(const float *__restrict__ x,
                                                               float *__restrict__ partial,
                                                               float *__restrict__ mr_last,
                                                               int32_t N, int32_t T,
                                                               float threshold) {
+    constexpr int kPerThread = kMaxregVoxels / 256;
     __shared__ float red[256];
     const int32_t t = blockIdx.y;
-    const int32_t v = blockIdx.x * 256 + threadIdx.x;
+    const float *row = x + int64_t(t) * 3 * N;
+    float xs[kPerThread][3];
+#pragma unroll
+    for (int k = 0; k < kPerThread; ++k) {
+        const int32_t v = int32_t(blockIdx.x) * kMaxregVoxels + k * 256 + int32_t(threadIdx.x);
+#pragma unroll
+        for (int o = 0; o < 3; ++o) xs[k][o] = v < N ? row[3LL * v + o] : 0.0f;
+    }
     float f2 = 0.0f;
-    if (v < N) {
-        const float *xv = x + int64_t(t) * 3 * N + 3LL * v;
-        const float x0 = xv[0], x1 = xv[1], x2 = xv[2];
-        const float sum = fabsf(x0) + fabsf(x1) + fabsf(x2);
+#pragma unroll
+    for (int k = 0; k < kPerThread; ++k) {
+        const int32_t v = int32_t(blockIdx.x) * kMaxregVoxels + k * 256 + int32_t(threadIdx.x);
+        if (v >= N) break;
+        const float sum = fabsf(xs[k][0]) + fabsf(xs[k][1]) + fabsf(xs[k][2]);
         float f = 0.0f;
         const bool on = sum > threshold;
         if (on) {
             f = sum - threshold;
-            f2 = f * f;
+            f2 = f2 + f * f;
         }
         if (t == T - 1) {
             float *mv = mr_last + 3LL * v;
-            mv[0] = on ? f * signum(x0) : 0.0f;
-            mv[1] = on ? f * signum(x1) : 0.0f;
-            mv[2] = on ? f * signum(x2) : 0.0f;
+#pragma unroll
+            for (int o = 0; o < 3; ++o) mv[o] = on ? f * signum(xs[k][o]) : 0.0f;
         }
     }
     red[threadIdx.x] = f2;
