@@ -234,7 +234,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the epoch on a single MI355X; "
-              "scenarios would shard one per GPU with no communication (see DESIGN.md).",
+              "scenarios would shard one per GPU with no communication.",
               file=sys.stderr)
         return 2
     if args.workload != "plain":
