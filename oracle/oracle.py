@@ -303,8 +303,9 @@ def calculate_final_f32(x, voxel_types, voxel_numbers):
     if types.shape != numbers.shape:
         raise ValueError("voxel_types and voxel_numbers differ in shape")
     T, n = x.shape
-    if numbers.size and (numbers.max() + 2 >= n or numbers.min() < -1
-                         or np.any(numbers[numbers >= 0] % 3)):
+    real = numbers[numbers >= 0]
+    if (numbers.size and numbers.min() < -1) or (real.size and (real.max() + 2 >= n
+                                                                or np.any(real % 3))):
         raise ValueError("voxel numbers must be -1 or multiples of 3 below n")
     out = {k: np.zeros(101, np.float32) for k in FINAL_METRICS}
     lib().ct_oracle_calculate_final_f32(x.ctypes.data, T, n, types.ctypes.data,

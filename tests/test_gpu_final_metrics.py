@@ -130,3 +130,15 @@ def test_final_metrics_edge_cases_and_errors(gpu):
     with pytest.raises(CTError):
         r.calculate_final(np.zeros(3, np.uint32), np.zeros(2, np.int32))
     r.close()
+    # a scenario without voxels (N = 0): no maxima, nothing predicted Pathological
+    g0 = synth.grid_geometry(np.zeros((2, 2, 2), dtype=bool), 2.5)
+    p0 = synth.make_problem(g0, num_steps=10, max_delay=3, seed=9).truth()
+    r0, x0 = simulated(gpu, p0)
+    types = np.array([PATHOLOGICAL, VENTRICLE, 0, PATHOLOGICAL], np.uint32)
+    none = np.full(4, -1, np.int32)
+    got = r0.calculate_final(types, none)
+    assert_bits(got, oracle.calculate_final_f32(x0, types, none))
+    assert np.all(got["recall_over_threshold"] == 0.0)
+    with pytest.raises(CTError):
+        r0.calculate_final(types[:1], np.zeros(1, np.int32))  # no voxel 0 to point at
+    r0.close()
