@@ -205,6 +205,32 @@ def bench_batched(args) -> int:
     return 0
 
 
+def bench_copy(args) -> int:
+    """Counter calibration: a plain elementwise pass (y = -x, PyTorch's own
+    kernel) over a buffer of known size, args.copy_mib MiB read and the same
+    written per dispatch. rocprofv3 FETCH_SIZE / WRITE_SIZE of this command
+    give the factors that turn the counters of the epoch kernels into bytes."""
+    import torch
+
+    n = (args.copy_mib << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        torch.neg(src, out=dst)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    print(json.dumps({"workload": "copy calibration", "mib_read": args.copy_mib,
+                      "mib_written": args.copy_mib, "dispatches": args.steps + 1,
+                      "ms_per_dispatch": ms,
+                      "read_plus_write_gbs": 2 * n * 4 / (ms * 1e-3) / 1e9}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,11 +244,13 @@ def main() -> int:
                     help="reference: the reference's per-step derivation; adjoint: exact "
                          "reverse-mode gradient with checkpoints")
     ap.add_argument("--checkpoint", type=int, default=64, help="adjoint checkpoint interval")
-    ap.add_argument("--workload", choices=["plain", "batched-uneven", "batched-adjoint"],
+    ap.add_argument("--workload", choices=["plain", "batched-uneven", "batched-adjoint", "copy"],
                     default="plain",
                     help="plain: the 120k-voxel epoch (the headline line); batched-uneven: 64 "
                          "uneven scenarios per batched epoch; batched-adjoint: 8 shells x 4 "
-                         "beats with the reverse-mode gradient")
+                         "beats with the reverse-mode gradient; copy: HBM counter calibration "
+                         "(a plain elementwise pass of --copy-mib MiB)")
+    ap.add_argument("--copy-mib", type=int, default=1024, help="copy workload: MiB per dispatch")
     ap.add_argument("--scenarios", type=int, default=0, help="batched workloads: scenario count")
     ap.add_argument("--voxels-per-group", type=int, default=0,
                     help="batched workloads: 0 = autotune")
@@ -237,6 +265,8 @@ def main() -> int:
               "scenarios would shard one per GPU with no communication.",
               file=sys.stderr)
         return 2
+    if args.workload == "copy":
+        return bench_copy(args)
     if args.workload != "plain":
         return bench_batched(args)
 

@@ -16,7 +16,6 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libcardiotrust_hip.so")
-PROBE = os.path.join(HERE, "copy_probe")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in ("kernels.hip", "adjoint.hip", "gradcheck.hip",
                                                   "api.cpp", "batch.cpp")]
 ARCH = os.environ.get("CT_OFFLOAD_ARCH", "gfx950")
@@ -45,13 +44,6 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed")
     os.replace(tmp, LIB)
-    probe = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17",
-             os.path.join(HERE, "csrc", "copy_probe.hip"), "-o", PROBE + ".tmp"]
-    res = subprocess.run(probe, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed on copy_probe.hip")
-    os.replace(PROBE + ".tmp", PROBE)
     return LIB
 
 

@@ -288,20 +288,23 @@ int finish_batch(ct_epoch_kernel &k, int64_t num_beats, bool reset_after, int32_
                                      c.smoothness_regularization_strength, r.N));
     const int64_t bs = c.batch_size == 0 ? int64_t(r.T) * num_beats : int64_t(r.T) * c.batch_size;
     const float scale = c.learning_rate / float(bs);
+    // derivatives from a timed-out resident launch are not applied
+    const ct::UpdateGuard guard{&r.barrier->abort.count, r.batch_abort};
     if (!c.freeze_gains) {
         if (c.optimizer == 1) {
-            CT_HIP(ct::launch_adam(s, r.gT, r.dGT, r.gm1, r.gm2, r.n * 78, sc, scale));
+            CT_HIP(ct::launch_adam(s, r.gT, r.dGT, r.gm1, r.gm2, r.n * 78, sc, scale, guard));
         } else {
-            CT_HIP(ct::launch_sgd(s, r.gT, r.dGT, r.n * 78, scale));
+            CT_HIP(ct::launch_sgd(s, r.gT, r.dGT, r.n * 78, scale, guard));
         }
     }
     if (!c.freeze_delays) {
         if (c.optimizer == 1) {
-            CT_HIP(ct::launch_adam(s, r.coefs, r.dA, r.cm1, r.cm2, int64_t(r.N) * 26, sc, scale));
+            CT_HIP(ct::launch_adam(s, r.coefs, r.dA, r.cm1, r.cm2, int64_t(r.N) * 26, sc, scale,
+                                   guard));
         } else {
-            CT_HIP(ct::launch_sgd(s, r.coefs, r.dA, int64_t(r.N) * 26, scale));
+            CT_HIP(ct::launch_sgd(s, r.coefs, r.dA, int64_t(r.N) * 26, scale, guard));
         }
-        CT_HIP(ct::launch_roll_delays(s, r.coefs, r.delays, int64_t(r.N) * 26));
+        CT_HIP(ct::launch_roll_delays(s, r.coefs, r.delays, int64_t(r.N) * 26, guard));
     }
     if (reset_after) CT_CHECK(reset_derivatives(r));
     CT_HIP(ct::launch_batch_metrics(s, r.loss, r.loss_mse, r.loss_mr, r.loss_b, r.loss_mse_b,
@@ -380,20 +383,24 @@ int plan_resident(ct_results_gpu &r) {
 
 // After resident launches: waits for the stream and reports a timed-out wait
 // (workgroups not all resident, or a wait limit too short), which leaves the
-// results of that launch invalid. The resident forward and backward paths are
-// then disabled for this problem; the abort word stays set.
-int check_resident(ct_results_gpu &r) {
-    if (!r.resident_used) return CT_OK;
+// results of that launch invalid; the updates queued behind it checked the
+// abort word and left the parameters unchanged. The resident forward and
+// backward paths are then disabled for this problem, and the word is cleared
+// (nothing waits on it any more, and the next updates must run).
+int check_resident(ct_results_gpu &r, bool always) {
+    if (!r.resident_used && !always) return CT_OK;
     r.resident_used = false;
-    ct::GridBarrier gb{};
+    uint32_t abort = 0;
     CT_HIP(hipStreamSynchronize(r.stream));
-    CT_HIP(hipMemcpy(&gb, r.barrier, sizeof(gb), hipMemcpyDeviceToHost));
-    if (gb.abort.count) {
+    CT_HIP(hipMemcpy(&abort, &r.barrier->abort.count, sizeof(abort), hipMemcpyDeviceToHost));
+    if (abort) {
         r.resident_ok = false;
         r.sweep_ok = false;
+        const uint32_t zero = 0;
+        CT_HIP(hipMemcpy(&r.barrier->abort.count, &zero, sizeof(zero), hipMemcpyHostToDevice));
         return fail(CT_ERR_DEVICE, "resident propagation: a wait timed out (workgroups not all "
-                                   "resident, or the wait limit too short); the per-step path "
-                                   "is used from now on");
+                                   "resident, or the wait limit too short); the epoch's update "
+                                   "was not applied, and the per-step path is used from now on");
     }
     return CT_OK;
 }
@@ -1196,8 +1203,11 @@ int ct_epoch_kernel_execute(ct_epoch_kernel *k) {
     } else {
         CT_CHECK(finish_batch(*k, r.B, false, update++));
     }
+    // a timed-out resident wait: the update was skipped on the device, so the
+    // epoch's counters (Adam step, batch index) stay where they were
+    CT_CHECK(check_resident(r));
     end_epoch(*k);
-    return check_resident(r);
+    return CT_OK;
 }
 
 int ct_results_set_propagation(ct_results_gpu *results, int32_t mode, int32_t *resident) {

@@ -279,13 +279,16 @@ int plan(ct_batch &b, int32_t vpg) {
     CT_CHECK(batch_alloc(b, &b.jobs, b.num_jobs));
     CT_CHECK(batch_alloc(b, &b.task_ptr, int64_t(all_ptr.size())));
     CT_CHECK(batch_alloc(b, &b.task_job, int64_t(all_task.size())));
-    CT_CHECK(batch_alloc(b, &b.barriers, b.num_jobs));
+    CT_CHECK(batch_alloc(b, &b.barriers, 1));
     CT_CHECK(batch_alloc(b, &b.flags, std::max<int64_t>(flag_of.back(), 1)));
+    // one abort word for every job: a timed-out wait ends the whole launch at
+    // the other jobs' next waits instead of leaving them to wait out their own
+    // limits for workgroups that have left
     for (size_t j = 0; j < all_jobs.size(); ++j) {
-        all_jobs[j].gb = b.barriers + j;
+        all_jobs[j].gb = b.barriers;
         all_jobs[j].flags = b.flags + flag_of[j];
     }
-    CT_HIP(hipMemset(b.barriers, 0, sizeof(ct::GridBarrier) * std::max(b.num_jobs, 1)));
+    CT_HIP(hipMemset(b.barriers, 0, sizeof(ct::GridBarrier)));
     CT_CHECK(upload(b.jobs, all_jobs.data(), sizeof(ct::ResidentJob) * all_jobs.size()));
     CT_CHECK(upload(b.task_ptr, all_ptr.data(), sizeof(int32_t) * all_ptr.size()));
     CT_CHECK(upload(b.task_job, all_task.data(), sizeof(int32_t) * all_task.size()));
@@ -326,10 +329,15 @@ int propagate_round(ct_batch &b, int32_t rd, bool timed) {
         b.prop_events.push_back({e0, e1});
         CT_HIP(hipEventRecord(e0, s));
     }
+    // the members' wait limits (ct_results_set_wait_limit): the shortest bounds
+    // every step; a job's first wait, which may include earlier jobs on its
+    // slots, gets up to 30 times as long (at most 60 s)
+    double step_s = 1e300;
+    for (const ct_epoch_kernel *k : b.kernels) step_s = std::min(step_s, k->r->wait_seconds);
+    const double first_s = std::min(60.0, 30.0 * step_s);
     CT_HIP(ct::launch_propagate_batch(s, b.slots, b.jobs + R.job0, b.task_ptr + R.ptr0,
-                                      b.task_job, b.barriers + R.job0,
-                                      int32_t(R.resident.size()), b.flags + R.flag0, R.nflags,
-                                      wait_ticks(*b.gpu, 60.0), wait_ticks(*b.gpu, 2.0)));
+                                      b.task_job, b.barriers, 1, b.flags + R.flag0, R.nflags,
+                                      wait_ticks(*b.gpu, first_s), wait_ticks(*b.gpu, step_s)));
     if (timed) CT_HIP(hipEventRecord(e1, s));
     return CT_OK;
 }
@@ -364,7 +372,17 @@ int on_side(ct_batch &b, int32_t s, F &&fn) {
     return rc;
 }
 
+int enqueue_epoch_body(ct_batch &b, bool timed);
+
+// The members' updates also check the batched launch's abort word.
 int enqueue_epoch(ct_batch &b, bool timed) {
+    for (ct_epoch_kernel *k : b.kernels) k->r->batch_abort = &b.barriers->abort.count;
+    const int rc = enqueue_epoch_body(b, timed);
+    for (ct_epoch_kernel *k : b.kernels) k->r->batch_abort = nullptr;
+    return rc;
+}
+
+int enqueue_epoch_body(ct_batch &b, bool timed) {
     const int32_t S = int32_t(b.kernels.size());
     std::vector<int64_t> in_batch(S, 0);
     std::vector<int32_t> update(S, 0);
@@ -403,19 +421,39 @@ int enqueue_epoch(ct_batch &b, bool timed) {
     return join_side(b);
 }
 
-// Reads the abort words of every barrier after an epoch.
-int check_barriers(ct_batch &b) {
+// Reads every abort word after an epoch: the batched launches' and each
+// scenario's own (its backward sweep, or its resident propagation outside the
+// batched launch; after a graph replay the host cannot tell which ran, so all
+// are read). A scenario whose update was skipped on the device (either word
+// set) keeps its epoch counters; the others advance theirs. Any abort drops
+// the graph and fails the call; the paths that timed out are not used again.
+// `epoch`: after an epoch (advance the counters), not after the autotuner's
+// propagation-only launches.
+int check_barriers(ct_batch &b, bool epoch) {
     CT_HIP(hipStreamSynchronize(b.gpu->stream));
-    if (b.num_jobs <= 0) return CT_OK;
-    std::vector<ct::GridBarrier> gb(b.num_jobs);
-    CT_CHECK(download(gb.data(), b.barriers, sizeof(ct::GridBarrier) * gb.size()));
-    for (const auto &g : gb)
-        if (g.abort.count) {
-            b.slots = 0;  // stop using the batched launch
-            drop_graph(b);
-            return fail(CT_ERR_DEVICE, "batched propagation: a barrier timed out (workgroups not "
-                                       "all resident); the per-scenario paths are used from now on");
-        }
+    uint32_t batch_abort = 0;
+    if (b.barriers)
+        CT_CHECK(download(&batch_abort, &b.barriers->abort.count, sizeof(batch_abort)));
+    if (batch_abort) {
+        b.slots = 0;  // stop using the batched launch
+        const uint32_t zero = 0;
+        CT_CHECK(upload(&b.barriers->abort.count, &zero, sizeof(zero)));
+    }
+    int rc = CT_OK;
+    if (batch_abort)
+        rc = fail(CT_ERR_DEVICE, "batched propagation: a wait timed out (workgroups not all "
+                                 "resident, or a wait limit too short); the epoch's updates were "
+                                 "not applied, and the per-scenario paths are used from now on");
+    for (ct_epoch_kernel *k : b.kernels) {
+        const int own = check_resident(*k->r, true);
+        if (epoch && own == CT_OK && !batch_abort) end_epoch(*k);
+        if (own != CT_OK && rc == CT_OK) rc = own;
+    }
+    if (rc != CT_OK) {
+        const std::string msg = g_error;
+        drop_graph(b);
+        return fail(rc, msg);
+    }
     return CT_OK;
 }
 
@@ -451,7 +489,7 @@ int autotune(ct_batch &b) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         CT_CHECK(rc);
-        CT_CHECK(check_barriers(b));
+        CT_CHECK(check_barriers(b, false));
         b.tuned.push_back(vpg);
         b.tuned_ms.push_back(ms);
         if (ms < best_ms) {
@@ -640,8 +678,7 @@ int ct_batch_execute(ct_batch *b) {
     for (size_t i = 0; i < B.kernels.size(); ++i) B.kernels[i]->profiling = prof[i];
     CT_CHECK(rc);
     CT_HIP(hipEventRecord(B.ev[1], s));
-    for (ct_epoch_kernel *k : B.kernels) end_epoch(*k);
-    CT_CHECK(check_barriers(B));
+    CT_CHECK(check_barriers(B, true));  // advances the members' epoch counters
     float ms = 0.0f;
     CT_HIP(hipEventElapsedTime(&ms, B.ev[0], B.ev[1]));
     B.last_epoch_ms = ms;

@@ -164,7 +164,16 @@ hipError_t launch_adjoint_gradient(hipStream_t s, const int32_t *nbr, const floa
                                    int64_t ychunk_len, float *dGT, float *dA, int32_t N, int32_t T,
                                    int32_t K);
 
-hipError_t launch_sgd(hipStream_t s, float *v, const float *d, int64_t len, float scale);
+// Sticky abort words that a parameter update checks before it writes: when
+// either is nonzero (a resident launch whose states the update's derivatives
+// came from timed out), the update leaves the parameters and moments as they
+// are. Null words are not read.
+struct UpdateGuard {
+    const uint32_t *word0;
+    const uint32_t *word1;
+};
+hipError_t launch_sgd(hipStream_t s, float *v, const float *d, int64_t len, float scale,
+                      const UpdateGuard &guard);
 // Per-update scalars of the parameter update, read by the kernels from
 // device memory so that a captured epoch replays with each epoch's values:
 // Adam bias corrections c1 = 1 - 0.9^step, c2 = 1 - 0.999^step and the
@@ -176,8 +185,9 @@ struct UpdateScalars {
     int32_t pad;
 };
 hipError_t launch_adam(hipStream_t s, float *v, const float *d, float *m1, float *m2, int64_t len,
-                       const UpdateScalars *sc, float scale);
-hipError_t launch_roll_delays(hipStream_t s, float *coefs, int32_t *delays, int64_t len);
+                       const UpdateScalars *sc, float scale, const UpdateGuard &guard);
+hipError_t launch_roll_delays(hipStream_t s, float *coefs, int32_t *delays, int64_t len,
+                              const UpdateGuard &guard);
 
 hipError_t launch_average_delays(hipStream_t s, const float *gT, const float *coefs,
                                  const int32_t *delays, float *avg, uint8_t *valid, int32_t N);

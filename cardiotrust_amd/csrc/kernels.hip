@@ -972,18 +972,27 @@ __global__ __launch_bounds__(kDeriveThreads, 4) void derive_kernel(
 }
 
 // update.rs:106-114, 158-168
+__device__ __forceinline__ bool guarded(const UpdateGuard &g) {
+    const auto word = [](const uint32_t *w) {
+        return w && __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+    };
+    return word(g.word0) || word(g.word1);
+}
+
 __global__ void sgd_kernel(float *__restrict__ v, const float *__restrict__ d, int64_t len,
-                           float scale) {
+                           float scale, UpdateGuard guard) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k < len) v[k] = v[k] - scale * d[k];
+    if (k >= len || guarded(guard)) return;
+    v[k] = v[k] - scale * d[k];
 }
 
 // update.rs:119-146, 173-200
 __global__ void adam_kernel(float *__restrict__ v, const float *__restrict__ d,
                             float *__restrict__ m1, float *__restrict__ m2, int64_t len,
-                            const UpdateScalars *__restrict__ sc, float scale) {
+                            const UpdateScalars *__restrict__ sc, float scale,
+                            UpdateGuard guard) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= len) return;
+    if (k >= len || guarded(guard)) return;
     const float c1 = sc->c1, c2 = sc->c2;
     const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
     const float omb1 = 1.0f - beta1, omb2 = 1.0f - beta2;
@@ -1000,9 +1009,9 @@ __global__ void adam_kernel(float *__restrict__ v, const float *__restrict__ d,
 
 // update.rs:206-228
 __global__ void roll_delays_kernel(float *__restrict__ coefs, int32_t *__restrict__ delays,
-                                   int64_t len) {
+                                   int64_t len, UpdateGuard guard) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= len) return;
+    if (k >= len || guarded(guard)) return;
     const float margin = 1e-4f;
     float a = coefs[k];
     int32_t D = delays[k];
@@ -1580,22 +1589,24 @@ hipError_t launch_derive(hipStream_t s, const int32_t *nbr, const float *coefs,
     return hipGetLastError();
 }
 
-hipError_t launch_sgd(hipStream_t s, float *v, const float *d, int64_t len, float scale) {
+hipError_t launch_sgd(hipStream_t s, float *v, const float *d, int64_t len, float scale,
+                      const UpdateGuard &guard) {
     if (len <= 0) return hipSuccess;
-    sgd_kernel<<<blocks_for(len, 256), 256, 0, s>>>(v, d, len, scale);
+    sgd_kernel<<<blocks_for(len, 256), 256, 0, s>>>(v, d, len, scale, guard);
     return hipGetLastError();
 }
 
 hipError_t launch_adam(hipStream_t s, float *v, const float *d, float *m1, float *m2, int64_t len,
-                       const UpdateScalars *sc, float scale) {
+                       const UpdateScalars *sc, float scale, const UpdateGuard &guard) {
     if (len <= 0) return hipSuccess;
-    adam_kernel<<<blocks_for(len, 256), 256, 0, s>>>(v, d, m1, m2, len, sc, scale);
+    adam_kernel<<<blocks_for(len, 256), 256, 0, s>>>(v, d, m1, m2, len, sc, scale, guard);
     return hipGetLastError();
 }
 
-hipError_t launch_roll_delays(hipStream_t s, float *coefs, int32_t *delays, int64_t len) {
+hipError_t launch_roll_delays(hipStream_t s, float *coefs, int32_t *delays, int64_t len,
+                              const UpdateGuard &guard) {
     if (len <= 0) return hipSuccess;
-    roll_delays_kernel<<<blocks_for(len, 256), 256, 0, s>>>(coefs, delays, len);
+    roll_delays_kernel<<<blocks_for(len, 256), 256, 0, s>>>(coefs, delays, len, guard);
     return hipGetLastError();
 }
 

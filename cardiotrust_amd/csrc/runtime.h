@@ -120,6 +120,9 @@ struct ct_results_gpu {
     int32_t resident_blocks = 0;
     bool sweep_ok = false;  // the resident adjoint sweep fits with resident_blocks workgroups
     ct::GridBarrier *barrier = nullptr;
+    // while a batch enqueues this scenario's epoch: the abort word of the
+    // batch's propagation launches, which the scenario's updates also check
+    const uint32_t *batch_abort = nullptr;
     // scratch
     float *partial = nullptr, *mr_partial = nullptr, *staging = nullptr, *step_mr = nullptr;
     double *objective = nullptr;  // J of the last forward (adjoint objective)
@@ -254,7 +257,11 @@ int begin_epoch(ct_epoch_kernel &k);
 void end_epoch(ct_epoch_kernel &k);
 ct::ResidentJob resident_job(ct_results_gpu &r, int32_t ckpt_interval);
 int prepare_propagation(ct_results_gpu &r, int32_t ckpt_interval);
-int check_resident(ct_results_gpu &r);
+// After resident launches (or, with `always`, unconditionally): synchronises
+// and reads the results' abort word. A timed-out wait disables the resident
+// paths for the problem, clears the word (so that later updates, which check
+// it, run again) and returns CT_ERR_DEVICE.
+int check_resident(ct_results_gpu &r, bool always = false);
 // Fixes the adjoint's q-ring size for graph capture: at least the current
 // largest delay + 1 + `headroom` (delays grow by at most one per update).
 // Reads the delays (synchronises); allocates the ring.

@@ -240,3 +240,56 @@ def test_batch_with_an_empty_scenario(gpu, ps):
     got, infos = run_batched(gpu, mixed, epochs=2, vpg=64)
     assert_same(alone, got)
     assert infos[-1]["scenarios"] == 3
+
+
+def test_batch_wait_timeout_leaves_parameters(gpu, ps):
+    """A failure forced from the host through the kernels' own bounded waits:
+    a wait limit of 0 s on a reverse-mode member bounds that member's backward
+    sweep and, as the shortest limit of the batch, every wait of the batched
+    propagation launch. A wait that is not satisfied at its first poll then
+    raises the abort word; every workgroup leaves at its next wait (so every
+    kernel ends normally), the members' updates read the word and skip, and
+    ct_batch_execute reports CT_ERR_DEVICE with every member's parameters and
+    epoch counters unchanged. With the limit restored the batch runs on the
+    paths that did not time out, and two more epochs equal two epochs of each
+    scenario alone bit for bit. (An attempt in which no wait has to wait
+    completes normally; one of three must time out.)"""
+    from cardiotrust_amd.gpu import Batch, CTError
+    note("case: batch of 5 with reverse-mode members, member 0 wait limit 0 s, then 2 epochs")
+    grads = [(1, 16), None, (1, 7), None, (1, 16)]
+    capacity = 5
+    ms = [Member(gpu, p, CFGS[i % len(CFGS)], capacity, orders(ps)[i], grads[i])
+          for i, p in enumerate(ps)]
+    b = Batch(gpu, [m.k for m in ms], voxels_per_group=96)
+    ms[0].r.set_wait_limit(0.0)
+    before = [m.results() for m in ms]
+    ok_epochs, failed = 0, False
+    for _ in range(3):
+        try:
+            b.execute()
+        except CTError as e:
+            assert "timed out" in str(e), e
+            failed = True
+            break
+        ok_epochs += 1
+        before = [m.results() for m in ms]
+    assert failed
+    after = [m.results() for m in ms]
+    for s, (x, y) in enumerate(zip(before, after)):
+        for k in ("gains", "coefs", "delays", "batch_index", "derivatives_step"):
+            assert np.array_equal(x[k], y[k]), (s, k)
+    ms[0].r.set_wait_limit(2.0)
+    for _ in range(2):
+        b.execute()
+    got = [m.results() for m in ms]
+    b.close()
+    for m in ms:
+        m.close()
+    alone = []
+    for i, p in enumerate(ps):
+        m = Member(gpu, p, CFGS[i % len(CFGS)], capacity, orders(ps)[i], grads[i])
+        for _ in range(ok_epochs + 2):
+            m.k.execute()
+        alone.append(m.results())
+        m.close()
+    assert_same(alone, got)

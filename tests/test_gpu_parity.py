@@ -181,7 +181,50 @@ def test_first_epoch_states_bit_exact_on_ragged_mask(gpu, mode):
     compare(g, o32, o64)
 
 
+def test_adam_first_update_within_1e4(gpu):
+    """One Adam update (3 beats, batch_size 0, beat order [2, 0, 1]) against
+    the oracle at the 1e-4 bar. The states are computed before the update, so
+    they are bit-exact. Adam's first step moves every parameter by
+    (lr / bs) * m1c / (sqrt(m2c) + eps), which is sign(d) to within 1e-7
+    whenever |d| >> eps: an entry whose derivative the GPU and the oracle give
+    with the same sign moves identically, so gains and coefficients must
+    agree at 1e-4 there. Where |d| is within the derivative bar itself
+    (|d_oracle| <= 1e-4 max|d_oracle|) the two FP32 summation orders may
+    round it to opposite signs, and the parameter may then differ by at most
+    2 lr / bs; the test shows how many entries that is."""
+    note("case: 3 beats, batch_size 0, Adam, beat order [2,0,1], 1 epoch")
+    p = blob_problem(3, T=48, beats=3)
+    cfg = Algorithm(learning_rate=0.01, batch_size=0, optimizer=Optimizer.Adam, freeze_delays=False)
+    order = [2, 0, 1]
+    g, _ = run_gpu_epochs(gpu, p, cfg, beat_order=order, optimizer=1)
+    o32 = run_oracle_epochs(p, cfg, np.float32, beat_order=order, optimizer=1)
+    o64 = run_oracle_epochs(p, cfg, np.float64, beat_order=order, optimizer=1)
+    assert g["derivatives_step"][0] == o32.d_step[0] == 2
+    close = tuple(k for k in CLOSE if k not in ("gains", "coefs"))
+    compare(g, o32, o64, exact=EXACT, close=close)
+    step = cfg.learning_rate / (p.num_steps * p.num_beats)
+    for name, dname in (("gains", "d_gains"), ("coefs", "d_coefs")):
+        d_o = getattr(o32, dname).astype(np.float64)
+        same = np.sign(g[dname]) == np.sign(d_o)
+        ref = getattr(o32, name).astype(np.float64)
+        diff = np.abs(g[name] - ref)
+        scale = np.max(np.abs(ref))
+        assert np.max(diff[same], initial=0.0) <= REL * scale, name
+        flipped = ~same
+        assert np.all(np.abs(d_o[flipped]) <= REL * np.max(np.abs(d_o))), name
+        assert np.all(diff[flipped] <= 2 * step * (1 + 1e-5) + REL * scale), name
+        note(f"  {name}: {int(flipped.sum())} of {flipped.size} entries with a derivative sign "
+             f"at FP32 noise; worst same-sign error {np.max(diff[same]) / scale:.2e}")
+
+
 def test_multibeat_batches_adam(gpu):
+    """Two epochs of two Adam updates each (batch_size 2 over 3 beats). The
+    bar is 1e-3, not 1e-4: every Adam step moves a parameter by about
+    lr / bs = 1e-4 whatever the derivative's size, so a derivative at the FP32
+    summation noise that the GPU and the oracle round to opposite signs leaves
+    the parameter 2e-4 apart (7e-4 of the largest gain), and the later updates
+    and states inherit it. test_adam_first_update_within_1e4 shows the same
+    update at 1e-4 once those entries are accounted for."""
     note("case: 3 beats, batch_size 2, Adam, beat order [2,0,1], 2 epochs")
     p = blob_problem(3, T=48, beats=3)
     cfg = Algorithm(learning_rate=0.01, batch_size=2, optimizer=Optimizer.Adam, freeze_delays=False)
@@ -436,6 +479,65 @@ def test_wait_timeout_aborts_and_falls_back_to_per_step(gpu):
     assert np.array_equal(m, m_ref)
     with pytest.raises(CTError):
         ResultsGPU(gpu, p).set_wait_limit(-1.0)
+
+
+def test_wait_timeout_in_epoch_leaves_parameters(gpu):
+    """The same host-forced timeout inside an epoch: the derivatives of a
+    timed-out propagation are not applied. The SGD update and the delay roll
+    read the results' abort word and skip, so after the CT_ERR_DEVICE the
+    gains, coefficients, delays and the batch index are those from before the
+    call. With the limit restored, the next epoch runs on the per-step path and
+    equals a per-step epoch from the same parameters bit for bit."""
+    from cardiotrust_amd.gpu import Buffer, CTError, EpochKernel, ResultsGPU
+    note("case: 24^3 shell, T=300, epoch with resident wait limit 0 s, then the per-step epoch")
+    g0 = synth.ellipsoid_shell(shape=(24, 24, 24), outer=(12, 12, 12), inner=(6, 6, 6))
+    p = synth.make_problem(g0, num_steps=300, max_delay=40, seed=15)
+    truth = ResultsGPU(gpu, p.truth())
+    truth.set_propagation(1)
+    p.measurements = truth.simulate()
+    truth.close()
+    cfg = Algorithm(learning_rate=0.5, freeze_delays=False)
+    params = ("gains", "coefs", "delays")
+
+    def epoch_object(mode):
+        r = ResultsGPU(gpu, p, num_batch_entries=4)
+        r.set_propagation(mode)
+        buf = Buffer(gpu, p.measurements)
+        return r, buf, EpochKernel(gpu, r, buf, cfg, 3 * p.num_voxels, p.num_sensors, p.num_steps)
+
+    r, buf, k = epoch_object(0)
+    assert r.set_propagation(0)
+    r.set_wait_limit(0.0)
+    before = r.update_from_gpu(list(params))
+    index = k.batch_index()
+    ok_epochs, failed = 0, False
+    for _ in range(3):
+        try:
+            k.execute()
+        except CTError as e:
+            assert "timed out" in str(e), e
+            failed = True
+            break
+        ok_epochs += 1
+        before = r.update_from_gpu(list(params))
+        index = k.batch_index()
+    assert failed
+    after = r.update_from_gpu(list(params))
+    for name in params:
+        assert np.array_equal(before[name], after[name]), name
+    assert k.batch_index() == index
+    assert not r.set_propagation(0)
+    r.set_wait_limit(2.0)
+    k.execute()
+    got = r.update_from_gpu(list(params) + ["system_states", "loss_batch"])
+    k.close(), buf.close(), r.close()
+    r, buf, k = epoch_object(1)
+    for _ in range(ok_epochs + 1):
+        k.execute()
+    want = r.update_from_gpu(list(params) + ["system_states", "loss_batch"])
+    k.close(), buf.close(), r.close()
+    for name in want:
+        assert np.array_equal(got[name], want[name]), name
 
 
 @pytest.mark.parametrize("mode", [1, 2], ids=["per-step", "resident"])
