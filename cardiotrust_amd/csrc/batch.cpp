@@ -435,7 +435,10 @@ int check_barriers(ct_batch &b, bool epoch) {
     if (b.barriers)
         CT_CHECK(download(&batch_abort, &b.barriers->abort.count, sizeof(batch_abort)));
     if (batch_abort) {
-        b.slots = 0;  // stop using the batched launch
+        // stop using the batched launch; the next execute plans again, with
+        // every scenario on its own propagation path
+        b.slots = 0;
+        b.plan_vpg = -1;
         const uint32_t zero = 0;
         CT_CHECK(upload(&b.barriers->abort.count, &zero, sizeof(zero)));
     }
@@ -450,8 +453,11 @@ int check_barriers(ct_batch &b, bool epoch) {
         if (own != CT_OK && rc == CT_OK) rc = own;
     }
     if (rc != CT_OK) {
+        // the scenarios whose resident paths timed out leave the batched
+        // launch too: plan again at the next execute
         const std::string msg = g_error;
         drop_graph(b);
+        b.plan_vpg = -1;
         return fail(rc, msg);
     }
     return CT_OK;
