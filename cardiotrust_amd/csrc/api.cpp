@@ -6,6 +6,8 @@
 // (src/core/algorithm.rs:124-281): per beat, the forward propagation over all
 // steps, then the deferred projection, residuals, losses and derivatives;
 // per batch, the smoothness term, the update and the batch metrics.
+#include <mutex>
+
 #include "runtime.h"
 
 namespace ctrt {
@@ -589,6 +591,112 @@ int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32
     return adjoint_backward(r, c, K, slots, 1, k);
 }
 
+// ---------------------------------------------------------------------------
+// Device scratch of ct_results_calculate_final_metrics_noalloc: a fixed array
+// in the library's code object (each device holds its own copy, from the
+// first use of the library on that device until the process ends), so the
+// entry point needs no allocation. Laid out per call as the histogram
+// (2 x 102 words), each voxel's maximum (N floats) and the staged cell
+// arrays (numbers, types: `cells` words each), every part 256-byte aligned.
+// Calls that overlap on one device share it and race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_address[kMaxDevices] = {};  // device address of `scratch`, per device
+
+// The device address of `scratch` on `device`, which must be current. In host
+// code the array's name stands for a host-side shadow, never for device
+// memory: kernels, copies and fills get this address.
+int scratch_address(int32_t device, void **out) {
+    if (device < 0 || device >= kMaxDevices)
+        return fail(CT_ERR_DEVICE, "device " + std::to_string(device) + " beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_address[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(CT_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                           hipGetErrorString(e));
+        g_scratch_address[device] = p;
+    }
+    *out = g_scratch_address[device];
+    return CT_OK;
+}
+
+// Argument checks shared by the two final-metrics entry points.
+int final_metrics_check(const ct_results_gpu *results, const uint32_t *voxel_types,
+                        const int32_t *voxel_numbers, int64_t cells) {
+    if (!results || (cells > 0 && (!voxel_types || !voxel_numbers)))
+        return fail(CT_ERR_INVALID, "null argument");
+    if (cells < 0 || cells > INT32_MAX)
+        return fail(CT_ERR_INVALID, "cell count must be in [0, 2^31 - 1]");
+    // predict_voxeltype indexes the states at number, +1, +2 (metrics.rs:429-433)
+    const int64_t n = 3LL * results->N;
+    for (int64_t c = 0; c < cells; ++c) {
+        const int32_t v = voxel_numbers[c];
+        if (v < -1 || (v >= 0 && (v % 3 != 0 || v >= n)))
+            return fail(CT_ERR_INVALID, "voxel number " + std::to_string(v) + " of cell " +
+                                            std::to_string(c) +
+                                            " is neither -1 nor a multiple of 3 below 3N");
+    }
+    return CT_OK;
+}
+
+constexpr int kFinalBins = ct::kSegmentationThresholds + 1;
+
+// The device part of the final metrics: stages the cell arrays into d_num /
+// d_types, runs the maxima and the histograms, and copies the histograms to
+// `hist` (2 * kFinalBins words). Blocks.
+int final_metrics_device(ct_results_gpu &r, const uint32_t *voxel_types,
+                         const int32_t *voxel_numbers, int64_t cells, uint32_t *d_hist,
+                         float *d_max, int32_t *d_num, uint32_t *d_types, uint32_t *hist) {
+    ct::SegmentationThresholds th;
+    for (int i = 0; i < ct::kSegmentationThresholds; ++i)
+        th.value[i] = float(i) / 100.0f;  // metrics.rs:233
+    hipStream_t s = r.stream;
+    int rc = CT_OK;
+    auto step = [&](hipError_t e) {
+        if (rc == CT_OK && e != hipSuccess)
+            rc = fail(CT_ERR_DEVICE, std::string("final metrics: ") + hipGetErrorString(e));
+    };
+    if (cells > 0) {
+        step(hipMemcpyAsync(d_num, voxel_numbers, sizeof(int32_t) * cells, hipMemcpyHostToDevice, s));
+        step(hipMemcpyAsync(d_types, voxel_types, sizeof(uint32_t) * cells, hipMemcpyHostToDevice,
+                            s));
+    }
+    if (rc == CT_OK)
+        step(ct::launch_segmentation_histogram(s, r.x, r.T, r.N, d_num, d_types, cells, th, d_max,
+                                               d_hist));
+    if (rc == CT_OK)
+        step(hipMemcpyAsync(hist, d_hist, sizeof(uint32_t) * 2 * kFinalBins, hipMemcpyDeviceToHost,
+                            s));
+    step(hipStreamSynchronize(s));
+    return rc;
+}
+
+// The counts of calculate_dice / _iou / _precision / _recall
+// (metrics.rs:253-404) at each threshold, from the histograms' prefix sums:
+// a cell predicted Pathological at threshold i has bin <= i.
+void final_metrics_from_histograms(const uint32_t *hist, float *dice, float *iou, float *precision,
+                                   float *recall) {
+    int64_t gt_pos = 0;
+    for (int k = 0; k < kFinalBins; ++k) gt_pos += hist[kFinalBins + k];
+    int64_t tp = 0, pred_pos = 0;
+    for (int i = 0; i < ct::kSegmentationThresholds; ++i) {
+        tp += hist[kFinalBins + i];
+        pred_pos += int64_t(hist[i]) + hist[kFinalBins + i];
+        const int64_t fp = pred_pos - tp, fn = gt_pos - tp;
+        const int64_t den = 2 * tp + fp + fn, uni = tp + fp + fn;
+        if (dice) dice[i] = den == 0 ? 0.0f : float(2 * tp) / float(den);
+        if (iou) iou[i] = uni == 0 ? 0.0f : float(tp) / float(uni);
+        if (precision) precision[i] = pred_pos == 0 ? 0.0f : float(tp) / float(pred_pos);
+        if (recall) recall[i] = gt_pos == 0 ? 1.0f : float(tp) / float(gt_pos);
+    }
+}
+
 }  // namespace ctrt
 
 using namespace ctrt;
@@ -605,6 +713,8 @@ int ct_gpu_new(int32_t device, ct_gpu **out) {
         return fail(CT_ERR_DEVICE, "no HIP device " + std::to_string(device) + " (" +
                                        std::to_string(count) + " present)");
     CT_HIP(hipSetDevice(device));
+    void *scratch_ptr = nullptr;  // resolved once per device, with the device current
+    CT_CHECK(scratch_address(device, &scratch_ptr));
     ct_gpu *g = new ct_gpu;
     g->device = device;
     int khz = 0;
@@ -1052,67 +1162,57 @@ int ct_results_spherical(ct_results_gpu *results, float sample_rate_hz, float *m
 int ct_results_calculate_final_metrics(ct_results_gpu *results, const uint32_t *voxel_types,
                                        const int32_t *voxel_numbers, int64_t cells, float *dice,
                                        float *iou, float *precision, float *recall) {
-    if (!results || (cells > 0 && (!voxel_types || !voxel_numbers)))
-        return fail(CT_ERR_INVALID, "null argument");
-    if (cells < 0 || cells > INT32_MAX)
-        return fail(CT_ERR_INVALID, "cell count must be in [0, 2^31 - 1]");
+    CT_CHECK(final_metrics_check(results, voxel_types, voxel_numbers, cells));
     ct_results_gpu &r = *results;
     CT_CHECK(check_gpu(r.gpu));
-    // predict_voxeltype indexes the states at number, +1, +2 (metrics.rs:429-433)
-    const int64_t n = 3LL * r.N;
-    for (int64_t c = 0; c < cells; ++c) {
-        const int32_t v = voxel_numbers[c];
-        if (v < -1 || (v >= 0 && (v % 3 != 0 || v >= n)))
-            return fail(CT_ERR_INVALID, "voxel number " + std::to_string(v) + " of cell " +
-                                            std::to_string(c) +
-                                            " is neither -1 nor a multiple of 3 below 3N");
-    }
-    constexpr int kThresholds = ct::kSegmentationThresholds, kBins = kThresholds + 1;
-    ct::SegmentationThresholds th;
-    for (int i = 0; i < kThresholds; ++i) th.value[i] = float(i) / 100.0f;  // metrics.rs:233
-    hipStream_t s = r.stream;
     const int64_t nvox = std::max<int32_t>(r.N, 1), ncell = std::max<int64_t>(cells, 1);
     char *dev = nullptr;
-    CT_HIP(hipMalloc(&dev, sizeof(uint32_t) * 2 * kBins + sizeof(float) * nvox +
+    CT_HIP(hipMalloc(&dev, sizeof(uint32_t) * 2 * kFinalBins + sizeof(float) * nvox +
                                (sizeof(int32_t) + sizeof(uint32_t)) * ncell));
     uint32_t *d_hist = reinterpret_cast<uint32_t *>(dev);
-    float *d_max = reinterpret_cast<float *>(d_hist + 2 * kBins);
+    float *d_max = reinterpret_cast<float *>(d_hist + 2 * kFinalBins);
     int32_t *d_num = reinterpret_cast<int32_t *>(d_max + nvox);
     uint32_t *d_types = reinterpret_cast<uint32_t *>(d_num + ncell);
-    uint32_t hist[2 * kBins] = {};
-    int rc = CT_OK;
-    auto step = [&](hipError_t e) {
-        if (rc == CT_OK && e != hipSuccess)
-            rc = fail(CT_ERR_DEVICE, std::string("final metrics: ") + hipGetErrorString(e));
-    };
-    if (cells > 0) {
-        step(hipMemcpyAsync(d_num, voxel_numbers, sizeof(int32_t) * cells, hipMemcpyHostToDevice, s));
-        step(hipMemcpyAsync(d_types, voxel_types, sizeof(uint32_t) * cells, hipMemcpyHostToDevice,
-                            s));
-    }
-    if (rc == CT_OK)
-        step(ct::launch_segmentation_histogram(s, r.x, r.T, r.N, d_num, d_types, cells, th, d_max,
-                                               d_hist));
-    if (rc == CT_OK) step(hipMemcpyAsync(hist, d_hist, sizeof(hist), hipMemcpyDeviceToHost, s));
-    step(hipStreamSynchronize(s));
+    uint32_t hist[2 * kFinalBins] = {};
+    const int rc = final_metrics_device(r, voxel_types, voxel_numbers, cells, d_hist, d_max, d_num,
+                                        d_types, hist);
     (void)hipFree(dev);
-    if (rc != CT_OK) return rc;
-    // The counts of calculate_dice / _iou / _precision / _recall
-    // (metrics.rs:253-404) at each threshold, from the histograms' prefix
-    // sums: a cell predicted Pathological at threshold i has bin <= i.
-    int64_t gt_pos = 0;
-    for (int k = 0; k < kBins; ++k) gt_pos += hist[kBins + k];
-    int64_t tp = 0, pred_pos = 0;
-    for (int i = 0; i < kThresholds; ++i) {
-        tp += hist[kBins + i];
-        pred_pos += int64_t(hist[i]) + hist[kBins + i];
-        const int64_t fp = pred_pos - tp, fn = gt_pos - tp;
-        const int64_t den = 2 * tp + fp + fn, uni = tp + fp + fn;
-        if (dice) dice[i] = den == 0 ? 0.0f : float(2 * tp) / float(den);
-        if (iou) iou[i] = uni == 0 ? 0.0f : float(tp) / float(uni);
-        if (precision) precision[i] = pred_pos == 0 ? 0.0f : float(tp) / float(pred_pos);
-        if (recall) recall[i] = gt_pos == 0 ? 1.0f : float(tp) / float(gt_pos);
-    }
+    CT_CHECK(rc);
+    final_metrics_from_histograms(hist, dice, iou, precision, recall);
+    return CT_OK;
+}
+
+int ct_results_calculate_final_metrics_noalloc(ct_results_gpu *results,
+                                               const uint32_t *voxel_types,
+                                               const int32_t *voxel_numbers, int64_t cells,
+                                               float *dice, float *iou, float *precision,
+                                               float *recall) {
+    CT_CHECK(final_metrics_check(results, voxel_types, voxel_numbers, cells));
+    ct_results_gpu &r = *results;
+    const auto aligned = [](size_t bytes) { return (bytes + 255) / 256 * 256; };
+    const size_t o_max = aligned(sizeof(uint32_t) * 2 * kFinalBins);
+    const size_t o_num = o_max + aligned(sizeof(float) * size_t(r.N));
+    const size_t o_types = o_num + aligned(sizeof(int32_t) * size_t(cells));
+    const size_t need = o_types + sizeof(uint32_t) * size_t(cells);
+    if (need > kNoAllocScratchBytes)
+        return fail(CT_ERR_INVALID,
+                    "final metrics without allocation: " + std::to_string(r.N) + " voxels and " +
+                        std::to_string(cells) + " cells need " + std::to_string(need) +
+                        " bytes of scratch, more than its " +
+                        std::to_string(kNoAllocScratchBytes) +
+                        " (4 B per voxel + 8 B per cell + 1 KiB); use "
+                        "ct_results_calculate_final_metrics");
+    CT_CHECK(check_gpu(r.gpu));
+    void *base = nullptr;
+    CT_CHECK(scratch_address(r.gpu->device, &base));
+    char *sp = static_cast<char *>(base);
+    uint32_t hist[2 * kFinalBins] = {};
+    CT_CHECK(final_metrics_device(r, voxel_types, voxel_numbers, cells,
+                                  reinterpret_cast<uint32_t *>(sp),
+                                  reinterpret_cast<float *>(sp + o_max),
+                                  reinterpret_cast<int32_t *>(sp + o_num),
+                                  reinterpret_cast<uint32_t *>(sp + o_types), hist));
+    final_metrics_from_histograms(hist, dice, iou, precision, recall);
     return CT_OK;
 }
 

@@ -25,7 +25,7 @@ EXPORTS = (
     "ct_measurements_to_gpu", "ct_buffer_free", "ct_results_to_gpu", "ct_results_gpu_free",
     "ct_results_update_from_gpu", "ct_results_simulate", "ct_results_simulate_noisy",
     "ct_measurement_matrix", "ct_model_connections", "ct_results_spherical",
-    "ct_results_calculate_final_metrics",
+    "ct_results_calculate_final_metrics", "ct_results_calculate_final_metrics_noalloc",
     "ct_epoch_kernel_last_loss", "ct_results_set_wait_limit",
     "ct_results_set_propagation",
     "ct_results_forward", "ct_results_backward", "ct_epoch_kernel_set_gradient",
@@ -126,6 +126,8 @@ def lib() -> ctypes.CDLL:
         "ct_results_spherical": ([vp, ctypes.c_float] + [vp] * 7, ctypes.c_int),
         "ct_results_calculate_final_metrics": ([vp, vp, vp, ctypes.c_int64] + [vp] * 4,
                                                ctypes.c_int),
+        "ct_results_calculate_final_metrics_noalloc": ([vp, vp, vp, ctypes.c_int64] + [vp] * 4,
+                                                       ctypes.c_int),
         "ct_epoch_kernel_last_loss": ([vp, vp, vp, vp, vp], ctypes.c_int),
         "ct_results_set_wait_limit": ([vp, ctypes.c_double], ctypes.c_int),
         "ct_measurement_matrix": ([vp, ctypes.c_int32, vp, ctypes.c_int32, vp, vp, ctypes.c_int32,
@@ -359,20 +361,23 @@ class ResultsGPU:
     FINAL_METRICS = ("dice_score_over_threshold", "iou_over_threshold",
                      "precision_over_threshold", "recall_over_threshold")
 
-    def calculate_final(self, voxel_types, voxel_numbers) -> dict:
+    def calculate_final(self, voxel_types, voxel_numbers, no_alloc: bool = False) -> dict:
         """metrics::calculate_final (algorithm/metrics.rs:225-241) over the
         device states (ct_results_calculate_final_metrics): the four (101,)
         arrays named as the reference's Metrics fields. voxel_types: VoxelType
         ordinals per grid cell (Pathological = 6); voxel_numbers: each cell's
-        first state index, -1 for None (the reference's .npy forms)."""
+        first state index, -1 for None (the reference's .npy forms).
+        no_alloc: use ct_results_calculate_final_metrics_noalloc (the
+        library's fixed device scratch instead of an allocation)."""
         types = np.ascontiguousarray(voxel_types, dtype=np.uint32).ravel()
         numbers = np.ascontiguousarray(voxel_numbers, dtype=np.int32).ravel()
         if types.shape != numbers.shape:
             raise CTError("voxel_types and voxel_numbers differ in shape")
         out = {k: np.zeros(101, np.float32) for k in self.FINAL_METRICS}
-        _check(lib().ct_results_calculate_final_metrics(
-            self.handle, _ptr(types), _ptr(numbers), types.size,
-            *(_ptr(out[k]) for k in self.FINAL_METRICS)), "calculate_final")
+        fn = (lib().ct_results_calculate_final_metrics_noalloc if no_alloc
+              else lib().ct_results_calculate_final_metrics)
+        _check(fn(self.handle, _ptr(types), _ptr(numbers), types.size,
+                  *(_ptr(out[k]) for k in self.FINAL_METRICS)), "calculate_final")
         return out
 
     def close(self) -> None:

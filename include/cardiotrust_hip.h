@@ -212,6 +212,21 @@ int ct_results_calculate_final_metrics(ct_results_gpu *results, const uint32_t *
                                        const int32_t *voxel_numbers, int64_t cells, float *dice,
                                        float *iou, float *precision, float *recall);
 
+/* ct_results_calculate_final_metrics without any device allocation (no
+ * reference counterpart: for callers that cannot allocate device memory,
+ * such as code under HIP graph capture, tight serving loops or harnesses
+ * that forbid hipMalloc). Same arguments, checks and results, bit for bit.
+ * Its device memory is a fixed 8 MiB array that each device holds from the
+ * library's first use there until the process ends: N voxels and `cells`
+ * cells must satisfy 4 N + 8 cells + 1 KiB (each part rounded up to 256 B)
+ * <= 8 MiB, else CT_ERR_INVALID. Calls on one device share that array:
+ * calls that overlap on one device race. Blocks. */
+int ct_results_calculate_final_metrics_noalloc(ct_results_gpu *results,
+                                               const uint32_t *voxel_types,
+                                               const int32_t *voxel_numbers, int64_t cells,
+                                               float *dice, float *iou, float *precision,
+                                               float *recall);
+
 /* Simulation::run with its measurement noise (src/core/data/simulation.rs:
  * 129-173): ct_results_simulate, then m[b][t][s] += sensor_std[s] * z with z
  * standard normal, drawn in the reference's order (sensor, then beat, then
