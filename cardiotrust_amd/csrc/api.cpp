@@ -237,9 +237,7 @@ int32_t update_group(const ct_epoch_kernel &k) {
     return k.cfg.batch_size == 0 ? k.r->B : k.cfg.batch_size;
 }
 
-bool shares_propagation(const ct_epoch_kernel &k) {
-    return k.share_mode > 0 || (k.share_mode < 0 && !uses_adjoint(k));
-}
+bool shares_propagation(const ct_epoch_kernel &k) { return k.share_mode != 0; }
 
 bool propagates_at(const ct_epoch_kernel &k, int32_t pos) {
     return !shares_propagation(k) || pos % update_group(k) == 0;

@@ -181,9 +181,10 @@ struct ct_epoch_kernel {
     int32_t checkpoint_interval = 64;
     // beats of one update batch share one propagation (their states are
     // identical: same parameters and drive); with the reverse-mode gradient
-    // also one backward pass for the batch's summed objective. -1 automatic:
-    // shared when the epoch runs the reference derivation (bit-identical),
-    // not under the reverse-mode gradient; 0 never; 1 always.
+    // also one backward pass for the batch's summed objective. -1 automatic
+    // (shared under both gradients: bit-identical results under the reference
+    // derivation, the same gradient up to FP32 rounding under reverse mode);
+    // 0 never; 1 always.
     int32_t share_mode = -1;
     std::vector<hipEvent_t> events;
     size_t used_events = 0;

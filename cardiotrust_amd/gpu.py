@@ -417,9 +417,9 @@ class EpochKernel:
     def set_share_propagation(self, enabled: Optional[bool]) -> None:
         """Propagate once per update batch (the beats' states are identical);
         with the reverse-mode gradient, one backward pass per update batch.
-        None = automatic (the default: shared under the reference derivation,
-        where the results are bit-identical, not under the reverse-mode
-        gradient)."""
+        None = automatic (the default: shared under both gradients; the
+        results are bit-identical under the reference derivation and the same
+        gradient up to FP32 rounding under the reverse-mode one)."""
         mode = -1 if enabled is None else int(bool(enabled))
         _check(lib().ct_epoch_kernel_set_share_propagation(self.handle, mode),
                "set_share_propagation")

@@ -291,8 +291,8 @@ int ct_epoch_kernel_set_gradient(ct_epoch_kernel *kernel, int32_t mode,
  * batch's summed objective sum_b J_b is differentiated with one backward pass
  * (forcing summed over the beats), instead of one pass per beat: the same
  * gradient up to FP32 rounding. `enabled`: 1 shared, 0 one propagation per
- * beat, -1 automatic (the default): shared when the epoch runs the reference
- * derivation, not under gradient mode 1. CT_ERR_INVALID for other values. */
+ * beat, -1 automatic (the default): shared under both gradient modes.
+ * CT_ERR_INVALID for other values. */
 int ct_epoch_kernel_set_share_propagation(ct_epoch_kernel *kernel, int32_t enabled);
 
 /* Forward / backward pair of one beat (the counterpart of an autograd

@@ -133,10 +133,11 @@ def test_shared_propagation_in_a_graph_replayed_batch(gpu):
 
 
 
-def test_default_shares_under_the_reference_derivation_only(gpu):
-    """The default (automatic) propagates once per update batch under the
-    reference derivation, bit-identical to one propagation per beat, and once
-    per beat under the reverse-mode gradient (the per-beat backward passes)."""
+def test_default_shares_under_both_gradients(gpu):
+    """The default (automatic) propagates once per update batch under both
+    gradients: under the reference derivation it is bit-identical to one
+    propagation per beat; under the reverse-mode gradient it is the explicit
+    shared mode, one forward and one backward pass per update batch."""
     note("case: 3 beats, default share mode vs explicit, both gradient modes")
     p = problem(5)
     cfg = Algorithm(learning_rate=0.02, freeze_delays=False)
@@ -146,7 +147,9 @@ def test_default_shares_under_the_reference_derivation_only(gpu):
     assert n_auto == [1, 1] and n_off == [3, 3], (n_auto, n_off)
     for f in FIELDS:
         assert np.array_equal(np.asarray(auto[f]).view(np.uint8), np.asarray(off[f]).view(np.uint8)), f
-    auto = run(gpu, p, cfg, "default", gradient=(1, 16))
-    off = run(gpu, p, cfg, False, gradient=(1, 16))
+    n_auto = []
+    auto = run(gpu, p, cfg, "default", gradient=(1, 16), launches=n_auto)
+    on = run(gpu, p, cfg, True, gradient=(1, 16))
+    assert n_auto == [1, 1], n_auto
     for f in FIELDS:
-        assert np.array_equal(np.asarray(auto[f]).view(np.uint8), np.asarray(off[f]).view(np.uint8)), f
+        assert np.array_equal(np.asarray(auto[f]).view(np.uint8), np.asarray(on[f]).view(np.uint8)), f
