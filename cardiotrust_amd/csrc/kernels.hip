@@ -772,10 +772,15 @@ __device__ __forceinline__ void load_row_chunk(const float *src, int64_t stride,
         }
 }
 
+// One connection's running state in the derivation. The coefficient
+// derivative is kept as the 3x3 moment M[in][o] = sum over steps of
+// s[o] e[in] (s = dy/da: FIR - IIR, or the Simple form's input difference),
+// contracted with the connection's gains once after the last step: the
+// reference's per-step sum over (in, o) of s[o] G[in][o] e[in] reassociated,
+// 9 multiply-adds per step instead of 36 operations.
 struct ConnState {
-    float G[3][3], g[3][3];
+    float g[3][3], M[3][3];
     float f[3], h[3], y[3], prev_in[3];
-    float A;
 };
 
 // Per-state factor of the gain derivative at one step (derivation.rs:315-326
@@ -797,26 +802,34 @@ __device__ __forceinline__ void gain_factors(const float xo[3], const float e[3]
 // in0 = +0 and starts from y = prev_in = fir = iir = +0, so updating its
 // filter and FIR unconditionally keeps them +0 (for a finite coefficient),
 // as the reference's guarded updates do. With kSteady the caller has checked
-// t >= D for every lane of the wave.
-template <bool kSteady>
+// t >= D for every lane of the wave. The filter outputs and FIR / IIR follow
+// the reference's operations exactly; the gain derivative fuses its
+// multiply-add (derivation.rs:325-326 multiplies, then adds).
+// kGains / kCoefs: the gain / coefficient derivatives are computed (not
+// frozen); kSimple: the Simple coefficient derivative instead of Textbook.
+// As template parameters they leave the unrolled step loop without branches.
+template <bool kSteady, bool kGains, bool kCoefs, bool kSimple>
 __device__ __forceinline__ void derive_step(ConnState &c, const float in0[3], const float e[3],
                                             const float cg[3], bool has, float a, int32_t D,
-                                            int32_t t, float reg, const DeriveParams &p) {
+                                            int32_t t) {
     float yl[3];
 #pragma unroll
     for (int o = 0; o < 3; ++o) {
         yl[o] = c.y[o];
         c.y[o] = fmaf(a, in0[o] - c.y[o], c.prev_in[o]);
     }
-    if (!p.freeze_gains) {
+    if (kGains) {
 #pragma unroll
         for (int in = 0; in < 3; ++in)
 #pragma unroll
-            for (int o = 0; o < 3; ++o) c.g[in][o] += c.y[o] * cg[in];
+            for (int o = 0; o < 3; ++o) c.g[in][o] = fmaf(c.y[o], cg[in], c.g[in][o]);
     }
-    if (!p.freeze_delays) {
+    if (kCoefs) {
         const bool on = kSteady || t >= D;
-        if (!p.simple) {
+        float s[3];
+        if (!kSimple) {
+            // derivation.rs:417-499: FIR / IIR advance from step D on; the
+            // combine runs every step with the sensitivities they hold
             if (on) {
 #pragma unroll
                 for (int o = 0; o < 3; ++o) {
@@ -825,22 +838,23 @@ __device__ __forceinline__ void derive_step(ConnState &c, const float in0[3], co
                 }
             }
 #pragma unroll
-            for (int in = 0; in < 3; ++in)
+            for (int o = 0; o < 3; ++o) s[o] = c.f[o] - c.h[o];
+        } else {
+            // derivation.rs:338-398: only existing connections past their delay
+            const bool use = has && on;
 #pragma unroll
-                for (int o = 0; o < 3; ++o)
-                    c.A += fmaf((c.f[o] - c.h[o]) * c.G[in][o] * e[in], p.mse_scaling, reg);
-        } else if (has && on) {
-#pragma unroll
-            for (int in = 0; in < 3; ++in)
-#pragma unroll
-                for (int o = 0; o < 3; ++o)
-                    c.A += fmaf((in0[o] - yl[o]) * c.G[in][o] * e[in], p.mse_scaling, reg);
+            for (int o = 0; o < 3; ++o) s[o] = use ? in0[o] - yl[o] : 0.0f;
         }
+#pragma unroll
+        for (int in = 0; in < 3; ++in)
+#pragma unroll
+            for (int o = 0; o < 3; ++o) c.M[in][o] = fmaf(s[o], e[in], c.M[in][o]);
     }
 #pragma unroll
     for (int o = 0; o < 3; ++o) c.prev_in[o] = in0[o];
 }
 
+template <bool kGains, bool kCoefs, bool kSimple>
 __global__ __launch_bounds__(kDeriveThreads, 4) void derive_kernel(
     const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
     const int32_t *__restrict__ delays, const float *__restrict__ initial_delays,
@@ -867,17 +881,14 @@ __global__ __launch_bounds__(kDeriveThreads, 4) void derive_kernel(
     const bool has = w >= 0;
     const float a = coefs[it];
     const int32_t D = delays[it];
-    const float delay = float(D) + (1.0f - a) / (a + 1.0f);
-    const float reg = p.diff_strength * powi5(initial_delays[it] - delay);
 
     ConnState c;
 #pragma unroll
     for (int in = 0; in < 3; ++in)
 #pragma unroll
         for (int o = 0; o < 3; ++o) {
-            const int64_t k = int64_t(3 * d + o) * n + 3LL * v + in;
-            c.G[in][o] = gT[k];
-            c.g[in][o] = dGT[k];
+            c.g[in][o] = dGT[int64_t(3 * d + o) * n + 3LL * v + in];
+            c.M[in][o] = 0.0f;
         }
 #pragma unroll
     for (int o = 0; o < 3; ++o) {
@@ -886,7 +897,6 @@ __global__ __launch_bounds__(kDeriveThreads, 4) void derive_kernel(
         c.y[o] = 0.0f;
         c.prev_in[o] = 0.0f;
     }
-    c.A = dA[it];
     const float *xw = xT + (has ? 3LL * w : 0) * T;
 
     for (int32_t t0 = 0; t0 < T; t0 += kDeriveChunk) {
@@ -941,7 +951,7 @@ __global__ __launch_bounds__(kDeriveThreads, 4) void derive_kernel(
                     const float in0[3] = {xin[0][j], xin[1][j], xin[2][j]};
                     const float e[3] = {ev[0][jj], ev[1][jj], ev[2][jj]};
                     const float cg[3] = {cv[0][jj], cv[1][jj], cv[2][jj]};
-                    derive_step<true>(c, in0, e, cg, has, a, D, t0 + j, reg, p);
+                    derive_step<true, kGains, kCoefs, kSimple>(c, in0, e, cg, has, a, D, t0 + j);
                 }
             }
         } else if (active) {
@@ -951,7 +961,8 @@ __global__ __launch_bounds__(kDeriveThreads, 4) void derive_kernel(
                     const float in0[3] = {xin[0][j], xin[1][j], xin[2][j]};
                     const float e[3] = {es[vl][0][j], es[vl][1][j], es[vl][2][j]};
                     const float cg[3] = {cs[vl][0][j], cs[vl][1][j], cs[vl][2][j]};
-                    derive_step<false>(c, in0, e, cg, has, a, D, t0 + j, reg, p);
+                    derive_step<false, kGains, kCoefs, kSimple>(c, in0, e, cg, has, a, D,
+                                                                t0 + j);
                 }
             }
         }
@@ -968,7 +979,21 @@ __global__ __launch_bounds__(kDeriveThreads, 4) void derive_kernel(
         fir[item * 3 + o] = c.f[o];
         iir[item * 3 + o] = c.h[o];
     }
-    dA[item] = c.A;
+    if (!kCoefs) return;
+    // the beat's coefficient derivative: the moment contracted with the gains,
+    // plus the difference regulariser's term, which the reference adds for
+    // each (in, o) of every step it combines (every step in the Textbook
+    // form, the steps t >= D of an existing connection in the Simple form)
+    float dot = 0.0f;
+#pragma unroll
+    for (int in = 0; in < 3; ++in)
+#pragma unroll
+        for (int o = 0; o < 3; ++o)
+            dot = fmaf(gT[int64_t(3 * d + o) * n + 3LL * v + in], c.M[in][o], dot);
+    const float delay = float(D) + (1.0f - a) / (a + 1.0f);
+    const float reg = p.diff_strength * powi5(initial_delays[item] - delay);
+    const int32_t combined = kSimple ? (has ? max(T - D, 0) : 0) : T;
+    dA[item] = dA[item] + fmaf(dot, p.mse_scaling, float(9 * combined) * reg);
 }
 
 // update.rs:106-114, 158-168
@@ -1584,8 +1609,18 @@ hipError_t launch_derive(hipStream_t s, const int32_t *nbr, const float *coefs,
                          const float *xT, const float *eT, float *fir, float *iir, float *dGT,
                          float *dA, int32_t N, int32_t T, const DeriveParams &p) {
     if (N <= 0) return hipSuccess;
-    derive_kernel<<<blocks_for(int64_t(N) * 26, kDeriveThreads), kDeriveThreads, 0, s>>>(
-        nbr, coefs, delays, initial_delays, gT, xT, eT, fir, iir, dGT, dA, N, T, p);
+    const unsigned blocks = blocks_for(int64_t(N) * 26, kDeriveThreads);
+#define CT_DERIVE(G, C, S)                                                                  \
+    derive_kernel<G, C, S><<<blocks, kDeriveThreads, 0, s>>>(                                \
+        nbr, coefs, delays, initial_delays, gT, xT, eT, fir, iir, dGT, dA, N, T, p)
+    const bool g = !p.freeze_gains, c = !p.freeze_delays;
+    if (g && c && !p.simple) CT_DERIVE(true, true, false);
+    else if (g && c) CT_DERIVE(true, true, true);
+    else if (g) CT_DERIVE(true, false, false);
+    else if (c && !p.simple) CT_DERIVE(false, true, false);
+    else if (c) CT_DERIVE(false, true, true);
+    else CT_DERIVE(false, false, false);
+#undef CT_DERIVE
     return hipGetLastError();
 }
 
