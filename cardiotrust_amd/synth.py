@@ -377,41 +377,63 @@ def config2(simulate=None, num_steps: int = 2000) -> Problem:
     """32^3 full cube (N = 32,768), delays 1-40, T = 2000, one beat."""
     return make_problem(cube(32), num_steps=num_steps, max_delay=40, seed=2, simulate=simulate)
 
+def _uneven_scenario(seed: int, i: int, n: int, beats: int, steps: int) -> Problem:
+    g = blob(seed * 1000 + i, n)
+    offsets = grid_motion_offsets((beats, 1, 1), (10.0 * (beats - 1), 0.0, 0.0))
+    return make_problem(g, num_steps=steps, max_delay=20, seed=seed * 1000 + i,
+                        num_beats=beats, beat_offsets_mm=offsets)
+
+def _shell_scenario(seed: int, i: int, ux: float, uy: float, beats: int,
+                    num_steps: int) -> Problem:
+    outer = (32.0, 32.0 - 2.0 * ux, 32.0)
+    inner = (16.0 + 2.0 * uy, 16.0, 16.0)
+    g = ellipsoid_shell(outer=outer, inner=inner)
+    region = sphere_region(g, seed * 1000 + i)
+    offsets = grid_motion_offsets((beats, 1, 1), (10.0 * (beats - 1), 0.0, 0.0))
+    return make_problem(g, num_steps=num_steps, max_delay=40, seed=seed * 1000 + i,
+                        num_beats=beats, beat_offsets_mm=offsets, slow_region=region)
+
+def _build(fn, specs, simulate, workers: int) -> list:
+    """fn(*spec) for every spec, in `workers` spawned processes when > 1 (they
+    run numpy only; the arrays are the same as built in this process), then
+    the ground truth's measurements with `simulate` when given."""
+    if workers > 1 and len(specs) > 1:
+        import multiprocessing as mp
+        from concurrent.futures import ProcessPoolExecutor
+        with ProcessPoolExecutor(max_workers=min(workers, len(specs)),
+                                 mp_context=mp.get_context("spawn")) as pool:
+            out = list(pool.map(fn, *zip(*specs)))
+    else:
+        out = [fn(*spec) for spec in specs]
+    if simulate is not None:
+        for p in out:
+            p.measurements = np.ascontiguousarray(simulate(p.truth()), dtype=np.float32)
+    return out
+
 def config3(simulate=None, count: int = 64, seed: int = 3, min_voxels: int = 1000,
             max_voxels: int = 40000, min_steps: int = 600, max_steps: int = 2000,
-            max_beats: int = 8) -> list:
+            max_beats: int = 8, workers: int = 1) -> list:
     """`count` scenarios of uneven size: voxels log-uniform in
     [min_voxels, max_voxels] (seeded blobs, 2.5 mm), beats uniform in
     1..max_beats (sensor array moved along x by 10 mm per beat, grid motion),
     steps uniform in [min_steps, max_steps] per scenario (the reference's
-    Measurements are (B, T, S): one length per scenario), delays 1-20."""
+    Measurements are (B, T, S): one length per scenario), delays 1-20.
+    `workers` > 1 builds the scenarios in that many processes."""
     u = uniform(seed, 40, 3 * count).reshape(count, 3)
-    out = []
+    specs = []
     for i in range(count):
         n = int(round(np.exp(np.log(min_voxels) + u[i, 0] * np.log(max_voxels / min_voxels))))
         beats = 1 + int(u[i, 1] * max_beats)
         steps = min_steps + int(u[i, 2] * (max_steps - min_steps + 1))
-        g = blob(seed * 1000 + i, n)
-        offsets = grid_motion_offsets((beats, 1, 1), (10.0 * (beats - 1), 0.0, 0.0))
-        out.append(make_problem(g, num_steps=steps, max_delay=20, seed=seed * 1000 + i,
-                                num_beats=beats, beat_offsets_mm=offsets, simulate=simulate))
-    return out
+        specs.append((seed, i, n, beats, steps))
+    return _build(_uneven_scenario, specs, simulate, workers)
 
 def config4(simulate=None, count: int = 8, seed: int = 4, beats: int = 4,
-            num_steps: int = 2000) -> list:
+            num_steps: int = 2000, workers: int = 1) -> list:
     """`count` ellipsoidal shells of about 120k voxels (64^3 grid, semi-axes
     varied by a few voxels per scenario) with a seeded slow sub-region (true
     delays doubled), `beats` beats with grid sensor motion, T = num_steps,
-    delays 1-40."""
+    delays 1-40. `workers` > 1 builds the scenarios in that many processes."""
     u = uniform(seed, 50, 2 * count).reshape(count, 2)
-    out = []
-    for i in range(count):
-        outer = (32.0, 32.0 - 2.0 * u[i, 0], 32.0)
-        inner = (16.0 + 2.0 * u[i, 1], 16.0, 16.0)
-        g = ellipsoid_shell(outer=outer, inner=inner)
-        region = sphere_region(g, seed * 1000 + i)
-        offsets = grid_motion_offsets((beats, 1, 1), (10.0 * (beats - 1), 0.0, 0.0))
-        out.append(make_problem(g, num_steps=num_steps, max_delay=40, seed=seed * 1000 + i,
-                                num_beats=beats, beat_offsets_mm=offsets, slow_region=region,
-                                simulate=simulate))
-    return out
+    specs = [(seed, i, float(u[i, 0]), float(u[i, 1]), beats, num_steps) for i in range(count)]
+    return _build(_shell_scenario, specs, simulate, workers)
