@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "epochs/s at 120k voxels x 2000 steps (fwd+grad+update), HBM GB/s vs peak"
 HBM_PEAK_GBS = 8000.0
+BUILD_WORKERS = 8  # processes that build the batched workloads' scenarios (numpy only)
 MFMA_FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
 # Algorithmic bytes of one propagation step per active voxel: 4 B x (234 gains
 # + 26 coefs + 26 delays + 2 x 78 filter outputs read/write + 3 states written
@@ -36,38 +37,80 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(problem, measurements, sample_steps: int, threads_cap: int = 16):
-    """Oracle FP32 epoch (single thread) on the first `sample_steps` steps of
-    the same problem, scaled to epochs/s of the full T."""
+def cpu_cores_for_baseline() -> int:
+    """Threads the CPU baseline may use: the cores this process may run on,
+    at most 16 (a one-GPU job's share of the host on the MI355X boxes, whatever
+    os.cpu_count() reports there)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(avail, 16))
+
+
+def cpu_info() -> dict:
+    """What /proc/cpuinfo says about the host (model, family / model /
+    stepping numbers, logical processors) and the core counts the OS reports."""
+    info = {"os_cpu_count": os.cpu_count()}
+    try:
+        info["affinity_cores"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    fields, processors = {}, 0
+    try:
+        for line in open("/proc/cpuinfo"):
+            if ":" not in line:
+                continue
+            k, v = (x.strip() for x in line.split(":", 1))
+            if k == "processor":
+                processors += 1
+            elif k in ("vendor_id", "model name", "cpu family", "model", "stepping", "cpu MHz",
+                       "cache size", "cpu cores", "siblings") and k not in fields:
+                fields[k] = v
+    except OSError:
+        pass
+    info["cpuinfo"] = fields
+    info["logical_processors"] = processors
+    return info
+
+
+def cpu_baseline(problem, measurements, sample_steps: int, cfg0_problem, gpu_cfg0_ms: float):
+    """The FP32 oracle's run_epoch on the host (the reference's CPU epoch,
+    restated operation for operation), three ways:
+      * cfg0 (10^3 cube, T = 1000): whole epochs on one thread, median of 3;
+      * cfg1 (the 120k-voxel shell, T = 2000): one thread over the first
+        `sample_steps` steps, scaled to the full T (a whole epoch takes
+        about 8 minutes);
+      * throughput: that cfg1 sample on every core the baseline may use at once,
+        one independent scenario per thread as the reference's scheduler runs
+        them (the oracle's ctypes calls release the GIL)."""
+    import statistics
+    import threading
+
     from oracle import oracle
     from cardiotrust_amd import Algorithm
     from cardiotrust_amd.synth import Problem
+
+    cfg = Algorithm(learning_rate=1e-3, freeze_delays=False)
+    times0 = []
+    for _ in range(3):
+        r0 = oracle.OracleResults(cfg0_problem, np.float32)
+        t0 = time.perf_counter()
+        r0.run_epoch(cfg)
+        times0.append(time.perf_counter() - t0)
+    med0 = statistics.median(times0)
 
     p = Problem(problem.geometry, problem.num_sensors, sample_steps, 1, problem.gains,
                 problem.output_state_indices, problem.coefs, problem.delays,
                 problem.initial_delays, problem.control_matrix,
                 problem.control_function[:sample_steps], problem.measurement_matrix,
                 np.ascontiguousarray(measurements[:, :sample_steps]))
-    cfg = Algorithm(learning_rate=1e-3, freeze_delays=False)
     r = oracle.OracleResults(p, np.float32)
     t0 = time.perf_counter()
     r.run_epoch(cfg)
     dt = time.perf_counter() - t0
     per_epoch = dt * problem.num_steps / sample_steps
-    out = {
-        "value": 1.0 / per_epoch,
-        "unit": "epochs/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"oracle FP32 run_epoch on the first {sample_steps} of {problem.num_steps} steps "
-                  f"of the same problem ({dt:.1f} s), scaled by T/{sample_steps}",
-        "cpu_model": cpu_model(),
-    }
-    # throughput variant: one independent scenario epoch per core, concurrently
-    # (the reference runs one scenario per thread); the oracle's ctypes calls
-    # release the GIL
-    import threading
-    threads = max(1, min(os.cpu_count() or 1, threads_cap))
+    threads = cpu_cores_for_baseline()
     copies = [oracle.OracleResults(p, np.float32) for _ in range(threads)]
     workers = [threading.Thread(target=c.run_epoch, args=(cfg,)) for c in copies]
     t0 = time.perf_counter()
@@ -76,90 +119,105 @@ def cpu_baseline(problem, measurements, sample_steps: int, threads_cap: int = 16
     for w in workers:
         w.join()
     dt_all = time.perf_counter() - t0
-    out["throughput"] = {
-        "value": threads / (dt_all * problem.num_steps / sample_steps),
-        "unit": "epochs/s", "cores": threads,
-        "sample": f"{threads} concurrent copies of the same sample epoch ({dt_all:.1f} s)",
+    return {
+        "value": 1.0 / per_epoch,
+        "unit": "epochs/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle FP32 run_epoch (the reference's CPU epoch restated) on the first "
+                  f"{sample_steps} of {problem.num_steps} steps of the 120k-voxel shell "
+                  f"({dt:.1f} s), scaled by T/{sample_steps}; one thread",
+        "host": cpu_info(),
+        "cfg0": {"value": 1.0 / med0, "unit": "epochs/s", "cores": 1,
+                 "ms_per_epoch_median_of_3": 1e3 * med0,
+                 "ms_per_epoch_runs": [1e3 * t for t in times0],
+                 "sample": "whole oracle FP32 epochs of the 10^3 cube, T = 1000, one thread",
+                 "gpu_ms_per_epoch": gpu_cfg0_ms,
+                 "gpu_vs_cpu": 1e3 * med0 / gpu_cfg0_ms if gpu_cfg0_ms else None},
+        "throughput": {
+            "value": threads / (dt_all * problem.num_steps / sample_steps),
+            "unit": "epochs/s", "cores": threads,
+            "sample": f"{threads} concurrent copies of the cfg1 sample epoch ({dt_all:.1f} s), "
+                      f"one per thread; {threads} = the cores this process may use, at most 16 "
+                      f"(one GPU's share of the host)",
+        },
     }
-    return out
 
 
-def cpu_model() -> str:
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return "unknown"
-
-
-def bench_batched(args) -> int:
+def run_batched(gpu, problems, label: str, adjoint: bool, warmup: int, steps: int,
+                checkpoint: int = 64, share: str = "auto", vpg: int = 0, graph: bool = True,
+                plain_epochs_per_s: float = 0.0, trajectory: bool = False,
+                gradient_check: int = 0) -> dict:
     """Many scenarios per epoch through ct_batch (one persistent propagation
     launch per round of beats, autotuned voxels per workgroup, hipGraph
-    replay), timed beside the same scenarios run one after another."""
-    from cardiotrust_amd import Algorithm, synth
-    from cardiotrust_amd.gpu import GPU, Batch, Buffer, EpochKernel, ResultsGPU
+    replay), timed beside the same scenarios run one after another. Returns
+    the JSON object of the workload."""
+    from cardiotrust_amd import Algorithm
+    from cardiotrust_amd.gpu import Batch, Buffer, EpochKernel, ResultsGPU
 
     t0 = time.perf_counter()
-    adjoint = args.workload == "batched-adjoint"
-    shared = args.share_propagation == "on" or (args.share_propagation == "auto" and not adjoint)
-    if adjoint:
-        problems = synth.config4(count=args.scenarios or 8, num_steps=args.num_steps)
-        label = (f"{len(problems)} ellipsoidal shells of ~120k voxels with slow sub-regions, "
-                 f"4 beats, T={args.num_steps}, reverse-mode gradient (checkpoint every "
-                 f"{args.checkpoint} steps), SGD lr 1e-3")
-    else:
-        problems = synth.config3(count=args.scenarios or 64)
-        label = (f"{len(problems)} blob scenarios, voxels log-uniform 1k-40k, 1-8 beats, "
-                 f"T uniform 600-2000, reference derivation, SGD lr 1e-3")
-    log(f"{len(problems)} synthetic scenarios ({time.perf_counter() - t0:.1f} s)")
-    gpu = GPU(0)
+    shared = share != "off"
     for p in problems:
         tr = ResultsGPU(gpu, p.truth())
         p.measurements = tr.simulate()
         tr.close()
-    log(f"ground truths simulated on GPU ({time.perf_counter() - t0:.1f} s)")
+    log(f"  ground truths simulated on GPU ({time.perf_counter() - t0:.1f} s)")
     cfg = Algorithm(learning_rate=1e-3, freeze_delays=False)
-    epochs = args.warmup + args.steps + 2
+    epochs = warmup + steps + 2
     members = []
     for p in problems:
         r = ResultsGPU(gpu, p, num_batch_entries=epochs + 1)
         b = Buffer(gpu, p.measurements)
         k = EpochKernel(gpu, r, b, cfg, 3 * p.num_voxels, p.num_sensors, p.num_steps)
         if adjoint:
-            k.set_gradient(1, args.checkpoint)
-        k.set_share_propagation({"auto": None, "on": True, "off": False}[args.share_propagation])
+            k.set_gradient(1, checkpoint)
+        k.set_share_propagation({"auto": None, "on": True, "off": False}[share])
         members.append((r, b, k))
     voxel_steps = sum(p.num_voxels * p.num_steps * p.num_beats for p in problems)
 
-    # the same scenarios one after another (one epoch each)
+    # the same scenarios one after another (one epoch each); the first one's
+    # per-phase times give the projections' MFMA utilisation
     gpu.synchronize()
     ts = time.perf_counter()
-    for r, b, k in members:
+    for i, (r, b, k) in enumerate(members):
+        k.set_profiling(i == 0)
         k.execute()
     gpu.synchronize()
     seq_s = time.perf_counter() - ts
-    log(f"sequential epoch of all scenarios: {seq_s * 1e3:.1f} ms")
+    prof0 = members[0][2].profile()
+    members[0][2].set_profiling(False)
+    p0 = problems[0]
+    flop0 = 2.0 * p0.num_sensors * 3 * p0.num_voxels * p0.num_steps * p0.num_beats
+    mfma = {"scenario": 0, "peak_tflops": MFMA_FP32_PEAK_TFLOPS,
+            "basis": "2 S n T flop per beat and contraction / the contraction's HIP-event "
+                     "time in scenario 0's own epoch"}
+    for key, kernel in (("project_ms", "project_kernel (H·Xᵀ)"),
+                        ("map_residuals_ms", "map_residuals_kernel (Hᵀ·R)")):
+        ms = prof0[key]
+        tf = flop0 / (ms * 1e-3) / 1e12 if ms else 0.0
+        mfma[key.replace("_ms", "")] = {"kernel": kernel, "gflop": flop0 / 1e9, "ms": ms,
+                                        "tflops": tf, "frac": tf / MFMA_FP32_PEAK_TFLOPS}
+    log(f"  sequential epoch of all scenarios: {seq_s * 1e3:.1f} ms")
 
-    batch = Batch(gpu, [k for _, _, k in members], voxels_per_group=args.voxels_per_group,
-                  graph=not args.no_graph)
+    batch = Batch(gpu, [k for _, _, k in members], voxels_per_group=vpg, graph=graph)
     batch.set_profiling(True)
     batch.execute()  # eager epoch (autotune, plan, buffers)
     info0 = batch.info()
-    log("autotune (voxels per workgroup -> ms of round 0): " + json.dumps(info0["autotune"]))
-    log(f"eager batched epoch: {info0['epoch_ms']:.1f} ms, batched propagation "
+    log("  autotune (voxels per workgroup -> ms of round 0): " + json.dumps(info0["autotune"]))
+    log(f"  eager batched epoch: {info0['epoch_ms']:.1f} ms, batched propagation "
         f"{info0['propagate_ms']:.1f} ms in {info0['propagate_launches']} launches")
-    for _ in range(max(args.warmup - 1, 0)):
+    for _ in range(max(warmup - 1, 0)):
         batch.execute()
     gpu.synchronize()
     ts = time.perf_counter()
-    for _ in range(args.steps):
+    for e in range(steps):
         batch.execute()
+        if (e + 1) % 10 == 0:
+            log(f"  batched epoch {e + 1} of {steps}")
     gpu.synchronize()
     elapsed = time.perf_counter() - ts
     info = batch.info()
-    ms_per_epoch = 1e3 * elapsed / args.steps
+    ms_per_epoch = 1e3 * elapsed / steps
     # roofline of the batched propagation launches (eager epoch, HIP events)
     prop_ms = info0["propagate_ms"]
     resident_vs = 0  # voxel-steps the batched launches propagated in one epoch
@@ -168,38 +226,134 @@ def bench_batched(args) -> int:
             launches = 1 if shared else p.num_beats  # batch_size 0
             resident_vs += p.num_voxels * p.num_steps * launches
     achieved = PROP_BYTES_PER_VOXEL_STEP * resident_vs / (prop_ms * 1e-3) / 1e9 if prop_ms else 0.0
+    scen_eps = len(problems) * 1000.0 / ms_per_epoch
     out = {
-        "metric": "batched epochs/s (all scenarios advance one epoch)",
-        "value": 1000.0 / ms_per_epoch,
-        "unit": "epochs/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_epoch,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": label, "scenarios": len(problems),
-                   "shared_propagation": shared, "share_mode": args.share_propagation,
-                   "voxel_steps_per_epoch": voxel_steps, "graph": bool(info["graph"]),
-                   "voxels_per_group": info["voxels_per_group"], "rounds": info["rounds"],
-                   "jobs": info["jobs"]},
-        "scenario_epochs_per_s": len(problems) * 1000.0 / ms_per_epoch,
+        "workload": label,
+        "epochs_per_s": 1000.0 / ms_per_epoch,
+        "ms_per_epoch": ms_per_epoch,
+        "timed_epochs": steps, "warmup_epochs": warmup,
+        "scenarios": len(problems),
+        "scenario_epochs_per_s": scen_eps,
+        "voxel_steps_per_epoch": voxel_steps,
         "voxel_steps_per_s": voxel_steps * 1000.0 / ms_per_epoch,
-        "sequential": {"ms_per_epoch": seq_s * 1e3,
-                       "speedup_of_batch": seq_s * 1e3 / ms_per_epoch},
-        "roofline": {"bound": "hbm", "kernel": "propagate_batch_kernel", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "propagate_ms_per_epoch": prop_ms,
-                     "predicted_ms": info0["predicted_propagate_ms"]},
+        "shared_propagation": shared, "graph": bool(info["graph"]),
+        "voxels_per_group": info["voxels_per_group"], "rounds": info["rounds"],
+        "jobs": info["jobs"], "side_streams": info["side_streams"],
+        "sequential": {"ms_per_epoch": seq_s * 1e3, "speedup_of_batch": seq_s * 1e3 / ms_per_epoch},
+        "hbm": {"kernel": "propagate_batch_kernel", "achieved_gbs": achieved,
+                "peak_gbs": HBM_PEAK_GBS, "frac": achieved / HBM_PEAK_GBS,
+                "basis": "algorithmic bytes (1792 B per propagated voxel-step) / the batched "
+                         "launches' HIP-event time in the eager epoch",
+                "propagate_ms_per_epoch": prop_ms},
+        "mfma": mfma,
     }
+    if plain_epochs_per_s:
+        out["vs_plain_epoch"] = {
+            "scenario_epochs_ratio": scen_eps / plain_epochs_per_s,
+            "definition": "scenario-epochs per second of the batch (every scenario advancing "
+                          "one epoch over all its beats) / epochs per second of the plain "
+                          "single-scenario epoch (120,120 voxels, T = 2000, one beat, reference "
+                          "derivation) measured in the same run",
+            "voxel_steps_ratio": (voxel_steps * 1000.0 / ms_per_epoch)
+                                  / (plain_epochs_per_s * 120120 * 2000),
+        }
+    if trajectory:
+        losses = np.array([r.update_from_gpu(["loss_batch"])["loss_batch"] for r, _, _ in members])
+        n = warmup + steps + 1  # the sequential epoch, then the batched ones
+        out["loss_trajectory"] = {
+            "epochs": n,
+            "mean_over_scenarios": losses[:, :n].mean(axis=0).tolist(),
+            "per_scenario_first_last": [[float(a[0]), float(a[n - 1])] for a in losses],
+        }
+    if gradient_check and adjoint:
+        out["gradient_check"] = fd_gradient_check(gpu, members[0], problems[0], checkpoint,
+                                                  gradient_check)
     batch.close()
     for r, b, k in members:
         k.close()
         b.close()
         r.close()
+    return out
+
+
+def fd_gradient_check(gpu, member, p, checkpoint: int, picks: int) -> dict:
+    """The reverse-mode gradient of scenario 0's summed objective sum_b J_b at
+    its current parameters (one epoch with learning rate 0 on a fresh epoch
+    kernel) against FP64 central differences (h = 1e-6) of sum_b J_b on
+    `picks` seeded gains and `picks` seeded coefficients above the FD noise
+    floor: the largest relative error per entry."""
+    from cardiotrust_amd import Algorithm
+    from cardiotrust_amd.gpu import EpochKernel
+
+    r, buf, _ = member
+    cfg = Algorithm(learning_rate=0.0, freeze_delays=False)
+    k = EpochKernel(gpu, r, buf, cfg, 3 * p.num_voxels, p.num_sensors, p.num_steps)
+    k.set_gradient(1, checkpoint)
+    k.set_batch_index(0)
+    k.execute()
+    d = r.update_from_gpu(["d_gains", "d_coefs"])
+    k.close()
+    h = 1e-6
+
+    def J(param=0, index=0, delta=0.0):
+        return sum(r.objective_f64(buf, beat, cfg, param, index, delta)
+                   for beat in range(p.num_beats))
+
+    J0 = J()
+    floor = 1e4 * abs(J0) * np.finfo(np.float64).eps / h
+    rng = np.random.default_rng(7)
+    out = {"h": h, "noise_floor": floor, "scenario": 0,
+           "definition": "|g - fd| / |g| per entry, g the FP32 reverse-mode gradient of "
+                         "sum_b J_b, fd its FP64 central difference"}
+    for name, param, g, pool in (
+            ("gains", 1, d["d_gains"].ravel(), np.flatnonzero(p.output_state_indices.ravel() >= 0)),
+            ("coefs", 2, d["d_coefs"].ravel(), np.flatnonzero(p.geometry.neighbours.ravel() >= 0))):
+        res = pool[np.abs(g[pool]) >= floor]
+        sel = rng.choice(res, min(picks, res.size), replace=False)
+        errs = [abs(g[i] - (J(param, int(i), h) - J(param, int(i), -h)) / (2 * h))
+                / max(abs(g[i]), 1e-300) for i in sel]
+        out[name] = {"entries": int(sel.size), "resolvable": int(res.size),
+                     "max_rel_err": float(max(errs)) if errs else None}
+    return out
+
+
+def batched_problems(workload: str, args):
+    from cardiotrust_amd import synth
+    if workload == "batched-adjoint":
+        problems = synth.config4(count=args.scenarios or 8, num_steps=args.num_steps,
+                                 workers=BUILD_WORKERS)
+        label = (f"{len(problems)} ellipsoidal shells of ~120k voxels with slow sub-regions, "
+                 f"4 beats, T={args.num_steps}, reverse-mode gradient (checkpoint every "
+                 f"{args.checkpoint} steps), SGD lr 1e-3")
+    else:
+        problems = synth.config3(count=args.scenarios or 64, workers=BUILD_WORKERS)
+        label = (f"{len(problems)} blob scenarios, voxels log-uniform 1k-40k, 1-8 beats, "
+                 f"T uniform 600-2000, reference derivation, SGD lr 1e-3")
+    return problems, label
+
+
+def bench_batched(args) -> int:
+    """One batched workload on its own: prints its own JSON line."""
+    from cardiotrust_amd.gpu import GPU
+
+    t0 = time.perf_counter()
+    problems, label = batched_problems(args.workload, args)
+    log(f"{len(problems)} synthetic scenarios ({time.perf_counter() - t0:.1f} s)")
+    gpu = GPU(0)
+    res = run_batched(gpu, problems, label, args.workload == "batched-adjoint", args.warmup,
+                      args.steps, args.checkpoint, args.share_propagation,
+                      args.voxels_per_group, not args.no_graph,
+                      trajectory=args.workload == "batched-adjoint")
+    out = {
+        "metric": "batched epochs/s (all scenarios advance one epoch)",
+        "value": res["epochs_per_s"], "unit": "epochs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["ms_per_epoch"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": label, "scenarios": len(problems),
+                   "shared_propagation": res["shared_propagation"],
+                   "share_mode": args.share_propagation},
+        "batched": res,
+    }
     print(json.dumps(out), flush=True)
     gpu.close()
     return 0
@@ -255,10 +409,14 @@ def main() -> int:
     ap.add_argument("--voxels-per-group", type=int, default=0,
                     help="batched workloads: 0 = autotune")
     ap.add_argument("--no-graph", action="store_true", help="batched workloads: eager epochs")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="plain workload: skip the two batched workloads of the headline line")
+    ap.add_argument("--batched-adjoint-epochs", type=int, default=50,
+                    help="plain workload: timed epochs of the 8-shell reverse-mode batch")
     ap.add_argument("--share-propagation", choices=["auto", "on", "off"], default="auto",
                     help="batched workloads: one propagation (and, with the reverse-mode "
                          "gradient, one backward pass) per update batch of beats; auto = "
-                         "the library default (shared under the reference derivation only)")
+                         "the library default (shared under both gradients)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the epoch on a single MI355X; "
@@ -320,11 +478,16 @@ def main() -> int:
     steps_per_launch = prof["propagate_steps"] / launches
     bytes_per_launch = PROP_BYTES_PER_VOXEL_STEP * N * steps_per_launch
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
-    traffic = None
+    traffic, traffic_source = None, None
     tf = os.path.join(ROOT, "profiles", "propagate_resident_cfg1_traffic.json")
     if resident and N == 120120 and os.path.exists(tf):
         # HBM bytes per step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-        traffic = json.load(open(tf))["bytes_per_step"] * steps_per_launch
+        rec = json.load(open(tf))
+        traffic = rec["bytes_per_step"] * steps_per_launch
+        traffic_source = (f"{rec['source']}: rocprofv3 FETCH_SIZE x {rec['fetch_scale']} + "
+                          f"WRITE_SIZE x {rec['write_scale']} per dispatch of "
+                          f"{rec['steps_per_dispatch']} steps ({rec['workload']}), per step "
+                          f"x {steps_per_launch:.0f} steps")
     loss = results.update_from_gpu(["loss_batch"])["loss_batch"]
     log(f"loss_batch per epoch: {loss.tolist()}")
 
@@ -357,7 +520,13 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
+            "achieved_basis": "algorithmic bytes, 1792 B per voxel-step (234 gains, 26 coefs, "
+                              "26 delays, 2 x 78 filter outputs, 3 + 3 states, 4 B each) x "
+                              "voxels x steps per launch / the launch's HIP-event time",
             "traffic": traffic,
+            "traffic_source": traffic_source,
+            "traffic_gbs": traffic / (launch_ms * 1e-3) / 1e9 if traffic else None,
+            "traffic_frac": traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
             "avg_launch_ms": launch_ms,
             "steps_per_launch": steps_per_launch,
             "us_per_step": 1e3 * launch_ms / steps_per_launch,
@@ -381,8 +550,47 @@ def main() -> int:
     kernel.close()
     actual.close()
     results.close()
+
+    p0, gpu_cfg0_ms = None, 0.0
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(problem, measurements, args.cpu_sample_steps)
+        # the plain epoch of cfg0 (10^3 cube, T = 1000) on the GPU, beside the CPU
+        # baseline's cfg0 epochs
+        p0 = synth.config0()
+        tr0 = ResultsGPU(gpu, p0.truth())
+        p0.measurements = tr0.simulate()
+        tr0.close()
+        r0 = ResultsGPU(gpu, p0, num_batch_entries=8)
+        b0 = Buffer(gpu, p0.measurements)
+        k0 = EpochKernel(gpu, r0, b0, cfg, 3 * p0.num_voxels, p0.num_sensors, p0.num_steps)
+        k0.execute()
+        gpu.synchronize()
+        t_0 = time.perf_counter()
+        for _ in range(5):
+            k0.execute()
+        gpu.synchronize()
+        gpu_cfg0_ms = 1e3 * (time.perf_counter() - t_0) / 5
+        k0.close(), b0.close(), r0.close()
+        log(f"cfg0 plain epoch on the GPU: {gpu_cfg0_ms:.2f} ms")
+
+    if not args.no_batched and args.gradient == "reference":
+        plain = out["value"]
+        batched = {}
+        for workload, warm, timed, extra in (
+                ("batched-uneven", 2, 5, {}),
+                ("batched-adjoint", 1, args.batched_adjoint_epochs,
+                 {"trajectory": True, "gradient_check": 3})):
+            t1 = time.perf_counter()
+            problems, label = batched_problems(workload, args)
+            log(f"{workload}: {len(problems)} scenarios built ({time.perf_counter() - t1:.1f} s)")
+            batched[workload.split("-")[1]] = run_batched(
+                gpu, problems, label, workload == "batched-adjoint", warm, timed,
+                args.checkpoint, plain_epochs_per_s=plain, **extra)
+            del problems
+            log(f"{workload}: done ({time.perf_counter() - t1:.1f} s)")
+        out["batched"] = batched
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(problem, measurements, args.cpu_sample_steps, p0,
+                                           gpu_cfg0_ms)
     print(json.dumps(out), flush=True)
     gpu.close()
     return 0
