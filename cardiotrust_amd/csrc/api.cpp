@@ -168,7 +168,7 @@ int project_beat(ct_results_gpu &r, int32_t beat, const float *mhat, ct_epoch_ke
     r.forward_beat = beat;
     Span sp(k, kProject);
     const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
-    CT_HIP(ct::launch_project(s, r.x, Hb, r.partial, r.xT, r.T, r.n, r.Sp, r.splits, r.kps));
+    CT_HIP(ct::launch_project(s, r.x, Hb, r.partial, r.xT, r.T, r.n, r.S, r.Sp, r.splits, r.kps));
     r.xT_fresh = true;
     CT_HIP(ct::launch_residuals(s, r.partial, r.splits,
                                 mhat ? mhat + int64_t(beat) * r.T * r.S : nullptr,
@@ -202,7 +202,7 @@ int derive_reference(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, ct_
     const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
     {
         Span sp(k, kMapResiduals);
-        CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.eT, r.n, r.T, r.Sp));
+        CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.eT, r.n, r.T, r.S, r.Sp));
     }
     if (c.freeze_gains && c.freeze_delays) return CT_OK;
     if (!r.xT_fresh) {
@@ -522,9 +522,9 @@ static int adjoint_forcing(ct_results_gpu &r, int32_t beat, bool accumulate, ct_
     const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
     Span sp(k, kMapResiduals);
     if (!accumulate) {
-        CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.eT, r.n, r.T, r.Sp));
+        CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.eT, r.n, r.T, r.S, r.Sp));
     } else {
-        CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.xbarT, r.n, r.T, r.Sp));  // scratch
+        CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.xbarT, r.n, r.T, r.S, r.Sp));  // scratch
         CT_HIP(ct::launch_add(s, r.eT, r.xbarT, r.n * r.T));
     }
     return CT_OK;
@@ -810,7 +810,7 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
     const int64_t n = r->n, T = r->T, S = r->S, Sp = r->Sp, B = r->B;
 
     // GEMM split-K plan: about 2048 workgroups for the projection.
-    const int64_t tiles = (T + ct::kProjTile - 1) / ct::kProjTile * (Sp / 128);
+    const int64_t tiles = (T + ct::kProjTile - 1) / ct::kProjTile * ct::project_sensor_tiles(int32_t(S));
     int64_t splits = std::max<int64_t>(1, std::min<int64_t>((n + 31) / 32, 2048 / std::max<int64_t>(tiles, 1)));
     int64_t kps = ((n + splits - 1) / splits + ct::kGemmTileK - 1) / ct::kGemmTileK * ct::kGemmTileK;
     kps = std::max<int64_t>(kps, ct::kGemmTileK);

@@ -15,6 +15,10 @@ namespace ct {
 
 constexpr int kGemmTileK = 32;
 constexpr int kProjTile = 128;  // steps per project workgroup (H·Xᵀ)
+constexpr int kProjRows = 112;  // sensors per project workgroup (seven 16-row MFMA blocks)
+// Sensor tiles of the projection for S sensors (the tiles cover rows up to
+// S rounded up to 112; H holds Sp >= S rows, those >= S zero).
+inline int32_t project_sensor_tiles(int32_t S) { return (S + kProjRows - 1) / kProjRows; }
 
 struct DeriveParams {
     float mse_scaling;
@@ -90,7 +94,7 @@ hipError_t launch_propagate_batch(hipStream_t s, int32_t slots, const ResidentJo
 // H·Xᵀ split-K partial sums; with xT not null the same pass also writes the
 // states transposed, xT (n, T).
 hipError_t launch_project(hipStream_t s, const float *x, const float *H, float *partial,
-                          float *xT, int32_t T, int64_t n, int32_t Sp, int32_t splits,
+                          float *xT, int32_t T, int64_t n, int32_t S, int32_t Sp, int32_t splits,
                           int64_t k_per_split);
 
 hipError_t launch_residuals(hipStream_t s, const float *partial, int32_t splits, const float *mhat,
@@ -110,7 +114,7 @@ hipError_t launch_maxreg_finalize(hipStream_t s, const float *partial, int32_t b
 hipError_t launch_transpose(hipStream_t s, const float *in, float *out, int64_t rows, int64_t cols);
 
 hipError_t launch_map_residuals(hipStream_t s, const float *H, const float *R, float *eT,
-                                int64_t n, int32_t T, int32_t Sp);
+                                int64_t n, int32_t T, int32_t S, int32_t Sp);
 
 hipError_t launch_derive(hipStream_t s, const int32_t *nbr, const float *coefs,
                          const int32_t *delays, const float *initial_delays, const float *gT,

@@ -405,31 +405,42 @@ __global__ __launch_bounds__(kResThreads) void propagate_batch_kernel(
 
 // ---------------------------------------------------------------------------
 // Projection H·Xᵀ over all steps (prediction.rs:112-130 for every t), FP32
-// MFMA 32x32x2. Split-K: workgroup (t-tile, split, s-tile) writes the partial
-// product of its K range; launch_residuals sums the splits in fixed order.
+// MFMA 16x16x4 over tiles of kProjRows = 112 sensors (seven blocks of 16), so
+// 108 sensors waste 4 rows instead of the 20 a 32-row granularity would.
+// Workgroup (t-tile, split, sensor tile) covers kProjTile steps x 112 sensors
+// over its K range and writes that partial product; launch_residuals sums the
+// splits in fixed order. Wave w computes steps [32 w, 32 w + 32) of the tile
+// for all 112 sensors: 14 accumulators, 9 LDS operand reads per 14 MFMAs.
 // ---------------------------------------------------------------------------
+constexpr int kProjRowsPad = kProjRows + 32;   // H tile row stride: rows k, k+1 on opposite bank halves
+constexpr int kProjStepsPad = kProjTile + 16;  // X tile row stride, the same property
 __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ x,
                                                       const float *__restrict__ H,
                                                       float *__restrict__ partial,
                                                       float *__restrict__ xT, int32_t T,
                                                       int64_t n, int32_t Sp, int64_t kps) {
-    // k-major tiles (rows padded by 4 floats): the MFMA operand reads of a
-    // wave are 32 consecutive floats, free of bank conflicts
-    __shared__ float As[kGemmTileK][kProjTile + 4];  // X[t][k], kProjTile steps
-    __shared__ float Bs[kGemmTileK][128 + 4];        // H[s][k], 128 sensors
+    // k-major tiles: a 16x16x4 operand read takes 16 consecutive floats of
+    // four k rows; strides = 16 mod 32 words put the two rows of each 32-lane
+    // half on disjoint banks
+    __shared__ float As[kGemmTileK][kProjStepsPad];  // X[t][k], kProjTile steps
+    __shared__ float Bs[kGemmTileK][kProjRowsPad];   // H[s][k], kProjRows sensors
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int32_t t0 = blockIdx.x * kProjTile;
     const int32_t split = blockIdx.y;
-    const int32_t s0 = blockIdx.z * 128;
+    const int32_t s0 = blockIdx.z * kProjRows;
     const int64_t k0 = int64_t(split) * kps;
     const int64_t k1 = min(n, k0 + kps);
-    floatx16 acc[kProjTile / 32];
+    typedef float floatx4 __attribute__((ext_vector_type(4)));
+    floatx4 acc[kProjRows / 16][2];
 #pragma unroll
-    for (int q = 0; q < kProjTile / 32; ++q) acc[q] = floatx16{0};
+    for (int sb = 0; sb < kProjRows / 16; ++sb)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb) acc[sb][tb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     // loader: 8 lanes cover one row's 32 k values (one 128-byte line). The
     // next k tile is loaded into registers while the MFMAs run on this one.
     const int lrow = tid >> 3, lk = (tid & 7) * 4;
-    float4_a4 xa[kProjTile / 32], ha[4];
+    constexpr int kHPasses = (kProjRows + 31) / 32;
+    float4_a4 xa[kProjTile / 32], ha[kHPasses];
     auto load_tile = [&](int64_t kb) __attribute__((always_inline)) {
         const bool full = kb + kGemmTileK <= k1;
 #pragma unroll
@@ -448,28 +459,33 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
             xa[pass] = v;
         }
 #pragma unroll
-        for (int pass = 0; pass < 4; ++pass) {
-            const float *src = H + int64_t(s0 + pass * 32 + lrow) * n + kb + lk;
+        for (int pass = 0; pass < kHPasses; ++pass) {
+            const int r = pass * 32 + lrow, srow = s0 + r;
             float4_a4 v = {0.0f, 0.0f, 0.0f, 0.0f};
-            if (full) {
-                v = *reinterpret_cast<const float4_a4 *>(src);
-            } else {
+            if (r < kProjRows && srow < Sp) {
+                const float *src = H + int64_t(srow) * n + kb + lk;
+                if (full) {
+                    v = *reinterpret_cast<const float4_a4 *>(src);
+                } else {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) v[j] = (kb + lk + j < k1) ? src[j] : 0.0f;
+                    for (int j = 0; j < 4; ++j) v[j] = (kb + lk + j < k1) ? src[j] : 0.0f;
+                }
             }
             ha[pass] = v;
         }
     };
     if (k0 < k1) load_tile(k0);
+    const int li = lane & 15, lq = lane >> 4;
     for (int64_t kb = k0; kb < k1; kb += kGemmTileK) {
 #pragma unroll
         for (int pass = 0; pass < kProjTile / 32; ++pass)
 #pragma unroll
             for (int j = 0; j < 4; ++j) As[lk + j][pass * 32 + lrow] = xa[pass][j];
 #pragma unroll
-        for (int pass = 0; pass < 4; ++pass)
+        for (int pass = 0; pass < kHPasses; ++pass)
+            if (pass * 32 + lrow < kProjRows)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) Bs[lk + j][pass * 32 + lrow] = ha[pass][j];
+                for (int j = 0; j < 4; ++j) Bs[lk + j][pass * 32 + lrow] = ha[pass][j];
         __syncthreads();
         if (kb + kGemmTileK < k1) load_tile(kb + kGemmTileK);
         if (xT && blockIdx.z == 0) {
@@ -493,25 +509,32 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
             }
         }
 #pragma unroll
-        for (int kk = 0; kk < kGemmTileK; kk += 2) {
-            const int kr = kk + (lane >> 5);
-            const float bv = Bs[kr][wave * 32 + (lane & 31)];
+        for (int kq = 0; kq < kGemmTileK; kq += 4) {
+            const int kr = kq + lq;
+            float bv[2];
 #pragma unroll
-            for (int q = 0; q < kProjTile / 32; ++q) {
-                const float av = As[kr][q * 32 + (lane & 31)];
-                acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[q], 0, 0, 0);
+            for (int tb = 0; tb < 2; ++tb) bv[tb] = As[kr][wave * 32 + tb * 16 + li];
+#pragma unroll
+            for (int sb = 0; sb < kProjRows / 16; ++sb) {
+                const float av = Bs[kr][sb * 16 + li];
+#pragma unroll
+                for (int tb = 0; tb < 2; ++tb)
+                    acc[sb][tb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[tb], acc[sb][tb], 0, 0, 0);
             }
         }
         __syncthreads();
     }
-    const int32_t s = s0 + wave * 32 + (lane & 31);
+    // lane l holds D[4 (l / 16) + r][l % 16]: sensor rows, step columns
 #pragma unroll
-    for (int q = 0; q < kProjTile / 32; ++q)
+    for (int sb = 0; sb < kProjRows / 16; ++sb)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int32_t t = t0 + q * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-            if (t < T) partial[(int64_t(split) * T + t) * Sp + s] = acc[q][r];
-        }
+        for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int32_t s = s0 + sb * 16 + 4 * lq + r;
+                const int32_t t = t0 + wave * 32 + tb * 16 + li;
+                if (t < T && s < Sp) partial[(int64_t(split) * T + t) * Sp + s] = acc[sb][tb][r];
+            }
 }
 
 // Residuals and per-step MSE (estimation.rs:146-152, metrics.rs:189-190).
@@ -659,12 +682,13 @@ __global__ __launch_bounds__(256) void transpose_kernel(const float *__restrict_
 
 // ---------------------------------------------------------------------------
 // Mapped residuals Hᵀ·Rᵀ for all steps (derivation.rs:538-551 for every t),
-// FP32 MFMA 32x32x2, K = padded sensor count. Output eT is (n, T).
+// FP32 MFMA 32x32x2, K = the sensors, S rounded up to even (the padded rows
+// of H and columns of R, all zero, are not multiplied). Output eT is (n, T).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void map_residuals_kernel(const float *__restrict__ H,
                                                             const float *__restrict__ R,
                                                             float *__restrict__ eT, int64_t n,
-                                                            int32_t T, int32_t Sp) {
+                                                            int32_t T, int32_t Se, int32_t Sp) {
     // tile: 128 states x kMapTile steps; k (sensors) in chunks of 32, k-major
     // LDS tiles (rows padded by 4 floats)
     __shared__ float Hs[kGemmTileK][128 + 4];       // H[s][i]
@@ -702,8 +726,8 @@ __global__ __launch_bounds__(256) void map_residuals_kernel(const float *__restr
             rv[pass] = v;
         }
     };
-    if (Sp > 0) load_tile(0);
-    for (int32_t sb = 0; sb < Sp; sb += kGemmTileK) {
+    if (Se > 0) load_tile(0);
+    for (int32_t sb = 0; sb < Se; sb += kGemmTileK) {
 #pragma unroll
         for (int pass = 0; pass < 4; ++pass)
 #pragma unroll
@@ -713,9 +737,11 @@ __global__ __launch_bounds__(256) void map_residuals_kernel(const float *__restr
 #pragma unroll
             for (int j = 0; j < 4; ++j) Rs[rq + j][pass * 32 + rrow] = rv[pass][j];
         __syncthreads();
-        if (sb + kGemmTileK < Sp) load_tile(sb + kGemmTileK);
+        if (sb + kGemmTileK < Se) load_tile(sb + kGemmTileK);
+        const int kend = min(kGemmTileK, Se - sb);  // even: the k pairs of one MFMA
 #pragma unroll
         for (int kk = 0; kk < kGemmTileK; kk += 2) {
+            if (kk >= kend) break;
             const int kr = kk + (lane >> 5);
             const float a = Hs[kr][wave * 32 + (lane & 31)];
 #pragma unroll
@@ -1555,9 +1581,10 @@ hipError_t launch_propagate_batch(hipStream_t s, int32_t slots, const ResidentJo
 }
 
 hipError_t launch_project(hipStream_t s, const float *x, const float *H, float *partial,
-                          float *xT, int32_t T, int64_t n, int32_t Sp, int32_t splits,
+                          float *xT, int32_t T, int64_t n, int32_t S, int32_t Sp, int32_t splits,
                           int64_t kps) {
-    dim3 grid(blocks_for(T, kProjTile), splits, Sp / 128);
+    if (S <= 0 || S > Sp) return hipErrorInvalidValue;
+    dim3 grid(blocks_for(T, kProjTile), splits, project_sensor_tiles(S));
     project_kernel<<<grid, 256, 0, s>>>(x, H, partial, xT, T, n, Sp, kps);
     return hipGetLastError();
 }
@@ -1597,10 +1624,11 @@ hipError_t launch_transpose(hipStream_t s, const float *in, float *out, int64_t 
 }
 
 hipError_t launch_map_residuals(hipStream_t s, const float *H, const float *R, float *eT,
-                                int64_t n, int32_t T, int32_t Sp) {
+                                int64_t n, int32_t T, int32_t S, int32_t Sp) {
     if (n <= 0) return hipSuccess;
+    if (S <= 0 || S > Sp) return hipErrorInvalidValue;
     dim3 grid(blocks_for(n, 128), blocks_for(T, kMapTile));
-    map_residuals_kernel<<<grid, 256, 0, s>>>(H, R, eT, n, T, Sp);
+    map_residuals_kernel<<<grid, 256, 0, s>>>(H, R, eT, n, T, (S + 1) / 2 * 2, Sp);
     return hipGetLastError();
 }
 
