@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libcardiotrust_hip.so")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in ("kernels.hip", "adjoint.hip", "gradcheck.hip",
-                                                  "api.cpp", "batch.cpp")]
+                                                  "api.cpp", "batch.cpp", "tiles.cpp")]
 ARCH = os.environ.get("CT_OFFLOAD_ARCH", "gfx950")
 
 

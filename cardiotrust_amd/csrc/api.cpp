@@ -9,6 +9,7 @@
 #include <mutex>
 
 #include "runtime.h"
+#include "tiles.h"
 
 namespace ctrt {
 
@@ -219,8 +220,12 @@ int derive_reference(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, ct_
     p.freeze_gains = c.freeze_gains;
     p.freeze_delays = c.freeze_delays;
     p.simple = c.ap_derivative == 1;
-    CT_HIP(ct::launch_derive(s, r.nbr, r.coefs, r.delays, r.initial_delays, r.gT, r.xT, r.eT,
-                             r.fir, r.iir, r.dGT, r.dA, r.N, r.T, p));
+    if (r.tiles.tiles > 0 && r.derive_mode != 1)
+        CT_HIP(ct::launch_derive_tiled(s, r.tiles, r.nbr, r.coefs, r.delays, r.initial_delays,
+                                       r.gT, r.xT, r.eT, r.fir, r.iir, r.dGT, r.dA, r.N, r.T, p));
+    else
+        CT_HIP(ct::launch_derive(s, r.nbr, r.coefs, r.delays, r.initial_delays, r.gT, r.xT,
+                                 r.eT, r.fir, r.iir, r.dGT, r.dA, r.N, r.T, p));
     return CT_OK;
 }
 
@@ -355,6 +360,34 @@ void end_epoch(ct_epoch_kernel &k) {
 // problem: at most kResMaxVoxels voxels per workgroup with every workgroup
 // resident on the device at once, neighbour indices below 2^17 - 1, delays
 // and step offsets that fit its 15-bit / 32-bit fields.
+// Voxel tiles of the tiled derivation, built once per problem from the host
+// neighbour table. Without them (no voxels, or the kernels' LDS limit cannot
+// be set) the derivation takes a lane per connection.
+int plan_tiles(ct_results_gpu &r) {
+    static std::once_flag prepared;
+    static hipError_t prepare_status = hipSuccess;
+    std::call_once(prepared, [] { prepare_status = ct::derive_tiled_prepare(); });
+    if (r.N <= 0 || prepare_status != hipSuccess) return CT_OK;
+    ctrt::VoxelTiles t;
+    ctrt::build_voxel_tiles(r.nbr_host, r.N, ct::kTileMaxVoxels, ct::kTileMaxHalo, ct::kNoSlot, t);
+    const int32_t G = t.tiles();
+    if (G <= 0) return CT_OK;
+    int32_t *vox_ptr = nullptr, *vox = nullptr, *halo_ptr = nullptr, *halo = nullptr;
+    uint8_t *slot = nullptr;
+    CT_CHECK(r.alloc(&vox_ptr, G + 1));
+    CT_CHECK(r.alloc(&vox, int64_t(t.vox.size())));
+    CT_CHECK(r.alloc(&halo_ptr, G + 1));
+    CT_CHECK(r.alloc(&halo, int64_t(t.halo.size())));
+    CT_CHECK(r.alloc(&slot, int64_t(t.slot.size())));
+    CT_CHECK(upload(vox_ptr, t.vox_ptr.data(), sizeof(int32_t) * t.vox_ptr.size()));
+    CT_CHECK(upload(vox, t.vox.data(), sizeof(int32_t) * t.vox.size()));
+    CT_CHECK(upload(halo_ptr, t.halo_ptr.data(), sizeof(int32_t) * t.halo_ptr.size()));
+    CT_CHECK(upload(halo, t.halo.data(), sizeof(int32_t) * t.halo.size()));
+    CT_CHECK(upload(slot, t.slot.data(), t.slot.size()));
+    r.tiles = ct::DeriveTiles{vox_ptr, vox, halo_ptr, halo, slot, G};
+    return CT_OK;
+}
+
 int plan_resident(ct_results_gpu &r) {
     r.resident_ok = false;
     r.resident_blocks = 0;
@@ -927,6 +960,7 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
             return cleanup(fail(CT_ERR_DEVICE, "upload failed"));
     }
     if ((rc = plan_resident(*r)) != CT_OK) return cleanup(rc);
+    if ((rc = plan_tiles(*r)) != CT_OK) return cleanup(rc);
     *out = r;
     return CT_OK;
 }
@@ -1315,6 +1349,16 @@ int ct_results_set_propagation(ct_results_gpu *results, int32_t mode, int32_t *r
         return fail(CT_ERR_INVALID, "the resident propagation kernel cannot hold this problem");
     results->propagation_mode = mode;
     if (resident) *resident = (results->resident_ok && mode != 1) ? 1 : 0;
+    return CT_OK;
+}
+
+int ct_results_set_derivation(ct_results_gpu *results, int32_t mode, int32_t *tiled) {
+    if (!results) return fail(CT_ERR_INVALID, "null argument");
+    if (mode < 0 || mode > 2) return fail(CT_ERR_INVALID, "derivation mode must be 0, 1 or 2");
+    if (mode == 2 && results->tiles.tiles <= 0 && results->N > 0)
+        return fail(CT_ERR_INVALID, "no voxel tiles were built for this problem");
+    results->derive_mode = mode;
+    if (tiled) *tiled = (results->tiles.tiles > 0 && mode != 1) ? 1 : 0;
     return CT_OK;
 }
 

@@ -121,6 +121,37 @@ hipError_t launch_derive(hipStream_t s, const int32_t *nbr, const float *coefs,
                          const float *xT, const float *eT, float *fir, float *iir, float *dGT,
                          float *dA, int32_t N, int32_t T, const DeriveParams &p);
 
+// The same derivation over voxel tiles: a workgroup takes the connections of
+// one tile (at most kTileMaxVoxels voxels) and stages the state rows of the
+// tile's halo (its voxels and all their neighbours, at most kTileMaxHalo) in
+// an LDS ring of kTileRing steps, so each row is read from memory once per
+// tile instead of once per connection that reads it. Connections with a
+// delay above kTileMaxDelay read their rows from memory. Tables (host-built
+// by build_voxel_tiles): tile g owns the tile-order positions
+// [vox_ptr[g], vox_ptr[g+1]) holding voxels vox[.], and halo voxels
+// halo[halo_ptr[g] .. halo_ptr[g+1]); slot[26 p + d] is the halo index of the
+// neighbour of connection d of the voxel at position p (kNoSlot if missing).
+constexpr int kTileThreads = 1024;
+constexpr int kTileMaxVoxels = kTileThreads / 26;  // 39
+constexpr int kTileMaxHalo = 152;
+constexpr uint8_t kNoSlot = 0xFF;
+struct DeriveTiles {
+    const int32_t *vox_ptr;
+    const int32_t *vox;
+    const int32_t *halo_ptr;
+    const int32_t *halo;
+    const uint8_t *slot;
+    int32_t tiles;
+};
+size_t derive_tiled_lds_bytes();
+// Sets the tiled kernels' LDS limit; once per process before their first launch.
+hipError_t derive_tiled_prepare();
+hipError_t launch_derive_tiled(hipStream_t s, const DeriveTiles &tiles, const int32_t *nbr,
+                               const float *coefs, const int32_t *delays,
+                               const float *initial_delays, const float *gT, const float *xT,
+                               const float *eT, float *fir, float *iir, float *dGT, float *dA,
+                               int32_t N, int32_t T, const DeriveParams &p);
+
 // Reverse-mode gradient (adjoint.hip).
 struct AdjointParams {
     float mse_scaling;

@@ -1022,6 +1022,218 @@ __global__ __launch_bounds__(kDeriveThreads, 4) void derive_kernel(
     dA[item] = dA[item] + fmaf(dot, p.mse_scaling, float(9 * combined) * reg);
 }
 
+// ---------------------------------------------------------------------------
+// Derivatives over voxel tiles (the same derivation.rs references as
+// derive_kernel). Every connection runs derive_kernel's arithmetic step for
+// step; only the source of its delayed inputs differs. Per chunk of
+// kDeriveChunk steps the workgroup copies that chunk of each of its halo rows
+// (three state rows per halo voxel) into an LDS ring holding the last
+// kTileRing steps, and each lane reads its neighbour's inputs x[t - D] from
+// there. Step t of a row sits at position t mod kTileRing; the chunk at
+// positions [0, kDeriveChunk) is also written behind the ring, so the inputs
+// of a lane's chunk, which start anywhere in the ring, are contiguous. The
+// loads of chunk c + 1 are in flight while the steps of chunk c run.
+// ---------------------------------------------------------------------------
+constexpr int kTileRing = 64;
+constexpr int kTileRowStride = kTileRing + kDeriveChunk;  // 80 floats per row
+constexpr int kTileMaxDelay = kTileRing - kDeriveChunk;   // the ring holds chunks c-3 .. c
+constexpr int kTileRingFloats = kTileMaxHalo * 3 * kTileRowStride;
+constexpr int kTileStageFloats = kTileMaxVoxels * 3 * kDeriveChunk;
+constexpr int kTileQuads = kTileMaxHalo * 3 * (kDeriveChunk / 4);  // 16-byte loads per chunk
+constexpr int kTileQuadsPerThread = (kTileQuads + kTileThreads - 1) / kTileThreads;
+static_assert(kTileMaxVoxels * kDeriveChunk <= kTileThreads, "one staging item per thread");
+static_assert(kTileRing % kDeriveChunk == 0 && (kTileRing & (kTileRing - 1)) == 0,
+              "chunks fill ring positions without wrapping");
+static_assert(kTileMaxHalo < kNoSlot, "halo indices fit the slot bytes");
+
+constexpr size_t kTileLdsBytes =
+    sizeof(float) * (size_t(kTileRingFloats) + 3 * kTileRowStride + 2 * size_t(kTileStageFloats));
+
+typedef float float4_a16 __attribute__((ext_vector_type(4), aligned(16)));
+
+template <bool kGains, bool kCoefs, bool kSimple>
+__global__ __launch_bounds__(kTileThreads) void derive_tiled_kernel(
+    DeriveTiles tl, const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
+    const int32_t *__restrict__ delays, const float *__restrict__ initial_delays,
+    const float *__restrict__ gT, const float *__restrict__ xT, const float *__restrict__ eT,
+    float *__restrict__ fir, float *__restrict__ iir, float *__restrict__ dGT,
+    float *__restrict__ dA, int32_t N, int32_t T, DeriveParams p) {
+    extern __shared__ __align__(16) float lds[];
+    float *ring = lds;                           // [halo][3][kTileRowStride]
+    float *zeros = ring + kTileRingFloats;       // [3][kTileRowStride], +0: missing connections
+    float *es = zeros + 3 * kTileRowStride;      // [voxel][3][chunk] mapped residuals
+    float *cs = es + kTileStageFloats;           // [voxel][3][chunk] gain-derivative factors
+    const int64_t n = 3LL * N;
+    const int32_t tile = xcd_block(blockIdx.x, gridDim.x);
+    const int32_t pbeg = tl.vox_ptr[tile], nvox = tl.vox_ptr[tile + 1] - pbeg;
+    const int32_t hbeg = tl.halo_ptr[tile], nhalo = tl.halo_ptr[tile + 1] - hbeg;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 3 * kTileRowStride; i += kTileThreads) zeros[i] = 0.0f;
+
+    // the lane's connection: voxel lv of the tile, direction d
+    const int lv = tid / 26, d = tid - 26 * lv;
+    const bool active = lv < nvox;
+    const int32_t v = active ? tl.vox[pbeg + lv] : 0;
+    const int64_t item = int64_t(v) * 26 + d;
+    const int32_t w = active ? nbr[item] : -1;
+    const bool has = w >= 0;
+    const float a = active ? coefs[item] : 0.0f;
+    const int32_t D = active ? delays[item] : 0;
+    const bool far = has && D > kTileMaxDelay;  // beyond the ring: reads its rows from memory
+    const int slot = (has && !far) ? int(tl.slot[int64_t(pbeg + lv) * 26 + d]) : -1;
+    const float *xw = xT + (has ? 3LL * w : 0) * T;
+    const float *rrow = slot >= 0 ? ring + slot * 3 * kTileRowStride : zeros;
+
+    ConnState c;
+#pragma unroll
+    for (int in = 0; in < 3; ++in)
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            c.g[in][o] = active ? dGT[int64_t(3 * d + o) * n + 3LL * v + in] : 0.0f;
+            c.M[in][o] = 0.0f;
+        }
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+        c.f[o] = active ? fir[item * 3 + o] : 0.0f;
+        c.h[o] = active ? iir[item * 3 + o] : 0.0f;
+        c.y[o] = 0.0f;
+        c.prev_in[o] = 0.0f;
+    }
+
+    // this thread's share of the chunk copies: 16-byte quads of halo rows
+    // (row r = 3 h + o holds state o of halo voxel h), and one (voxel, step)
+    // item of the tile's own residuals and states
+    int64_t qsrc[kTileQuadsPerThread];
+    int32_t qdst[kTileQuadsPerThread];
+#pragma unroll
+    for (int k = 0; k < kTileQuadsPerThread; ++k) {
+        const int qi = tid + k * kTileThreads;
+        qsrc[k] = -1;
+        qdst[k] = 0;
+        if (qi < nhalo * 3 * (kDeriveChunk / 4)) {
+            const int r = qi / (kDeriveChunk / 4), q4 = 4 * (qi % (kDeriveChunk / 4));
+            const int h = r / 3, o = r - 3 * h;
+            qsrc[k] = (3LL * tl.halo[hbeg + h] + o) * T + q4;
+            qdst[k] = r * kTileRowStride + q4;
+        }
+    }
+    const int sv = tid / kDeriveChunk, sj = tid - kDeriveChunk * sv;
+    const bool sown = sv < nvox;
+    const int64_t srow = sown ? 3LL * tl.vox[pbeg + sv] * T + sj : 0;
+
+    float4_a4 quad[kTileQuadsPerThread];
+    float se[3], sx[3];
+    const auto fetch = [&](int32_t t0) {
+        const bool whole = t0 + kDeriveChunk <= T;
+#pragma unroll
+        for (int k = 0; k < kTileQuadsPerThread; ++k) {
+            if (qsrc[k] < 0) continue;
+            if (whole) {
+                quad[k] = *reinterpret_cast<const float4_a4 *>(xT + qsrc[k] + t0);
+            } else {
+                const int32_t tq = t0 + int32_t(qdst[k] % kTileRowStride);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) quad[k][e] = tq + e < T ? xT[qsrc[k] + t0 + e] : 0.0f;
+            }
+        }
+        const bool live = sown && t0 + sj < T;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            se[q] = live ? eT[srow + int64_t(q) * T + t0] : 0.0f;
+            sx[q] = live ? xT[srow + int64_t(q) * T + t0] : 0.0f;
+        }
+    };
+    const auto stage = [&](int32_t t0) {
+        const int pos = t0 & (kTileRing - 1);
+#pragma unroll
+        for (int k = 0; k < kTileQuadsPerThread; ++k) {
+            if (qsrc[k] < 0) continue;
+            *reinterpret_cast<float4_a16 *>(ring + qdst[k] + pos) = quad[k];
+            if (pos == 0) *reinterpret_cast<float4_a16 *>(ring + qdst[k] + kTileRing) = quad[k];
+        }
+        if (sown) {
+            float cg[3];
+            gain_factors(sx, se, p, cg);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                es[(sv * 3 + q) * kDeriveChunk + sj] = se[q];
+                cs[(sv * 3 + q) * kDeriveChunk + sj] = cg[q];
+            }
+        }
+    };
+
+    fetch(0);
+    for (int32_t t0 = 0; t0 < T; t0 += kDeriveChunk) {
+        stage(t0);
+        __syncthreads();
+        if (t0 + kDeriveChunk < T) fetch(t0 + kDeriveChunk);
+        // the lane's 16 inputs of this chunk: rb[o * kTileRowStride + j] = x[t0 + j - D]
+        const float *rb = rrow + ((t0 - D) & (kTileRing - 1));
+        const float *el = es + lv * 3 * kDeriveChunk;
+        const float *cl = cs + lv * 3 * kDeriveChunk;
+        const bool steady = __all(!active || t0 >= D) && t0 + kDeriveChunk <= T;
+        const bool any_far = __any(far);
+        if (active && steady && !any_far) {
+#pragma unroll
+            for (int m = 0; m < kDeriveChunk; m += 2) {
+                float2_a8 ev[3], cv[3];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    ev[q] = *reinterpret_cast<const float2_a8 *>(el + q * kDeriveChunk + m);
+                    cv[q] = *reinterpret_cast<const float2_a8 *>(cl + q * kDeriveChunk + m);
+                }
+#pragma unroll
+                for (int jj = 0; jj < 2; ++jj) {
+                    const int j = m + jj;
+                    const float in0[3] = {rb[j], rb[kTileRowStride + j], rb[2 * kTileRowStride + j]};
+                    const float e[3] = {ev[0][jj], ev[1][jj], ev[2][jj]};
+                    const float cg[3] = {cv[0][jj], cv[1][jj], cv[2][jj]};
+                    derive_step<true, kGains, kCoefs, kSimple>(c, in0, e, cg, has, a, D, t0 + j);
+                }
+            }
+        } else if (active) {
+#pragma unroll
+            for (int j = 0; j < kDeriveChunk; ++j) {
+                const int32_t t = t0 + j;
+                if (t < T) {
+                    const int32_t s = t - D;
+                    float in0[3];
+#pragma unroll
+                    for (int o = 0; o < 3; ++o)
+                        in0[o] = s < 0 ? 0.0f
+                                       : (far ? xw[int64_t(o) * T + s] : rb[o * kTileRowStride + j]);
+                    const float e[3] = {el[j], el[kDeriveChunk + j], el[2 * kDeriveChunk + j]};
+                    const float cg[3] = {cl[j], cl[kDeriveChunk + j], cl[2 * kDeriveChunk + j]};
+                    derive_step<false, kGains, kCoefs, kSimple>(c, in0, e, cg, has, a, D, t);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+
+#pragma unroll
+    for (int in = 0; in < 3; ++in)
+#pragma unroll
+        for (int o = 0; o < 3; ++o) dGT[int64_t(3 * d + o) * n + 3LL * v + in] = c.g[in][o];
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+        fir[item * 3 + o] = c.f[o];
+        iir[item * 3 + o] = c.h[o];
+    }
+    if (!kCoefs) return;
+    float dot = 0.0f;
+#pragma unroll
+    for (int in = 0; in < 3; ++in)
+#pragma unroll
+        for (int o = 0; o < 3; ++o)
+            dot = fmaf(gT[int64_t(3 * d + o) * n + 3LL * v + in], c.M[in][o], dot);
+    const float delay = float(D) + (1.0f - a) / (a + 1.0f);
+    const float reg = p.diff_strength * powi5(initial_delays[item] - delay);
+    const int32_t combined = kSimple ? (has ? max(T - D, 0) : 0) : T;
+    dA[item] = dA[item] + fmaf(dot, p.mse_scaling, float(9 * combined) * reg);
+}
+
 // update.rs:106-114, 158-168
 __device__ __forceinline__ bool guarded(const UpdateGuard &g) {
     const auto word = [](const uint32_t *w) {
@@ -1649,6 +1861,44 @@ hipError_t launch_derive(hipStream_t s, const int32_t *nbr, const float *coefs,
     else if (c) CT_DERIVE(false, true, true);
     else CT_DERIVE(false, false, false);
 #undef CT_DERIVE
+    return hipGetLastError();
+}
+
+#define CT_DERIVE_VARIANTS(X) \
+    X(true, true, false) X(true, true, true) X(true, false, false) X(false, true, false) \
+    X(false, true, true) X(false, false, false)
+
+size_t derive_tiled_lds_bytes() { return kTileLdsBytes; }
+
+hipError_t derive_tiled_prepare() {
+    hipError_t e = hipSuccess;
+#define CT_TILED_ATTR(G, C, S)                                                               \
+    if (e == hipSuccess)                                                                     \
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(&derive_tiled_kernel<G, C, S>), \
+                                hipFuncAttributeMaxDynamicSharedMemorySize,                  \
+                                int(derive_tiled_lds_bytes()));
+    CT_DERIVE_VARIANTS(CT_TILED_ATTR)
+#undef CT_TILED_ATTR
+    return e;
+}
+
+hipError_t launch_derive_tiled(hipStream_t s, const DeriveTiles &tiles, const int32_t *nbr,
+                               const float *coefs, const int32_t *delays,
+                               const float *initial_delays, const float *gT, const float *xT,
+                               const float *eT, float *fir, float *iir, float *dGT, float *dA,
+                               int32_t N, int32_t T, const DeriveParams &p) {
+    if (N <= 0 || tiles.tiles <= 0) return hipSuccess;
+#define CT_DERIVE_TILED(G, C, S)                                                              \
+    derive_tiled_kernel<G, C, S><<<tiles.tiles, kTileThreads, derive_tiled_lds_bytes(), s>>>( \
+        tiles, nbr, coefs, delays, initial_delays, gT, xT, eT, fir, iir, dGT, dA, N, T, p)
+    const bool g = !p.freeze_gains, c = !p.freeze_delays;
+    if (g && c && !p.simple) CT_DERIVE_TILED(true, true, false);
+    else if (g && c) CT_DERIVE_TILED(true, true, true);
+    else if (g) CT_DERIVE_TILED(true, false, false);
+    else if (c && !p.simple) CT_DERIVE_TILED(false, true, false);
+    else if (c) CT_DERIVE_TILED(false, true, true);
+    else CT_DERIVE_TILED(false, false, false);
+#undef CT_DERIVE_TILED
     return hipGetLastError();
 }
 

@@ -115,6 +115,10 @@ struct ct_results_gpu {
     int32_t propagation_mode = 0;  // 0 auto, 1 per-step launches, 2 resident
     double wait_seconds = 2.0;     // bound on every resident wait (ct_results_set_wait_limit)
     bool xT_fresh = false;         // xT holds the current states (written by the projection)
+    // derivation path: over voxel tiles (derive_tiled_kernel) when their
+    // tables were built, else a lane per connection (derive_kernel)
+    int32_t derive_mode = 0;  // 0 automatic, 1 a lane per connection, 2 tiled
+    ct::DeriveTiles tiles{};  // device tables; tiles.tiles == 0 when not built
     bool resident_ok = false;
     bool resident_used = false;  // a resident launch is in flight since the last check
     int32_t resident_blocks = 0;

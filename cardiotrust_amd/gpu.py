@@ -27,7 +27,7 @@ EXPORTS = (
     "ct_measurement_matrix", "ct_model_connections", "ct_results_spherical",
     "ct_results_calculate_final_metrics", "ct_results_calculate_final_metrics_noalloc",
     "ct_epoch_kernel_last_loss", "ct_results_set_wait_limit",
-    "ct_results_set_propagation",
+    "ct_results_set_propagation", "ct_results_set_derivation",
     "ct_results_forward", "ct_results_backward", "ct_epoch_kernel_set_gradient",
     "ct_epoch_kernel_new",
     "ct_epoch_kernel_free", "ct_epoch_kernel_execute", "ct_epoch_kernel_set_freeze_delays",
@@ -135,6 +135,7 @@ def lib() -> ctypes.CDLL:
         "ct_model_connections": ([vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp, vp, vp,
                                   ctypes.c_float, ctypes.c_int32, vp, vp], ctypes.c_int),
         "ct_results_set_propagation": ([vp, i32, P(i32)], ctypes.c_int),
+        "ct_results_set_derivation": ([vp, i32, P(i32)], ctypes.c_int),
         "ct_results_forward": ([vp, vp, i32, P(CAlgorithm), i32, P(ctypes.c_double)],
                                ctypes.c_int),
         "ct_results_backward": ([vp, vp, i32, P(CAlgorithm), i32, i32], ctypes.c_int),
@@ -289,6 +290,14 @@ class ResultsGPU:
         res = ctypes.c_int32()
         _check(lib().ct_results_set_propagation(self.handle, int(mode), ctypes.byref(res)),
                "set_propagation")
+        return bool(res.value)
+
+    def set_derivation(self, mode: int) -> bool:
+        """0 automatic, 1 a lane per connection, 2 over voxel tiles; returns
+        whether the tiled kernel will run."""
+        res = ctypes.c_int32()
+        _check(lib().ct_results_set_derivation(self.handle, int(mode), ctypes.byref(res)),
+               "set_derivation")
         return bool(res.value)
 
     def objective_f64(self, actual: "Buffer", beat: int, config: Algorithm, param: int = 0,

@@ -250,6 +250,16 @@ int ct_results_simulate_noisy(ct_results_gpu *results, const float *sensor_std, 
  * `resident` (may be NULL) receives 1 if the resident path will run. */
 int ct_results_set_propagation(ct_results_gpu *results, int32_t mode, int32_t *resident);
 
+/* Derivation path (no reference counterpart; a tuning switch of this
+ * library): 0 = automatic (over voxel tiles when their tables were built at
+ * set-up), 1 = a lane per connection reading its neighbour's states from
+ * memory, 2 = over voxel tiles (a workgroup per spatially compact tile stages
+ * the states of the tile and its neighbours once and its connections read
+ * them from LDS; CT_ERR_INVALID if no tiles were built). Both paths compute
+ * the same values in the same order, bit for bit.
+ * `tiled` (may be NULL) receives 1 if the tiled path will run. */
+int ct_results_set_derivation(ct_results_gpu *results, int32_t mode, int32_t *tiled);
+
 /* Bound on every wait of the resident kernels, in seconds of wall clock
  * (default 2; no reference counterpart: a diagnostic switch of this library).
  * A wait that exceeds it sets the launch's sticky abort word, every workgroup
