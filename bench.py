@@ -144,10 +144,13 @@ def cpu_baseline(problem, measurements, sample_steps: int, cfg0_problem, gpu_cfg
     }
 
 
+DERIVATION = {"auto": 0, "lanes": 1, "tiles": 2}
+
+
 def run_batched(gpu, problems, label: str, adjoint: bool, warmup: int, steps: int,
-                checkpoint: int = 64, share: str = "auto", vpg: int = 0, graph: bool = True,
+                checkpoint: int = 16, share: str = "auto", vpg: int = 0, graph: bool = True,
                 plain_epochs_per_s: float = 0.0, trajectory: bool = False,
-                gradient_check: int = 0) -> dict:
+                gradient_check: int = 0, derivation: str = "auto") -> dict:
     """Many scenarios per epoch through ct_batch (one persistent propagation
     launch per round of beats, autotuned voxels per workgroup, hipGraph
     replay), timed beside the same scenarios run one after another. Returns
@@ -167,6 +170,7 @@ def run_batched(gpu, problems, label: str, adjoint: bool, warmup: int, steps: in
     members = []
     for p in problems:
         r = ResultsGPU(gpu, p, num_batch_entries=epochs + 1)
+        r.set_derivation(DERIVATION[derivation])
         b = Buffer(gpu, p.measurements)
         k = EpochKernel(gpu, r, b, cfg, 3 * p.num_voxels, p.num_sensors, p.num_steps)
         if adjoint:
@@ -343,7 +347,7 @@ def bench_batched(args) -> int:
     res = run_batched(gpu, problems, label, args.workload == "batched-adjoint", args.warmup,
                       args.steps, args.checkpoint, args.share_propagation,
                       args.voxels_per_group, not args.no_graph,
-                      trajectory=args.workload == "batched-adjoint")
+                      trajectory=args.workload == "batched-adjoint", derivation=args.derivation)
     out = {
         "metric": "batched epochs/s (all scenarios advance one epoch)",
         "value": res["epochs_per_s"], "unit": "epochs/s", "n_gpus": 1, "steps": args.steps,
@@ -397,7 +401,11 @@ def main() -> int:
     ap.add_argument("--gradient", choices=["reference", "adjoint"], default="reference",
                     help="reference: the reference's per-step derivation; adjoint: exact "
                          "reverse-mode gradient with checkpoints")
-    ap.add_argument("--checkpoint", type=int, default=64, help="adjoint checkpoint interval")
+    ap.add_argument("--checkpoint", type=int, default=16,
+                    help="adjoint checkpoint interval (16: the gradient pass runs over voxel tiles)")
+    ap.add_argument("--derivation", choices=["auto", "lanes", "tiles"], default="auto",
+                    help="derivation / gradient pass: over voxel tiles when built (auto), a lane "
+                         "per connection, or tiles")
     ap.add_argument("--workload", choices=["plain", "batched-uneven", "batched-adjoint", "copy"],
                     default="plain",
                     help="plain: the 120k-voxel epoch (the headline line); batched-uneven: 64 "
@@ -447,6 +455,8 @@ def main() -> int:
     mode = {"auto": 0, "per-step": 1, "resident": 2}[args.propagation]
     resident = results.set_propagation(mode)
     log(f"propagation path: {'resident' if resident else 'per-step'}")
+    tiled = results.set_derivation(DERIVATION[args.derivation])
+    log(f"derivation: {'voxel tiles' if tiled else 'a lane per connection'}")
     actual = Buffer(gpu, measurements)
     kernel = EpochKernel(gpu, results, actual, cfg, 3 * N, S, T)
     if args.gradient == "adjoint":
@@ -584,7 +594,7 @@ def main() -> int:
             log(f"{workload}: {len(problems)} scenarios built ({time.perf_counter() - t1:.1f} s)")
             batched[workload.split("-")[1]] = run_batched(
                 gpu, problems, label, workload == "batched-adjoint", warm, timed,
-                args.checkpoint, plain_epochs_per_s=plain, **extra)
+                args.checkpoint, plain_epochs_per_s=plain, derivation=args.derivation, **extra)
             del problems
             log(f"{workload}: done ({time.perf_counter() - t1:.1f} s)")
         out["batched"] = batched

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "barrier.h"
 #include "device.h"
@@ -379,6 +380,14 @@ __device__ __forceinline__ void load_rows16(const float *src, int64_t stride,
 }
 constexpr int kGradMaxVoxels = kGradThreads / 26 + 2;
 
+// One backward step of a connection's filter adjoint in the gradient pass:
+// ybar_t[o] = sum_i G[i][o] xbar_t[i] - a ybar_{t+1}[o], as fused
+// multiply-adds (both gradient kernels use this, so they agree bit for bit).
+__device__ __forceinline__ float ybar_step(const float G[3][3], const float xb[3], float a,
+                                           float ybn, int o) {
+    return fmaf(-a, ybn, fmaf(G[2][o], xb[2], fmaf(G[1][o], xb[1], G[0][o] * xb[0])));
+}
+
 __global__ __launch_bounds__(kGradThreads) void adjoint_gradient_kernel(
     const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
     const int32_t *__restrict__ delays, const float *__restrict__ gT,
@@ -503,18 +512,248 @@ __global__ __launch_bounds__(kGradThreads) void adjoint_gradient_kernel(
                     const float xb[3] = {xs[vl][0][j], xs[vl][1][j], xs[vl][2][j]};
 #pragma unroll
                     for (int o = 0; o < 3; ++o) {
-                        const float yb =
-                            G[0][o] * xb[0] + G[1][o] * xb[1] + G[2][o] * xb[2] - a * ybn[o];
+                        const float yb = ybar_step(G, xb, a, ybn[o], o);
                         const float ym = j > 0 ? yv[o][j - 1] : y0[o];
 #pragma unroll
-                        for (int in = 0; in < 3; ++in) g[in][o] += xb[in] * yv[o][j];
-                        A += yb * (xi[o][j] - ym);
+                        for (int in = 0; in < 3; ++in) g[in][o] = fmaf(xb[in], yv[o][j], g[in][o]);
+                        A = fmaf(yb, xi[o][j] - ym, A);
                         ybn[o] = yb;
                     }
                 }
             }
             __syncthreads();
         }
+    }
+    if (!active || !has) return;  // no connection: no dependence on these parameters
+#pragma unroll
+    for (int in = 0; in < 3; ++in)
+#pragma unroll
+        for (int o = 0; o < 3; ++o) dGT[int64_t(3 * d + o) * n + 3LL * v + in] += g[in][o];
+    dA[item] += A;
+}
+
+// Gradient pass over voxel tiles, for checkpoints every kTileChunk steps
+// (every segment one chunk, so no segment is replayed to find its chunk
+// starts). A workgroup takes the connections of one tile (derive_tiled_kernel's
+// tables) and walks the chunks from the last to the first; the delayed
+// inputs x_w[t - D] of its lanes come from an LDS ring of the tile's halo
+// rows that holds chunks c-3 .. c while chunk c runs, filled going backwards.
+// Each lane's arithmetic is adjoint_gradient_kernel's at K = kTileChunk,
+// operation for operation, so the two give the same bits.
+constexpr int kGradTileMaxDelay = kTileRing - kTileChunk - 1;  // the chunk's start input too
+constexpr int kGradRingFloats = kTileMaxHalo * 3 * kTileRowStride;
+constexpr int kGradStageFloats = kTileMaxVoxels * 3 * kTileChunk;
+constexpr int kGradQuads = kTileMaxHalo * 3 * (kTileChunk / 4);
+constexpr int kGradQuadsPerThread = (kGradQuads + kTileThreads - 1) / kTileThreads;
+constexpr size_t kGradTiledLdsBytes =
+    sizeof(float) * (size_t(kGradRingFloats) + 3 * kTileRowStride + size_t(kGradStageFloats));
+static_assert(kGradChunk == kTileChunk, "one gradient chunk per ring chunk");
+static_assert(kTileMaxVoxels * kTileChunk <= kTileThreads, "one staging item per thread");
+
+typedef float float4_a16 __attribute__((ext_vector_type(4), aligned(16)));
+
+__global__ __launch_bounds__(kTileThreads) void adjoint_gradient_tiled_kernel(
+    DeriveTiles tl, const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
+    const int32_t *__restrict__ delays, const float *__restrict__ gT,
+    const float *__restrict__ xT, const float *__restrict__ xbarT, const float *__restrict__ ckpt,
+    float *__restrict__ dGT, float *__restrict__ dA, int32_t N, int32_t T) {
+    extern __shared__ __align__(16) float lds[];
+    float *ring = lds;                         // [halo][3][kTileRowStride]
+    float *zeros = ring + kGradRingFloats;     // [3][kTileRowStride], +0: missing connections
+    float *xs = zeros + 3 * kTileRowStride;    // [voxel][3][chunk] adjoint states of the tile
+    const int64_t n = 3LL * N;
+    const int64_t M = int64_t(N) * 78;
+    const int32_t tile = xcd_block(blockIdx.x, gridDim.x);
+    const int32_t pbeg = tl.vox_ptr[tile], nvox = tl.vox_ptr[tile + 1] - pbeg;
+    const int32_t hbeg = tl.halo_ptr[tile], nhalo = tl.halo_ptr[tile + 1] - hbeg;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 3 * kTileRowStride; i += kTileThreads) zeros[i] = 0.0f;
+
+    const int lv = tid / 26, d = tid - 26 * lv;
+    const bool active = lv < nvox;
+    const int32_t v = active ? tl.vox[pbeg + lv] : 0;
+    const int64_t item = int64_t(v) * 26 + d;
+    const int32_t w = active ? nbr[item] : -1;
+    const bool has = w >= 0;
+    const float a = active ? coefs[item] : 0.0f;
+    const int32_t D = active ? delays[item] : 0;
+    const bool far = has && D > kGradTileMaxDelay;  // reads its rows from memory
+    const int slot = (has && !far) ? int(tl.slot[int64_t(pbeg + lv) * 26 + d]) : -1;
+    const float *xw = xT + (has ? 3LL * w : 0) * T;
+    const float *rrow = slot >= 0 ? ring + slot * 3 * kTileRowStride : zeros;
+
+    float G[3][3], g[3][3];
+#pragma unroll
+    for (int in = 0; in < 3; ++in)
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            G[in][o] = active ? gT[int64_t(3 * d + o) * n + 3LL * v + in] : 0.0f;
+            g[in][o] = 0.0f;
+        }
+    float A = 0.0f, ybn[3] = {0.0f, 0.0f, 0.0f};
+
+    // this thread's share of the copies: 16-byte quads of halo rows, and one
+    // (voxel, step) item of the tile's adjoint states
+    int64_t qsrc[kGradQuadsPerThread];
+    int32_t qdst[kGradQuadsPerThread];
+#pragma unroll
+    for (int k = 0; k < kGradQuadsPerThread; ++k) {
+        const int qi = tid + k * kTileThreads;
+        qsrc[k] = -1;
+        qdst[k] = 0;
+        if (qi < nhalo * 3 * (kTileChunk / 4)) {
+            const int r = qi / (kTileChunk / 4), q4 = 4 * (qi % (kTileChunk / 4));
+            const int h = r / 3, o = r - 3 * h;
+            qsrc[k] = (3LL * tl.halo[hbeg + h] + o) * T + q4;
+            qdst[k] = r * kTileRowStride + q4;
+        }
+    }
+    const int sv = tid / kTileChunk, sj = tid - kTileChunk * sv;
+    const bool sown = sv < nvox;
+    const int64_t srow = sown ? 3LL * tl.vox[pbeg + sv] * T + sj : 0;
+
+    float4_a4 quad[kGradQuadsPerThread];
+    float sb[3];
+    const auto fetch_rows = [&](int32_t cb) {  // chunk starting at step cb >= 0
+        const bool whole = cb + kTileChunk <= T;
+#pragma unroll
+        for (int k = 0; k < kGradQuadsPerThread; ++k) {
+            if (qsrc[k] < 0) continue;
+            if (whole) {
+                quad[k] = *reinterpret_cast<const float4_a4 *>(xT + qsrc[k] + cb);
+            } else {
+                const int32_t tq = cb + int32_t(qdst[k] % kTileRowStride);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) quad[k][e] = tq + e < T ? xT[qsrc[k] + cb + e] : 0.0f;
+            }
+        }
+    };
+    const auto stage_rows = [&](int32_t cb) {
+        const int pos = cb & (kTileRing - 1);
+#pragma unroll
+        for (int k = 0; k < kGradQuadsPerThread; ++k) {
+            if (qsrc[k] < 0) continue;
+            *reinterpret_cast<float4_a16 *>(ring + qdst[k] + pos) = quad[k];
+            if (pos == 0) *reinterpret_cast<float4_a16 *>(ring + qdst[k] + kTileRing) = quad[k];
+        }
+    };
+    const auto fetch_xbar = [&](int32_t cb) {
+        const bool live = sown && cb + sj < T;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) sb[q] = live ? xbarT[srow + int64_t(q) * T + cb] : 0.0f;
+    };
+    const auto stage_xbar = [&]() {
+        if (sown) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) xs[(sv * 3 + q) * kTileChunk + sj] = sb[q];
+        }
+    };
+
+    const int32_t nchunks = (T + kTileChunk - 1) / kTileChunk;
+    // chunks nchunks-1 .. nchunks-3 first; each iteration c then adds c-3
+    for (int32_t c = nchunks - 1; c >= max(nchunks - 3, 0); --c) {
+        fetch_rows(c * kTileChunk);
+        stage_rows(c * kTileChunk);
+    }
+    if (nchunks - 4 >= 0) fetch_rows((nchunks - 4) * kTileChunk);
+    fetch_xbar((nchunks - 1) * kTileChunk);
+    for (int32_t c = nchunks - 1; c >= 0; --c) {
+        const int32_t cbeg = c * kTileChunk, len = min(kTileChunk, T - cbeg);
+        if (c - 3 >= 0) stage_rows(cbeg - 3 * kTileChunk);
+        stage_xbar();
+        __syncthreads();
+        if (c - 4 >= 0) fetch_rows(cbeg - 4 * kTileChunk);
+        if (c - 1 >= 0) fetch_xbar(cbeg - kTileChunk);
+        // inputs: rb[o * kTileRowStride + j] = x_w[cbeg + j - D]; the chunk's
+        // start input x_w[cbeg - 1 - D] at its own ring position
+        const float *rb = rrow + ((cbeg - D) & (kTileRing - 1));
+        const float *rp = rrow + ((cbeg - 1 - D) & (kTileRing - 1));
+        const float *xl = xs + lv * 3 * kTileChunk;
+        // every lane of the wave in the ring, past its delay, and a whole chunk:
+        // no per-input checks
+        const bool fast = __all(!active || cbeg - 1 - D >= 0) && len == kTileChunk;
+        // kMode 0: no checks; 1: inputs before step 0 or past T read +0;
+        // 2: as 1, and lanes beyond the ring read memory
+        const auto walk = [&](auto mode_tag) __attribute__((always_inline)) {
+            constexpr int kMode = decltype(mode_tag)::value;
+            const float *xwb = xw + (cbeg - D);  // x_w[cbeg + j - D] = xwb[o T + j]
+            const auto input = [&](int o, int32_t j) -> float {
+                if (kMode == 0) return rb[o * kTileRowStride + j];
+                if (j >= len || cbeg + j - D < 0) return 0.0f;
+                if (kMode == 2 && far) return xwb[int64_t(o) * T + j];
+                return rb[o * kTileRowStride + j];
+            };
+            float y0[3], pin[3];
+#pragma unroll
+            for (int o = 0; o < 3; ++o) {
+                y0[o] = c == 0 ? 0.0f : ckpt[int64_t(c - 1) * M + item * 3 + o];
+                if (kMode == 0) pin[o] = rp[o * kTileRowStride];
+                else if (cbeg - 1 - D < 0) pin[o] = 0.0f;
+                else pin[o] = (kMode == 2 && far) ? xwb[int64_t(o) * T - 1] : rp[o * kTileRowStride];
+            }
+            // The chunk in two halves, the later first: the filter is run
+            // over the first half to its end state, then each half is
+            // recomputed into registers and walked back (half the registers
+            // of the whole chunk, for 8 more filter steps).
+            constexpr int kHalf = kTileChunk / 2;
+            float yh[3], ph[3];
+#pragma unroll
+            for (int o = 0; o < 3; ++o) {
+                float yc = y0[o], prev = pin[o];
+#pragma unroll
+                for (int j = 0; j < kHalf; ++j) {
+                    const float xin = input(o, j);
+                    yc = fmaf(a, xin - yc, prev);
+                    prev = xin;
+                }
+                yh[o] = yc;
+                ph[o] = prev;
+            }
+            // Inputs are read from LDS again where they are used next, not
+            // kept in registers from their first read: an empty asm with a
+            // memory clobber keeps the compiler from reusing the loaded values.
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int half = 1; half >= 0; --half) {
+                const int jb = half * kHalf;
+                float yv[3][kHalf], ys[3];
+#pragma unroll
+                for (int o = 0; o < 3; ++o) {
+                    ys[o] = half ? yh[o] : y0[o];  // y at step jb - 1
+                    float yc = ys[o], prev = half ? ph[o] : pin[o];
+#pragma unroll
+                    for (int j = 0; j < kHalf; ++j) {
+                        const float xin = input(o, jb + j);
+                        yc = fmaf(a, xin - yc, prev);
+                        yv[o][j] = yc;
+                        prev = xin;
+                    }
+                }
+                asm volatile("" ::: "memory");
+#pragma unroll
+                for (int j = kHalf - 1; j >= 0; --j) {
+                    if (kMode != 0 && jb + j >= len) continue;
+                    const float xb[3] = {xl[jb + j], xl[kTileChunk + jb + j],
+                                         xl[2 * kTileChunk + jb + j]};
+#pragma unroll
+                    for (int o = 0; o < 3; ++o) {
+                        const float yb = ybar_step(G, xb, a, ybn[o], o);
+                        const float ym = j > 0 ? yv[o][j - 1] : ys[o];
+#pragma unroll
+                        for (int in = 0; in < 3; ++in) g[in][o] = fmaf(xb[in], yv[o][j], g[in][o]);
+                        A = fmaf(yb, input(o, jb + j) - ym, A);
+                        ybn[o] = yb;
+                    }
+                }
+                asm volatile("" ::: "memory");
+            }
+        };
+        if (has) {
+            if (__any(far)) walk(std::integral_constant<int, 2>{});
+            else if (fast) walk(std::integral_constant<int, 0>{});
+            else walk(std::integral_constant<int, 1>{});
+        }
+        __syncthreads();
     }
     if (!active || !has) return;  // no connection: no dependence on these parameters
 #pragma unroll
@@ -588,6 +827,22 @@ int64_t adjoint_gradient_state_len(int32_t N, int32_t T, int32_t K) {
     // a segment spans min(K, T) steps; 6 floats per connection and chunk start
     const int64_t steps = std::max<int64_t>(1, std::min(K, T));
     return (steps + kGradChunk - 1) / kGradChunk * int64_t(N) * 26 * 6;
+}
+
+hipError_t adjoint_gradient_tiled_prepare() {
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(&adjoint_gradient_tiled_kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(kGradTiledLdsBytes));
+}
+
+hipError_t launch_adjoint_gradient_tiled(hipStream_t s, const DeriveTiles &tiles,
+                                         const int32_t *nbr, const float *coefs,
+                                         const int32_t *delays, const float *gT, const float *xT,
+                                         const float *xbarT, const float *ckpt, float *dGT,
+                                         float *dA, int32_t N, int32_t T) {
+    if (N <= 0 || T <= 0 || tiles.tiles <= 0) return hipSuccess;
+    adjoint_gradient_tiled_kernel<<<tiles.tiles, kTileThreads, kGradTiledLdsBytes, s>>>(
+        tiles, nbr, coefs, delays, gT, xT, xbarT, ckpt, dGT, dA, N, T);
+    return hipGetLastError();
 }
 
 hipError_t launch_adjoint_gradient(hipStream_t s, const int32_t *nbr, const float *coefs,

@@ -203,7 +203,7 @@ int derive_reference(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, ct_
     const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
     {
         Span sp(k, kMapResiduals);
-        CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.eT, r.n, r.T, r.S, r.Sp));
+        CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.eT, r.n, r.T, r.S, r.Sp, false));
     }
     if (c.freeze_gains && c.freeze_delays) return CT_OK;
     if (!r.xT_fresh) {
@@ -366,7 +366,10 @@ void end_epoch(ct_epoch_kernel &k) {
 int plan_tiles(ct_results_gpu &r) {
     static std::once_flag prepared;
     static hipError_t prepare_status = hipSuccess;
-    std::call_once(prepared, [] { prepare_status = ct::derive_tiled_prepare(); });
+    std::call_once(prepared, [] {
+        prepare_status = ct::derive_tiled_prepare();
+        if (prepare_status == hipSuccess) prepare_status = ct::adjoint_gradient_tiled_prepare();
+    });
     if (r.N <= 0 || prepare_status != hipSuccess) return CT_OK;
     ctrt::VoxelTiles t;
     ctrt::build_voxel_tiles(r.nbr_host, r.N, ct::kTileMaxVoxels, ct::kTileMaxHalo, ct::kNoSlot, t);
@@ -554,12 +557,7 @@ static int adjoint_forcing(ct_results_gpu &r, int32_t beat, bool accumulate, ct_
     hipStream_t s = r.stream;
     const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
     Span sp(k, kMapResiduals);
-    if (!accumulate) {
-        CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.eT, r.n, r.T, r.S, r.Sp));
-    } else {
-        CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.xbarT, r.n, r.T, r.S, r.Sp));  // scratch
-        CT_HIP(ct::launch_add(s, r.eT, r.xbarT, r.n * r.T));
-    }
+    CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.eT, r.n, r.T, r.S, r.Sp, accumulate));
     return CT_OK;
 }
 
@@ -605,8 +603,12 @@ static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K,
         CT_HIP(ct::launch_transpose(s, r.x, r.xT, T, n));
         r.xT_fresh = true;
     }
-    CT_HIP(ct::launch_adjoint_gradient(s, r.nbr, r.coefs, r.delays, r.gT, r.xT, r.xbarT, r.ckpt,
-                                       r.yseg, r.yseg_have, r.dGT, r.dA, r.N, r.T, K));
+    if (K == ct::kTileChunk && r.tiles.tiles > 0 && r.derive_mode != 1)
+        CT_HIP(ct::launch_adjoint_gradient_tiled(s, r.tiles, r.nbr, r.coefs, r.delays, r.gT, r.xT,
+                                                 r.xbarT, r.ckpt, r.dGT, r.dA, r.N, r.T));
+    else
+        CT_HIP(ct::launch_adjoint_gradient(s, r.nbr, r.coefs, r.delays, r.gT, r.xT, r.xbarT,
+                                           r.ckpt, r.yseg, r.yseg_have, r.dGT, r.dA, r.N, r.T, K));
     return CT_OK;
 }
 

@@ -113,8 +113,9 @@ hipError_t launch_maxreg_finalize(hipStream_t s, const float *partial, int32_t b
 
 hipError_t launch_transpose(hipStream_t s, const float *in, float *out, int64_t rows, int64_t cols);
 
+// eT = Hᵀ R, or with `accumulate` eT = eT + Hᵀ R (each element one FP32 add).
 hipError_t launch_map_residuals(hipStream_t s, const float *H, const float *R, float *eT,
-                                int64_t n, int32_t T, int32_t S, int32_t Sp);
+                                int64_t n, int32_t T, int32_t S, int32_t Sp, bool accumulate);
 
 hipError_t launch_derive(hipStream_t s, const int32_t *nbr, const float *coefs,
                          const int32_t *delays, const float *initial_delays, const float *gT,
@@ -131,7 +132,13 @@ hipError_t launch_derive(hipStream_t s, const int32_t *nbr, const float *coefs,
 // [vox_ptr[g], vox_ptr[g+1]) holding voxels vox[.], and halo voxels
 // halo[halo_ptr[g] .. halo_ptr[g+1]); slot[26 p + d] is the halo index of the
 // neighbour of connection d of the voxel at position p (kNoSlot if missing).
+// The LDS ring holds kTileRing steps of each halo row; the chunk at ring
+// positions [0, kTileChunk) is mirrored behind it, so a row takes
+// kTileRowStride floats.
 constexpr int kTileThreads = 1024;
+constexpr int kTileChunk = 16;
+constexpr int kTileRing = 64;
+constexpr int kTileRowStride = kTileRing + kTileChunk;
 constexpr int kTileMaxVoxels = kTileThreads / 26;  // 39
 constexpr int kTileMaxHalo = 152;
 constexpr uint8_t kNoSlot = 0xFF;
@@ -198,6 +205,16 @@ hipError_t launch_adjoint_gradient(hipStream_t s, const int32_t *nbr, const floa
                                    const float *xbarT, const float *ckpt, float *ychunk,
                                    int64_t ychunk_len, float *dGT, float *dA, int32_t N, int32_t T,
                                    int32_t K);
+// The gradient pass over voxel tiles (the derivation's tables) for
+// checkpoints every kTileChunk steps; the same bits as launch_adjoint_gradient
+// with K = kTileChunk. Connections with delays above 47 read their rows from
+// memory.
+hipError_t adjoint_gradient_tiled_prepare();
+hipError_t launch_adjoint_gradient_tiled(hipStream_t s, const DeriveTiles &tiles,
+                                         const int32_t *nbr, const float *coefs,
+                                         const int32_t *delays, const float *gT, const float *xT,
+                                         const float *xbarT, const float *ckpt, float *dGT,
+                                         float *dA, int32_t N, int32_t T);
 
 // Sticky abort words that a parameter update checks before it writes: when
 // either is nonzero (a resident launch whose states the update's derivatives
@@ -236,7 +253,6 @@ hipError_t launch_batch_metrics(hipStream_t s, const float *loss, const float *l
 hipError_t launch_expand_rows(hipStream_t s, const float *in, float *out, int64_t N);
 
 hipError_t launch_fill(hipStream_t s, float *p, float value, int64_t len);
-hipError_t launch_add(hipStream_t s, float *a, const float *b, int64_t len);  // a += b
 // Spherical states of x (T, 3N): (T, N) magnitude/theta/phi when not null,
 // their values at each voxel's maximum magnitude and the activation time (N);
 // *no_max = 1 if a voxel has no non-NaN magnitude.

@@ -688,7 +688,8 @@ __global__ __launch_bounds__(256) void transpose_kernel(const float *__restrict_
 __global__ __launch_bounds__(256) void map_residuals_kernel(const float *__restrict__ H,
                                                             const float *__restrict__ R,
                                                             float *__restrict__ eT, int64_t n,
-                                                            int32_t T, int32_t Se, int32_t Sp) {
+                                                            int32_t T, int32_t Se, int32_t Sp,
+                                                            int32_t accumulate) {
     // tile: 128 states x kMapTile steps; k (sensors) in chunks of 32, k-major
     // LDS tiles (rows padded by 4 floats)
     __shared__ float Hs[kGemmTileK][128 + 4];       // H[s][i]
@@ -758,7 +759,7 @@ __global__ __launch_bounds__(256) void map_residuals_kernel(const float *__restr
         for (int r = 0; r < 16; ++r) {
             const int64_t i = i0 + wave * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
             const int32_t t = t0 + q * 32 + (lane & 31);
-            if (i < n && t < T) eT[i * T + t] = acc[q][r];
+            if (i < n && t < T) eT[i * T + t] = accumulate ? eT[i * T + t] + acc[q][r] : acc[q][r];
         }
 }
 
@@ -1034,9 +1035,8 @@ __global__ __launch_bounds__(kDeriveThreads, 4) void derive_kernel(
 // of a lane's chunk, which start anywhere in the ring, are contiguous. The
 // loads of chunk c + 1 are in flight while the steps of chunk c run.
 // ---------------------------------------------------------------------------
-constexpr int kTileRing = 64;
-constexpr int kTileRowStride = kTileRing + kDeriveChunk;  // 80 floats per row
-constexpr int kTileMaxDelay = kTileRing - kDeriveChunk;   // the ring holds chunks c-3 .. c
+static_assert(kDeriveChunk == kTileChunk, "the ring is filled one derivation chunk at a time");
+constexpr int kTileMaxDelay = kTileRing - kDeriveChunk;  // the ring holds chunks c-3 .. c
 constexpr int kTileRingFloats = kTileMaxHalo * 3 * kTileRowStride;
 constexpr int kTileStageFloats = kTileMaxVoxels * 3 * kDeriveChunk;
 constexpr int kTileQuads = kTileMaxHalo * 3 * (kDeriveChunk / 4);  // 16-byte loads per chunk
@@ -1374,11 +1374,6 @@ __global__ void expand_rows_kernel(const float *__restrict__ in, float *__restri
     if (k >= N * 234) return;
     const int64_t row = k / 78, g = k % 78;
     out[k] = in[(row / 3) * 78 + g];
-}
-
-__global__ void add_kernel(float *__restrict__ a, const float *__restrict__ b, int64_t len) {
-    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k < len) a[k] = a[k] + b[k];
 }
 
 __global__ void fill_kernel(float *p, float value, int64_t len) {
@@ -1749,12 +1744,6 @@ hipError_t launch_segmentation_histogram(hipStream_t s, const float *x, int32_t 
     return hipGetLastError();
 }
 
-hipError_t launch_add(hipStream_t s, float *a, const float *b, int64_t len) {
-    if (len <= 0) return hipSuccess;
-    add_kernel<<<blocks_for(len, 256), 256, 0, s>>>(a, b, len);
-    return hipGetLastError();
-}
-
 hipError_t launch_barrier_reset(hipStream_t s, GridBarrier *gb, int32_t count) {
     if (count <= 0) return hipSuccess;
     barrier_reset_kernel<<<blocks_for(int64_t(count) * 10, 256), 256, 0, s>>>(gb, count);
@@ -1836,11 +1825,12 @@ hipError_t launch_transpose(hipStream_t s, const float *in, float *out, int64_t 
 }
 
 hipError_t launch_map_residuals(hipStream_t s, const float *H, const float *R, float *eT,
-                                int64_t n, int32_t T, int32_t S, int32_t Sp) {
+                                int64_t n, int32_t T, int32_t S, int32_t Sp, bool accumulate) {
     if (n <= 0) return hipSuccess;
     if (S <= 0 || S > Sp) return hipErrorInvalidValue;
     dim3 grid(blocks_for(n, 128), blocks_for(T, kMapTile));
-    map_residuals_kernel<<<grid, 256, 0, s>>>(H, R, eT, n, T, (S + 1) / 2 * 2, Sp);
+    map_residuals_kernel<<<grid, 256, 0, s>>>(H, R, eT, n, T, (S + 1) / 2 * 2, Sp,
+                                              accumulate ? 1 : 0);
     return hipGetLastError();
 }
 
