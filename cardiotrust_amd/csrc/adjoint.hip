@@ -24,6 +24,7 @@
 #include "barrier.h"
 #include "device.h"
 #include "lds_sort.h"
+#include "tile_ring.h"
 #include "kernels.h"
 
 namespace ct {
@@ -541,16 +542,9 @@ __global__ __launch_bounds__(kGradThreads) void adjoint_gradient_kernel(
 // Each lane's arithmetic is adjoint_gradient_kernel's at K = kTileChunk,
 // operation for operation, so the two give the same bits.
 constexpr int kGradTileMaxDelay = kTileRing - kTileChunk - 1;  // the chunk's start input too
-constexpr int kGradRingFloats = kTileMaxHalo * 3 * kTileRowStride;
-constexpr int kGradStageFloats = kTileMaxVoxels * 3 * kTileChunk;
-constexpr int kGradQuads = kTileMaxHalo * 3 * (kTileChunk / 4);
-constexpr int kGradQuadsPerThread = (kGradQuads + kTileThreads - 1) / kTileThreads;
 constexpr size_t kGradTiledLdsBytes =
-    sizeof(float) * (size_t(kGradRingFloats) + 3 * kTileRowStride + size_t(kGradStageFloats));
+    sizeof(float) * (size_t(kTileRingFloats) + kTileZeroFloats + size_t(kTileStageFloats));
 static_assert(kGradChunk == kTileChunk, "one gradient chunk per ring chunk");
-static_assert(kTileMaxVoxels * kTileChunk <= kTileThreads, "one staging item per thread");
-
-typedef float float4_a16 __attribute__((ext_vector_type(4), aligned(16)));
 
 __global__ __launch_bounds__(kTileThreads) void adjoint_gradient_tiled_kernel(
     DeriveTiles tl, const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
@@ -558,29 +552,25 @@ __global__ __launch_bounds__(kTileThreads) void adjoint_gradient_tiled_kernel(
     const float *__restrict__ xT, const float *__restrict__ xbarT, const float *__restrict__ ckpt,
     float *__restrict__ dGT, float *__restrict__ dA, int32_t N, int32_t T) {
     extern __shared__ __align__(16) float lds[];
-    float *ring = lds;                         // [halo][3][kTileRowStride]
-    float *zeros = ring + kGradRingFloats;     // [3][kTileRowStride], +0: missing connections
-    float *xs = zeros + 3 * kTileRowStride;    // [voxel][3][chunk] adjoint states of the tile
+    float *ring = lds;                       // [halo][3][kTileRowStride]
+    float *zeros = ring + kTileRingFloats;   // [3][kTileRowStride], +0: missing connections
+    float *xs = zeros + kTileZeroFloats;     // [voxel][3][chunk] adjoint states of the tile
     const int64_t n = 3LL * N;
     const int64_t M = int64_t(N) * 78;
     const int32_t tile = xcd_block(blockIdx.x, gridDim.x);
     const int32_t pbeg = tl.vox_ptr[tile], nvox = tl.vox_ptr[tile + 1] - pbeg;
     const int32_t hbeg = tl.halo_ptr[tile], nhalo = tl.halo_ptr[tile + 1] - hbeg;
     const int tid = threadIdx.x;
-    for (int i = tid; i < 3 * kTileRowStride; i += kTileThreads) zeros[i] = 0.0f;
 
-    const int lv = tid / 26, d = tid - 26 * lv;
-    const bool active = lv < nvox;
-    const int32_t v = active ? tl.vox[pbeg + lv] : 0;
-    const int64_t item = int64_t(v) * 26 + d;
-    const int32_t w = active ? nbr[item] : -1;
-    const bool has = w >= 0;
-    const float a = active ? coefs[item] : 0.0f;
-    const int32_t D = active ? delays[item] : 0;
-    const bool far = has && D > kGradTileMaxDelay;  // reads its rows from memory
-    const int slot = (has && !far) ? int(tl.slot[int64_t(pbeg + lv) * 26 + d]) : -1;
-    const float *xw = xT + (has ? 3LL * w : 0) * T;
-    const float *rrow = slot >= 0 ? ring + slot * 3 * kTileRowStride : zeros;
+    const TileLane L = tile_lane(tl, nbr, coefs, delays, pbeg, nvox, kGradTileMaxDelay);
+    for (int i = tid; i < kTileZeroFloats; i += kTileThreads) zeros[i] = 0.0f;
+    const bool active = L.active, has = L.has, far = L.far;
+    const int d = L.d;
+    const int32_t v = L.v, D = L.D;
+    const int64_t item = L.item;
+    const float a = L.a;
+    const float *xw = xT + (has ? 3LL * L.w : 0) * T;
+    const float *rrow = L.slot >= 0 ? ring + L.slot * 3 * kTileRowStride : zeros;
 
     float G[3][3], g[3][3];
 #pragma unroll
@@ -592,51 +582,14 @@ __global__ __launch_bounds__(kTileThreads) void adjoint_gradient_tiled_kernel(
         }
     float A = 0.0f, ybn[3] = {0.0f, 0.0f, 0.0f};
 
-    // this thread's share of the copies: 16-byte quads of halo rows, and one
-    // (voxel, step) item of the tile's adjoint states
-    int64_t qsrc[kGradQuadsPerThread];
-    int32_t qdst[kGradQuadsPerThread];
-#pragma unroll
-    for (int k = 0; k < kGradQuadsPerThread; ++k) {
-        const int qi = tid + k * kTileThreads;
-        qsrc[k] = -1;
-        qdst[k] = 0;
-        if (qi < nhalo * 3 * (kTileChunk / 4)) {
-            const int r = qi / (kTileChunk / 4), q4 = 4 * (qi % (kTileChunk / 4));
-            const int h = r / 3, o = r - 3 * h;
-            qsrc[k] = (3LL * tl.halo[hbeg + h] + o) * T + q4;
-            qdst[k] = r * kTileRowStride + q4;
-        }
-    }
+    // this thread's share of the copies: quads of halo rows, and one (voxel,
+    // step) item of the tile's adjoint states
+    TileRingCopy rc;
+    rc.init(tl, hbeg, nhalo, T);
     const int sv = tid / kTileChunk, sj = tid - kTileChunk * sv;
     const bool sown = sv < nvox;
     const int64_t srow = sown ? 3LL * tl.vox[pbeg + sv] * T + sj : 0;
-
-    float4_a4 quad[kGradQuadsPerThread];
     float sb[3];
-    const auto fetch_rows = [&](int32_t cb) {  // chunk starting at step cb >= 0
-        const bool whole = cb + kTileChunk <= T;
-#pragma unroll
-        for (int k = 0; k < kGradQuadsPerThread; ++k) {
-            if (qsrc[k] < 0) continue;
-            if (whole) {
-                quad[k] = *reinterpret_cast<const float4_a4 *>(xT + qsrc[k] + cb);
-            } else {
-                const int32_t tq = cb + int32_t(qdst[k] % kTileRowStride);
-#pragma unroll
-                for (int e = 0; e < 4; ++e) quad[k][e] = tq + e < T ? xT[qsrc[k] + cb + e] : 0.0f;
-            }
-        }
-    };
-    const auto stage_rows = [&](int32_t cb) {
-        const int pos = cb & (kTileRing - 1);
-#pragma unroll
-        for (int k = 0; k < kGradQuadsPerThread; ++k) {
-            if (qsrc[k] < 0) continue;
-            *reinterpret_cast<float4_a16 *>(ring + qdst[k] + pos) = quad[k];
-            if (pos == 0) *reinterpret_cast<float4_a16 *>(ring + qdst[k] + kTileRing) = quad[k];
-        }
-    };
     const auto fetch_xbar = [&](int32_t cb) {
         const bool live = sown && cb + sj < T;
 #pragma unroll
@@ -648,6 +601,8 @@ __global__ __launch_bounds__(kTileThreads) void adjoint_gradient_tiled_kernel(
             for (int q = 0; q < 3; ++q) xs[(sv * 3 + q) * kTileChunk + sj] = sb[q];
         }
     };
+    const auto fetch_rows = [&](int32_t cb) { rc.fetch(xT, cb, T); };
+    const auto stage_rows = [&](int32_t cb) { rc.stage(ring, cb); };
 
     const int32_t nchunks = (T + kTileChunk - 1) / kTileChunk;
     // chunks nchunks-1 .. nchunks-3 first; each iteration c then adds c-3
@@ -668,7 +623,7 @@ __global__ __launch_bounds__(kTileThreads) void adjoint_gradient_tiled_kernel(
         // start input x_w[cbeg - 1 - D] at its own ring position
         const float *rb = rrow + ((cbeg - D) & (kTileRing - 1));
         const float *rp = rrow + ((cbeg - 1 - D) & (kTileRing - 1));
-        const float *xl = xs + lv * 3 * kTileChunk;
+        const float *xl = xs + L.lv * 3 * kTileChunk;
         // every lane of the wave in the ring, past its delay, and a whole chunk:
         // no per-input checks
         const bool fast = __all(!active || cbeg - 1 - D >= 0) && len == kTileChunk;

@@ -133,8 +133,7 @@ hipError_t launch_derive(hipStream_t s, const int32_t *nbr, const float *coefs,
 // halo[halo_ptr[g] .. halo_ptr[g+1]); slot[26 p + d] is the halo index of the
 // neighbour of connection d of the voxel at position p (kNoSlot if missing).
 // The LDS ring holds kTileRing steps of each halo row; the chunk at ring
-// positions [0, kTileChunk) is mirrored behind it, so a row takes
-// kTileRowStride floats.
+// positions [0, kTileChunk) is mirrored behind it.
 constexpr int kTileThreads = 1024;
 constexpr int kTileChunk = 16;
 constexpr int kTileRing = 64;

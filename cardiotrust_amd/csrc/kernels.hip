@@ -11,6 +11,7 @@
 #include "barrier.h"
 #include "device.h"
 #include "lds_sort.h"
+#include "tile_ring.h"
 
 namespace ct {
 namespace {
@@ -1037,19 +1038,8 @@ __global__ __launch_bounds__(kDeriveThreads, 4) void derive_kernel(
 // ---------------------------------------------------------------------------
 static_assert(kDeriveChunk == kTileChunk, "the ring is filled one derivation chunk at a time");
 constexpr int kTileMaxDelay = kTileRing - kDeriveChunk;  // the ring holds chunks c-3 .. c
-constexpr int kTileRingFloats = kTileMaxHalo * 3 * kTileRowStride;
-constexpr int kTileStageFloats = kTileMaxVoxels * 3 * kDeriveChunk;
-constexpr int kTileQuads = kTileMaxHalo * 3 * (kDeriveChunk / 4);  // 16-byte loads per chunk
-constexpr int kTileQuadsPerThread = (kTileQuads + kTileThreads - 1) / kTileThreads;
-static_assert(kTileMaxVoxels * kDeriveChunk <= kTileThreads, "one staging item per thread");
-static_assert(kTileRing % kDeriveChunk == 0 && (kTileRing & (kTileRing - 1)) == 0,
-              "chunks fill ring positions without wrapping");
-static_assert(kTileMaxHalo < kNoSlot, "halo indices fit the slot bytes");
-
 constexpr size_t kTileLdsBytes =
-    sizeof(float) * (size_t(kTileRingFloats) + 3 * kTileRowStride + 2 * size_t(kTileStageFloats));
-
-typedef float float4_a16 __attribute__((ext_vector_type(4), aligned(16)));
+    sizeof(float) * (size_t(kTileRingFloats) + kTileZeroFloats + 2 * size_t(kTileStageFloats));
 
 template <bool kGains, bool kCoefs, bool kSimple>
 __global__ __launch_bounds__(kTileThreads) void derive_tiled_kernel(
@@ -1059,30 +1049,25 @@ __global__ __launch_bounds__(kTileThreads) void derive_tiled_kernel(
     float *__restrict__ fir, float *__restrict__ iir, float *__restrict__ dGT,
     float *__restrict__ dA, int32_t N, int32_t T, DeriveParams p) {
     extern __shared__ __align__(16) float lds[];
-    float *ring = lds;                           // [halo][3][kTileRowStride]
-    float *zeros = ring + kTileRingFloats;       // [3][kTileRowStride], +0: missing connections
-    float *es = zeros + 3 * kTileRowStride;      // [voxel][3][chunk] mapped residuals
-    float *cs = es + kTileStageFloats;           // [voxel][3][chunk] gain-derivative factors
+    float *ring = lds;                       // [halo][3][kTileRowStride]
+    float *zeros = ring + kTileRingFloats;   // [3][kTileRowStride], +0: missing connections
+    float *es = zeros + kTileZeroFloats;     // [voxel][3][chunk] mapped residuals
+    float *cs = es + kTileStageFloats;       // [voxel][3][chunk] gain-derivative factors
     const int64_t n = 3LL * N;
     const int32_t tile = xcd_block(blockIdx.x, gridDim.x);
     const int32_t pbeg = tl.vox_ptr[tile], nvox = tl.vox_ptr[tile + 1] - pbeg;
     const int32_t hbeg = tl.halo_ptr[tile], nhalo = tl.halo_ptr[tile + 1] - hbeg;
     const int tid = threadIdx.x;
-    for (int i = tid; i < 3 * kTileRowStride; i += kTileThreads) zeros[i] = 0.0f;
 
-    // the lane's connection: voxel lv of the tile, direction d
-    const int lv = tid / 26, d = tid - 26 * lv;
-    const bool active = lv < nvox;
-    const int32_t v = active ? tl.vox[pbeg + lv] : 0;
-    const int64_t item = int64_t(v) * 26 + d;
-    const int32_t w = active ? nbr[item] : -1;
-    const bool has = w >= 0;
-    const float a = active ? coefs[item] : 0.0f;
-    const int32_t D = active ? delays[item] : 0;
-    const bool far = has && D > kTileMaxDelay;  // beyond the ring: reads its rows from memory
-    const int slot = (has && !far) ? int(tl.slot[int64_t(pbeg + lv) * 26 + d]) : -1;
-    const float *xw = xT + (has ? 3LL * w : 0) * T;
-    const float *rrow = slot >= 0 ? ring + slot * 3 * kTileRowStride : zeros;
+    const TileLane L = tile_lane(tl, nbr, coefs, delays, pbeg, nvox, kTileMaxDelay);
+    for (int i = tid; i < kTileZeroFloats; i += kTileThreads) zeros[i] = 0.0f;
+    const bool active = L.active, has = L.has, far = L.far;
+    const int d = L.d;
+    const int32_t v = L.v, D = L.D;
+    const int64_t item = L.item;
+    const float a = L.a;
+    const float *xw = xT + (has ? 3LL * L.w : 0) * T;
+    const float *rrow = L.slot >= 0 ? ring + L.slot * 3 * kTileRowStride : zeros;
 
     ConnState c;
 #pragma unroll
@@ -1100,42 +1085,16 @@ __global__ __launch_bounds__(kTileThreads) void derive_tiled_kernel(
         c.prev_in[o] = 0.0f;
     }
 
-    // this thread's share of the chunk copies: 16-byte quads of halo rows
-    // (row r = 3 h + o holds state o of halo voxel h), and one (voxel, step)
-    // item of the tile's own residuals and states
-    int64_t qsrc[kTileQuadsPerThread];
-    int32_t qdst[kTileQuadsPerThread];
-#pragma unroll
-    for (int k = 0; k < kTileQuadsPerThread; ++k) {
-        const int qi = tid + k * kTileThreads;
-        qsrc[k] = -1;
-        qdst[k] = 0;
-        if (qi < nhalo * 3 * (kDeriveChunk / 4)) {
-            const int r = qi / (kDeriveChunk / 4), q4 = 4 * (qi % (kDeriveChunk / 4));
-            const int h = r / 3, o = r - 3 * h;
-            qsrc[k] = (3LL * tl.halo[hbeg + h] + o) * T + q4;
-            qdst[k] = r * kTileRowStride + q4;
-        }
-    }
+    // this thread's share of the chunk copies: quads of halo rows, and one
+    // (voxel, step) item of the tile's own residuals and states
+    TileRingCopy rc;
+    rc.init(tl, hbeg, nhalo, T);
     const int sv = tid / kDeriveChunk, sj = tid - kDeriveChunk * sv;
     const bool sown = sv < nvox;
     const int64_t srow = sown ? 3LL * tl.vox[pbeg + sv] * T + sj : 0;
-
-    float4_a4 quad[kTileQuadsPerThread];
     float se[3], sx[3];
     const auto fetch = [&](int32_t t0) {
-        const bool whole = t0 + kDeriveChunk <= T;
-#pragma unroll
-        for (int k = 0; k < kTileQuadsPerThread; ++k) {
-            if (qsrc[k] < 0) continue;
-            if (whole) {
-                quad[k] = *reinterpret_cast<const float4_a4 *>(xT + qsrc[k] + t0);
-            } else {
-                const int32_t tq = t0 + int32_t(qdst[k] % kTileRowStride);
-#pragma unroll
-                for (int e = 0; e < 4; ++e) quad[k][e] = tq + e < T ? xT[qsrc[k] + t0 + e] : 0.0f;
-            }
-        }
+        rc.fetch(xT, t0, T);
         const bool live = sown && t0 + sj < T;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
@@ -1144,13 +1103,7 @@ __global__ __launch_bounds__(kTileThreads) void derive_tiled_kernel(
         }
     };
     const auto stage = [&](int32_t t0) {
-        const int pos = t0 & (kTileRing - 1);
-#pragma unroll
-        for (int k = 0; k < kTileQuadsPerThread; ++k) {
-            if (qsrc[k] < 0) continue;
-            *reinterpret_cast<float4_a16 *>(ring + qdst[k] + pos) = quad[k];
-            if (pos == 0) *reinterpret_cast<float4_a16 *>(ring + qdst[k] + kTileRing) = quad[k];
-        }
+        rc.stage(ring, t0);
         if (sown) {
             float cg[3];
             gain_factors(sx, se, p, cg);
@@ -1169,8 +1122,9 @@ __global__ __launch_bounds__(kTileThreads) void derive_tiled_kernel(
         if (t0 + kDeriveChunk < T) fetch(t0 + kDeriveChunk);
         // the lane's 16 inputs of this chunk: rb[o * kTileRowStride + j] = x[t0 + j - D]
         const float *rb = rrow + ((t0 - D) & (kTileRing - 1));
-        const float *el = es + lv * 3 * kDeriveChunk;
-        const float *cl = cs + lv * 3 * kDeriveChunk;
+        const float *el = es + L.lv * 3 * kDeriveChunk;
+        const float *cl = cs + L.lv * 3 * kDeriveChunk;
+        const auto at = [](const float *base, int q, int j) { return base[q * kDeriveChunk + j]; };
         const bool steady = __all(!active || t0 >= D) && t0 + kDeriveChunk <= T;
         const bool any_far = __any(far);
         if (active && steady && !any_far) {
@@ -1202,8 +1156,8 @@ __global__ __launch_bounds__(kTileThreads) void derive_tiled_kernel(
                     for (int o = 0; o < 3; ++o)
                         in0[o] = s < 0 ? 0.0f
                                        : (far ? xw[int64_t(o) * T + s] : rb[o * kTileRowStride + j]);
-                    const float e[3] = {el[j], el[kDeriveChunk + j], el[2 * kDeriveChunk + j]};
-                    const float cg[3] = {cl[j], cl[kDeriveChunk + j], cl[2 * kDeriveChunk + j]};
+                    const float e[3] = {at(el, 0, j), at(el, 1, j), at(el, 2, j)};
+                    const float cg[3] = {at(cl, 0, j), at(cl, 1, j), at(cl, 2, j)};
                     derive_step<false, kGains, kCoefs, kSimple>(c, in0, e, cg, has, a, D, t);
                 }
             }
