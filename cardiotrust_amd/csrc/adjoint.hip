@@ -166,9 +166,14 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
     const float *__restrict__ coefs, float *__restrict__ gcons,
     const float *__restrict__ x, const float *__restrict__ E,
     float *__restrict__ xbar, GridBarrier *__restrict__ gb, int32_t N, int32_t T, AdjointParams p,
-    uint64_t wait_ticks) {
+    uint64_t wait_ticks, int32_t coherent_rows) {
     extern __shared__ float lds[];
     float *q_lds = lds;  // [voxel][26][3]
+    // the xbar history as one buffer resource: with `coherent`, rows are
+    // stored through the L2 and gathered past it (sc1), and the barrier needs
+    // no agent-scope fences
+    const bool coherent = coherent_rows != 0;
+    const __amdgpu_buffer_rsrc_t xbrs = __builtin_amdgcn_make_buffer_rsrc(xbar, 0, -1, kBufferWord3);
     int *ok_lds = reinterpret_cast<int *>(lds + kResMaxVoxels * 78);  // [4]
     uint8_t *dtab = reinterpret_cast<uint8_t *>(ok_lds + 4);          // [phase-1 position]
     uint64_t *keys = reinterpret_cast<uint64_t *>(lds);                 // set-up only, over q
@@ -272,7 +277,7 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
     for (int32_t s = T - 1; s >= 0; --s) {
 #pragma unroll 1
         for (uint32_t pass = 0; pass < 2; ++pass) {
-        if (pass == 1 && s < T - 1 && !team_wait(gb, gen, ok_lds, wait_ticks)) return;
+        if (pass == 1 && s < T - 1 && !team_wait(gb, gen, ok_lds, wait_ticks, coherent)) return;
         // phase 1: each item's q for step s (its ybar at tau = s + D). The
         // empty asm statement hides the lane index from the optimiser, so the
         // per-item addresses are recomputed each step instead of being kept
@@ -295,8 +300,10 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
                 const int32_t tau = s + D;
                 on[kk] = present && tau < T;
                 if (!on[kk]) continue;  // not this pass, absent, or tau beyond the beat
-                const float *xs = xbar + int64_t(tau) * n + 3LL * (pk[k] & 0x1FFFFu);
-                xv[kk] = *reinterpret_cast<const float3_a4 *>(xs);
+                const int64_t xe = int64_t(tau) * n + 3LL * (pk[k] & 0x1FFFFu);
+                xv[kk] = coherent
+                             ? __builtin_amdgcn_raw_buffer_load_b96(xbrs, uint32_t(xe) * 4u, 0, kCacheSC1)
+                             : *reinterpret_cast<const float3_a4 *>(xbar + xe);
                 if (k0 + kk < kSweepCachedItems) {
                     const int kc = k0 + kk < kSweepCachedItems ? k0 + kk : 0;
 #pragma unroll
@@ -344,9 +351,11 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
                 }
             }
 #pragma unroll
-            for (int o = 0; o < 3; ++o) xbar[row + o] = acc[o];
+            for (int o = 0; o < 3; ++o)
+                __hip_atomic_store(xbar + row + o, acc[o], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);  // written through the L2
         }
-        team_arrive(gb, gridDim.x, blockIdx.x, gen);
+        team_arrive(gb, gridDim.x, blockIdx.x, gen, coherent);
     }
 }
 
@@ -773,8 +782,10 @@ hipError_t launch_adjoint_sweep_resident(hipStream_t s, int32_t blocks, const in
     if (N <= 0 || T <= 0) return hipSuccess;
     hipError_t e = launch_barrier_reset(s, gb, 1);
     if (e != hipSuccess) return e;
+    const int32_t coherent = int64_t(T) * 3 * N * 4 < (int64_t(1) << 32) ? 1 : 0;
     adjoint_sweep_resident_kernel<<<blocks, kSweepThreads, sweep_lds_bytes(), s>>>(
-        cons_ptr, cons_idx, delays, gains_rows, coefs, gcons, x, E, xbar, gb, N, T, p, wait_ticks);
+        cons_ptr, cons_idx, delays, gains_rows, coefs, gcons, x, E, xbar, gb, N, T, p, wait_ticks,
+        coherent);
     return hipGetLastError();
 }
 

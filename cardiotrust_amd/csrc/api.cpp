@@ -133,6 +133,7 @@ ct::ResidentJob resident_job(ct_results_gpu &r, int32_t ckpt_interval) {
     j.ckpt_interval = ckpt_interval;
     j.slot0 = 0;
     j.width = r.resident_blocks;
+    j.coherent = int64_t(r.T) * r.n * 4 < (int64_t(1) << 32) ? 1 : 0;
     return j;
 }
 

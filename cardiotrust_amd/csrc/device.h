@@ -27,4 +27,11 @@ __device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t nb) {
     return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
+// Raw buffer resources (byte offsets, stride 0): word 3 of the descriptor
+// with the 32-bit data format, and the cache-policy operand that makes a
+// buffer access coherent at agent scope (sc1: it does not hit a stale copy in
+// this XCD's L2).
+constexpr int32_t kBufferWord3 = 0x00020000;
+constexpr int32_t kCacheSC1 = 16;
+
 }  // namespace ct

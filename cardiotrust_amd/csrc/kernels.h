@@ -74,6 +74,12 @@ struct ResidentJob {
     int32_t ckpt_interval;  // 0 = no checkpoints
     int32_t slot0;
     int32_t width;          // workgroups in the team
+    // 1: the states are exchanged between workgroups through agent-coherent
+    // accesses (stores written through the L2, gathers that bypass it), so a
+    // step needs no agent-scope release / acquire; requires 4 T n < 2^32
+    // (byte offsets of one buffer resource). 0: normal accesses with the
+    // agent-scope fences around every wait.
+    int32_t coherent;
 };
 // Most workgroups of a resident launch (one flag word each).
 constexpr int kMaxSlabFlags = 4096;

@@ -180,6 +180,10 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
     const float *__restrict__ b = j.b;
     const float *__restrict__ u = j.u;
     float *__restrict__ x = j.x;
+    // the states as one buffer resource (raw, byte offsets): gathers that
+    // bypass the L2 (sc1) when the rows are exchanged coherently
+    const bool coherent = j.coherent != 0;
+    const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc(x, 0, -1, kBufferWord3);
     const int64_t n = 3LL * N;
     const int32_t nv = ve - vb;
     const int32_t items = nv * 26;
@@ -279,7 +283,7 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
         const bool last = (t == T - 1);
         if (t > t_begin &&
             !slab_wait(j.flags, j.gb, smin, smax, me, uint32_t(t - t_begin), ok_lds,
-                       t == t_begin + 1 ? first_wait : step_wait))
+                       t == t_begin + 1 ? first_wait : step_wait, coherent))
             return false;  // host reports the abort
 #pragma unroll
         for (int k0 = 0; k0 < kResItems; k0 += kGather) {
@@ -301,7 +305,9 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
                 if (D <= t) {
                     // the neighbour's three states: one 12-byte load
                     const uint32_t r0 = uint32_t(t - D) * uint32_t(n) + 3u * w;
-                    const float3_a4 v = *reinterpret_cast<const float3_a4 *>(x + r0);
+                    const float3_a4 v =
+                        coherent ? __builtin_amdgcn_raw_buffer_load_b96(xrs, r0 * 4u, 0, kCacheSC1)
+                                 : *reinterpret_cast<const float3_a4 *>(x + r0);
                     in0[q][0] = v.x;
                     in0[q][1] = v.y;
                     in0[q][2] = v.z;
@@ -354,7 +360,9 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
                 for (int q = 0; q < 3 * kGainConns; ++q)
                     if (h + q < 78) acc = acc + g[q] * yv[h + q];
             }
-            x[int64_t(t) * n + i] = acc + ut * b[i];
+            // written through the L2: the other slabs read it coherently
+            __hip_atomic_store(x + int64_t(t) * n + i, acc + ut * b[i], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         }
         // filter outputs y_t for the reverse-mode gradient, every ckpt_interval steps
         const int32_t K = j.ckpt_interval;
@@ -362,7 +370,7 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
             float *dst = j.ckpt + int64_t((t + 1) / K - 1) * N * 78 + int64_t(vb) * 78;
             for (int idx = tid; idx < nv * 78; idx += kResThreads) dst[idx] = ys[idx];
         }
-        slab_publish(j.flags, me, uint32_t(t + 1 - t_begin));
+        slab_publish(j.flags, me, uint32_t(t + 1 - t_begin), coherent);
     }
     for (int idx = tid; idx < nv * 78; idx += kResThreads) j.y[int64_t(vb) * 78 + idx] = ys[idx];
     return true;
