@@ -532,7 +532,10 @@ def main() -> int:
             "frac": achieved / HBM_PEAK_GBS,
             "achieved_basis": "algorithmic bytes, 1792 B per voxel-step (234 gains, 26 coefs, "
                               "26 delays, 2 x 78 filter outputs, 3 + 3 states, 4 B each) x "
-                              "voxels x steps per launch / the launch's HIP-event time",
+                              "voxels x steps per launch / the launch's HIP-event time; the "
+                              "kernel keeps the filter outputs (624 B of the 1792) and the "
+                              "connection tables on chip, so this rate can exceed the HBM "
+                              "peak: traffic_frac is the measured HBM fraction",
             "traffic": traffic,
             "traffic_source": traffic_source,
             "traffic_gbs": traffic / (launch_ms * 1e-3) / 1e9 if traffic else None,
