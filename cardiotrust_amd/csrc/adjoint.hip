@@ -778,11 +778,12 @@ hipError_t launch_adjoint_sweep_resident(hipStream_t s, int32_t blocks, const in
                                          const float *gains_rows, const float *coefs,
                                          float *gcons, const float *x, const float *E,
                                          float *xbar, GridBarrier *gb, int32_t N, int32_t T,
-                                         const AdjointParams &p, uint64_t wait_ticks) {
+                                         const AdjointParams &p, uint64_t wait_ticks,
+                                         bool allow_coherent) {
     if (N <= 0 || T <= 0) return hipSuccess;
     hipError_t e = launch_barrier_reset(s, gb, 1);
     if (e != hipSuccess) return e;
-    const int32_t coherent = int64_t(T) * 3 * N * 4 < (int64_t(1) << 32) ? 1 : 0;
+    const int32_t coherent = allow_coherent && coherent_exchange(T, 3LL * N) ? 1 : 0;
     adjoint_sweep_resident_kernel<<<blocks, kSweepThreads, sweep_lds_bytes(), s>>>(
         cons_ptr, cons_idx, delays, gains_rows, coefs, gcons, x, E, xbar, gb, N, T, p, wait_ticks,
         coherent);

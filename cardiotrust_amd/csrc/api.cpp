@@ -133,7 +133,7 @@ ct::ResidentJob resident_job(ct_results_gpu &r, int32_t ckpt_interval) {
     j.ckpt_interval = ckpt_interval;
     j.slot0 = 0;
     j.width = r.resident_blocks;
-    j.coherent = int64_t(r.T) * r.n * 4 < (int64_t(1) << 32) ? 1 : 0;
+    j.coherent = r.propagation_mode != 3 && ct::coherent_exchange(r.T, r.n) ? 1 : 0;
     return j;
 }
 
@@ -586,7 +586,8 @@ static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K,
                                                      r.delays, r.staging, r.coefs, r.gcons, r.x,
                                                      r.E, r.xbar,
                                                      r.barrier, r.N, r.T, p,
-                                                     wait_ticks(*r.gpu, r.wait_seconds)));
+                                                     wait_ticks(*r.gpu, r.wait_seconds),
+                                                     r.propagation_mode != 3));
             r.resident_used = true;
         } else {
             CT_HIP(ct::launch_consumer_delays(s, r.cons_idx, r.delays, r.cons_delay, r.cons_len));
@@ -1347,8 +1348,9 @@ int ct_epoch_kernel_execute(ct_epoch_kernel *k) {
 
 int ct_results_set_propagation(ct_results_gpu *results, int32_t mode, int32_t *resident) {
     if (!results) return fail(CT_ERR_INVALID, "null argument");
-    if (mode < 0 || mode > 2) return fail(CT_ERR_INVALID, "propagation mode must be 0, 1 or 2");
-    if (mode == 2 && !results->resident_ok && results->N > 0)
+    if (mode < 0 || mode > 3)
+        return fail(CT_ERR_INVALID, "propagation mode must be 0, 1, 2 or 3");
+    if (mode >= 2 && !results->resident_ok && results->N > 0)
         return fail(CT_ERR_INVALID, "the resident propagation kernel cannot hold this problem");
     results->propagation_mode = mode;
     if (resident) *resident = (results->resident_ok && mode != 1) ? 1 : 0;

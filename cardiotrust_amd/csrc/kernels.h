@@ -196,7 +196,13 @@ hipError_t launch_adjoint_sweep_resident(hipStream_t s, int32_t blocks, const in
                                          const float *gains_rows, const float *coefs,
                                          float *gcons, const float *x, const float *E,
                                          float *xbar, GridBarrier *gb, int32_t N, int32_t T,
-                                         const AdjointParams &p, uint64_t wait_ticks);
+                                         const AdjointParams &p, uint64_t wait_ticks,
+                                         bool allow_coherent);
+// Whether a resident launch over T steps of n states exchanges the rows
+// coherently (byte offsets of one buffer resource below 2^32) when allowed.
+inline bool coherent_exchange(int32_t T, int64_t n) {
+    return int64_t(T) * n * 4 < (int64_t(1) << 32);
+}
 hipError_t launch_barrier_reset(hipStream_t s, GridBarrier *gb, int32_t count);
 // One job's barrier counters and its first `nflags` slab flags (flags may be null).
 hipError_t launch_resident_reset(hipStream_t s, GridBarrier *gb, uint32_t *flags, int32_t nflags);

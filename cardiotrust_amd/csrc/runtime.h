@@ -112,7 +112,10 @@ struct ct_results_gpu {
     int64_t sensor_std_have = 0;
     // propagation path: resident (one launch per beat) when the
     // problem fits on chip, else one launch per step
-    int32_t propagation_mode = 0;  // 0 auto, 1 per-step launches, 2 resident
+    // 0 auto, 1 per-step launches, 2 resident, 3 resident exchanging the
+    // state rows through agent-scope fences (what problems too large for the
+    // coherent exchange take)
+    int32_t propagation_mode = 0;
     double wait_seconds = 2.0;     // bound on every resident wait (ct_results_set_wait_limit)
     bool xT_fresh = false;         // xT holds the current states (written by the projection)
     // derivation path: over voxel tiles (derive_tiled_kernel) when their

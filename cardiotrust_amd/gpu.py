@@ -285,7 +285,8 @@ class ResultsGPU:
         return out
 
     def set_propagation(self, mode: int) -> bool:
-        """0 automatic, 1 one launch per step, 2 resident; returns whether the
+        """0 automatic, 1 one launch per step, 2 resident, 3 resident with the
+        state rows exchanged through agent-scope fences; returns whether the
         resident kernel will run."""
         res = ctypes.c_int32()
         _check(lib().ct_results_set_propagation(self.handle, int(mode), ctypes.byref(res)),

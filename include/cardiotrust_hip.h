@@ -246,7 +246,12 @@ int ct_results_simulate_noisy(ct_results_gpu *results, const float *sensor_std, 
  * launch per time step, 2 = resident (one launch per beat whose workgroups
  * all stay resident, keep the filter outputs in LDS and wait after every step
  * for the workgroups owning their neighbours; CT_ERR_INVALID if the problem
- * does not fit).
+ * does not fit), 3 = resident with the state rows exchanged through
+ * agent-scope release / acquire fences at every wait, the path that
+ * problems with 12 T N bytes of states >= 2^32 take (the others exchange
+ * them through agent-coherent stores and loads; both give the same bits).
+ * The resident backward sweep of the reverse-mode gradient follows the same
+ * setting.
  * `resident` (may be NULL) receives 1 if the resident path will run. */
 int ct_results_set_propagation(ct_results_gpu *results, int32_t mode, int32_t *resident);
 

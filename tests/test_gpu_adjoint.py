@@ -135,7 +135,7 @@ def test_adjoint_resident_and_per_step_forward_agree(gpu):
 
 
 def test_resident_sweep_matches_per_step_sweep_on_many_workgroups(gpu):
-    note("case: adjoint on a 24^3 shell (several resident workgroups), mode 1 vs mode 2")
+    note("case: adjoint on a 24^3 shell (several resident workgroups), mode 1 vs 2 vs 3")
     g0 = synth.ellipsoid_shell(shape=(24, 24, 24), outer=(12, 12, 12), inner=(6, 6, 6))
     p = synth.make_problem(g0, num_steps=120, max_delay=40, seed=9)
     from cardiotrust_amd.gpu import ResultsGPU
@@ -145,11 +145,12 @@ def test_resident_sweep_matches_per_step_sweep_on_many_workgroups(gpu):
     truth.close()
     cfg = Algorithm(maximum_regularization_threshold=0.05)
     a = gpu_grad(gpu, p, cfg, K=16, mode=1)
-    b = gpu_grad(gpu, p, cfg, K=16, mode=2)
     assert g0.num_voxels > 5000
-    assert a[0] == b[0]
-    assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
-    assert np.array_equal(a[2].view(np.uint32), b[2].view(np.uint32))
+    for mode in (2, 3):  # coherent row exchange, and agent-scope fences
+        b = gpu_grad(gpu, p, cfg, K=16, mode=mode)
+        assert a[0] == b[0]
+        assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+        assert np.array_equal(a[2].view(np.uint32), b[2].view(np.uint32))
 
 
 
