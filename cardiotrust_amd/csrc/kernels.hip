@@ -418,11 +418,16 @@ __global__ __launch_bounds__(kResThreads) void propagate_batch_kernel(
 // 108 sensors waste 4 rows instead of the 20 a 32-row granularity would.
 // Workgroup (t-tile, split, sensor tile) covers kProjTile steps x 112 sensors
 // over its K range and writes that partial product; launch_residuals sums the
-// splits in fixed order. Wave w computes steps [32 w, 32 w + 32) of the tile
-// for all 112 sensors: 14 accumulators, 9 LDS operand reads per 14 MFMAs.
+// splits in fixed order. Wave w computes steps [w kProjTile / 4, (w + 1)
+// kProjTile / 4) of the tile for all 112 sensors: at 128 steps, 14
+// accumulators and 9 LDS operand reads per 14 MFMAs. (64-step tiles time the
+// same; 256-step tiles need 178 VGPRs and run 38% slower.)
 // ---------------------------------------------------------------------------
 constexpr int kProjRowsPad = kProjRows + 32;   // H tile row stride: rows k, k+1 on opposite bank halves
 constexpr int kProjStepsPad = kProjTile + 16;  // X tile row stride, the same property
+constexpr int kProjWaveSteps = kProjTile / 4;  // steps of the tile per wave
+constexpr int kProjStepBlocks = kProjWaveSteps / 16;
+static_assert(kProjTile % 64 == 0, "whole 16-step MFMA blocks per wave");
 __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ x,
                                                       const float *__restrict__ H,
                                                       float *__restrict__ partial,
@@ -440,11 +445,11 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
     const int64_t k0 = int64_t(split) * kps;
     const int64_t k1 = min(n, k0 + kps);
     typedef float floatx4 __attribute__((ext_vector_type(4)));
-    floatx4 acc[kProjRows / 16][2];
+    floatx4 acc[kProjRows / 16][kProjStepBlocks];
 #pragma unroll
     for (int sb = 0; sb < kProjRows / 16; ++sb)
 #pragma unroll
-        for (int tb = 0; tb < 2; ++tb) acc[sb][tb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int tb = 0; tb < kProjStepBlocks; ++tb) acc[sb][tb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     // loader: 8 lanes cover one row's 32 k values (one 128-byte line). The
     // next k tile is loaded into registers while the MFMAs run on this one.
     const int lrow = tid >> 3, lk = (tid & 7) * 4;
@@ -499,35 +504,37 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
         if (kb + kGemmTileK < k1) load_tile(kb + kGemmTileK);
         if (xT && blockIdx.z == 0) {
             // the k-major tile is the states-by-step layout: one pass over x
-            // also writes xT (row k: 16 consecutive steps per thread)
-            const int kr = tid >> 3, c0 = (tid & 7) * 16;
+            // also writes xT (row k: kProjTile / 8 consecutive steps per thread)
+            constexpr int kRun = kProjTile / 8;
+            const int kr = tid >> 3, c0 = (tid & 7) * kRun;
             const int64_t k = kb + kr;
             if (k < k1) {
                 float *dst = xT + k * T + t0 + c0;
-                if (t0 + c0 + 16 <= T) {
+                if (t0 + c0 + kRun <= T) {
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
+                    for (int q = 0; q < kRun / 4; ++q) {
                         float4_a4 v;
 #pragma unroll
                         for (int j = 0; j < 4; ++j) v[j] = As[kr][c0 + 4 * q + j];
                         *reinterpret_cast<float4_a4 *>(dst + 4 * q) = v;
                     }
                 } else {
-                    for (int c = 0; c < 16 && t0 + c0 + c < T; ++c) dst[c] = As[kr][c0 + c];
+                    for (int c = 0; c < kRun && t0 + c0 + c < T; ++c) dst[c] = As[kr][c0 + c];
                 }
             }
         }
 #pragma unroll
         for (int kq = 0; kq < kGemmTileK; kq += 4) {
             const int kr = kq + lq;
-            float bv[2];
+            float bv[kProjStepBlocks];
 #pragma unroll
-            for (int tb = 0; tb < 2; ++tb) bv[tb] = As[kr][wave * 32 + tb * 16 + li];
+            for (int tb = 0; tb < kProjStepBlocks; ++tb)
+                bv[tb] = As[kr][wave * kProjWaveSteps + tb * 16 + li];
 #pragma unroll
             for (int sb = 0; sb < kProjRows / 16; ++sb) {
                 const float av = Bs[kr][sb * 16 + li];
 #pragma unroll
-                for (int tb = 0; tb < 2; ++tb)
+                for (int tb = 0; tb < kProjStepBlocks; ++tb)
                     acc[sb][tb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[tb], acc[sb][tb], 0, 0, 0);
             }
         }
@@ -537,11 +544,11 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
 #pragma unroll
     for (int sb = 0; sb < kProjRows / 16; ++sb)
 #pragma unroll
-        for (int tb = 0; tb < 2; ++tb)
+        for (int tb = 0; tb < kProjStepBlocks; ++tb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int32_t s = s0 + sb * 16 + 4 * lq + r;
-                const int32_t t = t0 + wave * 32 + tb * 16 + li;
+                const int32_t t = t0 + wave * kProjWaveSteps + tb * 16 + li;
                 if (t < T && s < Sp) partial[(int64_t(split) * T + t) * Sp + s] = acc[sb][tb][r];
             }
 }
