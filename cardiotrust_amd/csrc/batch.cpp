@@ -228,7 +228,12 @@ int plan(ct_batch &b, int32_t vpg) {
             R.propagate.push_back(s);
             const bool fits = scenario_resident(r) &&
                               (r.N + ct::kResMaxVoxels - 1) / ct::kResMaxVoxels <= b.slots;
-            if (fits && b.slots > 0) {
+            // a scenario that alone needs more than half the slots at the
+            // largest slab gains nothing from sharing the launch: it runs its
+            // own resident launch, whose kernel also keeps gain rows in
+            // registers (8 shells of 115k voxels: 53 against 58 ms each)
+            const bool alone = int64_t(r.N) * 2 > int64_t(b.slots) * ct::kResMaxVoxels;
+            if (fits && !alone && b.slots > 0) {
                 R.resident.push_back(s);
                 rs.push_back(&r);
             }
@@ -312,6 +317,14 @@ bool plan_current(const ct_batch &b) {
 int propagate_round(ct_batch &b, int32_t rd, bool timed) {
     const ct_batch::Round &R = b.rounds[rd];
     hipStream_t s = b.gpu->stream;
+    if (R.propagate.empty()) return CT_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timed) {  // the round's whole propagation: own launches and the batched one
+        CT_HIP(hipEventCreate(&e0));
+        CT_HIP(hipEventCreate(&e1));
+        b.prop_events.push_back({e0, e1});
+        CT_HIP(hipEventRecord(e0, s));
+    }
     for (int32_t sc : R.propagate) {
         ct_epoch_kernel &k = *b.kernels[sc];
         const bool in_launch =
@@ -321,13 +334,9 @@ int propagate_round(ct_batch &b, int32_t rd, bool timed) {
         else
             CT_CHECK(propagate_beat(*k.r, nullptr, b.plan_checkpoints[sc]));
     }
-    if (R.resident.empty()) return CT_OK;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (timed) {
-        CT_HIP(hipEventCreate(&e0));
-        CT_HIP(hipEventCreate(&e1));
-        b.prop_events.push_back({e0, e1});
-        CT_HIP(hipEventRecord(e0, s));
+    if (R.resident.empty()) {
+        if (timed) CT_HIP(hipEventRecord(e1, s));
+        return CT_OK;
     }
     // the members' wait limits (ct_results_set_wait_limit): the shortest bounds
     // every step; a job's first wait, which may include earlier jobs on its
@@ -475,7 +484,10 @@ int autotune(ct_batch &b) {
     std::vector<std::vector<int32_t>> seen;
     for (int32_t vpg : kCandidates) {
         CT_CHECK(plan(b, vpg));
-        if (b.rounds.empty() || b.rounds[0].resident.empty()) return CT_OK;
+        if (b.rounds.empty() || b.rounds[0].resident.empty()) {
+            b.voxels_per_group = best;  // nothing to tune: no batched launch in round 0
+            return CT_OK;
+        }
         // skip candidates that give the same team widths as one already timed
         std::vector<ct::ResidentJob> j0(b.rounds[0].resident.size());
         CT_CHECK(download(j0.data(), b.jobs + b.rounds[0].job0, sizeof(ct::ResidentJob) * j0.size()));

@@ -382,13 +382,16 @@ typedef struct {
     int32_t rounds;                       /* = largest number of beats */
     int32_t slots;                        /* workgroups of the batched launch (0 = unused) */
     int32_t voxels_per_group;             /* planned voxels per workgroup */
-    int32_t jobs;                         /* jobs over all rounds */
+    int32_t jobs;                         /* jobs of the batched launches over all rounds (a
+                                             scenario needing more than half the slots runs
+                                             its own resident launch instead) */
     int32_t graph;                        /* 1 if the last epoch replayed a captured graph */
     double predicted_propagate_ms;        /* planner's makespan estimate, all rounds */
     double epoch_ms;                      /* device time of the last epoch (HIP events) */
-    double propagate_ms;                  /* batched propagation launches of the last epoch
+    double propagate_ms;                  /* the last epoch's propagation, all rounds: the
+                                             batched launch and the scenarios' own ones
                                              (profiling on, eager epochs only) */
-    int64_t propagate_launches;
+    int64_t propagate_launches;           /* rounds with propagation in that epoch */
     int64_t voxel_steps;                  /* sum over scenarios of N * T * B */
     int32_t tuned_candidates;             /* autotuner: voxels-per-workgroup values timed */
     int32_t tuned_voxels[16];

@@ -163,7 +163,9 @@ def test_shell_adjoint_batch_8x120k_equals_alone_and_fd(gpu):
         diff = [f for f in FIELDS if got[f] != alone[i][f]]
         assert not diff, f"scenario {i}: {diff}"
     assert infos[-1]["graph"] == 1
-    assert infos[-1]["jobs"] == len(ps)  # one propagation per scenario and epoch
+    # each ~115k-voxel scenario fills the GPU alone: it runs its own resident
+    # launch instead of a job of the batched one
+    assert infos[-1]["jobs"] == 0
 
     # member 0 (learning rate 0): its gradient against FP64 central differences
     m, p, cfg = members[0], ps[0], cfgs[0]
