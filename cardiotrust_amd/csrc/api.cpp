@@ -849,8 +849,8 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
     // GEMM split-K plan: about 2048 workgroups for the projection.
     const int64_t tiles = (T + ct::kProjTile - 1) / ct::kProjTile * ct::project_sensor_tiles(int32_t(S));
     int64_t splits = std::max<int64_t>(1, std::min<int64_t>((n + 31) / 32, 2048 / std::max<int64_t>(tiles, 1)));
-    int64_t kps = ((n + splits - 1) / splits + ct::kGemmTileK - 1) / ct::kGemmTileK * ct::kGemmTileK;
-    kps = std::max<int64_t>(kps, ct::kGemmTileK);
+    int64_t kps = ((n + splits - 1) / splits + ct::kProjTileK - 1) / ct::kProjTileK * ct::kProjTileK;
+    kps = std::max<int64_t>(kps, ct::kProjTileK);
     splits = std::max<int64_t>(1, (n + kps - 1) / kps);
     r->splits = int32_t(splits);
     r->kps = kps;

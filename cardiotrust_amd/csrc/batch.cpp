@@ -38,6 +38,7 @@ struct ct_batch {
         int32_t flag0 = 0, nflags = 0;    // the round's slab flags (one per team member)
         int32_t task0 = 0;                // first task of the round
         int32_t ptr0 = 0;                 // first task_ptr entry (slots + 1 per round)
+        bool coherent = true;             // every job of the launch exchanges rows coherently
         double predicted_ms = 0.0;
     };
     std::vector<Round> rounds;
@@ -245,6 +246,7 @@ int plan(ct_batch &b, int32_t vpg) {
         for (int32_t s : R.resident) {
             ct_results_gpu &r = *b.kernels[s]->r;
             jobs.push_back(resident_job(r, b.plan_checkpoints[s]));
+            R.coherent = R.coherent && jobs.back().coherent != 0;
         }
         std::vector<std::vector<int32_t>> tasks;
         R.predicted_ms = rs.empty() ? 0.0 : place_round(rs, b.slots, vpg, jobs, tasks);
@@ -346,7 +348,8 @@ int propagate_round(ct_batch &b, int32_t rd, bool timed) {
     const double first_s = std::min(60.0, 30.0 * step_s);
     CT_HIP(ct::launch_propagate_batch(s, b.slots, b.jobs + R.job0, b.task_ptr + R.ptr0,
                                       b.task_job, b.barriers, 1, b.flags + R.flag0, R.nflags,
-                                      wait_ticks(*b.gpu, first_s), wait_ticks(*b.gpu, step_s)));
+                                      wait_ticks(*b.gpu, first_s), wait_ticks(*b.gpu, step_s),
+                                      R.coherent));
     if (timed) CT_HIP(hipEventRecord(e1, s));
     return CT_OK;
 }

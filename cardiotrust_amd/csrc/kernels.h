@@ -14,6 +14,7 @@
 namespace ct {
 
 constexpr int kGemmTileK = 32;
+constexpr int kProjTileK = 32;  // states per k tile of the projection H·Xᵀ (its split-K unit)
 constexpr int kProjTile = 128;  // steps per project workgroup (H·Xᵀ)
 constexpr int kProjRows = 112;  // sensors per project workgroup (seven 16-row MFMA blocks)
 // Sensor tiles of the projection for S sensors (the tiles cover rows up to
@@ -90,12 +91,14 @@ hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const Reside
 // Batched resident propagation: `slots` workgroups (all resident at once) run
 // the task lists task_job[task_ptr[q] .. task_ptr[q+1]) of slot q; `jobs`, the
 // `num_jobs` abort words and the jobs' `num_flags` slab flags are device
-// arrays, cleared here before the launch.
+// arrays, cleared here before the launch. `coherent`: every job has
+// coherent != 0 (otherwise all jobs take the fenced exchange).
 hipError_t batch_capacity(int32_t *blocks_per_cu);
 hipError_t launch_propagate_batch(hipStream_t s, int32_t slots, const ResidentJob *jobs,
                                   const int32_t *task_ptr, const int32_t *task_job,
                                   GridBarrier *barriers, int32_t num_jobs, uint32_t *flags,
-                                  int32_t num_flags, uint64_t first_wait, uint64_t step_wait);
+                                  int32_t num_flags, uint64_t first_wait, uint64_t step_wait,
+                                  bool coherent);
 
 // H·Xᵀ split-K partial sums; with xT not null the same pass also writes the
 // states transposed, xT (n, T).

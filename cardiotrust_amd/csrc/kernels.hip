@@ -8,6 +8,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "barrier.h"
 #include "device.h"
 #include "lds_sort.h"
@@ -161,12 +163,139 @@ constexpr int kResNeighbourBits = 17;  // voxel index field; delay or connection
 static_assert(kSortSize >= kResMaxVoxels * 26 && kSortSize < (1 << 14) + 1, "sort size");
 static_assert(kSortSize * 8 <= kResMaxVoxels * 78 * 4, "sort keys must fit over the filter outputs");
 
+// Steps [t_begin, t_end) of the slab set up by resident_run. kCoherent: the
+// job exchanges rows through agent-coherent accesses. `sorted`: the slab's
+// connections run in (delay, neighbour) order with the delays in dtab; both
+// forms are computed and selected, so the per-connection code has no branch.
+template <int kGather, int kGainConns, int kCachedRows, bool kCoherent>
+__device__ __forceinline__ bool resident_steps(
+    const ResidentJob &j, int32_t vb, int32_t nv, int32_t t_begin, int32_t t_end, bool sorted,
+    float *ys,
+    int *ok_lds, const uint8_t *dtab, const uint32_t (&pk)[kResItems],
+    const float (&ak)[kResItems], float (&pv)[kResItems][3],
+    const float (&gc)[kCachedRows > 0 ? kCachedRows : 1], int32_t smin, int32_t smax,
+    int32_t me, uint64_t first_wait, uint64_t step_wait) {
+    const int32_t N = j.N, T = j.T;
+    const float *__restrict__ gT = j.gT;
+    const float *__restrict__ b = j.b;
+    const float *__restrict__ u = j.u;
+    float *__restrict__ x = j.x;
+    // the states as one buffer resource (raw, byte offsets): with kCoherent the
+    // gathers carry sc1 and do not hit a stale copy in this XCD's L2
+    const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc(x, 0, -1, kBufferWord3);
+    const int64_t n = 3LL * N;
+    const int tid = threadIdx.x;
+    for (int32_t t = t_begin; t < t_end; ++t) {
+        if (t > t_begin &&
+            !slab_wait(j.flags, j.gb, smin, smax, me, uint32_t(t - t_begin), ok_lds,
+                       t == t_begin + 1 ? first_wait : step_wait, kCoherent))
+            return false;  // host reports the abort
+        if (t == T - 1) {
+            // the filter outputs before the last step (y_last: [v][3d+out], the
+            // layout of ys; a missing connection's +0 equals its cleared value)
+            for (int idx = tid; idx < nv * 78; idx += kResThreads)
+                j.y_last[int64_t(vb) * 78 + idx] = ys[idx];
+            __syncthreads();
+        }
+#pragma unroll
+        for (int k0 = 0; k0 < kResItems; k0 += kGather) {
+            float in0[kGather][3];
+#pragma unroll
+            for (int q = 0; q < kGather; ++q) {
+                const int k = k0 + q;
+#pragma unroll
+                for (int o = 0; o < 3; ++o) in0[q][o] = 0.0f;
+                if (k >= kResItems || pk[k] == kNoConnection) continue;
+                // An opaque copy: otherwise the compiler keeps every value derived
+                // from pk (addresses, delays) live across the step loop and spills.
+                uint32_t pko = pk[k];
+                asm volatile("" : "+v"(pko));
+                // 32-bit element offsets: the host only picks this kernel when T * n < 2^32
+                const uint32_t w = pko & ((1u << kResNeighbourBits) - 1);
+                const int32_t Ds = int32_t(dtab[tid + k * kResThreads]);
+                const int32_t D = sorted ? Ds : int32_t(pko >> kResNeighbourBits);
+                if (D <= t) {
+                    // the neighbour's three states: one 12-byte load
+                    const uint32_t r0 = uint32_t(t - D) * uint32_t(n) + 3u * w;
+                    float3_a4 v;
+                    if constexpr (kCoherent)
+                        v = __builtin_amdgcn_raw_buffer_load_b96(xrs, r0 * 4u, 0, kCacheSC1);
+                    else
+                        v = *reinterpret_cast<const float3_a4 *>(x + r0);
+                    in0[q][0] = v.x;
+                    in0[q][1] = v.y;
+                    in0[q][2] = v.z;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kGather; ++q) {
+                const int k = k0 + q;
+                if (k >= kResItems || pk[k] == kNoConnection) continue;
+                uint32_t pko = pk[k];
+                asm volatile("" : "+v"(pko));
+                const int it = sorted ? int(pko >> kResNeighbourBits) : tid + k * kResThreads;
+#pragma unroll
+                for (int o = 0; o < 3; ++o) {
+                    const float prev = ys[it * 3 + o];
+                    ys[it * 3 + o] = fmaf(ak[k], in0[q][o] - prev, pv[k][o]);
+                    pv[k][o] = in0[q][o];
+                }
+            }
+        }
+        __syncthreads();
+        // Phase 2 (a lane per state): x_t[i] = sum over the 78 (connection,
+        // output) pairs in order of G[i][.] * y, then + u_t b[i]. A missing
+        // connection's y is +0 (reset per beat, never written) and its term a
+        // signed zero, which leaves the sum (started at +0) unchanged: the
+        // sum over all 78 equals the reference's sum over the existing ones
+        // bit for bit whenever the missing connections' gains are finite.
+        const float ut = u[t];
+#pragma unroll 1
+        for (int k = 0; k < kResStates; ++k) {
+            const int sidx = tid + k * kResThreads;
+            if (sidx >= nv * 3) break;
+            const float *yv = ys + (sidx / 3) * 78;
+            const uint32_t i = uint32_t(3 * vb + sidx);
+            float acc = 0.0f;
+#pragma unroll
+            for (int h = 0; h < 78; h += 3 * kGainConns) {
+                float g[3 * kGainConns];
+                if (h + 3 * kGainConns <= kCachedRows && k == 0) {
+#pragma unroll
+                    for (int q = 0; q < 3 * kGainConns; ++q)
+                        g[q] = gc[h + q < kCachedRows ? h + q : 0];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 3 * kGainConns; ++q)
+                        if (h + q < 78) g[q] = (gT + size_t(h + q) * size_t(n))[i];  // row h+q, uniform
+                }
+#pragma unroll
+                for (int q = 0; q < 3 * kGainConns; ++q)
+                    if (h + q < 78) acc = acc + g[q] * yv[h + q];
+            }
+            // written through the L2: the other slabs read it coherently
+            __hip_atomic_store(x + int64_t(t) * n + i, acc + ut * b[i], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // filter outputs y_t for the reverse-mode gradient, every ckpt_interval steps
+        const int32_t K = j.ckpt_interval;
+        if (K > 0 && (t + 1) % K == 0 && t + 1 < T) {
+            float *dst = j.ckpt + int64_t((t + 1) / K - 1) * N * 78 + int64_t(vb) * 78;
+            for (int idx = tid; idx < nv * 78; idx += kResThreads) dst[idx] = ys[idx];
+        }
+        slab_publish(j.flags, me, uint32_t(t + 1 - t_begin), kCoherent);
+    }
+    for (int idx = tid; idx < nv * 78; idx += kResThreads) j.y[int64_t(vb) * 78 + idx] = ys[idx];
+    return true;
+}
+
+
 // Steps [t_begin, t_end) of voxels [vb, ve), one slab of job `j`. After each
 // step the slab publishes its progress in j.flags; before the next it waits
 // for the slabs owning its neighbours, the first wait up to `first_wait`
 // ticks (they may still be finishing earlier jobs), the others `step_wait`.
 // Returns false if the job aborted (its sticky abort word is set).
-template <int kGather, int kGainConns, int kCachedRows>
+template <int kGather, int kGainConns, int kCachedRows, bool kCoherent>
 __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, int32_t ve,
                                              int32_t t_begin, int32_t t_end,
                                              float *lds, uint64_t first_wait, uint64_t step_wait) {
@@ -177,13 +306,7 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
     const int32_t N = j.N, T = j.T;
     const int32_t *__restrict__ nbr = j.nbr;
     const float *__restrict__ gT = j.gT;
-    const float *__restrict__ b = j.b;
-    const float *__restrict__ u = j.u;
-    float *__restrict__ x = j.x;
-    // the states as one buffer resource (raw, byte offsets): gathers that
-    // bypass the L2 (sc1) when the rows are exchanged coherently
-    const bool coherent = j.coherent != 0;
-    const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc(x, 0, -1, kBufferWord3);
+    const float *__restrict__ x = j.x;
     const int64_t n = 3LL * N;
     const int32_t nv = ve - vb;
     const int32_t items = nv * 26;
@@ -279,112 +402,22 @@ __device__ __forceinline__ bool resident_run(const ResidentJob &j, int32_t vb, i
     for (int idx = tid; idx < nv * 78; idx += kResThreads) ys[idx] = j.y[int64_t(vb) * 78 + idx];
     __syncthreads();
 
-    for (int32_t t = t_begin; t < t_end; ++t) {
-        const bool last = (t == T - 1);
-        if (t > t_begin &&
-            !slab_wait(j.flags, j.gb, smin, smax, me, uint32_t(t - t_begin), ok_lds,
-                       t == t_begin + 1 ? first_wait : step_wait, coherent))
-            return false;  // host reports the abort
-#pragma unroll
-        for (int k0 = 0; k0 < kResItems; k0 += kGather) {
-            float in0[kGather][3];
-#pragma unroll
-            for (int q = 0; q < kGather; ++q) {
-                const int k = k0 + q;
-#pragma unroll
-                for (int o = 0; o < 3; ++o) in0[q][o] = 0.0f;
-                if (k >= kResItems || pk[k] == kNoConnection) continue;
-                // An opaque copy: otherwise the compiler keeps every value derived
-                // from pk (addresses, delays) live across the step loop and spills.
-                uint32_t pko = pk[k];
-                asm volatile("" : "+v"(pko));
-                // 32-bit element offsets: the host only picks this kernel when T * n < 2^32
-                const uint32_t w = pko & ((1u << kResNeighbourBits) - 1);
-                const int32_t D = sorted ? int32_t(dtab[tid + k * kResThreads])
-                                         : int32_t(pko >> kResNeighbourBits);
-                if (D <= t) {
-                    // the neighbour's three states: one 12-byte load
-                    const uint32_t r0 = uint32_t(t - D) * uint32_t(n) + 3u * w;
-                    const float3_a4 v =
-                        coherent ? __builtin_amdgcn_raw_buffer_load_b96(xrs, r0 * 4u, 0, kCacheSC1)
-                                 : *reinterpret_cast<const float3_a4 *>(x + r0);
-                    in0[q][0] = v.x;
-                    in0[q][1] = v.y;
-                    in0[q][2] = v.z;
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < kGather; ++q) {
-                const int k = k0 + q;
-                if (k >= kResItems || pk[k] == kNoConnection) continue;
-                uint32_t pko = pk[k];
-                asm volatile("" : "+v"(pko));
-                const int it = sorted ? int(pko >> kResNeighbourBits) : tid + k * kResThreads;
-#pragma unroll
-                for (int o = 0; o < 3; ++o) {
-                    const float prev = ys[it * 3 + o];
-                    if (last) j.y_last[(int64_t(vb) * 26 + it) * 3 + o] = prev;
-                    ys[it * 3 + o] = fmaf(ak[k], in0[q][o] - prev, pv[k][o]);
-                    pv[k][o] = in0[q][o];
-                }
-            }
-        }
-        __syncthreads();
-        // Phase 2 (a lane per state): x_t[i] = sum over the 78 (connection,
-        // output) pairs in order of G[i][.] * y, then + u_t b[i]. A missing
-        // connection's y is +0 (reset per beat, never written) and its term a
-        // signed zero, which leaves the sum (started at +0) unchanged: the
-        // sum over all 78 equals the reference's sum over the existing ones
-        // bit for bit whenever the missing connections' gains are finite.
-        const float ut = u[t];
-#pragma unroll 1
-        for (int k = 0; k < kResStates; ++k) {
-            const int sidx = tid + k * kResThreads;
-            if (sidx >= nv * 3) break;
-            const float *yv = ys + (sidx / 3) * 78;
-            const uint32_t i = uint32_t(3 * vb + sidx);
-            float acc = 0.0f;
-#pragma unroll
-            for (int h = 0; h < 78; h += 3 * kGainConns) {
-                float g[3 * kGainConns];
-                if (h + 3 * kGainConns <= kCachedRows && k == 0) {
-#pragma unroll
-                    for (int q = 0; q < 3 * kGainConns; ++q)
-                        g[q] = gc[h + q < kCachedRows ? h + q : 0];
-                } else {
-#pragma unroll
-                    for (int q = 0; q < 3 * kGainConns; ++q)
-                        if (h + q < 78) g[q] = (gT + size_t(h + q) * size_t(n))[i];  // row h+q, uniform
-                }
-#pragma unroll
-                for (int q = 0; q < 3 * kGainConns; ++q)
-                    if (h + q < 78) acc = acc + g[q] * yv[h + q];
-            }
-            // written through the L2: the other slabs read it coherently
-            __hip_atomic_store(x + int64_t(t) * n + i, acc + ut * b[i], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-        // filter outputs y_t for the reverse-mode gradient, every ckpt_interval steps
-        const int32_t K = j.ckpt_interval;
-        if (K > 0 && (t + 1) % K == 0 && t + 1 < T) {
-            float *dst = j.ckpt + int64_t((t + 1) / K - 1) * N * 78 + int64_t(vb) * 78;
-            for (int idx = tid; idx < nv * 78; idx += kResThreads) dst[idx] = ys[idx];
-        }
-        slab_publish(j.flags, me, uint32_t(t + 1 - t_begin), coherent);
-    }
-    for (int idx = tid; idx < nv * 78; idx += kResThreads) j.y[int64_t(vb) * 78 + idx] = ys[idx];
-    return true;
+    return resident_steps<kGather, kGainConns, kCachedRows, kCoherent>(
+        j, vb, nv, t_begin, t_end, sorted, ys, ok_lds, dtab, pk, ak, pv, gc, smin, smax, me,
+        first_wait, step_wait);
 }
 
 // One job over the whole grid: workgroup slab = xcd_block(blockIdx.x).
+// kCoherent = job.coherent != 0 (the launcher picks the instantiation).
+template <bool kCoherent>
 __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
     ResidentJob job, int32_t t_begin, int32_t t_end, uint64_t wait_ticks) {
     extern __shared__ float lds[];
     const int64_t slab = xcd_block(blockIdx.x, gridDim.x);
     const int32_t vb = int32_t(slab * job.N / gridDim.x);
     const int32_t ve = int32_t((slab + 1) * job.N / gridDim.x);
-    resident_run<kResBatch, kResGainConns, kResCachedRows>(job, vb, ve, t_begin, t_end, lds,
-                                           wait_ticks, wait_ticks);
+    resident_run<kResBatch, kResGainConns, kResCachedRows, kCoherent>(
+        job, vb, ve, t_begin, t_end, lds, wait_ticks, wait_ticks);
 }
 
 // Many jobs in one persistent launch. Logical slot = xcd_block(blockIdx.x), so
@@ -393,7 +426,10 @@ __global__ __launch_bounds__(kResThreads) void propagate_resident_kernel(
 // contiguous slots [slot0, slot0 + width) and member (q - slot0) owns voxels
 // [m N / width, (m+1) N / width). Every slot lists its jobs in one global
 // order, so the earliest unfinished job always has its whole team and the
-// launch cannot deadlock.
+// launch cannot deadlock. kCoherent: every job exchanges its rows coherently
+// (job.coherent != 0); otherwise every job takes the fenced exchange, which
+// is correct for any job.
+template <bool kCoherent>
 __global__ __launch_bounds__(kResThreads) void propagate_batch_kernel(
     const ResidentJob *__restrict__ jobs, const int32_t *__restrict__ task_ptr,
     const int32_t *__restrict__ task_job, uint64_t first_wait, uint64_t step_wait) {
@@ -405,8 +441,8 @@ __global__ __launch_bounds__(kResThreads) void propagate_batch_kernel(
         const int32_t member = slot - job.slot0;
         const int32_t vb = int32_t(int64_t(member) * job.N / job.width);
         const int32_t ve = int32_t(int64_t(member + 1) * job.N / job.width);
-        if (!resident_run<kBatchGather, kBatchGainConns, 0>(job, vb, ve, 0, job.T,
-                                                         lds, first_wait, step_wait))
+        if (!resident_run<kBatchGather, kBatchGainConns, 0, kCoherent>(job, vb, ve, 0, job.T, lds,
+                                                                   first_wait, step_wait))
             return;
         __syncthreads();  // LDS is reused by the next job
     }
@@ -436,12 +472,18 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
     // k-major tiles: a 16x16x4 operand read takes 16 consecutive floats of
     // four k rows; strides = 16 mod 32 words put the two rows of each 32-lane
     // half on disjoint banks
-    __shared__ float As[kGemmTileK][kProjStepsPad];  // X[t][k], kProjTile steps
-    __shared__ float Bs[kGemmTileK][kProjRowsPad];   // H[s][k], kProjRows sensors
+    __shared__ float As[kProjTileK][kProjStepsPad];  // X[t][k], kProjTile steps
+    __shared__ float Bs[kProjTileK][kProjRowsPad];   // H[s][k], kProjRows sensors
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int32_t t0 = blockIdx.x * kProjTile;
-    const int32_t split = blockIdx.y;
-    const int32_t s0 = blockIdx.z * kProjRows;
+    // Workgroups are numbered (step tile fastest, split, sensor tile) and
+    // remapped XCD-contiguously: the step tiles of one split, which read the
+    // same H columns, run on one XCD and share its L2.
+    const int32_t nb = gridDim.x * gridDim.y * gridDim.z;
+    const int32_t lin =
+        xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nb);
+    const int32_t t0 = (lin % gridDim.x) * kProjTile;
+    const int32_t split = (lin / gridDim.x) % gridDim.y;
+    const int32_t s0 = (lin / (gridDim.x * gridDim.y)) * kProjRows;
     const int64_t k0 = int64_t(split) * kps;
     const int64_t k1 = min(n, k0 + kps);
     typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -450,16 +492,19 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
     for (int sb = 0; sb < kProjRows / 16; ++sb)
 #pragma unroll
         for (int tb = 0; tb < kProjStepBlocks; ++tb) acc[sb][tb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-    // loader: 8 lanes cover one row's 32 k values (one 128-byte line). The
-    // next k tile is loaded into registers while the MFMAs run on this one.
-    const int lrow = tid >> 3, lk = (tid & 7) * 4;
-    constexpr int kHPasses = (kProjRows + 31) / 32;
-    float4_a4 xa[kProjTile / 32], ha[kHPasses];
+    // loader: kProjTileK / 4 lanes cover one row's k values (kProjTileK / 32
+    // lines of 128 bytes). The next k tile is loaded into registers while the
+    // MFMAs run on this one.
+    constexpr int kRowLanes = kProjTileK / 4, kRowsPerPass = 256 / kRowLanes;
+    const int lrow = tid / kRowLanes, lk = (tid % kRowLanes) * 4;
+    constexpr int kXPasses = kProjTile / kRowsPerPass;
+    constexpr int kHPasses = (kProjRows + kRowsPerPass - 1) / kRowsPerPass;
+    float4_a4 xa[kXPasses], ha[kHPasses];
     auto load_tile = [&](int64_t kb) __attribute__((always_inline)) {
-        const bool full = kb + kGemmTileK <= k1;
+        const bool full = kb + kProjTileK <= k1;
 #pragma unroll
-        for (int pass = 0; pass < kProjTile / 32; ++pass) {
-            const int32_t t = t0 + pass * 32 + lrow;
+        for (int pass = 0; pass < kXPasses; ++pass) {
+            const int32_t t = t0 + pass * kRowsPerPass + lrow;
             float4_a4 v = {0.0f, 0.0f, 0.0f, 0.0f};
             if (t < T) {
                 const float *src = x + int64_t(t) * n + kb + lk;
@@ -474,7 +519,7 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
         }
 #pragma unroll
         for (int pass = 0; pass < kHPasses; ++pass) {
-            const int r = pass * 32 + lrow, srow = s0 + r;
+            const int r = pass * kRowsPerPass + lrow, srow = s0 + r;
             float4_a4 v = {0.0f, 0.0f, 0.0f, 0.0f};
             if (r < kProjRows && srow < Sp) {
                 const float *src = H + int64_t(srow) * n + kb + lk;
@@ -490,23 +535,23 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
     };
     if (k0 < k1) load_tile(k0);
     const int li = lane & 15, lq = lane >> 4;
-    for (int64_t kb = k0; kb < k1; kb += kGemmTileK) {
+    for (int64_t kb = k0; kb < k1; kb += kProjTileK) {
 #pragma unroll
-        for (int pass = 0; pass < kProjTile / 32; ++pass)
+        for (int pass = 0; pass < kXPasses; ++pass)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) As[lk + j][pass * 32 + lrow] = xa[pass][j];
+            for (int j = 0; j < 4; ++j) As[lk + j][pass * kRowsPerPass + lrow] = xa[pass][j];
 #pragma unroll
         for (int pass = 0; pass < kHPasses; ++pass)
-            if (pass * 32 + lrow < kProjRows)
+            if (pass * kRowsPerPass + lrow < kProjRows)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) Bs[lk + j][pass * 32 + lrow] = ha[pass][j];
+                for (int j = 0; j < 4; ++j) Bs[lk + j][pass * kRowsPerPass + lrow] = ha[pass][j];
         __syncthreads();
-        if (kb + kGemmTileK < k1) load_tile(kb + kGemmTileK);
-        if (xT && blockIdx.z == 0) {
+        if (kb + kProjTileK < k1) load_tile(kb + kProjTileK);
+        if (xT && s0 == 0) {
             // the k-major tile is the states-by-step layout: one pass over x
-            // also writes xT (row k: kProjTile / 8 consecutive steps per thread)
-            constexpr int kRun = kProjTile / 8;
-            const int kr = tid >> 3, c0 = (tid & 7) * kRun;
+            // also writes xT (row k: kRun consecutive steps per thread)
+            constexpr int kRowThreads = 256 / kProjTileK, kRun = kProjTile / kRowThreads;
+            const int kr = tid / kRowThreads, c0 = (tid % kRowThreads) * kRun;
             const int64_t k = kb + kr;
             if (k < k1) {
                 float *dst = xT + k * T + t0 + c0;
@@ -539,9 +584,9 @@ __global__ __launch_bounds__(256) void project_kernel(const float *__restrict__ 
         // interleave the operand reads with the MFMAs (2.25 -> 2.16 ms)
         __builtin_amdgcn_iglp_opt(1);
 #pragma unroll
-        for (int kq = 0; kq < kGemmTileK; kq += 4) {
+        for (int kq = 0; kq < kProjTileK; kq += 4) {
             const int cur = (kq / 4) & 1;
-            if (kq + 4 < kGemmTileK) read_operands(kq + 4, cur ^ 1);
+            if (kq + 4 < kProjTileK) read_operands(kq + 4, cur ^ 1);
 #pragma unroll
             for (int sb = 0; sb < kProjRows / 16; ++sb)
 #pragma unroll
@@ -704,6 +749,112 @@ __global__ __launch_bounds__(256) void transpose_kernel(const float *__restrict_
     for (int j = ty; j < 32; j += 8) {
         const int64_t c = c0 + j, r = r0 + tx;
         if (r < rows && c < cols) out[c * rows + r] = tile[tx][j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mapped residuals Hᵀ·Rᵀ for up to 112 sensors (derivation.rs:538-551 for
+// every t), FP32 MFMA 32x32x2. A workgroup owns 128 states (32 per wave) and
+// a span of steps. Each lane loads its A operands for the whole contraction
+// once, H[2p + lane/32][its state] for every sensor pair p (56 registers), so
+// H is read from memory once per span; the loop over the span's 64-step
+// tiles then stages only R (L2-resident, 1 MB at T = 2000) in LDS, and the
+// next tile's R is loaded while the MFMAs run. The k pairs go in ascending
+// order, as in map_residuals_kernel, so both give the same bits.
+// ---------------------------------------------------------------------------
+constexpr int kMapRegPairs = kProjRows / 2;  // sensor pairs held in registers (S <= 112)
+constexpr int kMapRegTile = 64;              // steps per tile: two 32-step MFMA blocks
+constexpr int kMapRegSpan = 16;              // tiles per workgroup (1024 steps)
+constexpr int kMapRegStride = kMapRegTile + 4;  // LDS row (one sensor) in floats
+constexpr int kMapRegQuads = kProjRows / 16;    // float4 of R per thread and tile
+static_assert(kMapRegTile == 64 && kProjRows % 16 == 0, "R tile: 64 steps x 4 threads per step");
+__global__ __launch_bounds__(256) void map_residuals_regs_kernel(const float *__restrict__ H,
+                                                                 const float *__restrict__ R,
+                                                                 float *__restrict__ eT, int64_t n,
+                                                                 int32_t T, int32_t Se,
+                                                                 int32_t Sp, int32_t accumulate) {
+    // R tile, k-major: sensor k, tile step u = 32 q + l at Rs[k][2 l + q], so a
+    // lane's operands of both 32-step blocks are one 8-byte read
+    __shared__ float Rs[kProjRows][kMapRegStride];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t i0 = int64_t(blockIdx.x) * 128;
+    const int64_t iw = i0 + wave * 32 + (lane & 31);
+    const int kh = lane >> 5;
+    const int32_t tb = blockIdx.y * kMapRegSpan * kMapRegTile;
+    const int32_t te = min(T, tb + kMapRegSpan * kMapRegTile);
+    // unconditional loads from clamped addresses (all in flight at once), then
+    // zeros where the pair or the state is out of range
+    float a[kMapRegPairs];
+    const float *hcol = H + min(iw, n - 1) + int64_t(kh) * n;
+#pragma unroll
+    for (int p = 0; p < kMapRegPairs; ++p) a[p] = hcol[int64_t(min(2 * p, Se - 2)) * n];
+#pragma unroll
+    for (int p = 0; p < kMapRegPairs; ++p) a[p] = (2 * p < Se && iw < n) ? a[p] : 0.0f;
+    // eT rows [i0, min(n, i0 + 128)) as one buffer: stores past them are dropped
+    const int64_t rows = min(int64_t(128), n - i0);
+    const __amdgpu_buffer_rsrc_t ers =
+        __builtin_amdgcn_make_buffer_rsrc(eT + i0 * T, 0, int32_t(4 * rows * T), kBufferWord3);
+    const int32_t lane_off = 4 * ((wave * 32 + kh * 4) * T + (lane & 31));
+    // R loads: thread = tile step tt (4 threads per step), sensors 16 m + 4 c .. + 3
+    const int tt = tid >> 2, c4 = (tid & 3) * 4;
+    float *const rs_w = &Rs[c4][2 * (tt & 31) + (tt >> 5)];
+    float4_a4 rv[kMapRegQuads];
+    auto load_r = [&](int32_t t0) __attribute__((always_inline)) {
+        const bool in_t = t0 + tt < T;
+        const float *src = R + int64_t(t0 + tt) * Sp + c4;
+#pragma unroll
+        for (int m = 0; m < kMapRegQuads; ++m) {
+            float4_a4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (in_t && 16 * m + c4 < Se) v = *reinterpret_cast<const float4_a4 *>(src + 16 * m);
+            rv[m] = v;
+        }
+    };
+    load_r(tb);
+    const float *const rs_r = &Rs[kh][2 * (lane & 31)];
+    for (int32_t t0 = tb; t0 < te; t0 += kMapRegTile) {
+        __syncthreads();  // the previous tile's operand reads are done
+#pragma unroll
+        for (int m = 0; m < kMapRegQuads; ++m)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) rs_w[(16 * m + j) * kMapRegStride] = rv[m][j];
+        __syncthreads();
+        if (t0 + kMapRegTile < te) load_r(t0 + kMapRegTile);
+        floatx16 acc0 = floatx16{0}, acc1 = floatx16{0};
+        // The operands of pair p + 2 are read (unconditionally: every row is
+        // in the tile) while pair p's MFMAs run; the MFMAs stop at Se. The
+        // loop has no break, so it unrolls and a[] stays in registers.
+        // an opaque copy: otherwise the 56 pair conditions are hoisted out of
+        // the tile loop and spill
+        int32_t se = Se;
+        asm volatile("" : "+s"(se));
+        float2 bq[kMapRegPairs];
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+            bq[p] = *reinterpret_cast<const float2 *>(rs_r + 2 * p * kMapRegStride);
+#pragma unroll
+        for (int p = 0; p < kMapRegPairs; ++p) {
+            if (p + 2 < kMapRegPairs)
+                bq[p + 2] = *reinterpret_cast<const float2 *>(rs_r + 2 * (p + 2) * kMapRegStride);
+            if (2 * p < se) {
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[p], bq[p].x, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[p], bq[p].y, acc1, 0, 0, 0);
+            }
+        }
+        // lane l holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]: state rows, step
+        // columns. Stored through the workgroup's buffer resource: the row part
+        // of each offset is uniform (scalar), the lane part one register.
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            if (t0 + q * 32 + (lane & 31) >= te) continue;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int32_t so = 4 * (((r >> 2) * 8 + (r & 3)) * T + t0 + q * 32);
+                float v = q == 0 ? acc0[r] : acc1[r];
+                if (accumulate)
+                    v += __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ers, lane_off, so, 0));
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), ers, lane_off, so, 0);
+            }
+        }
     }
 }
 
@@ -1627,17 +1778,29 @@ size_t resident_lds_bytes() {
     return sizeof(float) * kResMaxVoxels * 78 + 16 + kResMaxVoxels * 26;
 }
 
+// Workgroups of a resident kernel that fit on one CU at once: the smaller
+// count of its two instantiations (the launchers pick one per launch).
+template <typename K>
+hipError_t resident_fit(K kernel0, K kernel1, int32_t *blocks_per_cu) {
+    int fit = 1 << 30;
+    for (K k : {kernel0, kernel1}) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           int(resident_lds_bytes()));
+        if (e != hipSuccess) return e;
+        int nb = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(k),
+                                                         kResThreads, resident_lds_bytes());
+        if (e != hipSuccess) return e;
+        fit = std::min(fit, nb);
+    }
+    *blocks_per_cu = fit;
+    return hipSuccess;
+}
+
 hipError_t resident_capacity(int32_t *blocks_per_cu) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&propagate_resident_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       int(resident_lds_bytes()));
-    if (e != hipSuccess) return e;
-    int nb = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &nb, reinterpret_cast<const void *>(&propagate_resident_kernel), kResThreads,
-        resident_lds_bytes());
-    *blocks_per_cu = nb;
-    return e;
+    return resident_fit(&propagate_resident_kernel<true>, &propagate_resident_kernel<false>,
+                        blocks_per_cu);
 }
 
 hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const ResidentJob &job,
@@ -1647,8 +1810,12 @@ hipError_t launch_propagate_resident(hipStream_t s, int32_t blocks, const Reside
     if (!job.flags || blocks > kMaxSlabFlags) return hipErrorInvalidValue;
     hipError_t e = launch_resident_reset(s, job.gb, job.flags, blocks);
     if (e != hipSuccess) return e;
-    propagate_resident_kernel<<<blocks, kResThreads, resident_lds_bytes(), s>>>(job, t_begin, t_end,
-                                                                                wait_ticks);
+    if (job.coherent)
+        propagate_resident_kernel<true><<<blocks, kResThreads, resident_lds_bytes(), s>>>(
+            job, t_begin, t_end, wait_ticks);
+    else
+        propagate_resident_kernel<false><<<blocks, kResThreads, resident_lds_bytes(), s>>>(
+            job, t_begin, t_end, wait_ticks);
     return hipGetLastError();
 }
 
@@ -1731,22 +1898,15 @@ hipError_t launch_barrier_reset(hipStream_t s, GridBarrier *gb, int32_t count) {
 }
 
 hipError_t batch_capacity(int32_t *blocks_per_cu) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&propagate_batch_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       int(resident_lds_bytes()));
-    if (e != hipSuccess) return e;
-    int nb = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &nb, reinterpret_cast<const void *>(&propagate_batch_kernel), kResThreads,
-        resident_lds_bytes());
-    *blocks_per_cu = nb;
-    return e;
+    return resident_fit(&propagate_batch_kernel<true>, &propagate_batch_kernel<false>,
+                        blocks_per_cu);
 }
 
 hipError_t launch_propagate_batch(hipStream_t s, int32_t slots, const ResidentJob *jobs,
                                   const int32_t *task_ptr, const int32_t *task_job,
                                   GridBarrier *barriers, int32_t num_jobs, uint32_t *flags,
-                                  int32_t num_flags, uint64_t first_wait, uint64_t step_wait) {
+                                  int32_t num_flags, uint64_t first_wait, uint64_t step_wait,
+                                  bool coherent) {
     if (slots <= 0 || num_jobs <= 0) return hipSuccess;
     barrier_reset_kernel<<<blocks_for(int64_t(num_jobs) * 10, 256), 256, 0, s>>>(barriers, num_jobs);
     hipError_t e = hipGetLastError();
@@ -1756,8 +1916,12 @@ hipError_t launch_propagate_batch(hipStream_t s, int32_t slots, const ResidentJo
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    propagate_batch_kernel<<<slots, kResThreads, resident_lds_bytes(), s>>>(jobs, task_ptr, task_job,
-                                                                            first_wait, step_wait);
+    if (coherent)
+        propagate_batch_kernel<true><<<slots, kResThreads, resident_lds_bytes(), s>>>(
+            jobs, task_ptr, task_job, first_wait, step_wait);
+    else
+        propagate_batch_kernel<false><<<slots, kResThreads, resident_lds_bytes(), s>>>(
+            jobs, task_ptr, task_job, first_wait, step_wait);
     return hipGetLastError();
 }
 
@@ -1808,9 +1972,15 @@ hipError_t launch_map_residuals(hipStream_t s, const float *H, const float *R, f
                                 int64_t n, int32_t T, int32_t S, int32_t Sp, bool accumulate) {
     if (n <= 0) return hipSuccess;
     if (S <= 0 || S > Sp) return hipErrorInvalidValue;
+    const int32_t Se = (S + 1) / 2 * 2;
+    // the register-held kernel's buffer offsets (4 B x 128 rows x T) fit in 31 bits
+    if (Se <= kProjRows && int64_t(T) * 128 * 4 < (int64_t(1) << 31)) {
+        dim3 grid(blocks_for(n, 128), blocks_for(T, kMapRegSpan * kMapRegTile));
+        map_residuals_regs_kernel<<<grid, 256, 0, s>>>(H, R, eT, n, T, Se, Sp, accumulate ? 1 : 0);
+        return hipGetLastError();
+    }
     dim3 grid(blocks_for(n, 128), blocks_for(T, kMapTile));
-    map_residuals_kernel<<<grid, 256, 0, s>>>(H, R, eT, n, T, (S + 1) / 2 * 2, Sp,
-                                              accumulate ? 1 : 0);
+    map_residuals_kernel<<<grid, 256, 0, s>>>(H, R, eT, n, T, Se, Sp, accumulate ? 1 : 0);
     return hipGetLastError();
 }
 

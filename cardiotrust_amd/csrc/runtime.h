@@ -155,7 +155,7 @@ struct ct_results_gpu {
     ct::UpdateScalars *scal = nullptr;       // (B) device, per update of an epoch
     ct::UpdateScalars *scal_host = nullptr;  // (B) pinned host staging
     int32_t splits = 1, mr_blocks = 1;
-    int64_t kps = ct::kGemmTileK;
+    int64_t kps = ct::kProjTileK;
 
     ~ct_results_gpu() {
         for (void *p : allocations) (void)hipFree(p);
