@@ -153,26 +153,30 @@ constexpr int kSweepItems = (kResMaxVoxels * 26 + kSweepThreads - 1) / kSweepThr
 constexpr int kSweepBatch = 1;  // items per load group: 1 measured fastest (102 ms vs 112 at 4)
 constexpr int kConsGainStride = 10;  // gcons floats per (voxel, consumer) slot
 // Items per lane whose consumer gains and coefficient stay in registers for
-// the whole sweep instead of being re-read from gcons every step.
-constexpr int kSweepCachedItems = 2;
+// the whole sweep instead of being re-read from gcons every step (the fenced
+// exchange needs more registers and keeps fewer).
+constexpr int kSweepCachedCoherent = 4;
+constexpr int kSweepCachedFenced = 2;
 constexpr uint32_t kSweepNoEntry = 0xFFFFFFFFu;
 static_assert(kSortSize >= kSweepItems * kSweepThreads || kSortSize >= kResMaxVoxels * 26,
               "every phase-1 position has a key");
 static_assert(kSortSize * 8 <= kResMaxVoxels * 78 * 4, "sort keys must fit over q");
 
+// kCoherent: the xbar rows are stored through the L2 and gathered past it
+// (sc1), and the barrier needs no agent-scope fences (the launcher picks the
+// instantiation).
+template <bool kCoherent>
 __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
     const int32_t *__restrict__ cons_ptr, const int32_t *__restrict__ cons_idx,
     const int32_t *__restrict__ delays, const float *__restrict__ gains_rows,
     const float *__restrict__ coefs, float *__restrict__ gcons,
     const float *__restrict__ x, const float *__restrict__ E,
     float *__restrict__ xbar, GridBarrier *__restrict__ gb, int32_t N, int32_t T, AdjointParams p,
-    uint64_t wait_ticks, int32_t coherent_rows) {
+    uint64_t wait_ticks) {
+    constexpr int kSweepCachedItems = kCoherent ? kSweepCachedCoherent : kSweepCachedFenced;
     extern __shared__ float lds[];
     float *q_lds = lds;  // [voxel][26][3]
-    // the xbar history as one buffer resource: with `coherent`, rows are
-    // stored through the L2 and gathered past it (sc1), and the barrier needs
-    // no agent-scope fences
-    const bool coherent = coherent_rows != 0;
+    // the xbar history as one buffer resource
     const __amdgpu_buffer_rsrc_t xbrs = __builtin_amdgcn_make_buffer_rsrc(xbar, 0, -1, kBufferWord3);
     int *ok_lds = reinterpret_cast<int *>(lds + kResMaxVoxels * 78);  // [4]
     uint8_t *dtab = reinterpret_cast<uint8_t *>(ok_lds + 4);          // [phase-1 position]
@@ -277,7 +281,7 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
     for (int32_t s = T - 1; s >= 0; --s) {
 #pragma unroll 1
         for (uint32_t pass = 0; pass < 2; ++pass) {
-        if (pass == 1 && s < T - 1 && !team_wait(gb, gen, ok_lds, wait_ticks, coherent)) return;
+        if (pass == 1 && s < T - 1 && !team_wait(gb, gen, ok_lds, wait_ticks, kCoherent)) return;
         // phase 1: each item's q for step s (its ybar at tau = s + D). The
         // empty asm statement hides the lane index from the optimiser, so the
         // per-item addresses are recomputed each step instead of being kept
@@ -295,15 +299,17 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
                 const int pos = lane + k * kSweepThreads;
                 const bool present = k0 + kk < kSweepItems && pk[k] != kSweepNoEntry &&
                                      ((late >> k) & 1u) == pass;
-                const int32_t D = !present ? 0
-                                  : sorted ? int32_t(dtab[pos]) : int32_t(pk[k] >> 17);
+                // both forms of the delay, selected (no branch on `sorted`)
+                const int32_t Ds = int32_t(dtab[pos]);
+                const int32_t D = !present ? 0 : sorted ? Ds : int32_t(pk[k] >> 17);
                 const int32_t tau = s + D;
                 on[kk] = present && tau < T;
                 if (!on[kk]) continue;  // not this pass, absent, or tau beyond the beat
                 const int64_t xe = int64_t(tau) * n + 3LL * (pk[k] & 0x1FFFFu);
-                xv[kk] = coherent
-                             ? __builtin_amdgcn_raw_buffer_load_b96(xbrs, uint32_t(xe) * 4u, 0, kCacheSC1)
-                             : *reinterpret_cast<const float3_a4 *>(xbar + xe);
+                if constexpr (kCoherent)
+                    xv[kk] = __builtin_amdgcn_raw_buffer_load_b96(xbrs, uint32_t(xe) * 4u, 0, kCacheSC1);
+                else
+                    xv[kk] = *reinterpret_cast<const float3_a4 *>(xbar + xe);
                 if (k0 + kk < kSweepCachedItems) {
                     const int kc = k0 + kk < kSweepCachedItems ? k0 + kk : 0;
 #pragma unroll
@@ -355,7 +361,7 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
                 __hip_atomic_store(xbar + row + o, acc[o], __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);  // written through the L2
         }
-        team_arrive(gb, gridDim.x, blockIdx.x, gen, coherent);
+        team_arrive(gb, gridDim.x, blockIdx.x, gen, kCoherent);
     }
 }
 
@@ -760,17 +766,22 @@ hipError_t launch_adjoint_step(hipStream_t s, const int32_t *nbr, const float *c
 
 size_t sweep_lds_bytes() { return sizeof(float) * kResMaxVoxels * 78 + 16 + kResMaxVoxels * 26; }
 
+// Workgroups of the sweep that fit on one CU at once: the smaller count of
+// its two instantiations (the launcher picks one per launch).
 hipError_t sweep_capacity(int32_t *blocks_per_cu) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&adjoint_sweep_resident_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       int(sweep_lds_bytes()));
-    if (e != hipSuccess) return e;
-    int nb = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &nb, reinterpret_cast<const void *>(&adjoint_sweep_resident_kernel), kSweepThreads,
-        sweep_lds_bytes());
-    *blocks_per_cu = nb;
-    return e;
+    int fit = 1 << 30;
+    for (const void *k : {reinterpret_cast<const void *>(&adjoint_sweep_resident_kernel<true>),
+                          reinterpret_cast<const void *>(&adjoint_sweep_resident_kernel<false>)}) {
+        hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           int(sweep_lds_bytes()));
+        if (e != hipSuccess) return e;
+        int nb = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, kSweepThreads, sweep_lds_bytes());
+        if (e != hipSuccess) return e;
+        fit = nb < fit ? nb : fit;
+    }
+    *blocks_per_cu = fit;
+    return hipSuccess;
 }
 
 hipError_t launch_adjoint_sweep_resident(hipStream_t s, int32_t blocks, const int32_t *cons_ptr,
@@ -783,10 +794,14 @@ hipError_t launch_adjoint_sweep_resident(hipStream_t s, int32_t blocks, const in
     if (N <= 0 || T <= 0) return hipSuccess;
     hipError_t e = launch_barrier_reset(s, gb, 1);
     if (e != hipSuccess) return e;
-    const int32_t coherent = allow_coherent && coherent_exchange(T, 3LL * N) ? 1 : 0;
-    adjoint_sweep_resident_kernel<<<blocks, kSweepThreads, sweep_lds_bytes(), s>>>(
-        cons_ptr, cons_idx, delays, gains_rows, coefs, gcons, x, E, xbar, gb, N, T, p, wait_ticks,
-        coherent);
+    if (allow_coherent && coherent_exchange(T, 3LL * N))
+        adjoint_sweep_resident_kernel<true><<<blocks, kSweepThreads, sweep_lds_bytes(), s>>>(
+            cons_ptr, cons_idx, delays, gains_rows, coefs, gcons, x, E, xbar, gb, N, T, p,
+            wait_ticks);
+    else
+        adjoint_sweep_resident_kernel<false><<<blocks, kSweepThreads, sweep_lds_bytes(), s>>>(
+            cons_ptr, cons_idx, delays, gains_rows, coefs, gcons, x, E, xbar, gb, N, T, p,
+            wait_ticks);
     return hipGetLastError();
 }
 

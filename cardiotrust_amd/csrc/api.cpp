@@ -205,6 +205,7 @@ int derive_reference(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, ct_
     {
         Span sp(k, kMapResiduals);
         CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.eT, r.n, r.T, r.S, r.Sp, false));
+        r.mapped_by_step = false;
     }
     if (c.freeze_gains && c.freeze_delays) return CT_OK;
     if (!r.xT_fresh) {
@@ -552,13 +553,17 @@ static int adjoint_prepare(ct_results_gpu &r, int32_t K, int64_t &slots) {
     return CT_OK;
 }
 
-// Hᵀ r of `beat` for every step into eT (n, T); with `accumulate` added to
-// what eT holds (beats of one update batch, in beat order).
+// Hᵀ r of `beat` for every step, by step into E (T, n) where the kernel
+// allows it (the sweep's layout), else into eT (n, T); with `accumulate`
+// added to what it holds (beats of one update batch, in beat order).
 static int adjoint_forcing(ct_results_gpu &r, int32_t beat, bool accumulate, ct_epoch_kernel *k) {
     hipStream_t s = r.stream;
     const float *Hb = r.H + int64_t(beat) * r.Sp * r.n;
     Span sp(k, kMapResiduals);
-    CT_HIP(ct::launch_map_residuals(s, Hb, r.R, r.eT, r.n, r.T, r.S, r.Sp, accumulate));
+    const bool by_step = ct::map_residuals_by_step_ok(r.S, r.T, r.n);
+    CT_HIP(ct::launch_map_residuals(s, Hb, r.R, by_step ? r.E : r.eT, r.n, r.T, r.S, r.Sp,
+                                    accumulate, by_step));
+    r.mapped_by_step = by_step;
     return CT_OK;
 }
 
@@ -571,7 +576,8 @@ static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K,
     const int64_t n = r.n, T = r.T, M = int64_t(r.N) * 78;
     {
         Span sp(k, kTranspose);
-        CT_HIP(ct::launch_transpose(s, r.eT, r.E, n, T));      // mapped residuals by step
+        if (!r.mapped_by_step)
+            CT_HIP(ct::launch_transpose(s, r.eT, r.E, n, T));  // mapped residuals by step
         CT_HIP(ct::launch_transpose(s, r.gT, r.staging, 78, n));  // gains by state row
     }
     ct::AdjointParams p;
@@ -1001,9 +1007,13 @@ int ct_results_update_from_gpu(ct_results_gpu *results, const ct_results_host *o
     CT_CHECK(download(o->system_states, r.x, sizeof(float) * T * n));
     CT_CHECK(download(o->measurements, r.m, sizeof(float) * r.B * T * S));
     CT_CHECK(download(o->residuals, r.R + (T - 1) * r.Sp, sizeof(float) * S));
-    if (o->mapped_residuals && n > 0)
-        CT_HIP(hipMemcpy2D(o->mapped_residuals, sizeof(float), r.eT + (T - 1), sizeof(float) * T,
-                           sizeof(float), n, hipMemcpyDeviceToHost));
+    if (o->mapped_residuals && n > 0) {
+        if (r.mapped_by_step)  // the reverse-mode forcing: row T - 1 of E
+            CT_CHECK(download(o->mapped_residuals, r.E + (T - 1) * n, sizeof(float) * n));
+        else
+            CT_HIP(hipMemcpy2D(o->mapped_residuals, sizeof(float), r.eT + (T - 1),
+                               sizeof(float) * T, sizeof(float), n, hipMemcpyDeviceToHost));
+    }
     CT_CHECK(download(o->maximum_regularization, r.mr_last, sizeof(float) * n));
     CT_CHECK(download(o->maximum_regularization_sum, r.mr_sum, sizeof(float)));
     if (o->average_delays && N > 0) {

@@ -123,8 +123,12 @@ hipError_t launch_maxreg_finalize(hipStream_t s, const float *partial, int32_t b
 hipError_t launch_transpose(hipStream_t s, const float *in, float *out, int64_t rows, int64_t cols);
 
 // eT = Hᵀ R, or with `accumulate` eT = eT + Hᵀ R (each element one FP32 add).
-hipError_t launch_map_residuals(hipStream_t s, const float *H, const float *R, float *eT,
-                                int64_t n, int32_t T, int32_t S, int32_t Sp, bool accumulate);
+// With `by_step` the output is E = eT transposed, (T, n) (the same values bit
+// for bit); only where map_residuals_by_step_ok(S, T, n).
+bool map_residuals_by_step_ok(int32_t S, int32_t T, int64_t n);
+hipError_t launch_map_residuals(hipStream_t s, const float *H, const float *R, float *out,
+                                int64_t n, int32_t T, int32_t S, int32_t Sp, bool accumulate,
+                                bool by_step = false);
 
 hipError_t launch_derive(hipStream_t s, const int32_t *nbr, const float *coefs,
                          const int32_t *delays, const float *initial_delays, const float *gT,

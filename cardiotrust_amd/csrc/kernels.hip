@@ -761,6 +761,10 @@ __global__ __launch_bounds__(256) void transpose_kernel(const float *__restrict_
 // tiles then stages only R (L2-resident, 1 MB at T = 2000) in LDS, and the
 // next tile's R is loaded while the MFMAs run. The k pairs go in ascending
 // order, as in map_residuals_kernel, so both give the same bits.
+// kByStep: the output is E = (Hᵀ·Rᵀ)ᵀ, (T, n) by step, for the reverse-mode
+// sweep: the same MFMAs with the operands swapped (each output element sums
+// the same products in the same order, so E[t][i] == eT[i][t] bit for bit),
+// which puts 32 consecutive states of one step in a lane group.
 // ---------------------------------------------------------------------------
 constexpr int kMapRegPairs = kProjRows / 2;  // sensor pairs held in registers (S <= 112)
 constexpr int kMapRegTile = 64;              // steps per tile: two 32-step MFMA blocks
@@ -768,6 +772,7 @@ constexpr int kMapRegSpan = 16;              // tiles per workgroup (1024 steps)
 constexpr int kMapRegStride = kMapRegTile + 4;  // LDS row (one sensor) in floats
 constexpr int kMapRegQuads = kProjRows / 16;    // float4 of R per thread and tile
 static_assert(kMapRegTile == 64 && kProjRows % 16 == 0, "R tile: 64 steps x 4 threads per step");
+template <bool kByStep>
 __global__ __launch_bounds__(256) void map_residuals_regs_kernel(const float *__restrict__ H,
                                                                  const float *__restrict__ R,
                                                                  float *__restrict__ eT, int64_t n,
@@ -791,10 +796,12 @@ __global__ __launch_bounds__(256) void map_residuals_regs_kernel(const float *__
 #pragma unroll
     for (int p = 0; p < kMapRegPairs; ++p) a[p] = (2 * p < Se && iw < n) ? a[p] : 0.0f;
     // eT rows [i0, min(n, i0 + 128)) as one buffer: stores past them are dropped
-    const int64_t rows = min(int64_t(128), n - i0);
+    // (by step: E from row t0 of the tile, columns from i0, bounds checked per lane)
+    const int64_t rows = kByStep ? 0 : min(int64_t(128), n - i0);
     const __amdgpu_buffer_rsrc_t ers =
         __builtin_amdgcn_make_buffer_rsrc(eT + i0 * T, 0, int32_t(4 * rows * T), kBufferWord3);
-    const int32_t lane_off = 4 * ((wave * 32 + kh * 4) * T + (lane & 31));
+    const int32_t lane_off = kByStep ? 4 * (kh * 4 * int32_t(n) + wave * 32 + (lane & 31))
+                                     : 4 * ((wave * 32 + kh * 4) * T + (lane & 31));
     // R loads: thread = tile step tt (4 threads per step), sensors 16 m + 4 c .. + 3
     const int tt = tid >> 2, c4 = (tid & 3) * 4;
     float *const rs_w = &Rs[c4][2 * (tt & 31) + (tt >> 5)];
@@ -836,9 +843,36 @@ __global__ __launch_bounds__(256) void map_residuals_regs_kernel(const float *__
             if (p + 2 < kMapRegPairs)
                 bq[p + 2] = *reinterpret_cast<const float2 *>(rs_r + 2 * (p + 2) * kMapRegStride);
             if (2 * p < se) {
-                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[p], bq[p].x, acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[p], bq[p].y, acc1, 0, 0, 0);
+                if constexpr (kByStep) {
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(bq[p].x, a[p], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(bq[p].y, a[p], acc1, 0, 0, 0);
+                } else {
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[p], bq[p].x, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[p], bq[p].y, acc1, 0, 0, 0);
+                }
             }
+        }
+        if constexpr (kByStep) {
+            // lane l holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]: step rows,
+            // state columns, stored from row t0 through a buffer resource with
+            // the row part of each offset uniform
+            const __amdgpu_buffer_rsrc_t trs =
+                __builtin_amdgcn_make_buffer_rsrc(eT + int64_t(t0) * n + i0, 0, -1, kBufferWord3);
+            const bool in_i = iw < n;
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int32_t row = q * 32 + (r >> 2) * 8 + (r & 3);  // + 4 kh in lane_off
+                    if (!in_i || t0 + row + 4 * kh >= te) continue;
+                    const int32_t so = 4 * row * int32_t(n);
+                    float v = q == 0 ? acc0[r] : acc1[r];
+                    if (accumulate)
+                        v += __uint_as_float(
+                            __builtin_amdgcn_raw_buffer_load_b32(trs, lane_off, so, 0));
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), trs, lane_off, so, 0);
+                }
+            continue;
         }
         // lane l holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]: state rows, step
         // columns. Stored through the workgroup's buffer resource: the row part
@@ -1968,15 +2002,30 @@ hipError_t launch_transpose(hipStream_t s, const float *in, float *out, int64_t 
     return hipGetLastError();
 }
 
-hipError_t launch_map_residuals(hipStream_t s, const float *H, const float *R, float *eT,
-                                int64_t n, int32_t T, int32_t S, int32_t Sp, bool accumulate) {
+bool map_residuals_by_step_ok(int32_t S, int32_t T, int64_t n) {
+    // the register-held kernel: at most 112 sensors; its buffer offsets
+    // (4 B x 128 rows x T, and 4 B x 64 rows x n by step) fit in 31 bits
+    return S > 0 && (S + 1) / 2 * 2 <= kProjRows && int64_t(T) * 128 * 4 < (int64_t(1) << 31) &&
+           n * 64 * 4 < (int64_t(1) << 31);
+}
+
+hipError_t launch_map_residuals(hipStream_t s, const float *H, const float *R, float *out,
+                                int64_t n, int32_t T, int32_t S, int32_t Sp, bool accumulate,
+                                bool by_step) {
     if (n <= 0) return hipSuccess;
     if (S <= 0 || S > Sp) return hipErrorInvalidValue;
+    if (by_step && !map_residuals_by_step_ok(S, T, n)) return hipErrorInvalidValue;
+    float *eT = out;
     const int32_t Se = (S + 1) / 2 * 2;
     // the register-held kernel's buffer offsets (4 B x 128 rows x T) fit in 31 bits
     if (Se <= kProjRows && int64_t(T) * 128 * 4 < (int64_t(1) << 31)) {
         dim3 grid(blocks_for(n, 128), blocks_for(T, kMapRegSpan * kMapRegTile));
-        map_residuals_regs_kernel<<<grid, 256, 0, s>>>(H, R, eT, n, T, Se, Sp, accumulate ? 1 : 0);
+        if (by_step)
+            map_residuals_regs_kernel<true><<<grid, 256, 0, s>>>(H, R, out, n, T, Se, Sp,
+                                                                 accumulate ? 1 : 0);
+        else
+            map_residuals_regs_kernel<false><<<grid, 256, 0, s>>>(H, R, out, n, T, Se, Sp,
+                                                                  accumulate ? 1 : 0);
         return hipGetLastError();
     }
     dim3 grid(blocks_for(n, 128), blocks_for(T, kMapTile));

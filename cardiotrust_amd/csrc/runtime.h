@@ -141,6 +141,9 @@ struct ct_results_gpu {
     int64_t cons_len = 0;
     int32_t ckpt_interval = 0;  // checkpoint interval of the last forward pass (0 = none)
     float *E = nullptr, *xbar = nullptr, *xbarT = nullptr, *ybar = nullptr;
+    // the latest mapped residuals are in E (by step, the reverse-mode forcing)
+    // rather than eT
+    bool mapped_by_step = false;
     float *qring = nullptr, *ckpt = nullptr, *yseg = nullptr;
     float *gcons = nullptr;  // (N, 26, 10) consumer-ordered gains and coefficients
     uint32_t *flags = nullptr;  // kMaxSlabFlags per-slab step counters (resident launches)
