@@ -716,23 +716,41 @@ __global__ __launch_bounds__(256) void maxreg_step_sums_kernel(const float *__re
 // derivation.rs:523) and the per-step losses (metrics.rs:191-195).
 // Also the objective of the reverse-mode gradient (in double):
 // J = sum_t mse_strength/2 * loss_mse[t] + reg_strength/2 * step_mr[t].
-__global__ void maxreg_prefix_kernel(const float *__restrict__ step_mr, float *__restrict__ mr_sum,
-                                     const float *__restrict__ loss_mse,
-                                     float *__restrict__ loss_mr, float *__restrict__ loss,
-                                     double *__restrict__ objective, float mse_strength,
-                                     float reg_strength, int32_t T) {
-    if (threadIdx.x != 0) return;
-    float cum = *mr_sum;
+// The running maximum-regularisation sum and the per-step losses
+// (metrics.rs:189-198): a sequential scan in the reference's order, so one
+// thread runs it; the inputs are staged in LDS by the workgroup chunk by
+// chunk, so the scan does not wait on a global load per step.
+constexpr int kScanChunk = 2048;
+__global__ __launch_bounds__(256) void maxreg_prefix_kernel(
+    const float *__restrict__ step_mr, float *__restrict__ mr_sum,
+    const float *__restrict__ loss_mse, float *__restrict__ loss_mr, float *__restrict__ loss,
+    double *__restrict__ objective, float mse_strength, float reg_strength, int32_t T) {
+    __shared__ float smr[kScanChunk], smse[kScanChunk];
+    const int tid = threadIdx.x;
+    float cum = 0.0f;
     double J = 0.0;
-    for (int32_t t = 0; t < T; ++t) {
-        cum = cum + step_mr[t];
-        loss_mr[t] = cum;
-        loss[t] = fmaf(reg_strength, cum, loss_mse[t]);
-        J += 0.5 * double(mse_strength) * double(loss_mse[t]) +
-             0.5 * double(reg_strength) * double(step_mr[t]);
+    if (tid == 0) cum = *mr_sum;
+    for (int32_t c0 = 0; c0 < T; c0 += kScanChunk) {
+        const int32_t len = min(kScanChunk, T - c0);
+        for (int32_t i = tid; i < len; i += blockDim.x) {
+            smr[i] = step_mr[c0 + i];
+            smse[i] = loss_mse[c0 + i];
+        }
+        __syncthreads();
+        if (tid == 0)
+            for (int32_t i = 0; i < len; ++i) {
+                cum = cum + smr[i];
+                loss_mr[c0 + i] = cum;
+                loss[c0 + i] = fmaf(reg_strength, cum, smse[i]);
+                J += 0.5 * double(mse_strength) * double(smse[i]) +
+                     0.5 * double(reg_strength) * double(smr[i]);
+            }
+        __syncthreads();
     }
-    *mr_sum = cum;
-    *objective = J;
+    if (tid == 0) {
+        *mr_sum = cum;
+        *objective = J;
+    }
 }
 
 __global__ __launch_bounds__(256) void transpose_kernel(const float *__restrict__ in,
@@ -1508,28 +1526,49 @@ __global__ void smoothness_kernel(const int32_t *__restrict__ nbr, const float *
     for (int d = 0; d < 26; ++d) dA[int64_t(v) * 26 + d] += strength * diff;
 }
 
-__device__ float unrolled_mean(const float *xs, int32_t len) {
-    float p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int32_t i = 0;
-    for (; len - i >= 8; i += 8)
-        for (int j = 0; j < 8; ++j) p[j] = p[j] + xs[i + j];
-    float acc = 0.0f;
-    acc = acc + (p[0] + p[4]);
-    acc = acc + (p[1] + p[5]);
-    acc = acc + (p[2] + p[6]);
-    acc = acc + (p[3] + p[7]);
-    for (; i < len; ++i) acc = acc + xs[i];
-    return acc / float(len);
-}
-
-// metrics.rs:204-219
-__global__ void batch_metrics_kernel(const float *loss, const float *loss_mse,
-                                     const float *loss_mr, float *loss_b, float *loss_mse_b,
-                                     float *loss_mr_b, const UpdateScalars *sc, int32_t T) {
-    const int32_t idx = sc->batch_index;
-    if (threadIdx.x == 0) loss_mse_b[idx] = unrolled_mean(loss_mse, T);
-    if (threadIdx.x == 1) loss_mr_b[idx] = unrolled_mean(loss_mr, T);
-    if (threadIdx.x == 2) loss_b[idx] = unrolled_mean(loss, T);
+// metrics.rs:204-219: the means of the beat's per-step losses, in ndarray's
+// order for a 1-D f32 mean (eight interleaved partial sums, combined
+// pairwise, then the tail, then / T). The three arrays go through LDS in
+// chunks; eight lanes per array keep the eight partial sums, and thread q < 3
+// combines array q's.
+__global__ __launch_bounds__(256) void batch_metrics_kernel(const float *loss, const float *loss_mse,
+                                                            const float *loss_mr, float *loss_b,
+                                                            float *loss_mse_b, float *loss_mr_b,
+                                                            const UpdateScalars *sc, int32_t T) {
+    __shared__ float sx[3][kScanChunk];
+    __shared__ float part[3][8];
+    const int tid = threadIdx.x, arr = tid / 8, j = tid % 8;
+    const auto src = [&](int q) { return q == 0 ? loss_mse : q == 1 ? loss_mr : loss; };
+    float pj = 0.0f;
+    // the whole-chunk part of the array: chunks are multiples of 8 elements,
+    // so the interleaving continues across chunks; the tail (len % 8) is the
+    // last chunk's remainder
+    const int32_t body = T / 8 * 8;
+    for (int32_t c0 = 0; c0 < body; c0 += kScanChunk) {
+        const int32_t len = min(kScanChunk, body - c0);
+        for (int32_t i = tid; i < 3 * len; i += blockDim.x) {
+            const int32_t q = i / len, e = i - q * len;
+            sx[q][e] = src(q)[c0 + e];
+        }
+        __syncthreads();
+        if (arr < 3)
+            for (int32_t i = 0; i < len; i += 8) pj = pj + sx[arr][i + j];
+        __syncthreads();
+    }
+    if (arr < 3) part[arr][j] = pj;
+    __syncthreads();
+    if (tid < 3) {
+        const float *p = part[tid];
+        float acc = 0.0f;
+        acc = acc + (p[0] + p[4]);
+        acc = acc + (p[1] + p[5]);
+        acc = acc + (p[2] + p[6]);
+        acc = acc + (p[3] + p[7]);
+        const float *xs = src(tid);
+        for (int32_t i = body; i < T; ++i) acc = acc + xs[i];
+        float *dst = tid == 0 ? loss_mse_b : tid == 1 ? loss_mr_b : loss_b;
+        dst[sc->batch_index] = acc / float(T);
+    }
 }
 
 // (N, 78) per-connection filter rows -> (3N, 78) per-state rows.
@@ -1990,7 +2029,7 @@ hipError_t launch_maxreg_finalize(hipStream_t s, const float *partial, int32_t b
     maxreg_step_sums_kernel<<<T, 256, 0, s>>>(partial, blocks, step_mr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    maxreg_prefix_kernel<<<1, 64, 0, s>>>(step_mr, mr_sum, loss_mse, loss_mr, loss, objective,
+    maxreg_prefix_kernel<<<1, 256, 0, s>>>(step_mr, mr_sum, loss_mse, loss_mr, loss, objective,
                                           mse_strength, reg_strength, T);
     return hipGetLastError();
 }
@@ -2129,7 +2168,7 @@ hipError_t launch_smoothness(hipStream_t s, const int32_t *nbr, const float *avg
 hipError_t launch_batch_metrics(hipStream_t s, const float *loss, const float *loss_mse,
                                 const float *loss_mr, float *loss_b, float *loss_mse_b,
                                 float *loss_mr_b, const UpdateScalars *sc, int32_t T) {
-    batch_metrics_kernel<<<1, 64, 0, s>>>(loss, loss_mse, loss_mr, loss_b, loss_mse_b, loss_mr_b,
+    batch_metrics_kernel<<<1, 256, 0, s>>>(loss, loss_mse, loss_mr, loss_b, loss_mse_b, loss_mr_b,
                                           sc, T);
     return hipGetLastError();
 }
