@@ -142,22 +142,25 @@ typedef float float3_a4 __attribute__((ext_vector_type(3), aligned(4)));
 constexpr int kResThreads = 512;
 constexpr int kResItems = (kResMaxVoxels * 26 + kResThreads - 1) / kResThreads;
 constexpr int kResStates = (kResMaxVoxels * 3 + kResThreads - 1) / kResThreads;
-// Gathers issued per group in phase 1, and connections per gains group in
-// phase 2 (26 = 9 + 9 + 8), measured on the 120k-voxel shell with sorted
-// connections (ms per 2000 steps): 2 gathers / thirds 71.6, 1 / thirds 73.9,
-// 3 / thirds 72.0, 4 / thirds 72.6, 2 / halves 72.9, 2 / sixths 77.8.
+// Gathers issued per group in phase 1 (2, 3 or 4 time the same) and
+// connections per gains group in phase 2. The gain rows of each lane's first
+// state held in registers for the launch are whole groups (gains do not
+// change within a beat); the measured traffic is mostly gains, so every
+// cached row saves time. Per 2000-step launch at the 120k-voxel shell
+// (same-run A/B): 9-connection groups with one cached (27 rows) 53.2 ms,
+// none cached 55.7; 5-connection groups with two cached (30 rows) 52.2;
+// 4-connection groups with three cached (36 rows) 51.4, at 4 spilled
+// registers; two groups of 6 (36 rows) spill 45, of 3 or 1 connections more.
 constexpr int kResBatch = 2;
-constexpr int kResGainConns = 9;
-// Gain rows of each lane's first state held in registers for the launch: one
-// group of 27 (68.2-68.9 ms per 2000 steps against 70.4-71.1 without). The
-// kernel then needs 219 VGPRs, but 36 cached rows (a partial second group)
-// already spill 177 and 54 spill 291.
-constexpr int kResCachedRows = 27;
-// The batched kernel also holds its job's fields: its own measured values
-// (ms per epoch of the 64-scenario uneven batch): 3 gathers / halves 1751,
-// 2 / halves 1775, 1 / halves 1984, 2 / thirds 1830, 1 / thirds 1893.
+constexpr int kResGainConns = 4;
+constexpr int kResCachedRows = 36;
+// The batched kernel also holds its job's fields: 3 gathers per group,
+// 9-connection groups and one cached group of 27 rows fit without spills
+// (propagation of the 64-scenario uneven batch: 230.3 ms without the cached
+// rows, 215.6 ms with them).
 constexpr int kBatchGather = 3;
-constexpr int kBatchGainConns = 13;
+constexpr int kBatchGainConns = 9;
+constexpr int kBatchCachedRows = 27;
 constexpr uint32_t kNoConnection = 0xFFFFFFFFu;
 constexpr int kResNeighbourBits = 17;  // voxel index field; delay or connection in the upper 15 bits
 static_assert(kSortSize >= kResMaxVoxels * 26 && kSortSize < (1 << 14) + 1, "sort size");
@@ -441,7 +444,7 @@ __global__ __launch_bounds__(kResThreads) void propagate_batch_kernel(
         const int32_t member = slot - job.slot0;
         const int32_t vb = int32_t(int64_t(member) * job.N / job.width);
         const int32_t ve = int32_t(int64_t(member + 1) * job.N / job.width);
-        if (!resident_run<kBatchGather, kBatchGainConns, 0, kCoherent>(job, vb, ve, 0, job.T, lds,
+        if (!resident_run<kBatchGather, kBatchGainConns, kBatchCachedRows, kCoherent>(job, vb, ve, 0, job.T, lds,
                                                                    first_wait, step_wait))
             return;
         __syncthreads();  // LDS is reused by the next job

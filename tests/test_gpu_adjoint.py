@@ -177,3 +177,40 @@ def test_resident_sweep_long_delays_bitwise(gpu):
     assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
     assert np.array_equal(a[2].view(np.uint32), b[2].view(np.uint32))
     assert np.abs(a[1]).max() > 0
+
+
+@pytest.mark.parametrize("sensors_per_axis", [6, 9])
+def test_reverse_mode_forcing_equals_the_mapped_residuals(gpu, sensors_per_axis):
+    """The reverse-mode epoch writes Hᵀ·r by step (E[t][i]) where at most 112
+    sensors allow it, else by state (eT[i][t]) and transposes; the reference
+    derivation always writes eT. From the same start both epochs have the
+    same states and residuals, so the downloaded mapped residuals (step T-1)
+    must agree bit for bit. T = 150 leaves a partial 64-step tile; N is not a
+    multiple of the 128-state tile."""
+    from cardiotrust_amd.gpu import Buffer, EpochKernel, ResultsGPU
+    p = blob(21, shape=(7, 6, 5), T=150, maxd=12)
+    if sensors_per_axis != 6:
+        g = synth.grid_geometry(synth.uniform(21, 99, 210).reshape(7, 6, 5) < 0.7, 2.5)
+        p = synth.make_problem(g, num_steps=150, max_delay=12, seed=21, gain_sigma=0.25,
+                               simulate=oracle.simulate_f32, sensors_per_axis=sensors_per_axis)
+    note(f"case: forcing layout, {p.num_sensors} sensors, N={p.num_voxels}, T={p.num_steps}")
+    assert (3 * p.num_voxels) % 128 != 0
+    cfg = Algorithm(learning_rate=0.1, maximum_regularization_threshold=0.05)
+    got = {}
+    for mode in ("reference", "adjoint"):
+        r = ResultsGPU(gpu, p)
+        buf = Buffer(gpu, p.measurements)
+        k = EpochKernel(gpu, r, buf, cfg, 3 * p.num_voxels, p.num_sensors, p.num_steps)
+        if mode == "adjoint":
+            k.set_gradient(1, 16)
+        k.execute()
+        got[mode] = r.update_from_gpu(["mapped_residuals", "residuals", "system_states"])
+        k.close()
+        buf.close()
+        r.close()
+    ref, adj = got["reference"], got["adjoint"]
+    assert np.array_equal(ref["system_states"].view(np.uint32), adj["system_states"].view(np.uint32))
+    assert np.array_equal(ref["residuals"].view(np.uint32), adj["residuals"].view(np.uint32))
+    assert np.any(ref["mapped_residuals"] != 0)
+    assert np.array_equal(ref["mapped_residuals"].view(np.uint32),
+                          adj["mapped_residuals"].view(np.uint32))
