@@ -852,9 +852,15 @@ int ct_results_to_gpu(ct_gpu *gpu, const ct_results_desc *desc, ct_results_gpu *
     r->nbr_host = nbr;
     const int64_t n = r->n, T = r->T, S = r->S, Sp = r->Sp, B = r->B;
 
-    // GEMM split-K plan: about 2048 workgroups for the projection.
+    // GEMM split-K plan: about 2048 workgroups for the projection, each
+    // split at least kMinSplitStates states long. (A small scenario split
+    // down to single 32-state k tiles writes partial sums larger than its
+    // states: 93 splits x T x 128 floats for 3000 states.)
+    constexpr int64_t kMinSplitStates = 512;
     const int64_t tiles = (T + ct::kProjTile - 1) / ct::kProjTile * ct::project_sensor_tiles(int32_t(S));
-    int64_t splits = std::max<int64_t>(1, std::min<int64_t>((n + 31) / 32, 2048 / std::max<int64_t>(tiles, 1)));
+    int64_t splits = std::max<int64_t>(
+        1, std::min<int64_t>((n + kMinSplitStates - 1) / kMinSplitStates,
+                             2048 / std::max<int64_t>(tiles, 1)));
     int64_t kps = ((n + splits - 1) / splits + ct::kProjTileK - 1) / ct::kProjTileK * ct::kProjTileK;
     kps = std::max<int64_t>(kps, ct::kProjTileK);
     splits = std::max<int64_t>(1, (n + kps - 1) / kps);
