@@ -299,8 +299,10 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
                 const int pos = lane + k * kSweepThreads;
                 const bool present = k0 + kk < kSweepItems && pk[k] != kSweepNoEntry &&
                                      ((late >> k) & 1u) == pass;
-                // both forms of the delay, selected (no branch on `sorted`)
-                const int32_t Ds = int32_t(dtab[pos]);
+                // both forms of the delay, selected (no branch on `sorted`); an
+                // absent item reads position 0 (positions past the slab's
+                // items lie beyond the byte table)
+                const int32_t Ds = int32_t(dtab[present ? pos : 0]);
                 const int32_t D = !present ? 0 : sorted ? Ds : int32_t(pk[k] >> 17);
                 const int32_t tau = s + D;
                 on[kk] = present && tau < T;
