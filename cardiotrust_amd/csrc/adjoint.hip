@@ -566,7 +566,7 @@ static_assert(kGradChunk == kTileChunk, "one gradient chunk per ring chunk");
 __global__ __launch_bounds__(kTileThreads) void adjoint_gradient_tiled_kernel(
     DeriveTiles tl, const int32_t *__restrict__ nbr, const float *__restrict__ coefs,
     const int32_t *__restrict__ delays, const float *__restrict__ gT,
-    const float *__restrict__ xT, const float *__restrict__ xbarT, const float *__restrict__ ckpt,
+    const float *__restrict__ xT, const float *__restrict__ xbar, const float *__restrict__ ckpt,
     float *__restrict__ dGT, float *__restrict__ dA, int32_t N, int32_t T) {
     extern __shared__ __align__(16) float lds[];
     float *ring = lds;                       // [halo][3][kTileRowStride]
@@ -600,17 +600,20 @@ __global__ __launch_bounds__(kTileThreads) void adjoint_gradient_tiled_kernel(
     float A = 0.0f, ybn[3] = {0.0f, 0.0f, 0.0f};
 
     // this thread's share of the copies: quads of halo rows, and one (voxel,
-    // step) item of the tile's adjoint states
+    // step) item of the tile's adjoint states, read from x̄ by step (T, n):
+    // its voxel's three states at one step, 12 contiguous bytes (the tile's
+    // voxels come in runs along the grid's fastest axis)
     TileRingCopy rc;
     rc.init(tl, hbeg, nhalo, T);
     const int sv = tid / kTileChunk, sj = tid - kTileChunk * sv;
     const bool sown = sv < nvox;
-    const int64_t srow = sown ? 3LL * tl.vox[pbeg + sv] * T + sj : 0;
+    const int64_t scol = sown ? 3LL * tl.vox[pbeg + sv] : 0;
     float sb[3];
     const auto fetch_xbar = [&](int32_t cb) {
         const bool live = sown && cb + sj < T;
+        const float *src = xbar + int64_t(live ? cb + sj : 0) * n + scol;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) sb[q] = live ? xbarT[srow + int64_t(q) * T + cb] : 0.0f;
+        for (int q = 0; q < 3; ++q) sb[q] = live ? src[q] : 0.0f;
     };
     const auto stage_xbar = [&]() {
         if (sown) {
@@ -821,11 +824,11 @@ hipError_t adjoint_gradient_tiled_prepare() {
 hipError_t launch_adjoint_gradient_tiled(hipStream_t s, const DeriveTiles &tiles,
                                          const int32_t *nbr, const float *coefs,
                                          const int32_t *delays, const float *gT, const float *xT,
-                                         const float *xbarT, const float *ckpt, float *dGT,
+                                         const float *xbar, const float *ckpt, float *dGT,
                                          float *dA, int32_t N, int32_t T) {
     if (N <= 0 || T <= 0 || tiles.tiles <= 0) return hipSuccess;
     adjoint_gradient_tiled_kernel<<<tiles.tiles, kTileThreads, kGradTiledLdsBytes, s>>>(
-        tiles, nbr, coefs, delays, gT, xT, xbarT, ckpt, dGT, dA, N, T);
+        tiles, nbr, coefs, delays, gT, xT, xbar, ckpt, dGT, dA, N, T);
     return hipGetLastError();
 }
 

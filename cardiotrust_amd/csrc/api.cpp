@@ -606,17 +606,19 @@ static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K,
         }
     }
     Span sp(k, kDerive);
-    CT_HIP(ct::launch_transpose(s, r.xbar, r.xbarT, T, n));
     if (!r.xT_fresh) {
         CT_HIP(ct::launch_transpose(s, r.x, r.xT, T, n));
         r.xT_fresh = true;
     }
-    if (K == ct::kTileChunk && r.tiles.tiles > 0 && r.derive_mode != 1)
+    if (K == ct::kTileChunk && r.tiles.tiles > 0 && r.derive_mode != 1) {
+        // reads x̄ by step: no transpose
         CT_HIP(ct::launch_adjoint_gradient_tiled(s, r.tiles, r.nbr, r.coefs, r.delays, r.gT, r.xT,
-                                                 r.xbarT, r.ckpt, r.dGT, r.dA, r.N, r.T));
-    else
+                                                 r.xbar, r.ckpt, r.dGT, r.dA, r.N, r.T));
+    } else {
+        CT_HIP(ct::launch_transpose(s, r.xbar, r.xbarT, T, n));
         CT_HIP(ct::launch_adjoint_gradient(s, r.nbr, r.coefs, r.delays, r.gT, r.xT, r.xbarT,
                                            r.ckpt, r.yseg, r.yseg_have, r.dGT, r.dA, r.N, r.T, K));
+    }
     return CT_OK;
 }
 

@@ -226,12 +226,13 @@ hipError_t launch_adjoint_gradient(hipStream_t s, const int32_t *nbr, const floa
 // The gradient pass over voxel tiles (the derivation's tables) for
 // checkpoints every kTileChunk steps; the same bits as launch_adjoint_gradient
 // with K = kTileChunk. Connections with delays above 47 read their rows from
-// memory.
+// memory. It reads x̄ by step, (T, n), as the sweep wrote it (the
+// lane-per-connection pass reads x̄ᵀ).
 hipError_t adjoint_gradient_tiled_prepare();
 hipError_t launch_adjoint_gradient_tiled(hipStream_t s, const DeriveTiles &tiles,
                                          const int32_t *nbr, const float *coefs,
                                          const int32_t *delays, const float *gT, const float *xT,
-                                         const float *xbarT, const float *ckpt, float *dGT,
+                                         const float *xbar, const float *ckpt, float *dGT,
                                          float *dA, int32_t N, int32_t T);
 
 // Sticky abort words that a parameter update checks before it writes: when
