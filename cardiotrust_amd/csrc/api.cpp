@@ -236,6 +236,8 @@ bool uses_adjoint(const ct_epoch_kernel &k) {
 }
 
 static int adjoint_prepare(ct_results_gpu &r, int32_t K, int64_t &slots);
+template <typename V>
+int ensure(ct_results_gpu &r, V **p, int64_t &have, int64_t count, int64_t unit);
 static int adjoint_forcing(ct_results_gpu &r, int32_t beat, bool accumulate, ct_epoch_kernel *k);
 static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K, int64_t slots,
                             int32_t num_beats, ct_epoch_kernel *k);
@@ -267,10 +269,29 @@ int finish_beat(ct_epoch_kernel &k, int32_t pos) {
         if (r.ckpt_interval != k.checkpoint_interval)
             return fail(CT_ERR_STATE, "shared propagation without the adjoint's checkpoints");
     }
-    CT_CHECK(adjoint_forcing(r, beat, pos != first, &k));
-    if (pos != last) return CT_OK;
-    return adjoint_backward(r, k.cfg, k.checkpoint_interval, r.pending_slots, last - first + 1,
-                            &k);
+    const int32_t count = last - first + 1;
+    if (count > 1 && ct::map_residuals_beats_ok(r.S, r.T, r.n)) {
+        // the batch's forcing in one pass over its beats: keep each beat's
+        // residuals, sum the products in the MFMA accumulators at the last
+        const int64_t slot = int64_t(r.T) * r.Sp;
+        if (pos == first) CT_CHECK(ensure(r, &r.Rbeats, r.Rbeats_have, count * slot, 1));
+        CT_HIP(ct::launch_copy(r.stream, r.Rbeats + (pos - first) * slot, r.R, slot));
+        if (pos != last) return CT_OK;
+        Span sp(&k, kMapResiduals);
+        for (int32_t c0 = 0; c0 < count; c0 += ct::kMapMaxBeats) {
+            ct::MapBeats mb{};
+            mb.count = std::min<int32_t>(ct::kMapMaxBeats, count - c0);
+            for (int32_t q = 0; q < mb.count; ++q)
+                mb.h_off[q] = int64_t(k.order[first + c0 + q]) * r.Sp * r.n;
+            CT_HIP(ct::launch_map_residuals_beats(r.stream, r.H, mb, r.Rbeats + c0 * slot, slot,
+                                                  r.E, r.n, r.T, r.S, r.Sp, c0 > 0));
+        }
+        r.mapped_by_step = true;
+    } else {
+        CT_CHECK(adjoint_forcing(r, beat, pos != first, &k));
+        if (pos != last) return CT_OK;
+    }
+    return adjoint_backward(r, k.cfg, k.checkpoint_interval, r.pending_slots, count, &k);
 }
 
 static int run_beat(ct_epoch_kernel &k, int32_t pos) {
@@ -524,12 +545,10 @@ static int adjoint_prepare(ct_results_gpu &r, int32_t K, int64_t &slots) {
     hipStream_t s = r.stream;
     const int64_t n = r.n, T = r.T, M = int64_t(r.N) * 78;
     CT_CHECK(build_consumers(r));
-    int64_t have_e = r.E ? n * T : 0, have_xb = r.xbar ? n * T : 0, have_xbt = r.xbarT ? n * T : 0,
-            have_yb = r.ybar ? M : 0;
+    int64_t have_e = r.E ? n * T : 0, have_xb = r.xbar ? n * T : 0, have_yb = r.ybar ? M : 0;
     CT_CHECK(ensure(r, &r.E, have_e, n * T, 1));
     CT_CHECK(ensure(r, &r.xbar, have_xb, n * T, 1));
-    CT_CHECK(ensure(r, &r.xbarT, have_xbt, n * T, 1));
-    CT_CHECK(ensure(r, &r.ybar, have_yb, M, 1));
+    CT_CHECK(ensure(r, &r.ybar, have_yb, M, 1));  // (x̄ᵀ only when the gradient pass needs it)
     // ring of q over R >= Dmax + 1 slots (R a power of two)
     slots = r.ring_slots_fixed;
     if (slots <= 0) {
@@ -615,6 +634,8 @@ static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K,
         CT_HIP(ct::launch_adjoint_gradient_tiled(s, r.tiles, r.nbr, r.coefs, r.delays, r.gT, r.xT,
                                                  r.xbar, r.ckpt, r.dGT, r.dA, r.N, r.T));
     } else {
+        int64_t have_xbt = r.xbarT ? n * T : 0;
+        CT_CHECK(ensure(r, &r.xbarT, have_xbt, n * T, 1));
         CT_HIP(ct::launch_transpose(s, r.xbar, r.xbarT, T, n));
         CT_HIP(ct::launch_adjoint_gradient(s, r.nbr, r.coefs, r.delays, r.gT, r.xT, r.xbarT,
                                            r.ckpt, r.yseg, r.yseg_have, r.dGT, r.dA, r.N, r.T, K));

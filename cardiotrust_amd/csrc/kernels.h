@@ -126,6 +126,18 @@ hipError_t launch_transpose(hipStream_t s, const float *in, float *out, int64_t 
 // With `by_step` the output is E = eT transposed, (T, n) (the same values bit
 // for bit); only where map_residuals_by_step_ok(S, T, n).
 bool map_residuals_by_step_ok(int32_t S, int32_t T, int64_t n);
+// E (by step, (T, n)) = the sum over up to kMapMaxBeats beats of H_bᵀ·R_bᵀ, with
+// `accumulate` added to what E holds; H_b = H + h_off[b] (elements), R_b = R +
+// b r_stride. Only where map_residuals_beats_ok(S, T, n).
+constexpr int kMapMaxBeats = 8;
+bool map_residuals_beats_ok(int32_t S, int32_t T, int64_t n);
+struct MapBeats {
+    int64_t h_off[kMapMaxBeats];
+    int32_t count;
+};
+hipError_t launch_map_residuals_beats(hipStream_t s, const float *H, const MapBeats &beats,
+                                      const float *R, int64_t r_stride, float *E, int64_t n,
+                                      int32_t T, int32_t S, int32_t Sp, bool accumulate);
 hipError_t launch_map_residuals(hipStream_t s, const float *H, const float *R, float *out,
                                 int64_t n, int32_t T, int32_t S, int32_t Sp, bool accumulate,
                                 bool by_step = false);
@@ -271,6 +283,8 @@ hipError_t launch_batch_metrics(hipStream_t s, const float *loss, const float *l
 
 hipError_t launch_expand_rows(hipStream_t s, const float *in, float *out, int64_t N);
 
+// dst[0 .. len) = src[0 .. len) (a kernel, so it may be captured in a graph).
+hipError_t launch_copy(hipStream_t s, float *dst, const float *src, int64_t len);
 hipError_t launch_fill(hipStream_t s, float *p, float value, int64_t len);
 // Spherical states of x (T, 3N): (T, N) magnitude/theta/phi when not null,
 // their values at each voxel's maximum magnitude and the activation time (N);

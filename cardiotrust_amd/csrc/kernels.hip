@@ -914,6 +914,110 @@ __global__ __launch_bounds__(256) void map_residuals_regs_kernel(const float *__
 }
 
 // ---------------------------------------------------------------------------
+// The reverse-mode forcing of an update batch that shares one propagation:
+// E = sum over its beats b of H_bᵀ·R_bᵀ, by step (T, n), up to 112 sensors,
+// in one pass. The per-beat kernel would add each beat's product into E in
+// memory (a read and a write of 4 n T bytes per beat); here the products of
+// all beats accumulate in the MFMA accumulators. A workgroup owns 128 states
+// and 128 steps (two 64-step tiles, both accumulators live); per beat
+// it loads its lanes' H operands once (56 registers) and runs the two tiles
+// with R staged in LDS as in map_residuals_regs_kernel. The sensor pairs go
+// in ascending order within a beat, the beats in the order given.
+// ---------------------------------------------------------------------------
+constexpr int kMapBeatTiles = 2;  // 64-step tiles per workgroup
+__global__ __launch_bounds__(256) void map_residuals_beats_kernel(
+    const float *__restrict__ H, MapBeats mb, const float *__restrict__ R, int64_t r_stride,
+    float *__restrict__ E, int64_t n, int32_t T, int32_t Se, int32_t Sp, int32_t accumulate) {
+    __shared__ float Rs[kProjRows][kMapRegStride];  // as in map_residuals_regs_kernel
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int kh = lane >> 5;
+    // step spans fastest, remapped XCD-contiguously: the spans of one state
+    // tile, which load the same H columns, share an L2
+    const int32_t spans = (T + kMapBeatTiles * kMapRegTile - 1) / (kMapBeatTiles * kMapRegTile);
+    const int32_t lin = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t i0 = int64_t(lin / spans) * 128;
+    const int32_t tb = (lin % spans) * kMapBeatTiles * kMapRegTile;
+    const int32_t te = min(T, tb + kMapBeatTiles * kMapRegTile);
+    const int64_t iw = i0 + wave * 32 + (lane & 31);
+    const float *hcol = H + min(iw, n - 1) + int64_t(kh) * n;
+    const int tt = tid >> 2, c4 = (tid & 3) * 4;
+    float *const rs_w = &Rs[c4][2 * (tt & 31) + (tt >> 5)];
+    const float *const rs_r = &Rs[kh][2 * (lane & 31)];
+    floatx16 acc[kMapBeatTiles][2];
+#pragma unroll
+    for (int j = 0; j < kMapBeatTiles; ++j) acc[j][0] = acc[j][1] = floatx16{0};
+    for (int32_t b = 0; b < mb.count; ++b) {
+        float a[kMapRegPairs];
+        const float *hb = hcol + mb.h_off[b];
+#pragma unroll
+        for (int p = 0; p < kMapRegPairs; ++p) a[p] = hb[int64_t(min(2 * p, Se - 2)) * n];
+#pragma unroll
+        for (int p = 0; p < kMapRegPairs; ++p) a[p] = (2 * p < Se && iw < n) ? a[p] : 0.0f;
+        const float *rb = R + int64_t(b) * r_stride;
+#pragma unroll
+        for (int j = 0; j < kMapBeatTiles; ++j) {
+            const int32_t t0 = tb + j * kMapRegTile;
+            if (t0 >= te) continue;  // (no break: acc[] must stay in registers)
+            float4_a4 rv[kMapRegQuads];
+            {
+                const bool in_t = t0 + tt < T;
+                const float *src = rb + int64_t(t0 + tt) * Sp + c4;
+#pragma unroll
+                for (int m = 0; m < kMapRegQuads; ++m) {
+                    float4_a4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+                    if (in_t && 16 * m + c4 < Se)
+                        v = *reinterpret_cast<const float4_a4 *>(src + 16 * m);
+                    rv[m] = v;
+                }
+            }
+            __syncthreads();  // the previous tile's operand reads are done
+#pragma unroll
+            for (int m = 0; m < kMapRegQuads; ++m)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) rs_w[(16 * m + q) * kMapRegStride] = rv[m][q];
+            __syncthreads();
+            int32_t se = Se;
+            asm volatile("" : "+s"(se));
+            float2 bq[kMapRegPairs];
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+                bq[p] = *reinterpret_cast<const float2 *>(rs_r + 2 * p * kMapRegStride);
+#pragma unroll
+            for (int p = 0; p < kMapRegPairs; ++p) {
+                if (p + 2 < kMapRegPairs)
+                    bq[p + 2] =
+                        *reinterpret_cast<const float2 *>(rs_r + 2 * (p + 2) * kMapRegStride);
+                if (2 * p < se) {
+                    acc[j][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(bq[p].x, a[p], acc[j][0], 0, 0, 0);
+                    acc[j][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(bq[p].y, a[p], acc[j][1], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // lane l holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]: step rows, state
+    // columns, stored from row tb through a buffer resource with the row part
+    // of each offset uniform
+    const __amdgpu_buffer_rsrc_t trs =
+        __builtin_amdgcn_make_buffer_rsrc(E + int64_t(tb) * n + i0, 0, -1, kBufferWord3);
+    const int32_t lane_off = 4 * (kh * 4 * int32_t(n) + wave * 32 + (lane & 31));
+    const bool in_i = iw < n;
+#pragma unroll
+    for (int j = 0; j < kMapBeatTiles; ++j)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int32_t row = j * kMapRegTile + q * 32 + (r >> 2) * 8 + (r & 3);  // + 4 kh
+                if (!in_i || tb + row + 4 * kh >= te) continue;
+                const int32_t so = 4 * row * int32_t(n);
+                float v = acc[j][q][r];
+                if (accumulate)
+                    v += __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(trs, lane_off, so, 0));
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), trs, lane_off, so, 0);
+            }
+}
+
+// ---------------------------------------------------------------------------
 // Mapped residuals Hᵀ·Rᵀ for all steps (derivation.rs:538-551 for every t),
 // FP32 MFMA 32x32x2, K = the sensors, S rounded up to even (the padded rows
 // of H and columns of R, all zero, are not multiplied). Output eT is (n, T).
@@ -1583,6 +1687,11 @@ __global__ void expand_rows_kernel(const float *__restrict__ in, float *__restri
     out[k] = in[(row / 3) * 78 + g];
 }
 
+__global__ void copy_kernel(float *__restrict__ dst, const float *__restrict__ src, int64_t len) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < len) dst[k] = src[k];
+}
+
 __global__ void fill_kernel(float *p, float value, int64_t len) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k < len) p[k] = value;
@@ -2044,6 +2153,27 @@ hipError_t launch_transpose(hipStream_t s, const float *in, float *out, int64_t 
     return hipGetLastError();
 }
 
+hipError_t launch_map_residuals_beats(hipStream_t s, const float *H, const MapBeats &beats,
+                                      const float *R, int64_t r_stride, float *E, int64_t n,
+                                      int32_t T, int32_t S, int32_t Sp, bool accumulate) {
+    if (n <= 0 || beats.count <= 0) return hipSuccess;
+    if (beats.count > kMapMaxBeats || !map_residuals_beats_ok(S, T, n) || S > Sp)
+        return hipErrorInvalidValue;
+    const int64_t spans = (T + kMapBeatTiles * kMapRegTile - 1) / (kMapBeatTiles * kMapRegTile);
+    const int64_t blocks = spans * ((n + 127) / 128);
+    if (blocks > (int64_t(1) << 31) - 1) return hipErrorInvalidValue;
+    map_residuals_beats_kernel<<<unsigned(blocks), 256, 0, s>>>(H, beats, R, r_stride, E, n, T,
+                                                                (S + 1) / 2 * 2, Sp,
+                                                                accumulate ? 1 : 0);
+    return hipGetLastError();
+}
+
+bool map_residuals_beats_ok(int32_t S, int32_t T, int64_t n) {
+    // the by-step kernel's limits, and the byte offset of a 128-row span
+    return map_residuals_by_step_ok(S, T, n) &&
+           n * kMapBeatTiles * kMapRegTile * 4 < (int64_t(1) << 31);
+}
+
 bool map_residuals_by_step_ok(int32_t S, int32_t T, int64_t n) {
     // the register-held kernel: at most 112 sensors; its buffer offsets
     // (4 B x 128 rows x T, and 4 B x 64 rows x n by step) fit in 31 bits
@@ -2179,6 +2309,12 @@ hipError_t launch_batch_metrics(hipStream_t s, const float *loss, const float *l
 hipError_t launch_expand_rows(hipStream_t s, const float *in, float *out, int64_t N) {
     if (N <= 0) return hipSuccess;
     expand_rows_kernel<<<blocks_for(N * 234, 256), 256, 0, s>>>(in, out, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(hipStream_t s, float *dst, const float *src, int64_t len) {
+    if (len <= 0) return hipSuccess;
+    copy_kernel<<<blocks_for(len, 256), 256, 0, s>>>(dst, src, len);
     return hipGetLastError();
 }
 

@@ -141,6 +141,10 @@ struct ct_results_gpu {
     int64_t cons_len = 0;
     int32_t ckpt_interval = 0;  // checkpoint interval of the last forward pass (0 = none)
     float *E = nullptr, *xbar = nullptr, *xbarT = nullptr, *ybar = nullptr;
+    // residuals of the beats of one update batch that shares a propagation
+    // (reverse mode), one (T, Sp) slot per position of the batch
+    float *Rbeats = nullptr;
+    int64_t Rbeats_have = 0;
     // the latest mapped residuals are in E (by step, the reverse-mode forcing)
     // rather than eT
     bool mapped_by_step = false;
