@@ -153,3 +153,25 @@ def test_default_shares_under_both_gradients(gpu):
     assert n_auto == [1, 1], n_auto
     for f in FIELDS:
         assert np.array_equal(np.asarray(auto[f]).view(np.uint8), np.asarray(on[f]).view(np.uint8)), f
+
+
+def test_shared_reverse_mode_over_more_beats_than_one_forcing_pass(gpu):
+    """Ten beats in one update batch: the forcing of the shared backward pass
+    sums up to eight beats per launch, so it takes two launches, the second
+    adding to the first's output. Against the per-beat passes and the FP64
+    oracle summed over the beats, with an explicit beat order."""
+    note("case: 10 beats, batch_size 0, reverse mode shared vs per beat, beat order reversed")
+    p = problem(6, beats=10, T=48, n=300)
+    cfg = Algorithm(learning_rate=0.0, freeze_delays=False, maximum_regularization_threshold=0.1)
+    order = list(range(p.num_beats))[::-1]
+    got = run(gpu, p, cfg, True, epochs=1, gradient=(1, 16), order=order)
+    per_beat = run(gpu, p, cfg, False, epochs=1, gradient=(1, 16), order=order)
+    o = oracle.OracleResults(p, np.float64)
+    dG = sum(o.adjoint(cfg, beat=b)[1] for b in range(p.num_beats))
+    dA = sum(o.adjoint(cfg, beat=b)[2] for b in range(p.num_beats))
+    eg, ea = rel_inf(got["d_gains"], dG), rel_inf(got["d_coefs"], dA)
+    note(f"  vs FP64 sum over beats: dG {eg:.2e}, dA {ea:.2e}; vs per-beat passes: "
+         f"{rel_inf(got['d_gains'], per_beat['d_gains']):.2e}")
+    assert eg <= 1e-3 and ea <= 1e-3
+    assert rel_inf(got["d_gains"], per_beat["d_gains"]) <= 1e-4
+    assert rel_inf(got["d_coefs"], per_beat["d_coefs"]) <= 1e-4
