@@ -623,6 +623,14 @@ __global__ __launch_bounds__(kTileThreads) void adjoint_gradient_tiled_kernel(
     };
     const auto fetch_rows = [&](int32_t cb) { rc.fetch(xT, cb, T); };
     const auto stage_rows = [&](int32_t cb) { rc.stage(ring, cb); };
+    // chunk c's starting filter outputs (checkpoint c - 1, +0 for chunk 0),
+    // loaded one chunk ahead with the copies
+    float y0n[3];
+    const auto fetch_start = [&](int32_t c) {
+#pragma unroll
+        for (int o = 0; o < 3; ++o)
+            y0n[o] = c == 0 ? 0.0f : ckpt[int64_t(c - 1) * M + item * 3 + o];
+    };
 
     const int32_t nchunks = (T + kTileChunk - 1) / kTileChunk;
     // chunks nchunks-1 .. nchunks-3 first; each iteration c then adds c-3
@@ -632,13 +640,23 @@ __global__ __launch_bounds__(kTileThreads) void adjoint_gradient_tiled_kernel(
     }
     if (nchunks - 4 >= 0) fetch_rows((nchunks - 4) * kTileChunk);
     fetch_xbar((nchunks - 1) * kTileChunk);
+    fetch_start(nchunks - 1);
     for (int32_t c = nchunks - 1; c >= 0; --c) {
         const int32_t cbeg = c * kTileChunk, len = min(kTileChunk, T - cbeg);
         if (c - 3 >= 0) stage_rows(cbeg - 3 * kTileChunk);
         stage_xbar();
+        const float y0c[3] = {y0n[0], y0n[1], y0n[2]};
+        // Every load issued so far has landed (used above). Stated
+        // explicitly, since the compiler cannot see it through the per-thread
+        // conditions of the copies: without this it waits for the next
+        // chunk's loads right after issuing them, before this chunk's steps.
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (c - 4 >= 0) fetch_rows(cbeg - 4 * kTileChunk);
-        if (c - 1 >= 0) fetch_xbar(cbeg - kTileChunk);
+        if (c - 1 >= 0) {
+            fetch_xbar(cbeg - kTileChunk);
+            fetch_start(c - 1);
+        }
         // inputs: rb[o * kTileRowStride + j] = x_w[cbeg + j - D]; the chunk's
         // start input x_w[cbeg - 1 - D] at its own ring position
         const float *rb = rrow + ((cbeg - D) & (kTileRing - 1));
@@ -661,7 +679,7 @@ __global__ __launch_bounds__(kTileThreads) void adjoint_gradient_tiled_kernel(
             float y0[3], pin[3];
 #pragma unroll
             for (int o = 0; o < 3; ++o) {
-                y0[o] = c == 0 ? 0.0f : ckpt[int64_t(c - 1) * M + item * 3 + o];
+                y0[o] = y0c[o];
                 if (kMode == 0) pin[o] = rp[o * kTileRowStride];
                 else if (cbeg - 1 - D < 0) pin[o] = 0.0f;
                 else pin[o] = (kMode == 2 && far) ? xwb[int64_t(o) * T - 1] : rp[o * kTileRowStride];

@@ -1454,6 +1454,11 @@ __global__ __launch_bounds__(kTileThreads) void derive_tiled_kernel(
     fetch(0);
     for (int32_t t0 = 0; t0 < T; t0 += kDeriveChunk) {
         stage(t0);
+        // Every load issued so far has landed (stage used them). Stated
+        // explicitly, since the compiler cannot see it through the per-thread
+        // conditions of the copies: without this it waits for the next
+        // chunk's loads right after issuing them, before this chunk's steps.
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (t0 + kDeriveChunk < T) fetch(t0 + kDeriveChunk);
         // the lane's 16 inputs of this chunk: rb[o * kTileRowStride + j] = x[t0 + j - D]
