@@ -340,8 +340,10 @@ def batched_problems(workload: str, args):
                  f"{args.checkpoint} steps), SGD lr 1e-3")
     else:
         problems = synth.config3(count=args.scenarios or 64, workers=BUILD_WORKERS)
+        gradient = ("reference derivation" if workload == "batched-uneven" else
+                    f"reverse-mode gradient (checkpoint every {args.checkpoint} steps)")
         label = (f"{len(problems)} blob scenarios, voxels log-uniform 1k-40k, 1-8 beats, "
-                 f"T uniform 600-2000, reference derivation, SGD lr 1e-3")
+                 f"T uniform 600-2000, {gradient}, SGD lr 1e-3")
     return problems, label
 
 
@@ -353,7 +355,8 @@ def bench_batched(args) -> int:
     problems, label = batched_problems(args.workload, args)
     log(f"{len(problems)} synthetic scenarios ({time.perf_counter() - t0:.1f} s)")
     gpu = GPU(0)
-    res = run_batched(gpu, problems, label, args.workload == "batched-adjoint", args.warmup,
+    adjoint = args.workload in ("batched-adjoint", "batched-uneven-adjoint")
+    res = run_batched(gpu, problems, label, adjoint, args.warmup,
                       args.steps, args.checkpoint, args.share_propagation,
                       args.voxels_per_group, not args.no_graph,
                       trajectory=args.workload == "batched-adjoint", derivation=args.derivation)
@@ -415,11 +418,14 @@ def main() -> int:
     ap.add_argument("--derivation", choices=["auto", "lanes", "tiles"], default="auto",
                     help="derivation / gradient pass: over voxel tiles when built (auto), a lane "
                          "per connection, or tiles")
-    ap.add_argument("--workload", choices=["plain", "batched-uneven", "batched-adjoint", "copy"],
+    ap.add_argument("--workload", choices=["plain", "batched-uneven", "batched-adjoint",
+                                           "batched-uneven-adjoint", "copy"],
                     default="plain",
                     help="plain: the 120k-voxel epoch (the headline line); batched-uneven: 64 "
                          "uneven scenarios per batched epoch; batched-adjoint: 8 shells x 4 "
-                         "beats with the reverse-mode gradient; copy: HBM counter calibration "
+                         "beats with the reverse-mode gradient; batched-uneven-adjoint: the 64 "
+                         "uneven scenarios with the reverse-mode gradient; copy: HBM counter "
+                         "calibration "
                          "(a plain elementwise pass of --copy-mib MiB)")
     ap.add_argument("--copy-mib", type=int, default=1024, help="copy workload: MiB per dispatch")
     ap.add_argument("--scenarios", type=int, default=0, help="batched workloads: scenario count")
@@ -427,7 +433,7 @@ def main() -> int:
                     help="batched workloads: 0 = autotune")
     ap.add_argument("--no-graph", action="store_true", help="batched workloads: eager epochs")
     ap.add_argument("--no-batched", action="store_true",
-                    help="plain workload: skip the two batched workloads of the headline line")
+                    help="plain workload: skip the three batched workloads of the headline line")
     ap.add_argument("--batched-adjoint-epochs", type=int, default=50,
                     help="plain workload: timed epochs of the 8-shell reverse-mode batch")
     ap.add_argument("--share-propagation", choices=["auto", "on", "off"], default="auto",
@@ -597,15 +603,18 @@ def main() -> int:
     if not args.no_batched and args.gradient == "reference":
         plain = out["value"]
         batched = {}
+        keys = {"batched-uneven": "uneven", "batched-uneven-adjoint": "uneven_adjoint",
+                "batched-adjoint": "adjoint"}
         for workload, warm, timed, extra in (
                 ("batched-uneven", 2, 5, {}),
+                ("batched-uneven-adjoint", 1, 3, {}),
                 ("batched-adjoint", 1, args.batched_adjoint_epochs,
                  {"trajectory": True, "gradient_check": 3})):
             t1 = time.perf_counter()
             problems, label = batched_problems(workload, args)
             log(f"{workload}: {len(problems)} scenarios built ({time.perf_counter() - t1:.1f} s)")
-            batched[workload.split("-")[1]] = run_batched(
-                gpu, problems, label, workload == "batched-adjoint", warm, timed,
+            batched[keys[workload]] = run_batched(
+                gpu, problems, label, workload != "batched-uneven", warm, timed,
                 args.checkpoint, plain_epochs_per_s=plain, derivation=args.derivation, **extra)
             del problems
             log(f"{workload}: done ({time.perf_counter() - t1:.1f} s)")

@@ -157,24 +157,38 @@ constexpr int kConsGainStride = 10;  // gcons floats per (voxel, consumer) slot
 // exchange needs more registers and keeps fewer).
 constexpr int kSweepCachedCoherent = 4;
 constexpr int kSweepCachedFenced = 2;
+// The batched sweep also holds its job's fields: one item fits (two spill
+// 22 registers). It takes jobs with the coherent exchange only.
+constexpr int kBatchSweepCached = 1;
 constexpr uint32_t kSweepNoEntry = 0xFFFFFFFFu;
 static_assert(kSortSize >= kSweepItems * kSweepThreads || kSortSize >= kResMaxVoxels * 26,
               "every phase-1 position has a key");
 static_assert(kSortSize * 8 <= kResMaxVoxels * 78 * 4, "sort keys must fit over q");
 
-// kCoherent: the xbar rows are stored through the L2 and gathered past it
-// (sc1), and the barrier needs no agent-scope fences (the launcher picks the
-// instantiation).
-template <bool kCoherent>
-__global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
-    const int32_t *__restrict__ cons_ptr, const int32_t *__restrict__ cons_idx,
-    const int32_t *__restrict__ delays, const float *__restrict__ gains_rows,
-    const float *__restrict__ coefs, float *__restrict__ gcons,
-    const float *__restrict__ x, const float *__restrict__ E,
-    float *__restrict__ xbar, GridBarrier *__restrict__ gb, int32_t N, int32_t T, AdjointParams p,
-    uint64_t wait_ticks) {
-    constexpr int kSweepCachedItems = kCoherent ? kSweepCachedCoherent : kSweepCachedFenced;
+// The backward sweep of slab `slab` of the `slabs` slabs of job j, barrier
+// member `member` of its team; false if the job aborted (a wait timed out,
+// here or in another member). kCoherent: the xbar rows are stored through
+// the L2 and gathered past it (sc1), and the barrier needs no agent-scope
+// fences. `abort`: the sticky word a timed-out wait raises and every wait
+// polls (null: the job barrier's own). The first wait gets `first_wait`
+// ticks, the others `step_wait`.
+template <bool kCoherent, int kSweepCachedItems>
+__device__ __forceinline__ bool sweep_slab(const SweepJob &j, int32_t slab, int32_t slabs,
+                                           uint32_t member, uint32_t *abort,
+                                           uint64_t first_wait, uint64_t step_wait) {
     extern __shared__ float lds[];
+    const int32_t *__restrict__ cons_ptr = j.cons_ptr;
+    const int32_t *__restrict__ cons_idx = j.cons_idx;
+    const int32_t *__restrict__ delays = j.delays;
+    const float *__restrict__ gains_rows = j.gains_rows;
+    const float *__restrict__ coefs = j.coefs;
+    float *__restrict__ gcons = j.gcons;
+    const float *__restrict__ x = j.x;
+    const float *__restrict__ E = j.E;
+    float *__restrict__ xbar = j.xbar;
+    GridBarrier *__restrict__ gb = j.gb;
+    const int32_t N = j.N, T = j.T;
+    const AdjointParams p = j.p;
     float *q_lds = lds;  // [voxel][26][3]
     // the xbar history as one buffer resource
     const __amdgpu_buffer_rsrc_t xbrs = __builtin_amdgcn_make_buffer_rsrc(xbar, 0, -1, kBufferWord3);
@@ -182,9 +196,8 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
     uint8_t *dtab = reinterpret_cast<uint8_t *>(ok_lds + 4);          // [phase-1 position]
     uint64_t *keys = reinterpret_cast<uint64_t *>(lds);                 // set-up only, over q
     const int64_t n = 3LL * N;
-    const int32_t slab = xcd_block(blockIdx.x, gridDim.x);
-    const int32_t vb = int32_t(int64_t(slab) * N / gridDim.x);
-    const int32_t ve = int32_t(int64_t(slab + 1) * N / gridDim.x);
+    const int32_t vb = int32_t(int64_t(slab) * N / slabs);
+    const int32_t ve = int32_t(int64_t(slab + 1) * N / slabs);
     const int32_t nv = ve - vb, items = nv * 26;
     const int tid = threadIdx.x;
     uint32_t gen = 0;
@@ -281,7 +294,9 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
     for (int32_t s = T - 1; s >= 0; --s) {
 #pragma unroll 1
         for (uint32_t pass = 0; pass < 2; ++pass) {
-        if (pass == 1 && s < T - 1 && !team_wait(gb, gen, ok_lds, wait_ticks, kCoherent)) return;
+        if (pass == 1 && s < T - 1 &&
+            !team_wait(gb, gen, ok_lds, s == T - 2 ? first_wait : step_wait, kCoherent, abort))
+            return false;
         // phase 1: each item's q for step s (its ybar at tau = s + D). The
         // empty asm statement hides the lane index from the optimiser, so the
         // per-item addresses are recomputed each step instead of being kept
@@ -363,7 +378,60 @@ __global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
                 __hip_atomic_store(xbar + row + o, acc[o], __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);  // written through the L2
         }
-        team_arrive(gb, gridDim.x, blockIdx.x, gen, kCoherent);
+        team_arrive(gb, uint32_t(slabs), member, gen, kCoherent);
+    }
+    return true;
+}
+
+// One scenario's sweep: a slab per workgroup (launch_adjoint_sweep_resident
+// picks the instantiation).
+template <bool kCoherent>
+__global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_resident_kernel(
+    const int32_t *__restrict__ cons_ptr, const int32_t *__restrict__ cons_idx,
+    const int32_t *__restrict__ delays, const float *__restrict__ gains_rows,
+    const float *__restrict__ coefs, float *__restrict__ gcons,
+    const float *__restrict__ x, const float *__restrict__ E,
+    float *__restrict__ xbar, GridBarrier *__restrict__ gb, int32_t N, int32_t T, AdjointParams p,
+    uint64_t wait_ticks) {
+    SweepJob j;
+    j.cons_ptr = cons_ptr;
+    j.cons_idx = cons_idx;
+    j.delays = delays;
+    j.gains_rows = gains_rows;
+    j.coefs = coefs;
+    j.gcons = gcons;
+    j.x = x;
+    j.E = E;
+    j.xbar = xbar;
+    j.gb = gb;
+    j.N = N;
+    j.T = T;
+    j.p = p;
+    j.slot0 = 0;
+    j.width = int32_t(gridDim.x);
+    j.coherent = kCoherent ? 1 : 0;
+    (void)sweep_slab<kCoherent, kCoherent ? kSweepCachedCoherent : kSweepCachedFenced>(
+        j, xcd_block(blockIdx.x, gridDim.x), int32_t(gridDim.x), blockIdx.x, nullptr, wait_ticks,
+        wait_ticks);
+}
+
+// Many scenarios' sweeps in one launch: slot q walks its task list, taking
+// slab q - slot0 of each job's team in turn (the host's placement makes the
+// lists deadlock-free, as for propagate_batch_kernel). Every job exchanges
+// its x̄ rows through agent-coherent accesses.
+__global__ __launch_bounds__(kSweepThreads) void adjoint_sweep_batch_kernel(
+    const SweepJob *__restrict__ jobs, const int32_t *__restrict__ task_ptr,
+    const int32_t *__restrict__ task_job, uint32_t *abort, uint64_t first_wait,
+    uint64_t step_wait) {
+    const int32_t slot = xcd_block(blockIdx.x, gridDim.x);
+    const int32_t q1 = task_ptr[slot + 1];
+    for (int32_t q = task_ptr[slot]; q < q1; ++q) {
+        const SweepJob job = jobs[task_job[q]];
+        const int32_t member = slot - job.slot0;
+        if (!sweep_slab<true, kBatchSweepCached>(job, member, job.width, uint32_t(member), abort,
+                                                 first_wait, step_wait))
+            return;
+        __syncthreads();  // LDS is reused by the next job
     }
 }
 
@@ -805,6 +873,30 @@ hipError_t sweep_capacity(int32_t *blocks_per_cu) {
     }
     *blocks_per_cu = fit;
     return hipSuccess;
+}
+
+hipError_t sweep_batch_capacity(int32_t *blocks_per_cu) {
+    const void *k = reinterpret_cast<const void *>(&adjoint_sweep_batch_kernel);
+    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       int(sweep_lds_bytes()));
+    if (e != hipSuccess) return e;
+    int nb = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, kSweepThreads, sweep_lds_bytes());
+    if (e != hipSuccess) return e;
+    *blocks_per_cu = nb;
+    return hipSuccess;
+}
+
+hipError_t launch_adjoint_sweep_batch(hipStream_t s, int32_t slots, const SweepJob *jobs,
+                                      GridBarrier *job_barriers, int32_t num_jobs,
+                                      const int32_t *task_ptr, const int32_t *task_job,
+                                      uint32_t *abort, uint64_t first_wait, uint64_t step_wait) {
+    if (slots <= 0 || num_jobs <= 0) return hipSuccess;
+    hipError_t e = launch_barrier_reset(s, job_barriers, num_jobs);
+    if (e != hipSuccess) return e;
+    adjoint_sweep_batch_kernel<<<slots, kSweepThreads, sweep_lds_bytes(), s>>>(
+        jobs, task_ptr, task_job, abort, first_wait, step_wait);
+    return hipGetLastError();
 }
 
 hipError_t launch_adjoint_sweep_resident(hipStream_t s, int32_t blocks, const int32_t *cons_ptr,

@@ -241,6 +241,7 @@ int ensure(ct_results_gpu &r, V **p, int64_t &have, int64_t count, int64_t unit)
 static int adjoint_forcing(ct_results_gpu &r, int32_t beat, bool accumulate, ct_epoch_kernel *k);
 static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K, int64_t slots,
                             int32_t num_beats, ct_epoch_kernel *k);
+static bool resident_sweep(const ct_results_gpu &r, int64_t slots);
 
 int32_t update_group(const ct_epoch_kernel &k) {
     return k.cfg.batch_size == 0 ? k.r->B : k.cfg.batch_size;
@@ -290,6 +291,11 @@ int finish_beat(ct_epoch_kernel &k, int32_t pos) {
     } else {
         CT_CHECK(adjoint_forcing(r, beat, pos != first, &k));
         if (pos != last) return CT_OK;
+    }
+    if (k.defer_backward) {
+        // a batch runs the pass, with the sweeps of other scenarios in one launch
+        k.deferred_beats = count;
+        return CT_OK;
     }
     return adjoint_backward(r, k.cfg, k.checkpoint_interval, r.pending_slots, count, &k);
 }
@@ -586,44 +592,67 @@ static int adjoint_forcing(ct_results_gpu &r, int32_t beat, bool accumulate, ct_
     return CT_OK;
 }
 
-// Backward sweep and gradient pass for the forcing in eT, which sums
-// `num_beats` beats that share the forward states (each beat contributes the
-// maximum-regularisation term once).
-static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K, int64_t slots,
-                            int32_t num_beats, ct_epoch_kernel *k) {
-    hipStream_t s = r.stream;
-    const int64_t n = r.n, T = r.T, M = int64_t(r.N) * 78;
-    {
-        Span sp(k, kTranspose);
-        if (!r.mapped_by_step)
-            CT_HIP(ct::launch_transpose(s, r.eT, r.E, n, T));  // mapped residuals by step
-        CT_HIP(ct::launch_transpose(s, r.gT, r.staging, 78, n));  // gains by state row
-    }
+// Parameters of the backward sweep for a forcing that sums `num_beats` beats
+// sharing the forward states (each beat contributes the maximum-regularisation
+// term once).
+ct::AdjointParams adjoint_params(const ct_results_gpu &r, const ct_algorithm &c,
+                                 int32_t num_beats) {
     ct::AdjointParams p;
     p.mse_scaling = 1.0f / float(r.S) * c.mse_strength;
     p.reg_scaling = c.maximum_regularization_strength * float(num_beats);
     p.threshold = c.maximum_regularization_threshold;
-    {
-        Span sp(k, kAdjointSweep);
-        if (resident_sweep(r, slots)) {
-            // all steps in one launch, recurrences owned by the consuming voxels
-            CT_HIP(ct::launch_adjoint_sweep_resident(s, r.resident_blocks, r.cons_ptr, r.cons_idx,
-                                                     r.delays, r.staging, r.coefs, r.gcons, r.x,
-                                                     r.E, r.xbar,
-                                                     r.barrier, r.N, r.T, p,
-                                                     wait_ticks(*r.gpu, r.wait_seconds),
-                                                     r.propagation_mode != 3));
-            r.resident_used = true;
-        } else {
-            CT_HIP(ct::launch_consumer_delays(s, r.cons_idx, r.delays, r.cons_delay, r.cons_len));
-            CT_HIP(ct::launch_fill(s, r.ybar, 0.0f, M));
-            for (int32_t step = r.T - 1; step >= 0; --step)
-                CT_HIP(ct::launch_adjoint_step(s, r.nbr, r.coefs, r.staging, r.cons_ptr,
-                                               r.cons_idx, r.cons_delay, r.x, r.E, r.ybar,
-                                               r.qring, r.xbar, r.N, r.T, step,
-                                               uint32_t(slots - 1), p));
-        }
+    return p;
+}
+
+bool sweep_batchable(const ct_results_gpu &r) {
+    return resident_sweep(r, r.pending_slots) && r.propagation_mode != 3 &&
+           ct::coherent_exchange(r.T, r.n);
+}
+
+int prepare_backward(ct_results_gpu &r, int32_t K) {
+    return adjoint_prepare(r, K, r.pending_slots);
+}
+
+// Backward pass, part 1: the forcing by step (transposed from eT if it was
+// not written so) and the gains by state row, the sweep's inputs.
+int adjoint_backward_begin(ct_results_gpu &r, ct_epoch_kernel *k) {
+    hipStream_t s = r.stream;
+    Span sp(k, kTranspose);
+    if (!r.mapped_by_step)
+        CT_HIP(ct::launch_transpose(s, r.eT, r.E, r.n, r.T));  // mapped residuals by step
+    CT_HIP(ct::launch_transpose(s, r.gT, r.staging, 78, r.n));  // gains by state row
+    return CT_OK;
+}
+
+// Part 2 on the scenario's own: the resident sweep (all steps in one launch,
+// recurrences owned by the consuming voxels) or the per-step launches.
+static int adjoint_sweep_own(ct_results_gpu &r, const ct::AdjointParams &p, int64_t slots,
+                             ct_epoch_kernel *k) {
+    hipStream_t s = r.stream;
+    const int64_t M = int64_t(r.N) * 78;
+    Span sp(k, kAdjointSweep);
+    if (resident_sweep(r, slots)) {
+        CT_HIP(ct::launch_adjoint_sweep_resident(s, r.resident_blocks, r.cons_ptr, r.cons_idx,
+                                                 r.delays, r.staging, r.coefs, r.gcons, r.x,
+                                                 r.E, r.xbar, r.barrier, r.N, r.T, p,
+                                                 wait_ticks(*r.gpu, r.wait_seconds),
+                                                 r.propagation_mode != 3));
+        r.resident_used = true;
+    } else {
+        CT_HIP(ct::launch_consumer_delays(s, r.cons_idx, r.delays, r.cons_delay, r.cons_len));
+        CT_HIP(ct::launch_fill(s, r.ybar, 0.0f, M));
+        for (int32_t step = r.T - 1; step >= 0; --step)
+            CT_HIP(ct::launch_adjoint_step(s, r.nbr, r.coefs, r.staging, r.cons_ptr, r.cons_idx,
+                                           r.cons_delay, r.x, r.E, r.ybar, r.qring, r.xbar, r.N,
+                                           r.T, step, uint32_t(slots - 1), p));
     }
+    return CT_OK;
+}
+
+// Part 3: the gradient pass over x̄ (checkpoints every K steps).
+int adjoint_backward_end(ct_results_gpu &r, int32_t K, ct_epoch_kernel *k) {
+    hipStream_t s = r.stream;
+    const int64_t n = r.n, T = r.T;
     Span sp(k, kDerive);
     if (!r.xT_fresh) {
         CT_HIP(ct::launch_transpose(s, r.x, r.xT, T, n));
@@ -641,6 +670,22 @@ static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K,
                                            r.ckpt, r.yseg, r.yseg_have, r.dGT, r.dA, r.N, r.T, K));
     }
     return CT_OK;
+}
+
+// Backward sweep and gradient pass for the forcing in E (or eT), which sums
+// `num_beats` beats that share the forward states.
+static int adjoint_backward(ct_results_gpu &r, const ct_algorithm &c, int32_t K, int64_t slots,
+                            int32_t num_beats, ct_epoch_kernel *k) {
+    CT_CHECK(adjoint_backward_begin(r, k));
+    CT_CHECK(adjoint_sweep_own(r, adjoint_params(r, c, num_beats), slots, k));
+    return adjoint_backward_end(r, K, k);
+}
+
+int adjoint_backward_deferred(ct_epoch_kernel &k) {
+    const int32_t beats = k.deferred_beats;
+    k.deferred_beats = 0;
+    if (beats <= 0) return CT_OK;
+    return adjoint_backward(*k.r, k.cfg, k.checkpoint_interval, k.r->pending_slots, beats, &k);
 }
 
 int derive_adjoint(ct_results_gpu &r, int32_t beat, const ct_algorithm &c, int32_t K,

@@ -48,23 +48,25 @@ __device__ __forceinline__ void team_arrive(GridBarrier *gb, uint32_t members, u
 
 // Waits (at most `wait_ticks` of wall clock) until generation `gen` is
 // published; false if the team aborted or the wait timed out (which raises
-// the team's sticky abort word). With `coherent` the rows are read with
-// agent-coherent loads and no acquire fence follows.
+// the sticky abort word: `abort`, or the barrier's own when null). With
+// `coherent` the rows are read with agent-coherent loads and no acquire fence
+// follows.
 __device__ __forceinline__ bool team_wait(GridBarrier *gb, uint32_t gen, int *ok_lds,
-                                          uint64_t wait_ticks, bool coherent) {
+                                          uint64_t wait_ticks, bool coherent,
+                                          uint32_t *abort = nullptr) {
     if (threadIdx.x == 0) {
+        uint32_t *const ab = abort ? abort : &gb->abort.count;
         int ok = 1;
         const uint64_t t0 = uint64_t(wall_clock64());
         uint32_t polls = 0;
         while (__hip_atomic_load(&gb->generation.count, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_AGENT) < gen) {
-            if (__hip_atomic_load(&gb->abort.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            if (__hip_atomic_load(ab, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                 ok = 0;
                 break;
             }
             if ((polls++ & 63u) == 0 && uint64_t(wall_clock64()) - t0 > wait_ticks) {
-                __hip_atomic_store(&gb->abort.count, 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(ab, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
                 break;
             }

@@ -10,11 +10,15 @@
 // scenario's arrays, and every workgroup slot walks its own task list. The
 // planner packs the jobs (team width = ceil(N / voxels per workgroup) slots,
 // length ~ T steps) onto the slots longest first, and the autotuner picks the
-// voxels per workgroup by timing the first round. Optionally the whole epoch
-// is captured once in a hipGraph and replayed.
+// voxels per workgroup by timing the first round. Under the reverse-mode
+// gradient the backward sweeps of the scenarios whose update batch ends in a
+// round run the same way, in ONE persistent launch (adjoint_sweep_batch_kernel)
+// placed by the same planner. Optionally the whole epoch is captured once in a
+// hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstring>
 
 #include "device.h"
 #include "runtime.h"
@@ -51,6 +55,23 @@ struct ct_batch {
     uint32_t *flags = nullptr;  // slab flags of every job, job after job
     int32_t num_jobs = 0;
 
+    // backward sweeps: per round, the reverse-mode scenarios (sharing one
+    // backward pass per update batch) whose update batch ends in the round
+    // and whose sweep runs in the round's batched sweep launch
+    struct SweepRound {
+        std::vector<int32_t> scenarios;         // job order
+        std::vector<ct::AdjointParams> params;  // each job's sweep parameters
+        int32_t job0 = 0;                       // first job / barrier of the round
+        int32_t ptr0 = 0;                       // first task_ptr entry (sweep_slots + 1 per round)
+    };
+    std::vector<SweepRound> sweep_rounds;
+    int32_t sweep_slots = 0;  // workgroups of the batched sweep launch (0 = unused)
+    ct::SweepJob *sweep_jobs = nullptr;
+    int32_t *sweep_task_ptr = nullptr, *sweep_task_job = nullptr;
+    ct::GridBarrier *sweep_barriers = nullptr;
+    int32_t num_sweep_jobs = 0;
+    bool sweep_replan = false;  // a planned sweep job no longer matched its scenario
+
     // graph of one epoch
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -72,11 +93,14 @@ struct ct_batch {
 
     // Side streams (default priority) for the scenarios' work after each
     // round's propagation: scenario s runs on side[s % side.size()], after the
-    // `fork` event recorded behind the propagation; the next propagation waits
-    // for every side stream's `done` event. Used only while no member runs the
-    // reverse-mode gradient: its backward sweep is a persistent kernel whose
-    // workgroups must all be resident at once, which two sweeps on two streams
-    // would contend for. Otherwise everything runs on the GPU's stream.
+    // `fork` event recorded behind the propagation; the next persistent launch
+    // (propagation or batched sweep) waits for every side stream's `done`
+    // event, so no side work overlaps one. Used while every reverse-mode
+    // member shares one backward pass per update batch: those passes are
+    // deferred to the flushes, whose sweeps run on the GPU's stream. A member
+    // that would sweep inside its beat (a persistent kernel whose workgroups
+    // must all be resident at once, which two sweeps on two streams would
+    // contend for) puts everything on the GPU's stream.
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> done;
     hipEvent_t fork = nullptr;
@@ -116,8 +140,9 @@ bool scenario_resident(const ct_results_gpu &r) {
 // workgroup slots (longest first). Fills job slot0 / width and the per-slot
 // task lists in one global order (start time, then placement), which keeps
 // the launch deadlock-free. Returns the predicted makespan (ms).
+template <typename Job>
 double place_round(const std::vector<ct_results_gpu *> &rs, int32_t slots, int32_t vpg,
-                   std::vector<ct::ResidentJob> &jobs, std::vector<std::vector<int32_t>> &tasks) {
+                   std::vector<Job> &jobs, std::vector<std::vector<int32_t>> &tasks) {
     const int32_t nj = int32_t(rs.size());
     std::vector<double> cost(nj);
     std::vector<int32_t> order(nj);
@@ -196,6 +221,109 @@ void drop_graph(ct_batch &b) {
     b.exec = nullptr;
     b.graph = nullptr;
     b.warmed = false;
+}
+
+// The batched backward sweeps. A reverse-mode scenario that shares one
+// backward pass per update batch leaves that pass (and the update after it)
+// pending when its update batch ends; the pending passes are flushed together
+// just before a round in which one of their scenarios propagates again (its
+// states would be overwritten), and at the end of the epoch. Flush point p <
+// rounds is the one before round p's propagation, p = rounds the last. The
+// jobs of a point are its pending scenarios whose sweep the resident kernel
+// with the coherent exchange runs (sizes as for the batched propagation: a
+// scenario needing more than half the slots sweeps on its own). Allocates the
+// scenarios' backward buffers now: the job table holds their addresses.
+int plan_sweeps(ct_batch &b, int32_t vpg) {
+    const int32_t S = int32_t(b.kernels.size());
+    const int32_t rounds = int32_t(b.rounds.size());
+    const int32_t slots = b.slots > 0 ? b.sweep_slots : 0;
+    b.sweep_rounds.assign(rounds + 1, {});
+    b.sweep_replan = false;
+    std::vector<ct::SweepJob> all_jobs;
+    std::vector<int32_t> all_ptr, all_task;
+    std::vector<int32_t> pending(S, 0);  // beats of the pending pass
+    for (int32_t pt = 0; pt <= rounds; ++pt) {
+        ct_batch::SweepRound &W = b.sweep_rounds[pt];
+        bool flush = pt == rounds;
+        if (!flush)
+            for (int32_t s : b.rounds[pt].propagate) flush = flush || pending[s] > 0;
+        std::vector<ct_results_gpu *> rs;
+        std::vector<ct::SweepJob> jobs;
+        for (int32_t s = 0; s < S && flush; ++s) {
+            if (pending[s] <= 0) continue;
+            const int32_t beats = pending[s];
+            pending[s] = 0;
+            ct_epoch_kernel &k = *b.kernels[s];
+            ct_results_gpu &r = *k.r;
+            if (slots <= 0 || (r.N + ct::kResMaxVoxels - 1) / ct::kResMaxVoxels > slots ||
+                int64_t(r.N) * 2 > int64_t(slots) * ct::kResMaxVoxels)
+                continue;
+            CT_CHECK(prepare_backward(r, k.checkpoint_interval));
+            if (!sweep_batchable(r)) continue;
+            const ct::AdjointParams p = adjoint_params(r, k.cfg, beats);
+            ct::SweepJob j{};
+            j.cons_ptr = r.cons_ptr;
+            j.cons_idx = r.cons_idx;
+            j.delays = r.delays;
+            j.gains_rows = r.staging;
+            j.coefs = r.coefs;
+            j.gcons = r.gcons;
+            j.x = r.x;
+            j.E = r.E;
+            j.xbar = r.xbar;
+            j.N = r.N;
+            j.T = r.T;
+            j.p = p;
+            j.coherent = 1;
+            W.scenarios.push_back(s);
+            W.params.push_back(p);
+            jobs.push_back(j);
+            rs.push_back(&r);
+        }
+        // the passes the round's beats leave pending
+        for (int32_t s = 0; pt < rounds && s < S; ++s) {
+            const ct_epoch_kernel &k = *b.kernels[s];
+            const ct_results_gpu &r = *k.r;
+            if (!uses_adjoint(k) || !shares_propagation(k) || r.N <= 0 || r.B <= pt) continue;
+            const int32_t group = update_group(k), first = pt - pt % group;
+            const int32_t last = std::min<int32_t>(first + group, r.B) - 1;
+            if (pt == last) pending[s] = last - first + 1;
+        }
+        W.job0 = int32_t(all_jobs.size());
+        W.ptr0 = int32_t(all_ptr.size());
+        std::vector<std::vector<int32_t>> tasks;
+        if (!rs.empty()) (void)place_round(rs, slots, vpg, jobs, tasks);
+        all_jobs.insert(all_jobs.end(), jobs.begin(), jobs.end());
+        int32_t acc = int32_t(all_task.size());
+        for (int32_t q = 0; q < slots; ++q) {
+            all_ptr.push_back(acc);
+            if (!tasks.empty())
+                for (int32_t j : tasks[q]) {
+                    all_task.push_back(j);
+                    ++acc;
+                }
+        }
+        all_ptr.push_back(acc);
+    }
+    batch_free(b, b.sweep_jobs);
+    batch_free(b, b.sweep_task_ptr);
+    batch_free(b, b.sweep_task_job);
+    batch_free(b, b.sweep_barriers);
+    b.sweep_jobs = nullptr;
+    b.sweep_task_ptr = b.sweep_task_job = nullptr;
+    b.sweep_barriers = nullptr;
+    b.num_sweep_jobs = int32_t(all_jobs.size());
+    if (all_jobs.empty()) return CT_OK;
+    CT_CHECK(batch_alloc(b, &b.sweep_jobs, b.num_sweep_jobs));
+    CT_CHECK(batch_alloc(b, &b.sweep_task_ptr, int64_t(all_ptr.size())));
+    CT_CHECK(batch_alloc(b, &b.sweep_task_job, int64_t(all_task.size())));
+    CT_CHECK(batch_alloc(b, &b.sweep_barriers, b.num_sweep_jobs));
+    for (size_t j = 0; j < all_jobs.size(); ++j) all_jobs[j].gb = b.sweep_barriers + j;
+    CT_HIP(hipMemset(b.sweep_barriers, 0, sizeof(ct::GridBarrier) * all_jobs.size()));
+    CT_CHECK(upload(b.sweep_jobs, all_jobs.data(), sizeof(ct::SweepJob) * all_jobs.size()));
+    CT_CHECK(upload(b.sweep_task_ptr, all_ptr.data(), sizeof(int32_t) * all_ptr.size()));
+    CT_CHECK(upload(b.sweep_task_job, all_task.data(), sizeof(int32_t) * all_task.size()));
+    return CT_OK;
 }
 
 // Builds the rounds and the device job tables for `vpg` voxels per group.
@@ -299,12 +427,13 @@ int plan(ct_batch &b, int32_t vpg) {
     CT_CHECK(upload(b.jobs, all_jobs.data(), sizeof(ct::ResidentJob) * all_jobs.size()));
     CT_CHECK(upload(b.task_ptr, all_ptr.data(), sizeof(int32_t) * all_ptr.size()));
     CT_CHECK(upload(b.task_job, all_task.data(), sizeof(int32_t) * all_task.size()));
+    CT_CHECK(plan_sweeps(b, vpg));
     b.plan_vpg = vpg;
     return CT_OK;
 }
 
 bool plan_current(const ct_batch &b) {
-    if (b.plan_vpg != b.voxels_per_group) return false;
+    if (b.plan_vpg != b.voxels_per_group || b.sweep_replan) return false;
     for (size_t s = 0; s < b.kernels.size(); ++s) {
         const ct_epoch_kernel &k = *b.kernels[s];
         if (b.plan_checkpoints[s] != batch_checkpoint(k) ||
@@ -388,24 +517,106 @@ int enqueue_epoch_body(ct_batch &b, bool timed);
 
 // The members' updates also check the batched launch's abort word.
 int enqueue_epoch(ct_batch &b, bool timed) {
-    for (ct_epoch_kernel *k : b.kernels) k->r->batch_abort = &b.barriers->abort.count;
+    for (ct_epoch_kernel *k : b.kernels) {
+        k->r->batch_abort = &b.barriers->abort.count;
+        k->defer_backward = true;
+        k->deferred_beats = 0;
+    }
     const int rc = enqueue_epoch_body(b, timed);
-    for (ct_epoch_kernel *k : b.kernels) k->r->batch_abort = nullptr;
+    for (ct_epoch_kernel *k : b.kernels) {
+        k->r->batch_abort = nullptr;
+        k->defer_backward = false;
+        k->deferred_beats = 0;
+    }
     return rc;
+}
+
+// Flush point `pt` (see plan_sweeps): if it is the last, or a scenario of
+// round pt's propagation has a pending backward pass, every pending pass
+// runs now. The planned jobs whose scenario still matches (pending pass,
+// sweep parameters, resident sweep possible) get their sweep inputs, one
+// batched sweep launch and their gradient passes; any other pending pass runs
+// whole on its scenario's own, after the launch (overwriting what the launch
+// wrote for it), and the next epoch plans again. Then the updates that waited
+// for those passes: `upd[s]` beats of scenario s (update number update[s]).
+int flush_backward(ct_batch &b, int32_t pt, std::vector<int64_t> &upd,
+                   std::vector<int32_t> &update) {
+    const int32_t rounds = int32_t(b.rounds.size());
+    bool flush = pt >= rounds;
+    if (!flush)
+        for (int32_t s : b.rounds[pt].propagate) flush = flush || b.kernels[s]->deferred_beats > 0;
+    if (!flush) return CT_OK;
+    const ct_batch::SweepRound *W =
+        pt < int32_t(b.sweep_rounds.size()) ? &b.sweep_rounds[pt] : nullptr;
+    std::vector<int32_t> in_launch;
+    CT_CHECK(join_side(b));  // the pending passes' forcing may be on side streams
+    if (W && !W->scenarios.empty()) {
+        for (size_t i = 0; i < W->scenarios.size(); ++i) {
+            ct_epoch_kernel &k = *b.kernels[W->scenarios[i]];
+            const ct::AdjointParams p = adjoint_params(*k.r, k.cfg, k.deferred_beats);
+            const bool ok = k.deferred_beats > 0 && sweep_batchable(*k.r) &&
+                            std::memcmp(&p, &W->params[i], sizeof(p)) == 0;
+            if (!ok) {
+                b.sweep_replan = true;
+                continue;
+            }
+            CT_CHECK(adjoint_backward_begin(*k.r, &k));
+            in_launch.push_back(W->scenarios[i]);
+        }
+        if (!in_launch.empty()) {
+            double step_s = 1e300;
+            for (const ct_epoch_kernel *k : b.kernels) step_s = std::min(step_s, k->r->wait_seconds);
+            const double first_s = std::min(60.0, 30.0 * step_s);
+            CT_HIP(ct::launch_adjoint_sweep_batch(
+                b.gpu->stream, b.sweep_slots, b.sweep_jobs + W->job0, b.sweep_barriers + W->job0,
+                int32_t(W->scenarios.size()), b.sweep_task_ptr + W->ptr0, b.sweep_task_job,
+                &b.barriers->abort.count, wait_ticks(*b.gpu, first_s), wait_ticks(*b.gpu, step_s)));
+        }
+    }
+    // the other pending passes whole, sweeps included, on the GPU's stream
+    const int32_t S = int32_t(b.kernels.size());
+    for (int32_t s = 0; s < S; ++s) {
+        ct_epoch_kernel &k = *b.kernels[s];
+        if (k.deferred_beats > 0 &&
+            std::find(in_launch.begin(), in_launch.end(), s) == in_launch.end())
+            CT_CHECK(adjoint_backward_deferred(k));
+    }
+    // gradient passes of the batched sweeps and the waiting updates on the
+    // side streams, joined before the next persistent launch
+    CT_CHECK(fork_side(b));
+    for (int32_t s = 0; s < S; ++s) {
+        ct_epoch_kernel &k = *b.kernels[s];
+        CT_CHECK(on_side(b, s, [&] {
+            if (k.deferred_beats > 0) {
+                k.deferred_beats = 0;
+                CT_CHECK(adjoint_backward_end(*k.r, k.checkpoint_interval, &k));
+            }
+            if (upd[s] > 0) {
+                CT_CHECK(finish_batch(k, upd[s], true, update[s]++));
+                upd[s] = 0;
+            }
+            return CT_OK;
+        }));
+    }
+    return join_side(b);
 }
 
 int enqueue_epoch_body(ct_batch &b, bool timed) {
     const int32_t S = int32_t(b.kernels.size());
-    std::vector<int64_t> in_batch(S, 0);
+    std::vector<int64_t> in_batch(S, 0), upd(S, 0);
     std::vector<int32_t> update(S, 0);
     b.side_active = !b.side.empty() &&
-                    std::none_of(b.kernels.begin(), b.kernels.end(),
-                                 [](const ct_epoch_kernel *k) { return uses_adjoint(*k); });
+                    std::none_of(b.kernels.begin(), b.kernels.end(), [](const ct_epoch_kernel *k) {
+                        return uses_adjoint(*k) && !shares_propagation(*k);
+                    });
     for (int32_t s = 0; s < S; ++s) CT_CHECK(reset_derivatives(*b.kernels[s]->r));
     for (int32_t rd = 0; rd < int32_t(b.rounds.size()); ++rd) {
         // the round's propagation overwrites states the previous round's
         // beats read on the side streams
         if (rd > 0) CT_CHECK(join_side(b));
+        // pending backward passes (reverse-mode scenarios only, which leave
+        // the side streams unused) before states they read are overwritten
+        CT_CHECK(flush_backward(b, rd, upd, update));
         CT_CHECK(propagate_round(b, rd, timed));
         CT_CHECK(fork_side(b));
         for (int32_t s : b.rounds[rd].scenarios) {
@@ -413,13 +624,19 @@ int enqueue_epoch_body(ct_batch &b, bool timed) {
             CT_CHECK(on_side(b, s, [&] {
                 CT_CHECK(finish_beat(k, rd));
                 if (k.cfg.batch_size != 0 && ++in_batch[s] == k.cfg.batch_size) {
-                    CT_CHECK(finish_batch(k, in_batch[s], true, update[s]++));
+                    // after the backward pass, if the beat left it pending
+                    if (k.deferred_beats > 0)
+                        upd[s] = in_batch[s];
+                    else
+                        CT_CHECK(finish_batch(k, in_batch[s], true, update[s]++));
                     in_batch[s] = 0;
                 }
                 return CT_OK;
             }));
         }
     }
+    CT_CHECK(join_side(b));
+    CT_CHECK(flush_backward(b, int32_t(b.rounds.size()), upd, update));
     for (int32_t s = 0; s < S; ++s) {
         ct_epoch_kernel &k = *b.kernels[s];
         CT_CHECK(on_side(b, s, [&] {
@@ -608,6 +825,9 @@ int ct_batch_new(ct_gpu *gpu, ct_epoch_kernel *const *kernels, int32_t count, ct
             hipSuccess &&
         ct::batch_capacity(&per_cu) == hipSuccess && cus > 0 && per_cu > 0)
         b->slots = cus * per_cu;
+    int32_t sweep_per_cu = 0;
+    if (cus > 0 && ct::sweep_batch_capacity(&sweep_per_cu) == hipSuccess && sweep_per_cu > 0)
+        b->sweep_slots = cus * sweep_per_cu;
     (void)hipGetLastError();
     for (int j = 0; j < kSideStreams; ++j) {
         hipStream_t st = nullptr;
@@ -737,6 +957,7 @@ int ct_batch_get_info(ct_batch *b, ct_batch_info *out) {
         out->tuned_ms[i] = b->tuned_ms[i];
     }
     out->side_streams = b->side_active ? int32_t(b->side.size()) : 0;
+    out->sweep_jobs = b->num_sweep_jobs;
     return CT_OK;
 }
 

@@ -189,6 +189,41 @@ struct AdjointParams {
     float reg_scaling;
     float threshold;
 };
+// One backward sweep of a batched sweep launch: a scenario's arrays as the
+// resident sweep takes them (gains by state row, consumer table, forcing E
+// and states x by step, x̄ written by step), the team placement slot0 / width
+// (slab m of the team holds voxels [m N / width, (m + 1) N / width)), and the
+// job's own barrier; a timed-out wait raises the launch's shared abort word.
+struct SweepJob {
+    const int32_t *cons_ptr;
+    const int32_t *cons_idx;
+    const int32_t *delays;
+    const float *gains_rows;
+    const float *coefs;
+    float *gcons;
+    const float *x;
+    const float *E;
+    float *xbar;
+    GridBarrier *gb;
+    int32_t N;
+    int32_t T;
+    AdjointParams p;
+    int32_t slot0;
+    int32_t width;
+    int32_t coherent;  // 1: the job exchanges x̄ rows through agent-coherent accesses
+};
+// Workgroups of the batched sweep that fit on one CU at once.
+hipError_t sweep_batch_capacity(int32_t *blocks_per_cu);
+// Batched backward sweeps: `slots` workgroups (all resident at once) run the
+// task lists task_job[task_ptr[q] .. task_ptr[q+1]) of slot q over `jobs`;
+// the jobs' barriers (job_barriers[0 .. num_jobs)) are cleared here before
+// the launch, `abort` (sticky) is not. A job's first wait gets `first_wait`
+// ticks (its members may still be finishing earlier jobs), the others
+// `step_wait`. Every job must have coherent != 0.
+hipError_t launch_adjoint_sweep_batch(hipStream_t s, int32_t slots, const SweepJob *jobs,
+                                      GridBarrier *job_barriers, int32_t num_jobs,
+                                      const int32_t *task_ptr, const int32_t *task_job,
+                                      uint32_t *abort, uint64_t first_wait, uint64_t step_wait);
 hipError_t launch_max_delay(hipStream_t s, const int32_t *nbr, const int32_t *delays, int64_t len,
                             int32_t *out);
 hipError_t launch_consumer_delays(hipStream_t s, const int32_t *cons_idx, const int32_t *delays,

@@ -200,6 +200,12 @@ struct ct_epoch_kernel {
     // derivation, the same gradient up to FP32 rounding under reverse mode);
     // 0 never; 1 always.
     int32_t share_mode = -1;
+    // set by a batch for its epoch: the backward pass due at the end of an
+    // update batch (shared propagation) is left to the batch, which runs the
+    // sweeps of a round's scenarios in one launch; deferred_beats > 0 marks
+    // such a pending pass and the beats its forcing sums
+    bool defer_backward = false;
+    int32_t deferred_beats = 0;
     std::vector<hipEvent_t> events;
     size_t used_events = 0;
     struct SpanRec {
@@ -267,6 +273,19 @@ bool uses_adjoint(const ct_epoch_kernel &k);
 // Whether the beats of one update batch share one propagation (share_mode).
 bool shares_propagation(const ct_epoch_kernel &k);
 int finish_batch(ct_epoch_kernel &k, int64_t num_beats, bool reset_after, int32_t update);
+// Reverse-mode backward pass in parts, for a batch that sweeps many scenarios
+// in one launch: the buffers and q-ring size (prepare_backward), the sweep's
+// parameters for a forcing over `num_beats` beats, whether the resident
+// sweep with the coherent exchange can run the problem, the inputs of the
+// sweep (begin) and the gradient pass after it (end); and a deferred pass run
+// whole on the scenario's own (adjoint_backward_deferred).
+int prepare_backward(ct_results_gpu &r, int32_t K);
+ct::AdjointParams adjoint_params(const ct_results_gpu &r, const ct_algorithm &c,
+                                 int32_t num_beats);
+bool sweep_batchable(const ct_results_gpu &r);
+int adjoint_backward_begin(ct_results_gpu &r, ct_epoch_kernel *k);
+int adjoint_backward_end(ct_results_gpu &r, int32_t K, ct_epoch_kernel *k);
+int adjoint_backward_deferred(ct_epoch_kernel &k);
 int32_t updates_per_epoch(const ct_epoch_kernel &k);
 int begin_epoch(ct_epoch_kernel &k);
 void end_epoch(ct_epoch_kernel &k);

@@ -88,12 +88,13 @@ class BatchInfo(ctypes.Structure):
         ("propagate_ms", ctypes.c_double), ("propagate_launches", ctypes.c_int64),
         ("voxel_steps", ctypes.c_int64), ("tuned_candidates", ctypes.c_int32),
         ("tuned_voxels", ctypes.c_int32 * 16), ("tuned_ms", ctypes.c_double * 16),
-        ("side_streams", ctypes.c_int32),
+        ("side_streams", ctypes.c_int32), ("sweep_jobs", ctypes.c_int32),
     ]
 
     def as_dict(self) -> Dict[str, object]:
         d = {name: getattr(self, name) for name, _ in self._fields_[:11]}
         d["side_streams"] = self.side_streams
+        d["sweep_jobs"] = self.sweep_jobs
         n = self.tuned_candidates
         d["autotune"] = {int(self.tuned_voxels[i]): float(self.tuned_ms[i]) for i in range(n)}
         return d

@@ -397,9 +397,14 @@ typedef struct {
     int32_t tuned_voxels[16];
     double tuned_ms[16];                  /* time of round 0's batched propagation for each */
     int32_t side_streams;                 /* streams the last epoch ran the scenarios' beats on
-                                             besides the GPU's own (0 while a member runs the
-                                             reverse-mode gradient, whose backward sweep must
-                                             have the device to itself) */
+                                             besides the GPU's own (0 while a reverse-mode
+                                             member does not share its backward pass over an
+                                             update batch: its sweep would run inside a beat,
+                                             and a sweep must have the device to itself) */
+    int32_t sweep_jobs;                   /* backward sweeps of the batched sweep launches over
+                                             all rounds (reverse-mode members that share one
+                                             backward pass per update batch; a scenario needing
+                                             more than half the slots sweeps on its own) */
 } ct_batch_info;
 
 /* All kernels must live on `gpu`, with distinct results objects. The batch

@@ -65,7 +65,7 @@ CFGS = [
 
 
 class Member:
-    def __init__(self, gpu, p, spec, epochs, order=None, gradient=None, mode=0):
+    def __init__(self, gpu, p, spec, epochs, order=None, gradient=None, mode=0, share=None):
         from cardiotrust_amd.gpu import Buffer, EpochKernel, ResultsGPU
         nbatch = epochs * (p.num_beats if spec["cfg"].batch_size else 1)
         self.r = ResultsGPU(gpu, p, num_batch_entries=nbatch, optimizer=spec["optimizer"])
@@ -77,6 +77,8 @@ class Member:
             self.k.set_beat_order(order)
         if gradient is not None:
             self.k.set_gradient(*gradient)
+        if share is not None:
+            self.k.set_share_propagation(share)
 
     def results(self):
         out = self.r.update_from_gpu(list(FIELDS))
@@ -101,11 +103,12 @@ def orders(ps):
     return [list(range(p.num_beats))[::-1] if p.num_beats > 1 else None for p in ps]
 
 
-def run_alone(gpu, ps, epochs, gradients=None, modes=None):
+def run_alone(gpu, ps, epochs, gradients=None, modes=None, shares=None):
     outs = []
     for i, p in enumerate(ps):
         m = Member(gpu, p, CFGS[i % len(CFGS)], epochs, orders(ps)[i],
-                   gradients[i] if gradients else None, modes[i] if modes else 0)
+                   gradients[i] if gradients else None, modes[i] if modes else 0,
+                   shares[i] if shares else None)
         for _ in range(epochs):
             m.k.execute()
         outs.append(m.results())
@@ -113,10 +116,11 @@ def run_alone(gpu, ps, epochs, gradients=None, modes=None):
     return outs
 
 
-def run_batched(gpu, ps, epochs, vpg=0, graph=False, gradients=None, modes=None):
+def run_batched(gpu, ps, epochs, vpg=0, graph=False, gradients=None, modes=None, shares=None):
     from cardiotrust_amd.gpu import Batch
     ms = [Member(gpu, p, CFGS[i % len(CFGS)], epochs, orders(ps)[i],
-                 gradients[i] if gradients else None, modes[i] if modes else 0)
+                 gradients[i] if gradients else None, modes[i] if modes else 0,
+                 shares[i] if shares else None)
           for i, p in enumerate(ps)]
     b = Batch(gpu, [m.k for m in ms], voxels_per_group=vpg, graph=graph)
     infos = []
@@ -187,9 +191,48 @@ def test_batch_adjoint_members_and_graph(gpu, ps):
     assert_same(alone, got)
     got_g, infos = run_batched(gpu, ps, epochs=3, vpg=96, graph=True, gradients=grads)
     assert infos[-1]["graph"] == 1
-    # a reverse-mode member's backward sweep needs the device to itself
-    assert infos[-1]["side_streams"] == 0
+    # the reverse-mode members share one backward pass per update batch, so
+    # their sweeps wait for the batched sweep launch and the beats' other
+    # work runs on the side streams
+    assert infos[-1]["side_streams"] == 3
+    # the three members' sweeps (one update batch each) in the batched sweep launches
+    assert infos[-1]["sweep_jobs"] == 3
     assert_same(alone, got_g)
+
+
+def test_batch_all_reverse_mode_sweeps_per_round(gpu, ps):
+    """Every member in reverse mode. A member's backward pass (and its update)
+    waits until the member propagates again or the epoch ends, and the pending
+    passes sweep together in one batched launch. Scenario 1 updates after each
+    beat (batch_size 1, Adam): its first pass is flushed before round 1's
+    propagation together with those of scenarios 2 and 4 (one beat each), its
+    second at the end with those of scenarios 0 and 3. Eager and
+    graph-replayed epochs against each scenario alone, bit for bit."""
+    note("case: batch of 5, all reverse mode K=16, sweeps batched per round, eager and graph")
+    grads = [(1, 16)] * len(ps)
+    alone = run_alone(gpu, ps, epochs=3, gradients=grads)
+    got, infos = run_batched(gpu, ps, epochs=3, vpg=128, gradients=grads)
+    groups = sum(propagations(p, CFGS[i % len(CFGS)]) for i, p in enumerate(ps))
+    assert infos[-1]["sweep_jobs"] == groups == 6, infos[-1]
+    assert_same(alone, got)
+    got_g, infos = run_batched(gpu, ps, epochs=3, vpg=128, graph=True, gradients=grads)
+    assert infos[-1]["graph"] == 1 and infos[-1]["sweep_jobs"] == groups
+    assert_same(alone, got_g)
+
+
+def test_batch_with_a_member_sweeping_inside_its_beats(gpu, ps):
+    """Scenario 0 in reverse mode with one backward pass per beat (shared
+    propagation off): its sweeps run inside its beats, so the batch keeps
+    every scenario on the GPU's stream; the other reverse-mode member still
+    sweeps in the batched launch. Against each scenario alone, bit for bit."""
+    note("case: batch with a reverse-mode member that does not share (sweeps in its beats)")
+    grads = [(1, 16), None, (1, 16), None, None]
+    shares = [False, None, None, None, None]
+    alone = run_alone(gpu, ps, epochs=2, gradients=grads, shares=shares)
+    got, infos = run_batched(gpu, ps, epochs=2, vpg=96, gradients=grads, shares=shares)
+    assert infos[-1]["side_streams"] == 0
+    assert infos[-1]["sweep_jobs"] == 1
+    assert_same(alone, got)
 
 
 def test_batch_with_a_per_step_member(gpu, ps):
