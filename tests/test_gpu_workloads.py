@@ -1,4 +1,4 @@
-"""The two batched workloads at their stated sizes.
+"""The batched workloads at their stated sizes.
 
 Uneven workload: 64 seeded blob scenarios (voxels log-uniform in 1k-40k,
 1-8 beats, 600-2000 steps, delays 1-20), the reference derivation, default
@@ -8,6 +8,11 @@ through ct_epoch_kernel_execute, bit for bit (compared by SHA-256, so the host
 holds one member's arrays at a time), and the cheapest scenario's first epoch
 alone is compared with the FP32 oracle (states and delays bit-exact, the rest
 at 1e-4).
+
+The same 64 scenarios in reverse mode (checkpoints every 16 steps): every
+scenario's backward pass is left pending by its last beat and all 64 sweeps
+run as jobs of one batched sweep launch at the end of the epoch; two epochs
+(one eager, one graph replay) must equal each scenario alone bit for bit.
 
 Shell workload: 8 ellipsoidal shells of about 120k voxels with slow
 sub-regions, 4 beats, T = 2000, the reverse-mode gradient with checkpoints
@@ -127,6 +132,42 @@ def test_uneven_64_scenario_batch_equals_each_alone(gpu):
         assert not diff, f"scenario {i}: {diff}"
         m.close()
     assert infos[-1]["jobs"] == len(ps)  # one shared propagation per scenario
+    assert infos[-1]["side_streams"] == 3
+    assert infos[-1]["graph"] == 1
+
+
+def test_uneven_64_scenario_reverse_mode_batch_equals_each_alone(gpu):
+    from cardiotrust_amd.gpu import Batch
+    t0 = time.perf_counter()
+    ps = synth.config3(count=64, workers=BUILD_WORKERS)
+    note(f"case: 64 uneven scenarios in reverse mode (K=16), {sum(p.num_voxels for p in ps)} "
+         f"voxels, built in {time.perf_counter() - t0:.0f} s")
+    simulate_truths(gpu, ps)
+    cfg = Algorithm(learning_rate=1e-3, freeze_delays=False)
+    alone = []
+    for p in ps:
+        m = Member(gpu, p, cfg, 2, gradient=(1, 16))
+        m.k.execute()
+        m.k.execute()
+        alone.append(digest(m.r))
+        m.close()
+    note(f"  alone: 2 epochs of each scenario done ({time.perf_counter() - t0:.0f} s)")
+    members = [Member(gpu, p, cfg, 2, gradient=(1, 16)) for p in ps]
+    b = Batch(gpu, [m.k for m in members], graph=True)
+    infos = []
+    for _ in range(2):
+        b.execute()
+        infos.append(b.info())
+    b.close()
+    note(f"  batch: {infos[-1]['jobs']} propagation jobs, {infos[-1]['sweep_jobs']} sweep jobs, "
+         f"side streams {infos[-1]['side_streams']}, epochs {infos[0]['epoch_ms']:.0f} / "
+         f"{infos[1]['epoch_ms']:.0f} ms (graph {infos[1]['graph']})")
+    for i, m in enumerate(members):
+        got = digest(m.r)
+        diff = [f for f in FIELDS if got[f] != alone[i][f]]
+        assert not diff, f"scenario {i}: {diff}"
+        m.close()
+    assert infos[-1]["sweep_jobs"] == len(ps)  # one backward pass per scenario, one launch
     assert infos[-1]["side_streams"] == 3
     assert infos[-1]["graph"] == 1
 
