@@ -8,6 +8,7 @@
 // per batch, the smoothness term, the update and the batch metrics.
 #include <mutex>
 
+#include "device.h"
 #include "runtime.h"
 #include "tiles.h"
 
@@ -402,6 +403,7 @@ int plan_tiles(ct_results_gpu &r) {
     if (r.N <= 0 || prepare_status != hipSuccess) return CT_OK;
     ctrt::VoxelTiles t;
     ctrt::build_voxel_tiles(r.nbr_host, r.N, ct::kTileMaxVoxels, ct::kTileMaxHalo, ct::kNoSlot, t);
+    ctrt::order_tiles_for_dispatch(t, ct::kNumXcds);
     const int32_t G = t.tiles();
     if (G <= 0) return CT_OK;
     int32_t *vox_ptr = nullptr, *vox = nullptr, *halo_ptr = nullptr, *halo = nullptr;

@@ -133,4 +133,32 @@ void build_voxel_tiles(const std::vector<int32_t> &nbr, int32_t N, int max_voxel
     close();
 }
 
+void order_tiles_for_dispatch(VoxelTiles &t, int groups) {
+    const int32_t G = t.tiles();
+    if (G <= 1 || groups < 1) return;
+    std::vector<int32_t> perm(G);
+    for (int32_t g = 0; g < G; ++g) perm[g] = g;
+    const int32_t q = G / groups, r = G % groups;
+    for (int32_t x = 0, begin = 0; x < groups; ++x) {
+        const int32_t len = q + (x < r ? 1 : 0);
+        std::stable_sort(perm.begin() + begin, perm.begin() + begin + len, [&](int32_t a, int32_t b) {
+            return t.vox_ptr[a + 1] - t.vox_ptr[a] > t.vox_ptr[b + 1] - t.vox_ptr[b];
+        });
+        begin += len;
+    }
+    VoxelTiles o;
+    o.vox_ptr.push_back(0);
+    o.halo_ptr.push_back(0);
+    for (int32_t g : perm) {
+        o.vox.insert(o.vox.end(), t.vox.begin() + t.vox_ptr[g], t.vox.begin() + t.vox_ptr[g + 1]);
+        o.slot.insert(o.slot.end(), t.slot.begin() + size_t(t.vox_ptr[g]) * 26,
+                      t.slot.begin() + size_t(t.vox_ptr[g + 1]) * 26);
+        o.halo.insert(o.halo.end(), t.halo.begin() + t.halo_ptr[g],
+                      t.halo.begin() + t.halo_ptr[g + 1]);
+        o.vox_ptr.push_back(int32_t(o.vox.size()));
+        o.halo_ptr.push_back(int32_t(o.halo.size()));
+    }
+    t = std::move(o);
+}
+
 }  // namespace ctrt

@@ -28,4 +28,12 @@ struct VoxelTiles {
 void build_voxel_tiles(const std::vector<int32_t> &nbr, int32_t N, int max_voxels, int max_halo,
                        uint8_t no_slot, VoxelTiles &out);
 
+// Reorders the tiles for dispatch over `groups` equal ranges of tile
+// indices (the device's XCDs: launch block b runs tile xcd_block(b), so each
+// XCD takes one contiguous range in increasing order): within each range the
+// tiles with more voxels come first, so the shortest run last and the tail
+// of the launch is short. The ranges keep their tiles, and with them the
+// spatial locality the L2 of each XCD shares.
+void order_tiles_for_dispatch(VoxelTiles &t, int groups);
+
 }  // namespace ctrt

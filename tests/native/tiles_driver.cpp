@@ -27,6 +27,7 @@ int main(int argc, char **argv) {
     std::fclose(f);
     ctrt::VoxelTiles t;
     ctrt::build_voxel_tiles(nbr, N, std::atoi(argv[3]), std::atoi(argv[4]), 0xFF, t);
+    ctrt::order_tiles_for_dispatch(t, 8);  // as the library uploads them
     const std::string p = argv[5];
     const bool ok = write(p + ".vox_ptr.i32", t.vox_ptr) && write(p + ".vox.i32", t.vox) &&
                     write(p + ".halo_ptr.i32", t.halo_ptr) && write(p + ".halo.i32", t.halo) &&
