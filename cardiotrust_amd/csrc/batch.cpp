@@ -11,10 +11,10 @@
 // planner packs the jobs (team width = ceil(N / voxels per workgroup) slots,
 // length ~ T steps) onto the slots longest first, and the autotuner picks the
 // voxels per workgroup by timing the first round. Under the reverse-mode
-// gradient the backward sweeps of the scenarios whose update batch ends in a
-// round run the same way, in ONE persistent launch (adjoint_sweep_batch_kernel)
-// placed by the same planner. Optionally the whole epoch is captured once in a
-// hipGraph and replayed.
+// gradient the backward passes left pending by the scenarios' update batches
+// are flushed together, their sweeps in ONE persistent launch
+// (adjoint_sweep_batch_kernel) placed by the same planner. Optionally the
+// whole epoch is captured once in a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -55,14 +55,14 @@ struct ct_batch {
     uint32_t *flags = nullptr;  // slab flags of every job, job after job
     int32_t num_jobs = 0;
 
-    // backward sweeps: per round, the reverse-mode scenarios (sharing one
-    // backward pass per update batch) whose update batch ends in the round
-    // and whose sweep runs in the round's batched sweep launch
+    // backward sweeps: per flush point (before round p's propagation, p =
+    // rounds after the last round; see plan_sweeps), the pending passes of
+    // reverse-mode scenarios whose sweeps run in that point's batched launch
     struct SweepRound {
         std::vector<int32_t> scenarios;         // job order
         std::vector<ct::AdjointParams> params;  // each job's sweep parameters
-        int32_t job0 = 0;                       // first job / barrier of the round
-        int32_t ptr0 = 0;                       // first task_ptr entry (sweep_slots + 1 per round)
+        int32_t job0 = 0;                       // first job / barrier of the point
+        int32_t ptr0 = 0;                       // first task_ptr entry (sweep_slots + 1 per point)
     };
     std::vector<SweepRound> sweep_rounds;
     int32_t sweep_slots = 0;  // workgroups of the batched sweep launch (0 = unused)
