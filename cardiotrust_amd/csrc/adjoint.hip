@@ -157,9 +157,12 @@ constexpr int kConsGainStride = 10;  // gcons floats per (voxel, consumer) slot
 // exchange needs more registers and keeps fewer).
 constexpr int kSweepCachedCoherent = 4;
 constexpr int kSweepCachedFenced = 2;
-// The batched sweep also holds its job's fields: one item fits (two spill
-// 22 registers). It takes jobs with the coherent exchange only.
-constexpr int kBatchSweepCached = 1;
+// The batched sweep also holds its job's fields, so fewer items fit without
+// spills (1: 8 spilled registers, 2: 22, 3: 32); the spills stay out of the
+// step loop, and the cached gains win. Batched epoch of 64 uneven scenarios
+// in reverse mode (same-run A/Bs): one item 652.8 ms, two 646.9 ms; two
+// 653.0 ms, three 649.6 ms. It takes jobs with the coherent exchange only.
+constexpr int kBatchSweepCached = 3;
 constexpr uint32_t kSweepNoEntry = 0xFFFFFFFFu;
 static_assert(kSortSize >= kSweepItems * kSweepThreads || kSortSize >= kResMaxVoxels * 26,
               "every phase-1 position has a key");
