@@ -157,7 +157,9 @@ constexpr int kResCachedRows = 36;
 // The batched kernel also holds its job's fields: 3 gathers per group,
 // 9-connection groups and one cached group of 27 rows fit without spills
 // (propagation of the 64-scenario uneven batch: 230.3 ms without the cached
-// rows, 215.6 ms with them).
+// rows, 215.6 ms with them). The single-scenario setting (4-connection groups,
+// 36 rows) spills 29 registers here, inside the step loop: 229.8 against
+// 221.8 ms (same-run A/B; 234.1 ms with 2 gathers per group).
 constexpr int kBatchGather = 3;
 constexpr int kBatchGainConns = 9;
 constexpr int kBatchCachedRows = 27;
