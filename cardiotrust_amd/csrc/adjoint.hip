@@ -154,8 +154,11 @@ constexpr int kSweepBatch = 1;  // items per load group: 1 measured fastest (102
 constexpr int kConsGainStride = 10;  // gcons floats per (voxel, consumer) slot
 // Items per lane whose consumer gains and coefficient stay in registers for
 // the whole sweep instead of being re-read from gcons every step (the fenced
-// exchange needs more registers and keeps fewer).
-constexpr int kSweepCachedCoherent = 4;
+// exchange needs more registers and keeps fewer). Coherent sweep of the
+// 120k-voxel shell (same-run A/B): 4 items 54.0 ms, 5 items 53.1 ms (2
+// spilled registers, outside the step loop), 6 items 56.9 ms and 8 items
+// 65.7 ms (10 and 40 spilled, inside it).
+constexpr int kSweepCachedCoherent = 5;
 constexpr int kSweepCachedFenced = 2;
 // The batched sweep also holds its job's fields, so fewer items fit without
 // spills (1: 8 spilled registers, 2: 22, 3: 32); the spills stay out of the
