@@ -32,12 +32,13 @@ def driver(tmp_path_factory):
     return exe
 
 
-def build(driver, tmp_path, nbr, max_voxels=MAX_VOXELS, max_halo=MAX_HALO):
+def build(driver, tmp_path, nbr, max_voxels=MAX_VOXELS, max_halo=MAX_HALO, raw=False):
     N = nbr.shape[0]
     src = tmp_path / "nbr.i32"
     np.ascontiguousarray(nbr, dtype=np.int32).tofile(src)
     prefix = str(tmp_path / "tiles")
-    subprocess.run([driver, str(src), str(N), str(max_voxels), str(max_halo), prefix], check=True)
+    subprocess.run([driver, str(src), str(N), str(max_voxels), str(max_halo), prefix] +
+                   (["raw"] if raw else []), check=True)
     load = lambda name, dt: np.fromfile(f"{prefix}.{name}", dtype=dt)
     return (load("vox_ptr.i32", np.int32), load("vox.i32", np.int32), load("halo_ptr.i32", np.int32),
             load("halo.i32", np.int32), load("slot.u8", np.uint8))
@@ -90,3 +91,27 @@ def test_tiles_of_a_table_that_is_no_grid(driver, tmp_path):
     nbr = rng.integers(-1, N, size=(N, 26)).astype(np.int32)
     check(nbr, build(driver, tmp_path, nbr))
     check(nbr, build(driver, tmp_path, nbr, max_voxels=3, max_halo=60), 3, 60)
+
+
+def test_dispatch_order_keeps_each_xcd_range_and_puts_large_tiles_first(driver, tmp_path):
+    """The library uploads the tiles in dispatch order: launch block b runs
+    tile xcd_block(b), so XCD k takes one contiguous range of tile indices
+    (the first G mod 8 ranges one tile longer) in increasing order. Within
+    each range the tiles must be the builder's tiles of that range, largest
+    first (ties in builder order)."""
+    g = synth.ellipsoid_shell()
+    nbr = g.neighbours.reshape(-1, 26)
+    raw = build(driver, tmp_path, nbr, raw=True)
+    got = build(driver, tmp_path, nbr)
+    check(nbr, got)
+    G = raw[0].size - 1
+    assert got[0].size - 1 == G
+    key = lambda t, i: tuple(t[1][t[0][i]:t[0][i + 1]])  # a tile: its voxels in order
+    q, r = divmod(G, 8)
+    begin = 0
+    for x in range(8):
+        end = begin + q + (1 if x < r else 0)
+        want = sorted(range(begin, end), key=lambda i: (-(raw[0][i + 1] - raw[0][i]), i))
+        assert [key(got, i) for i in range(begin, end)] == [key(raw, i) for i in want], x
+        begin = end
+    assert begin == G
