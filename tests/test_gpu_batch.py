@@ -220,6 +220,21 @@ def test_batch_all_reverse_mode_sweeps_per_round(gpu, ps):
     assert_same(alone, got_g)
 
 
+def test_batch_graph_recaptures_past_16_reverse_mode_epochs(gpu, ps):
+    """Reverse-mode members in a graph-replayed batch: the captured epoch's
+    q rings hold room for 16 epochs of delay growth, so the batch captures
+    again after 16 replays. 18 epochs (one eager, replays across the
+    recapture), the backward passes in the batched sweep launches, against
+    each scenario alone, bit for bit."""
+    note("case: batch of 5, all reverse mode K=16, graph, 18 epochs across the recapture")
+    grads = [(1, 16)] * len(ps)
+    alone = run_alone(gpu, ps, epochs=18, gradients=grads)
+    got, infos = run_batched(gpu, ps, epochs=18, vpg=128, graph=True, gradients=grads)
+    assert [i["graph"] for i in infos] == [0] + [1] * 17
+    assert infos[-1]["sweep_jobs"] == 6
+    assert_same(alone, got)
+
+
 def test_batch_with_a_member_sweeping_inside_its_beats(gpu, ps):
     """Scenario 0 in reverse mode with one backward pass per beat (shared
     propagation off): its sweeps run inside its beats, so the batch keeps
